@@ -1,0 +1,99 @@
+"""ctypes binding of libgomc_hip.so (the C ABI in include/gomc_hip.h).
+
+The library is loaded from this package directory only.  If it is missing
+the import fails loudly: there is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+from .systems import SystemDesc
+
+LIB_PATH = Path(__file__).resolve().parent / "libgomc_hip.so"
+HEADER = Path(__file__).resolve().parent.parent / "include" / "gomc_hip.h"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_lp = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "gomc_box_create": (C.c_int, [C.POINTER(SystemDesc), C.c_int, C.POINTER(_vp)]),
+    "gomc_box_destroy": (C.c_int, [_vp]),
+    "gomc_last_error": (C.c_char_p, [_vp]),
+    "gomc_box_info": (C.c_int, [_vp, _lp]),
+    "gomc_set_coords": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "gomc_get_coords": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "gomc_commit_mol": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "gomc_build_cells": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "gomc_box_inter": (C.c_int, [_vp, _dp, _dp]),
+    "gomc_mol_inter": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "gomc_particle_inter": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                      _dp, _dp, _ip]),
+    "gomc_recip_init": (C.c_int, [_vp, _dp]),
+    "gomc_recip_sums": (C.c_int, [_vp, _dp]),
+    "gomc_mol_recip": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    "gomc_swap_recip": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "gomc_recip_accept": (C.c_int, [_vp]),
+    "gomc_recip_copy_ref_to_new": (C.c_int, [_vp]),
+    "gomc_recip_get": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "gomc_mol_trial": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    "gomc_system_total": (C.c_int, [_vp, _dp]),
+    "gomc_run_moves": (C.c_int64, [_vp, C.c_int64, C.c_int64, _ip, _ip, _ip, _ip, _dp, _dp, _dp]),
+    "gomc_energies": (C.c_int, [_vp, _dp]),
+    "gomc_get_scales": (C.c_int, [_vp, _dp, _dp]),
+    "gomc_batch_create": (C.c_int, [C.POINTER(SystemDesc), C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(_vp)]),
+    "gomc_batch_destroy": (C.c_int, [_vp]),
+    "gomc_batch_last_error": (C.c_char_p, [_vp]),
+    "gomc_batch_run": (C.c_int, [_vp, _lp]),
+    "gomc_batch_stats": (C.c_int, [_vp, _dp, _lp, _lp]),
+    "gomc_batch_get_coords": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "gomc_batch_set_trace": (C.c_int, [_vp, C.c_int, C.c_int64]),
+    "gomc_batch_get_trace": (C.c_int, [_vp, _ip, _dp, _dp, _dp, _lp]),
+    "gomc_batch_last_kernel_ms": (C.c_int, [_vp, _dp]),
+    "gomc_place_molecules": (C.c_int, [C.c_int, _ip, _dp, C.c_int, _ip, C.c_double, C.c_uint64,
+                                       C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]),
+}
+
+
+def header_symbols(path: Path = HEADER) -> list[str]:
+    """Function names declared in include/gomc_hip.h."""
+    text = re.sub(r"/\*.*?\*/", "", path.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(gomc_[a-z0-9_]+)\s*\(", text)))
+
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Load libgomc_hip.so and bind every declared entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m gomc_amd.build` "
+                          "(the HIP extension is required; there is no CPU fallback)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def ptr(a):
+    """ctypes pointer to a contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    if a.dtype.name == "float64":
+        return a.ctypes.data_as(_dp)
+    if a.dtype.name == "int32":
+        return a.ctypes.data_as(_ip)
+    if a.dtype.name == "int64":
+        return a.ctypes.data_as(_lp)
+    raise TypeError(a.dtype)

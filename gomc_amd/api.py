@@ -1,0 +1,239 @@
+"""Python handles over the C ABI: `Box` (one box, the plain path) and
+`Batch` (many boxes, one persistent launch).  Both call libgomc_hip.so for
+every energy; nothing here computes an energy on the CPU."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import ptr
+from .systems import SystemDesc, System
+
+ENERGY_NAMES = ("inter", "tail", "real", "recip", "self", "correction", "total")
+
+
+class GomcError(RuntimeError):
+    pass
+
+
+class Box:
+    """One simulation box resident on a gfx950 device (gomc_box)."""
+
+    def __init__(self, system: System, device: int = 0):
+        self.lib = _lib.load()
+        self.system = system
+        self._desc = system.desc()
+        h = C.c_void_p()
+        rc = self.lib.gomc_box_create(C.byref(self._desc), device, C.byref(h))
+        if rc != 0:
+            raise GomcError(f"gomc_box_create failed ({rc})")
+        self.h = h
+        info = np.zeros(7, dtype=np.int64)
+        self._check(self.lib.gomc_box_info(self.h, ptr(info)))
+        self.natoms, self.nmols, self.nkvec, self.ncells = (int(v) for v in info[:4])
+        self.edge = tuple(int(v) for v in info[4:])
+        self.mol_start = system.mol_start()
+
+    def _check(self, rc):
+        if rc < 0:
+            raise GomcError(self.lib.gomc_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gomc_box_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ----
+    def coords(self) -> np.ndarray:
+        x, y, z = (np.zeros(self.natoms) for _ in range(3))
+        self._check(self.lib.gomc_get_coords(self.h, ptr(x), ptr(y), ptr(z)))
+        return np.stack([x, y, z], axis=1)
+
+    def set_coords(self, xyz: np.ndarray):
+        x, y, z = (np.ascontiguousarray(xyz[:, i], dtype=np.float64) for i in range(3))
+        self._check(self.lib.gomc_set_coords(self.h, ptr(x), ptr(y), ptr(z)))
+
+    def mol_len(self, m: int) -> int:
+        return len(self.system.kinds[int(self.system.mol_kind[m])].atom_types)
+
+    # ---- energies ----
+    def box_inter(self) -> tuple[float, float]:
+        lj, real = C.c_double(), C.c_double()
+        self._check(self.lib.gomc_box_inter(self.h, C.byref(lj), C.byref(real)))
+        return lj.value, real.value
+
+    def system_total(self) -> np.ndarray:
+        out = np.zeros(7)
+        self._check(self.lib.gomc_system_total(self.h, ptr(out)))
+        return out
+
+    def energies(self) -> np.ndarray:
+        out = np.zeros(7)
+        self._check(self.lib.gomc_energies(self.h, ptr(out)))
+        return out
+
+    def _xyz3(self, xyz):
+        xyz = np.asarray(xyz, dtype=np.float64)
+        return tuple(np.ascontiguousarray(xyz[:, i]) for i in range(3))
+
+    def mol_inter(self, m: int, new_xyz) -> tuple[float, float, int]:
+        x, y, z = self._xyz3(new_xyz)
+        lj, real, ov = C.c_double(), C.c_double(), C.c_int32()
+        self._check(self.lib.gomc_mol_inter(self.h, m, ptr(x), ptr(y), ptr(z), C.byref(lj),
+                                            C.byref(real), C.byref(ov)))
+        return lj.value, real.value, ov.value
+
+    def mol_recip(self, m: int, new_xyz) -> float:
+        x, y, z = self._xyz3(new_xyz)
+        e = C.c_double()
+        self._check(self.lib.gomc_mol_recip(self.h, m, ptr(x), ptr(y), ptr(z), C.byref(e)))
+        return e.value
+
+    def mol_trial(self, m: int, new_xyz) -> np.ndarray:
+        x, y, z = self._xyz3(new_xyz)
+        out = np.zeros(4)
+        self._check(self.lib.gomc_mol_trial(self.h, m, ptr(x), ptr(y), ptr(z), ptr(out)))
+        return out
+
+    def swap_recip(self, kind: int, insert: bool, xyz) -> float:
+        x, y, z = self._xyz3(xyz)
+        e = C.c_double()
+        self._check(self.lib.gomc_swap_recip(self.h, kind, int(insert), ptr(x), ptr(y), ptr(z),
+                                             C.byref(e)))
+        return e.value
+
+    def particle_inter(self, atom_type: int, charge: float, trials, skip_mol: int = -1):
+        x, y, z = self._xyz3(trials)
+        n = len(x)
+        en, real = np.zeros(n), np.zeros(n)
+        ov = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.gomc_particle_inter(self.h, atom_type, charge, skip_mol, n, ptr(x),
+                                                 ptr(y), ptr(z), ptr(en), ptr(real), ptr(ov)))
+        return en, real, ov
+
+    def recip_init(self) -> float:
+        e = C.c_double()
+        self._check(self.lib.gomc_recip_init(self.h, C.byref(e)))
+        return e.value
+
+    def recip_sums(self) -> float:
+        e = C.c_double()
+        self._check(self.lib.gomc_recip_sums(self.h, C.byref(e)))
+        return e.value
+
+    def recip_accept(self):
+        self._check(self.lib.gomc_recip_accept(self.h))
+
+    def recip_get(self):
+        arrs = [np.zeros(self.nkvec) for _ in range(6)]
+        self._check(self.lib.gomc_recip_get(self.h, *(ptr(a) for a in arrs)))
+        return dict(zip(("kx", "ky", "kz", "prefact", "sum_r", "sum_i"), arrs))
+
+    def commit_mol(self, m: int, xyz):
+        x, y, z = self._xyz3(xyz)
+        self._check(self.lib.gomc_commit_mol(self.h, m, ptr(x), ptr(y), ptr(z)))
+
+    def build_cells(self):
+        cv = np.zeros(self.natoms, dtype=np.int32)
+        cs = np.zeros(self.ncells + 1, dtype=np.int32)
+        ac = np.zeros(self.natoms, dtype=np.int32)
+        self._check(self.lib.gomc_build_cells(self.h, ptr(cv), ptr(cs), ptr(ac)))
+        return cv, cs, ac
+
+    # ---- MC ----
+    def run(self, nsteps: int, start_step: int = 0, record: bool = True):
+        if not record:
+            return self._check(self.lib.gomc_run_moves(self.h, start_step, nsteps, None, None, None,
+                                                       None, None, None, None)), None
+        rec = dict(kind=np.zeros(nsteps, np.int32), mol=np.zeros(nsteps, np.int32),
+                   acc=np.zeros(nsteps, np.int32), ovl=np.zeros(nsteps, np.int32),
+                   dlj=np.zeros(nsteps), dreal=np.zeros(nsteps), drecip=np.zeros(nsteps))
+        n = self._check(self.lib.gomc_run_moves(self.h, start_step, nsteps, ptr(rec["kind"]),
+                                                ptr(rec["mol"]), ptr(rec["acc"]), ptr(rec["ovl"]),
+                                                ptr(rec["dlj"]), ptr(rec["dreal"]),
+                                                ptr(rec["drecip"])))
+        return n, rec
+
+    def scales(self):
+        nk = len(self.system.kinds)
+        d, r = np.zeros(nk), np.zeros(nk)
+        self._check(self.lib.gomc_get_scales(self.h, ptr(d), ptr(r)))
+        return d, r
+
+
+class Batch:
+    """Many independent boxes advanced by one persistent launch."""
+
+    def __init__(self, systems: list[System], device: int = 0, precision: int = 0):
+        self.lib = _lib.load()
+        self.systems = systems
+        arr = (SystemDesc * len(systems))()
+        for i, s in enumerate(systems):
+            arr[i] = s.desc()
+        self._descs = arr
+        h = C.c_void_p()
+        rc = self.lib.gomc_batch_create(arr, len(systems), device, precision, C.byref(h))
+        if rc != 0:
+            raise GomcError(f"gomc_batch_create failed ({rc})")
+        self.h = h
+        self.nboxes = len(systems)
+
+    def _check(self, rc):
+        if rc < 0:
+            raise GomcError(self.lib.gomc_batch_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gomc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, moves=None):
+        mv = None if moves is None else np.ascontiguousarray(moves, dtype=np.int64)
+        self._check(self.lib.gomc_batch_run(self.h, ptr(mv) if mv is not None else None))
+
+    def stats(self):
+        e = np.zeros((self.nboxes, 7))
+        acc = np.zeros(self.nboxes, dtype=np.int64)
+        done = np.zeros(self.nboxes, dtype=np.int64)
+        self._check(self.lib.gomc_batch_stats(self.h, ptr(e), ptr(acc), ptr(done)))
+        return e, acc, done
+
+    def coords(self, box: int) -> np.ndarray:
+        n = self.systems[box].natoms
+        x, y, z = (np.zeros(n) for _ in range(3))
+        self._check(self.lib.gomc_batch_get_coords(self.h, box, ptr(x), ptr(y), ptr(z)))
+        return np.stack([x, y, z], axis=1)
+
+    def set_trace(self, box: int, cap: int):
+        self._check(self.lib.gomc_batch_set_trace(self.h, box, cap))
+
+    def trace(self):
+        n = C.c_int64()
+        self._check(self.lib.gomc_batch_get_trace(self.h, None, None, None, None, C.byref(n)))
+        k = n.value
+        acc = np.zeros(k, np.int32)
+        dlj, dreal, drecip = np.zeros(k), np.zeros(k), np.zeros(k)
+        self._check(self.lib.gomc_batch_get_trace(self.h, ptr(acc), ptr(dlj), ptr(dreal),
+                                                  ptr(drecip), C.byref(n)))
+        return dict(acc=acc, dlj=dlj, dreal=dreal, drecip=drecip)
+
+    def last_kernel_ms(self) -> float:
+        v = C.c_double()
+        self._check(self.lib.gomc_batch_last_kernel_ms(self.h, C.byref(v)))
+        return v.value

@@ -1,0 +1,103 @@
+// Batched persistent MC over many independent boxes: shared layout of the
+// per-box parameter block used by the host (batch_setup.cpp) and the
+// kernel (batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/gomc_hip.h"
+
+namespace gomc {
+
+constexpr int kBatchThreads = 1024;  // one workgroup (16 waves) per box
+constexpr int kBatchMaxAtoms = 8;    // atoms per molecule
+constexpr int kBatchMaxTypes = 4;    // LJ atom types per box
+constexpr int kBatchMaxN = 32;       // largest |n| of a k-vector component
+constexpr int kDrawChunk = 64;       // moves whose Philox draws are made at once
+constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of every box stream
+
+// Read-only description of one box (host-filled, 8-byte aligned fields).
+struct BatchBox {
+  double L[3], half[3];
+  double rCutSq, rCutLJSq, rCutCoulSq, rCutLowSq;
+  double alpha, beta, pDisp;
+  double kc[3];                       // (1/L) * 2pi, the k-vector spacing
+  double sigSq[kBatchMaxAtoms * kBatchMaxAtoms];  // per (atom p, atom q) of the kind
+  double epsCn[kBatchMaxAtoms * kBatchMaxAtoms];
+  double nHalf[kBatchMaxAtoms * kBatchMaxAtoms];
+  double qq[kBatchMaxAtoms * kBatchMaxAtoms];     // q_p q_q qqFact
+  double charge[kBatchMaxAtoms];
+  int64_t atomOff, molOff, kOff;      // offsets into the flat arrays
+  int64_t perAdjust;
+  uint64_t seed;
+  int natoms, nmols, molLen, K;
+  int nmax[3], ewald, elect, pad;
+};
+
+static_assert(sizeof(BatchBox) % 8 == 0, "BatchBox is copied as 64-bit words");
+
+// Mutable per-box state kept in HBM between launches.
+struct BatchState {
+  double energy[7];   // inter, tc, real, recip, self, correction, total
+  double scale[2];    // displacement, rotation
+  int64_t step;       // moves made so far (the Philox counter)
+  int64_t tries[2], accepted[2], tempTries[2], tempAccepted[2];
+  int64_t pad;
+};
+
+struct BatchTrace {
+  int box;
+  int64_t cap;
+  int32_t *acc;
+  double *dlj, *dreal, *drecip;
+};
+
+struct BatchArgs {
+  const BatchBox *boxes;
+  BatchState *state;
+  double *x, *y, *z;          // [sum atoms]
+  double *cx, *cy, *cz;       // [sum mols] centres of mass
+  const int32_t *kn;          // [sum K] packed (nx, ny+128, nz+128) bytes
+  const double *pref;         // [sum K]
+  double *sumR, *sumI;        // [sum K] reference structure factors
+  const int64_t *moves;       // [nboxes] moves to make in this launch
+  BatchTrace trace;
+};
+
+}  // namespace gomc
+
+struct gomc_batch {
+  int device = 0;
+  int precision = 0;
+  int nboxes = 0;
+  int kpt = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double lastKernelMs = 0.0;
+  std::vector<gomc::BatchBox> hboxes;
+  gomc::BatchBox *boxes = nullptr;
+  gomc::BatchState *state = nullptr;
+  double *x = nullptr, *y = nullptr, *z = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
+  int32_t *kn = nullptr;
+  double *pref = nullptr, *sumR = nullptr, *sumI = nullptr;
+  int64_t *moves = nullptr;
+  int64_t totalAtoms = 0, totalMols = 0, totalK = 0;
+  gomc::BatchTrace trace{-1, 0, nullptr, nullptr, nullptr, nullptr};
+  int64_t traceLen = 0;
+};
+
+namespace gomc {
+// Launch the MC kernel instance for `kpt` k-vectors per thread.
+hipError_t LaunchBatch(int kpt, int precision, int nboxes, const BatchArgs &a, hipStream_t s);
+// Largest k-vectors-per-thread any compiled instance serves.
+int BatchMaxKpt();
+int BatchRoundKpt(int kpt);
+void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double volume,
+                  std::vector<double> &kx, std::vector<double> &ky, std::vector<double> &kz,
+                  std::vector<double> &pref, std::vector<int32_t> *packed);
+}  // namespace gomc

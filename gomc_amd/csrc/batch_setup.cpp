@@ -1,0 +1,311 @@
+// Host side of the batched path: builds every box once through the plain
+// path (initial energies, structure factors, COMs), packs all boxes into
+// flat HBM arrays, and launches the persistent MC kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "batch.hpp"
+#include "box.hpp"
+
+using namespace gomc;
+
+namespace {
+
+int BFail(gomc_batch *bt, int code, const std::string &msg) {
+  if (bt) bt->err = msg;
+  return code;
+}
+
+#define BHIP(bt, call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) return BFail(bt, GOMC_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+void FreeBatch(gomc_batch *bt) {
+  void *ptrs[] = {bt->boxes, bt->state, bt->x,    bt->y,    bt->z,    bt->cx,         bt->cy,
+                  bt->cz,    bt->kn,    bt->pref, bt->sumR, bt->sumI, bt->moves,      bt->trace.acc,
+                  bt->trace.dlj, bt->trace.dreal, bt->trace.drecip};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (bt->ev0) (void)hipEventDestroy(bt->ev0);
+  if (bt->ev1) (void)hipEventDestroy(bt->ev1);
+  if (bt->stream) (void)hipStreamDestroy(bt->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gomc_batch_last_error(const gomc_batch *bt) { return bt ? bt->err.c_str() : "null batch"; }
+
+int gomc_batch_destroy(gomc_batch *bt) {
+  if (!bt) return GOMC_ERR_ARG;
+  (void)hipSetDevice(bt->device);
+  FreeBatch(bt);
+  delete bt;
+  return GOMC_OK;
+}
+
+int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int precision,
+                      gomc_batch **out) {
+  if (!descs || nboxes <= 0 || !out || precision < 0 || precision > 1) return GOMC_ERR_ARG;
+  *out = nullptr;
+  gomc_batch *bt = new gomc_batch();
+  bt->device = device;
+  bt->precision = precision;
+  bt->nboxes = nboxes;
+  auto fail = [&](int code, const std::string &msg) {
+    std::string m = msg;
+    gomc_batch_destroy(bt);
+    (void)m;
+    return code;
+  };
+  std::vector<double> hx, hy, hz, hcx, hcy, hcz, hpref, hsr, hsi;
+  std::vector<int32_t> hkn;
+  std::vector<BatchState> hstate(nboxes);
+  bt->hboxes.resize(nboxes);
+  int kmax = 0;
+  for (int i = 0; i < nboxes; ++i) {
+    const gomc_system_desc &d = descs[i];
+    if (d.n_kinds != 1) return fail(GOMC_ERR_ARG, "batched boxes hold one molecule kind");
+    const int len = d.kind_natoms[0];
+    if (len <= 0 || len > kBatchMaxAtoms) return fail(GOMC_ERR_ARG, "molecule size out of range");
+    if (d.n_atom_types > kBatchMaxTypes) return fail(GOMC_ERR_ARG, "too many atom types");
+    gomc_box *b = nullptr;
+    int rc = gomc_box_create(&d, device, &b);
+    if (rc) return fail(rc, "box setup failed");
+    BatchBox &P = bt->hboxes[i];
+    std::memset(&P, 0, sizeof(P));
+    for (int a = 0; a < 3; ++a) {
+      P.L[a] = b->geom.L[a];
+      P.half[a] = b->geom.half[a];
+      P.kc[a] = (1.0 / b->geom.L[a]) * (2.0 * M_PI);
+      P.nmax[a] = int(b->recipRcut * b->geom.L[a] / (2.0 * M_PI)) + 1;
+      if (P.nmax[a] > kBatchMaxN) {
+        gomc_box_destroy(b);
+        return fail(GOMC_ERR_ARG, "box too large for the batched phase tables");
+      }
+    }
+    P.rCutSq = b->geom.rCutSq;
+    P.rCutLJSq = b->geom.rCutLJSq;
+    P.rCutCoulSq = b->geom.rCutCoulSq;
+    P.rCutLowSq = b->geom.rCutLowSq;
+    P.alpha = b->alpha;
+    P.beta = b->beta;
+    P.pDisp = b->movePerc[0] / (b->movePerc[0] + b->movePerc[1]);
+    for (int p = 0; p < len; ++p) {
+      P.charge[p] = b->kindCharge[0][p];
+      for (int q = 0; q < len; ++q) {
+        const int idx = b->kindType[0][p] + b->kindType[0][q] * b->ntypes;
+        const int pq = p * kBatchMaxAtoms + q;
+        P.sigSq[pq] = b->sigSq[idx];
+        P.epsCn[pq] = b->epsCn[idx];
+        P.nHalf[pq] = b->nHalf[idx];
+        if (b->epsCn[idx] != 0.0 && b->nHalf[idx] != 6.0) {
+          gomc_box_destroy(b);
+          return fail(GOMC_ERR_ARG, "batched boxes use the 12-6 Lennard-Jones form");
+        }
+        P.qq[pq] = b->kindCharge[0][p] * b->kindCharge[0][q] * kQQFact;
+      }
+    }
+    P.atomOff = int64_t(hx.size());
+    P.molOff = int64_t(hcx.size());
+    P.kOff = int64_t(hpref.size());
+    P.perAdjust = int64_t(b->perAdjust);
+    P.seed = d.seed;
+    P.natoms = b->natoms;
+    P.nmols = b->nmols;
+    P.molLen = len;
+    P.ewald = b->ewald;
+    P.elect = b->elect;
+    hx.insert(hx.end(), b->hx.begin(), b->hx.end());
+    hy.insert(hy.end(), b->hy.begin(), b->hy.end());
+    hz.insert(hz.end(), b->hz.begin(), b->hz.end());
+    hcx.insert(hcx.end(), b->comx.begin(), b->comx.end());
+    hcy.insert(hcy.end(), b->comy.begin(), b->comy.end());
+    hcz.insert(hcz.end(), b->comz.begin(), b->comz.end());
+    if (b->ewald) {
+      std::vector<double> kx, ky, kz, pref;
+      std::vector<int32_t> packed;
+      KVectorsOrth(b->geom.L, b->alpha * b->alpha, b->recipRcut, b->volume, kx, ky, kz, pref, &packed);
+      std::vector<double> sr(b->K), si(b->K);
+      rc = gomc_recip_get(b, nullptr, nullptr, nullptr, nullptr, sr.data(), si.data());
+      if (rc || int(pref.size()) != b->K) {
+        gomc_box_destroy(b);
+        return fail(GOMC_ERR_STATE, "k-vector setup mismatch");
+      }
+      P.K = b->K;
+      kmax = std::max(kmax, b->K);
+      hpref.insert(hpref.end(), pref.begin(), pref.end());
+      hkn.insert(hkn.end(), packed.begin(), packed.end());
+      hsr.insert(hsr.end(), sr.begin(), sr.end());
+      hsi.insert(hsi.end(), si.begin(), si.end());
+    }
+    BatchState &S = hstate[i];
+    std::memset(&S, 0, sizeof(S));
+    S.energy[0] = b->pot.inter;
+    S.energy[1] = b->pot.tc;
+    S.energy[2] = b->pot.real;
+    S.energy[3] = b->pot.recip;
+    S.energy[4] = b->pot.self;
+    S.energy[5] = b->pot.corr;
+    S.energy[6] = b->pot.total;
+    S.scale[0] = b->moves.scale[0][0];
+    S.scale[1] = b->moves.scale[1][0];
+    gomc_box_destroy(b);
+  }
+  bt->totalAtoms = int64_t(hx.size());
+  bt->totalMols = int64_t(hcx.size());
+  bt->totalK = int64_t(hpref.size());
+  const int kptNeed = (kmax + kBatchThreads - 1) / kBatchThreads;
+  bt->kpt = BatchRoundKpt(std::max(kptNeed, 1));
+  if (const char *f = std::getenv("GOMC_BATCH_KPT")) {
+    // 0 keeps S(k) in HBM; otherwise the register-resident instance, if it fits.
+    const int want = std::atoi(f);
+    if (want == 0 || (BatchRoundKpt(want) == want && want >= kptNeed)) bt->kpt = want;
+  }
+  if (hipSetDevice(device) != hipSuccess) return fail(GOMC_ERR_NODEV, "hipSetDevice");
+  auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess; };
+  bool ok = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking) == hipSuccess;
+  ok &= hipEventCreate(&bt->ev0) == hipSuccess;
+  ok &= hipEventCreate(&bt->ev1) == hipSuccess;
+  ok &= alloc((void **)&bt->boxes, sizeof(BatchBox) * nboxes);
+  ok &= alloc((void **)&bt->state, sizeof(BatchState) * nboxes);
+  ok &= alloc((void **)&bt->moves, sizeof(int64_t) * nboxes);
+  ok &= alloc((void **)&bt->x, sizeof(double) * hx.size());
+  ok &= alloc((void **)&bt->y, sizeof(double) * hx.size());
+  ok &= alloc((void **)&bt->z, sizeof(double) * hx.size());
+  ok &= alloc((void **)&bt->cx, sizeof(double) * hcx.size());
+  ok &= alloc((void **)&bt->cy, sizeof(double) * hcx.size());
+  ok &= alloc((void **)&bt->cz, sizeof(double) * hcx.size());
+  ok &= alloc((void **)&bt->kn, sizeof(int32_t) * hkn.size());
+  ok &= alloc((void **)&bt->pref, sizeof(double) * hpref.size());
+  ok &= alloc((void **)&bt->sumR, sizeof(double) * hsr.size());
+  ok &= alloc((void **)&bt->sumI, sizeof(double) * hsi.size());
+  if (!ok) return fail(GOMC_ERR_HIP, "batch allocation failed");
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  ok &= up(bt->boxes, bt->hboxes.data(), sizeof(BatchBox) * nboxes);
+  ok &= up(bt->state, hstate.data(), sizeof(BatchState) * nboxes);
+  ok &= up(bt->x, hx.data(), sizeof(double) * hx.size());
+  ok &= up(bt->y, hy.data(), sizeof(double) * hy.size());
+  ok &= up(bt->z, hz.data(), sizeof(double) * hz.size());
+  ok &= up(bt->cx, hcx.data(), sizeof(double) * hcx.size());
+  ok &= up(bt->cy, hcy.data(), sizeof(double) * hcy.size());
+  ok &= up(bt->cz, hcz.data(), sizeof(double) * hcz.size());
+  ok &= up(bt->kn, hkn.data(), sizeof(int32_t) * hkn.size());
+  ok &= up(bt->pref, hpref.data(), sizeof(double) * hpref.size());
+  ok &= up(bt->sumR, hsr.data(), sizeof(double) * hsr.size());
+  ok &= up(bt->sumI, hsi.data(), sizeof(double) * hsi.size());
+  if (!ok) return fail(GOMC_ERR_HIP, "batch upload failed");
+  *out = bt;
+  return GOMC_OK;
+}
+
+int gomc_batch_run(gomc_batch *bt, const int64_t *moves) {
+  if (!bt) return GOMC_ERR_ARG;
+  BHIP(bt, hipSetDevice(bt->device));
+  std::vector<int64_t> mv(bt->nboxes);
+  for (int i = 0; i < bt->nboxes; ++i) {
+    mv[i] = moves ? moves[i] : int64_t(bt->hboxes[i].nmols);
+    if (mv[i] < 0) return BFail(bt, GOMC_ERR_ARG, "negative move count");
+  }
+  BHIP(bt, hipMemcpyAsync(bt->moves, mv.data(), sizeof(int64_t) * bt->nboxes, hipMemcpyHostToDevice,
+                          bt->stream));
+  BatchArgs a;
+  a.boxes = bt->boxes;
+  a.state = bt->state;
+  a.x = bt->x;
+  a.y = bt->y;
+  a.z = bt->z;
+  a.cx = bt->cx;
+  a.cy = bt->cy;
+  a.cz = bt->cz;
+  a.kn = bt->kn;
+  a.pref = bt->pref;
+  a.sumR = bt->sumR;
+  a.sumI = bt->sumI;
+  a.moves = bt->moves;
+  a.trace = bt->trace;
+  BHIP(bt, hipEventRecord(bt->ev0, bt->stream));
+  BHIP(bt, LaunchBatch(bt->kpt, bt->precision, bt->nboxes, a, bt->stream));
+  BHIP(bt, hipEventRecord(bt->ev1, bt->stream));
+  BHIP(bt, hipStreamSynchronize(bt->stream));
+  float ms = 0.0f;
+  BHIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
+  bt->lastKernelMs = ms;
+  if (bt->trace.box >= 0) bt->traceLen = std::min<int64_t>(bt->trace.cap, mv[bt->trace.box]);
+  return GOMC_OK;
+}
+
+int gomc_batch_stats(const gomc_batch *bt_, double *energies, int64_t *accepted, int64_t *moves_done) {
+  gomc_batch *bt = const_cast<gomc_batch *>(bt_);
+  if (!bt) return GOMC_ERR_ARG;
+  BHIP(bt, hipSetDevice(bt->device));
+  std::vector<BatchState> st(bt->nboxes);
+  BHIP(bt, hipMemcpy(st.data(), bt->state, sizeof(BatchState) * bt->nboxes, hipMemcpyDeviceToHost));
+  for (int i = 0; i < bt->nboxes; ++i) {
+    if (energies)
+      for (int e = 0; e < 7; ++e) energies[i * 7 + e] = st[i].energy[e];
+    if (accepted) accepted[i] = st[i].accepted[0] + st[i].accepted[1];
+    if (moves_done) moves_done[i] = st[i].step;
+  }
+  return GOMC_OK;
+}
+
+int gomc_batch_get_coords(const gomc_batch *bt_, int box, double *x, double *y, double *z) {
+  gomc_batch *bt = const_cast<gomc_batch *>(bt_);
+  if (!bt || box < 0 || box >= bt->nboxes) return GOMC_ERR_ARG;
+  BHIP(bt, hipSetDevice(bt->device));
+  const BatchBox &P = bt->hboxes[box];
+  const size_t n = sizeof(double) * P.natoms;
+  BHIP(bt, hipMemcpy(x, bt->x + P.atomOff, n, hipMemcpyDeviceToHost));
+  BHIP(bt, hipMemcpy(y, bt->y + P.atomOff, n, hipMemcpyDeviceToHost));
+  BHIP(bt, hipMemcpy(z, bt->z + P.atomOff, n, hipMemcpyDeviceToHost));
+  return GOMC_OK;
+}
+
+int gomc_batch_set_trace(gomc_batch *bt, int box, int64_t cap) {
+  if (!bt || box >= bt->nboxes || cap < 0) return GOMC_ERR_ARG;
+  BHIP(bt, hipSetDevice(bt->device));
+  void *old[] = {bt->trace.acc, bt->trace.dlj, bt->trace.dreal, bt->trace.drecip};
+  for (void *p : old)
+    if (p) (void)hipFree(p);
+  bt->trace = BatchTrace{-1, 0, nullptr, nullptr, nullptr, nullptr};
+  bt->traceLen = 0;
+  if (box < 0 || cap == 0) return GOMC_OK;
+  BHIP(bt, hipMalloc(&bt->trace.acc, sizeof(int32_t) * cap));
+  BHIP(bt, hipMalloc(&bt->trace.dlj, sizeof(double) * cap));
+  BHIP(bt, hipMalloc(&bt->trace.dreal, sizeof(double) * cap));
+  BHIP(bt, hipMalloc(&bt->trace.drecip, sizeof(double) * cap));
+  bt->trace.box = box;
+  bt->trace.cap = cap;
+  return GOMC_OK;
+}
+
+int gomc_batch_get_trace(const gomc_batch *bt_, int32_t *acc, double *dlj, double *dreal,
+                         double *drecip, int64_t *n) {
+  gomc_batch *bt = const_cast<gomc_batch *>(bt_);
+  if (!bt) return GOMC_ERR_ARG;
+  BHIP(bt, hipSetDevice(bt->device));
+  const int64_t len = bt->traceLen;
+  if (n) *n = len;
+  if (len == 0) return GOMC_OK;
+  if (acc) BHIP(bt, hipMemcpy(acc, bt->trace.acc, sizeof(int32_t) * len, hipMemcpyDeviceToHost));
+  if (dlj) BHIP(bt, hipMemcpy(dlj, bt->trace.dlj, sizeof(double) * len, hipMemcpyDeviceToHost));
+  if (dreal) BHIP(bt, hipMemcpy(dreal, bt->trace.dreal, sizeof(double) * len, hipMemcpyDeviceToHost));
+  if (drecip) BHIP(bt, hipMemcpy(drecip, bt->trace.drecip, sizeof(double) * len, hipMemcpyDeviceToHost));
+  return GOMC_OK;
+}
+
+int gomc_batch_last_kernel_ms(const gomc_batch *bt, double *ms) {
+  if (!bt || !ms) return GOMC_ERR_ARG;
+  *ms = bt->lastKernelMs;
+  return GOMC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,981 @@
+// Plain single-box path on gfx950: box state resident in HBM, the device
+// cell list, the full-box pair sum, the fused single-molecule trial
+// (MoleculeInter + MolReciprocal in one launch), the structure factors,
+// swap deltas and trial-position energies, and their C entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "box.hpp"
+
+using namespace gomc;
+
+namespace gomc {
+
+int Fail(gomc_box *b, int code, const std::string &msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+int HipFail(gomc_box *b, hipError_t e, const char *what) {
+  return Fail(b, GOMC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace gomc
+
+#define HIP_TRY(b, call)                                   \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return gomc::HipFail(b, e_, #call); \
+  } while (0)
+
+namespace {
+
+// ----------------------------------------------------------------------
+// Completion of a multi-block reduction: each block publishes its partial,
+// the last block to arrive sums all partials in block order and writes
+// `nout` values to the host-mapped result, then re-arms the counter.
+__device__ void FinishPartials(const Partial4 &mine, Partial4 *partials, unsigned *counter,
+                               double *result) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = mine;
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    volatile Partial4 *vp = partials;
+    for (unsigned i = 0; i < gridDim.x; ++i) {
+      s0 += vp[i].lj;
+      s1 += vp[i].real;
+      s2 += vp[i].recip;
+      s3 += vp[i].overlap;
+    }
+    result[0] = s0;
+    result[1] = s1;
+    result[2] = s2;
+    result[3] = s3;
+    *counter = 0u;
+    __threadfence_system();
+  }
+}
+
+// ---------------------------------------------------------------- trial --
+struct TrialArgs {
+  BoxGeom g;
+  const double *x, *y, *z, *q;
+  const int *type;
+  const double *sigSq, *epsCn, *nHalf;
+  int ntypes, natoms, mstart, mlen;
+  int nPairBlocks, doPair, doRecip, K;
+  double ox[kMaxMolAtoms], oy[kMaxMolAtoms], oz[kMaxMolAtoms];
+  double nx[kMaxMolAtoms], ny[kMaxMolAtoms], nz[kMaxMolAtoms];
+  double mq[kMaxMolAtoms];
+  int mtype[kMaxMolAtoms];
+  const double *kx, *ky, *kz, *pref, *sumRref, *sumIref;
+  double *sumRnew, *sumInew;
+  // accepted coordinates of molecule [cstart, cstart+clen) not yet in HBM
+  int cstart, clen;
+  double cx[kMaxMolAtoms], cy[kMaxMolAtoms], cz[kMaxMolAtoms];
+  double *wx, *wy, *wz;
+  Partial4 *partials;
+  unsigned *counter;
+  double *result;
+};
+
+// Blocks [0, nPairBlocks): one atom j per thread against the old and new
+// positions of the molecule (CalculateEnergy.cpp:593-679).  Blocks after
+// that: one k-vector per thread, the structure-factor delta and the new
+// per-k energy (Ewald.cpp:436-467).
+__global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
+  __shared__ double red[kBlock / kWave];
+  Partial4 part{0, 0, 0, 0};
+  const int tid = threadIdx.x;
+  if (blockIdx.x == 0 && tid < a.clen) {
+    a.wx[a.cstart + tid] = a.cx[tid];
+    a.wy[a.cstart + tid] = a.cy[tid];
+    a.wz[a.cstart + tid] = a.cz[tid];
+  }
+  if (int(blockIdx.x) < a.nPairBlocks) {
+    double lj = 0.0, real = 0.0, ov = 0.0;
+    const int j = blockIdx.x * kBlock + tid;
+    if (a.doPair && j < a.natoms && (j < a.mstart || j >= a.mstart + a.mlen)) {
+      double xj, yj, zj;
+      const int cj = j - a.cstart;
+      if (cj >= 0 && cj < a.clen) {
+        xj = a.cx[cj];
+        yj = a.cy[cj];
+        zj = a.cz[cj];
+      } else {
+        xj = a.x[j];
+        yj = a.y[j];
+        zj = a.z[j];
+      }
+      const double qj = a.q[j];
+      const int tj = a.type[j];
+      for (int p = 0; p < a.mlen; ++p) {
+        const int idx = a.mtype[p] + tj * a.ntypes;
+        const double qiqj = a.mq[p] * qj * kQQFact;
+        double d2 = DistSq(a.g, a.ox[p], a.oy[p], a.oz[p], xj, yj, zj);
+        if (a.g.rCutSq > d2) {
+          if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2))
+            real += -CoulombEnergy(a.g, d2, qiqj);
+          if (!(a.g.rCutLJSq < d2)) lj += -LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+        }
+        d2 = DistSq(a.g, a.nx[p], a.ny[p], a.nz[p], xj, yj, zj);
+        if (a.g.rCutSq > d2) {
+          if (d2 < a.g.rCutLowSq) ov = 1.0;
+          if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2))
+            real += CoulombEnergy(a.g, d2, qiqj);
+          if (!(a.g.rCutLJSq < d2)) lj += LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+        }
+      }
+    }
+    part.lj = BlockSum(lj, red);
+    part.real = BlockSum(real, red);
+    part.overlap = BlockSum(ov, red);
+  } else {
+    double e = 0.0;
+    const int k = (blockIdx.x - a.nPairBlocks) * kBlock + tid;
+    if (a.doRecip && k < a.K) {
+      const double kxk = a.kx[k], kyk = a.ky[k], kzk = a.kz[k];
+      double sRN = 0.0, sIN = 0.0, sRO = 0.0, sIO = 0.0;
+      for (int p = 0; p < a.mlen; ++p) {
+        if (fabs(a.mq[p]) < 0.000000001) continue;
+        double dNew = a.nx[p] * kxk + a.ny[p] * kyk + a.nz[p] * kzk;
+        double dOld = a.ox[p] * kxk + a.oy[p] * kyk + a.oz[p] * kzk;
+        double sn, cn, so, co;
+        sincos(dNew, &sn, &cn);
+        sincos(dOld, &so, &co);
+        sRN += a.mq[p] * cn;
+        sIN += a.mq[p] * sn;
+        sRO += a.mq[p] * co;
+        sIO += a.mq[p] * so;
+      }
+      double R = a.sumRref[k] + (sRN - sRO);
+      double I = a.sumIref[k] + (sIN - sIO);
+      a.sumRnew[k] = R;
+      a.sumInew[k] = I;
+      e = (R * R + I * I) * a.pref[k];
+    }
+    part.recip = BlockSum(e, red);
+  }
+  FinishPartials(part, a.partials, a.counter, a.result);
+}
+
+// ------------------------------------------------------------ cell list --
+// CellList::PositionToCell (src/CellList.h:88-101).
+__device__ __forceinline__ int PositionToCell(double px, double py, double pz, const double *cs,
+                                              const int *edge) {
+  int ix = int(px / cs[0]), iy = int(py / cs[1]), iz = int(pz / cs[2]);
+  ix -= (ix == edge[0] ? 1 : 0);
+  iy -= (iy == edge[1] ? 1 : 0);
+  iz -= (iz == edge[2] ? 1 : 0);
+  // Coordinates are validated to [0, L) on upload; the clamp only keeps a
+  // bad value from indexing outside the cell arrays.
+  ix = min(max(ix, 0), edge[0] - 1);
+  iy = min(max(iy, 0), edge[1] - 1);
+  iz = min(max(iz, 0), edge[2] - 1);
+  return ix * edge[1] * edge[2] + iy * edge[2] + iz;
+}
+
+struct CellArgs {
+  const double *x, *y, *z;
+  int natoms, ncell;
+  double cs[3];
+  int edge[3];
+  int *cellCount, *cellCursor, *cellStart, *cellVec, *atomCell;
+};
+
+__global__ void __launch_bounds__(kBlock) CellCountKernel(const CellArgs a) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  int c = PositionToCell(a.x[i], a.y[i], a.z[i], a.cs, a.edge);
+  a.atomCell[i] = c;
+  atomicAdd(&a.cellCount[c], 1);
+}
+
+// Exclusive scan of the per-cell counts (one workgroup; ncell is small).
+__global__ void __launch_bounds__(1024) CellScanKernel(const CellArgs a) {
+  __shared__ int chunkSum[1024];
+  const int t = threadIdx.x, n = a.ncell;
+  const int per = (n + 1023) / 1024;
+  const int lo = min(n, t * per), hi = min(n, lo + per);
+  int s = 0;
+  for (int c = lo; c < hi; ++c) s += a.cellCount[c];
+  chunkSum[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int i = 0; i < 1024; ++i) {
+      int v = chunkSum[i];
+      chunkSum[i] = run;
+      run += v;
+    }
+    a.cellStart[n] = run;
+  }
+  __syncthreads();
+  int run = chunkSum[t];
+  for (int c = lo; c < hi; ++c) {
+    a.cellStart[c] = run;
+    a.cellCursor[c] = run;
+    run += a.cellCount[c];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) CellScatterKernel(const CellArgs a) {
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.natoms) return;
+  int pos = atomicAdd(&a.cellCursor[a.atomCell[i]], 1);
+  a.cellVec[pos] = i;
+}
+
+// Ascending order inside each cell (CellList.cpp:278-280) by rank: each
+// atom's slot is the number of smaller indices in its cell.
+__global__ void __launch_bounds__(kBlock) CellSortKernel(const CellArgs a) {
+  extern __shared__ int idx[];
+  const int c = blockIdx.x, s = a.cellStart[c], n = a.cellStart[c + 1] - s;
+  for (int i = threadIdx.x; i < n; i += kBlock) idx[i] = a.cellVec[s + i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kBlock) {
+    int v = idx[i], r = 0;
+    for (int j = 0; j < n; ++j) r += (idx[j] < v);
+    a.cellVec[s + r] = v;
+  }
+}
+
+// ------------------------------------------------------------- box inter --
+// One workgroup per (cell, neighbour slot); pairs i < j of different
+// molecules (CalculateEnergy.cpp:205-250).  The neighbour cell's atoms are
+// staged in LDS; threads stride over the (i, j) product.
+struct BoxInterArgs {
+  BoxGeom g;
+  const double *x, *y, *z, *q;
+  const int *type, *mol;
+  const double *sigSq, *epsCn, *nHalf;
+  int ntypes;
+  const int *cellStart, *cellVec, *nbr;
+  Partial4 *partials;
+  unsigned *counter;
+  double *result;
+};
+
+__global__ void __launch_bounds__(kBlock) BoxInterKernel(const BoxInterArgs a) {
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
+  __shared__ int sidx[kBlock], stype[kBlock], smol[kBlock];
+  __shared__ double red[kBlock / kWave];
+  const int cell = blockIdx.x / 27, nb = a.nbr[blockIdx.x];
+  const int cs = a.cellStart[cell], ce = a.cellStart[cell + 1];
+  const int ns = a.cellStart[nb], ne = a.cellStart[nb + 1];
+  const int ni = ce - cs;
+  double lj = 0.0, real = 0.0;
+  for (int tile = ns; tile < ne; tile += kBlock) {
+    const int tn = min(kBlock, ne - tile);
+    if (threadIdx.x < tn) {
+      int j = a.cellVec[tile + threadIdx.x];
+      sidx[threadIdx.x] = j;
+      sx[threadIdx.x] = a.x[j];
+      sy[threadIdx.x] = a.y[j];
+      sz[threadIdx.x] = a.z[j];
+      sq[threadIdx.x] = a.q[j];
+      stype[threadIdx.x] = a.type[j];
+      smol[threadIdx.x] = a.mol[j];
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < ni * tn; p += kBlock) {
+      const int i = a.cellVec[cs + p / tn], jj = p % tn, j = sidx[jj];
+      if (i < j && a.mol[i] != smol[jj]) {
+        double d2 = DistSq(a.g, a.x[i], a.y[i], a.z[i], sx[jj], sy[jj], sz[jj]);
+        if (a.g.rCutSq > d2) {
+          if (a.g.elect) {
+            double qiqj = a.q[i] * sq[jj] * kQQFact;
+            if (qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += CoulombEnergy(a.g, d2, qiqj);
+          }
+          if (!(a.g.rCutLJSq < d2)) {
+            int idx = a.type[i] + stype[jj] * a.ntypes;
+            lj += LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  Partial4 part{0, 0, 0, 0};
+  part.lj = BlockSum(lj, red);
+  part.real = BlockSum(real, red);
+  FinishPartials(part, a.partials, a.counter, a.result);
+}
+
+// ------------------------------------------------------ structure factor --
+// S(k) = sum_j q_j (cos, sin)(k . r_j) over all charged atoms, one k per
+// thread, atoms staged through LDS (Ewald.cpp:193-274), then the per-k
+// energy prefact (Re^2 + Im^2) (Ewald.cpp:375-406).
+struct SumsArgs {
+  const double *x, *y, *z, *q;
+  int natoms, K;
+  const double *kx, *ky, *kz, *pref;
+  double *sumR, *sumI;
+  Partial4 *partials;
+  unsigned *counter;
+  double *result;
+};
+
+__global__ void __launch_bounds__(kBlock) RecipSumsKernel(const SumsArgs a) {
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
+  __shared__ double red[kBlock / kWave];
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = k < a.K;
+  const double kxk = active ? a.kx[k] : 0.0, kyk = active ? a.ky[k] : 0.0,
+               kzk = active ? a.kz[k] : 0.0;
+  double sR = 0.0, sI = 0.0;
+  for (int tile = 0; tile < a.natoms; tile += kBlock) {
+    const int tn = min(kBlock, a.natoms - tile);
+    if (threadIdx.x < tn) {
+      int j = tile + threadIdx.x;
+      sx[threadIdx.x] = a.x[j];
+      sy[threadIdx.x] = a.y[j];
+      sz[threadIdx.x] = a.z[j];
+      sq[threadIdx.x] = a.q[j];
+    }
+    __syncthreads();
+    if (active)
+      for (int jj = 0; jj < tn; ++jj) {
+        if (fabs(sq[jj]) < 0.000000001) continue;
+        double dot = sx[jj] * kxk + sy[jj] * kyk + sz[jj] * kzk;
+        double s, c;
+        sincos(dot, &s, &c);
+        sR += sq[jj] * c;
+        sI += sq[jj] * s;
+      }
+    __syncthreads();
+  }
+  double e = 0.0;
+  if (active) {
+    a.sumR[k] = sR;
+    a.sumI[k] = sI;
+    e = (sR * sR + sI * sI) * a.pref[k];
+  }
+  Partial4 part{0, 0, 0, 0};
+  part.recip = BlockSum(e, red);
+  FinishPartials(part, a.partials, a.counter, a.result);
+}
+
+// Insertion (+) or removal (-) of one molecule: SwapDestRecip /
+// SwapSourceRecip (Ewald.cpp:478-531, 657-710).
+struct SwapArgs {
+  int K, n;
+  double sign;
+  double px[kMaxMolAtoms], py[kMaxMolAtoms], pz[kMaxMolAtoms], pq[kMaxMolAtoms];
+  const double *kx, *ky, *kz, *pref, *sumRref, *sumIref;
+  double *sumRnew, *sumInew;
+  Partial4 *partials;
+  unsigned *counter;
+  double *result;
+};
+
+__global__ void __launch_bounds__(kBlock) SwapRecipKernel(const SwapArgs a) {
+  __shared__ double red[kBlock / kWave];
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  double e = 0.0;
+  if (k < a.K) {
+    double sR = 0.0, sI = 0.0;
+    for (int p = 0; p < a.n; ++p) {
+      if (fabs(a.pq[p]) < 0.000000001) continue;
+      double dot = a.px[p] * a.kx[k] + a.py[p] * a.ky[k] + a.pz[p] * a.kz[k];
+      double s, c;
+      sincos(dot, &s, &c);
+      sR += a.pq[p] * c;
+      sI += a.pq[p] * s;
+    }
+    double R = (a.sign > 0) ? a.sumRref[k] + sR : a.sumRref[k] - sR;
+    double I = (a.sign > 0) ? a.sumIref[k] + sI : a.sumIref[k] - sI;
+    a.sumRnew[k] = R;
+    a.sumInew[k] = I;
+    e = (R * R + I * I) * a.pref[k];
+  }
+  Partial4 part{0, 0, 0, 0};
+  part.recip = BlockSum(e, red);
+  FinishPartials(part, a.partials, a.counter, a.result);
+}
+
+// Trial positions of one atom against all atoms (CalculateEnergy.cpp:741-783):
+// one workgroup per trial.
+struct ParticleArgs {
+  BoxGeom g;
+  const double *x, *y, *z, *q;
+  const int *type, *mol;
+  const double *sigSq, *epsCn, *nHalf;
+  int ntypes, natoms, atype, skipMol;
+  double charge;
+  const double *tx, *ty, *tz;
+  double *en, *real, *overlap;
+};
+
+__global__ void __launch_bounds__(kBlock) ParticleInterKernel(const ParticleArgs a) {
+  __shared__ double red[kBlock / kWave];
+  const int t = blockIdx.x;
+  const double px = a.tx[t], py = a.ty[t], pz = a.tz[t];
+  double lj = 0.0, real = 0.0, ov = 0.0;
+  for (int j = threadIdx.x; j < a.natoms; j += kBlock) {
+    if (a.mol[j] == a.skipMol) continue;
+    double d2 = DistSq(a.g, px, py, pz, a.x[j], a.y[j], a.z[j]);
+    if (a.g.rCutSq > d2) {
+      if (d2 < a.g.rCutLowSq) ov = 1.0;
+      if (!(a.g.rCutLJSq < d2)) {
+        int idx = a.atype + a.type[j] * a.ntypes;
+        lj += LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+      }
+      if (a.g.elect) {
+        double qiqj = a.q[j] * a.charge * kQQFact;
+        if (qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += CoulombEnergy(a.g, d2, qiqj);
+      }
+    }
+  }
+  lj = BlockSum(lj, red);
+  real = BlockSum(real, red);
+  ov = BlockSum(ov, red);
+  if (threadIdx.x == 0) {
+    a.en[t] = lj;
+    a.real[t] = real;
+    a.overlap[t] = ov;
+  }
+}
+
+}  // namespace
+
+namespace gomc {
+
+static int EnsurePartials(gomc_box *b, int nblocks) {
+  if (nblocks <= b->maxPartials) return GOMC_OK;
+  if (b->partials) (void)hipFree(b->partials);
+  b->partials = nullptr;
+  int n = std::max(nblocks, 1024);
+  HIP_TRY(b, hipMalloc(&b->partials, sizeof(Partial4) * n));
+  b->maxPartials = n;
+  return GOMC_OK;
+}
+
+static int Sync(gomc_box *b) {
+  HIP_TRY(b, hipStreamSynchronize(b->stream));
+  HIP_TRY(b, hipGetLastError());
+  return GOMC_OK;
+}
+
+int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const double *oz,
+                const double *nx, const double *ny, const double *nz, bool doPair,
+                bool doRecip, double out[4]) {
+  if (m < 0 || m >= b->nmols) return Fail(b, GOMC_ERR_ARG, "molecule index out of range");
+  const int len = b->molLen[m];
+  if (len > kMaxMolAtoms) return Fail(b, GOMC_ERR_ARG, "molecule has more than 16 atoms");
+  if (doRecip && !b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+  TrialArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = b->geom;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.q = b->q;
+  a.type = b->type;
+  a.sigSq = b->tSigSq;
+  a.epsCn = b->tEpsCn;
+  a.nHalf = b->tNHalf;
+  a.ntypes = b->ntypes;
+  a.natoms = b->natoms;
+  a.mstart = b->molStart[m];
+  a.mlen = len;
+  const int kind = b->molKind[m];
+  for (int p = 0; p < len; ++p) {
+    a.ox[p] = ox[p];
+    a.oy[p] = oy[p];
+    a.oz[p] = oz[p];
+    a.nx[p] = nx[p];
+    a.ny[p] = ny[p];
+    a.nz[p] = nz[p];
+    a.mq[p] = b->kindCharge[kind][p];
+    a.mtype[p] = b->kindType[kind][p];
+  }
+  a.doPair = doPair;
+  a.doRecip = doRecip && b->ewald;
+  a.nPairBlocks = doPair ? (b->natoms + kBlock - 1) / kBlock : 0;
+  const int nRecip = a.doRecip ? (b->K + kBlock - 1) / kBlock : 0;
+  a.K = b->K;
+  a.kx = b->kx;
+  a.ky = b->ky;
+  a.kz = b->kz;
+  a.pref = b->pref;
+  a.sumRref = b->sumRref;
+  a.sumIref = b->sumIref;
+  a.sumRnew = b->sumRnew;
+  a.sumInew = b->sumInew;
+  a.cstart = 0;
+  a.clen = 0;
+  if (b->pendingMol >= 0) {
+    a.cstart = b->molStart[b->pendingMol];
+    a.clen = b->molLen[b->pendingMol];
+    for (int p = 0; p < a.clen; ++p) {
+      a.cx[p] = b->pend[0][p];
+      a.cy[p] = b->pend[1][p];
+      a.cz[p] = b->pend[2][p];
+    }
+  }
+  a.wx = b->x;
+  a.wy = b->y;
+  a.wz = b->z;
+  int nblocks = a.nPairBlocks + nRecip;
+  if (nblocks == 0) nblocks = 1;
+  int rc = EnsurePartials(b, nblocks);
+  if (rc) return rc;
+  a.partials = b->partials;
+  a.counter = b->counter;
+  a.result = b->devRes;
+  hipLaunchKernelGGL(TrialKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  rc = Sync(b);
+  if (rc) return rc;
+  b->pendingMol = -1;
+  for (int i = 0; i < 3; ++i) out[i] = b->hostRes[i];
+  out[3] = b->hostRes[3] > 0.0 ? 1.0 : 0.0;  // any block saw a pair inside RcutLow
+  return GOMC_OK;
+}
+
+static int BuildCells(gomc_box *b) {
+  CellArgs a;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.natoms = b->natoms;
+  a.ncell = b->ncell;
+  for (int i = 0; i < 3; ++i) {
+    a.cs[i] = b->cellSize[i];
+    a.edge[i] = b->edge[i];
+  }
+  a.cellCount = b->cellCount;
+  a.cellCursor = b->cellCursor;
+  a.cellStart = b->cellStart;
+  a.cellVec = b->cellVec;
+  a.atomCell = b->atomCell;
+  HIP_TRY(b, hipMemsetAsync(b->cellCount, 0, sizeof(int) * b->ncell, b->stream));
+  const int g = (b->natoms + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(CellCountKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
+  hipLaunchKernelGGL(CellScanKernel, dim3(1), dim3(1024), 0, b->stream, a);
+  hipLaunchKernelGGL(CellScatterKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
+  // largest cell bounds the LDS of the sort
+  std::vector<int> counts(b->ncell);
+  HIP_TRY(b, hipMemcpyAsync(counts.data(), b->cellCount, sizeof(int) * b->ncell,
+                            hipMemcpyDeviceToHost, b->stream));
+  int rc = Sync(b);
+  if (rc) return rc;
+  int mx = *std::max_element(counts.begin(), counts.end());
+  if (mx * sizeof(int) > 60 * 1024) return Fail(b, GOMC_ERR_ARG, "cell too full for device sort");
+  hipLaunchKernelGGL(CellSortKernel, dim3(b->ncell), dim3(kBlock), std::max(mx, 1) * sizeof(int),
+                     b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  return GOMC_OK;
+}
+
+int BoxInter(gomc_box *b, double *lj, double *real) {
+  int rc = BuildCells(b);
+  if (rc) return rc;
+  BoxInterArgs a;
+  a.g = b->geom;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.q = b->q;
+  a.type = b->type;
+  a.mol = b->mol;
+  a.sigSq = b->tSigSq;
+  a.epsCn = b->tEpsCn;
+  a.nHalf = b->tNHalf;
+  a.ntypes = b->ntypes;
+  a.cellStart = b->cellStart;
+  a.cellVec = b->cellVec;
+  a.nbr = b->nbr;
+  const int nblocks = b->ncell * 27;
+  rc = EnsurePartials(b, nblocks);
+  if (rc) return rc;
+  a.partials = b->partials;
+  a.counter = b->counter;
+  a.result = b->devRes;
+  hipLaunchKernelGGL(BoxInterKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  rc = Sync(b);
+  if (rc) return rc;
+  *lj = b->hostRes[0];
+  *real = b->hostRes[1];
+  return GOMC_OK;
+}
+
+// BoxReciprocalSums into the "new" sums; returns the reciprocal energy.
+int RecipSums(gomc_box *b, double *erecip) {
+  if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+  SumsArgs a;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.q = b->q;
+  a.natoms = b->natoms;
+  a.K = b->K;
+  a.kx = b->kx;
+  a.ky = b->ky;
+  a.kz = b->kz;
+  a.pref = b->pref;
+  a.sumR = b->sumRnew;
+  a.sumI = b->sumInew;
+  const int nblocks = std::max(1, (b->K + kBlock - 1) / kBlock);
+  int rc = EnsurePartials(b, nblocks);
+  if (rc) return rc;
+  a.partials = b->partials;
+  a.counter = b->counter;
+  a.result = b->devRes;
+  hipLaunchKernelGGL(RecipSumsKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  rc = Sync(b);
+  if (rc) return rc;
+  *erecip = b->hostRes[2];
+  return GOMC_OK;
+}
+
+static double BoxHostMinImage(double d, int axis, const gomc_box *b) {
+  const double ax = b->geom.L[axis], half = b->geom.half[axis];
+  if (d > half)
+    d -= ax;
+  else if (d < -half)
+    d += ax;
+  return d;
+}
+
+// Ewald::BoxSelf, MolCorrection summed over molecules, and
+// CalculateEnergy::EnergyCorrection, from the host copy (O(molecules)).
+void HostCorrectionSelfLRC(gomc_box *b, double &self, double &corr, double &lrc) {
+  self = 0.0;
+  for (int k = 0; k < b->nkinds; ++k) {
+    double molSelf = 0.0;
+    for (double c : b->kindCharge[k]) molSelf += (c * c);
+    self += (molSelf * unsigned(b->kindCount[k]));
+  }
+  self *= -1.0 * b->alpha * kQQFact * M_2_SQRTPI * 0.5;
+  corr = 0.0;
+  for (int m = 0; m < b->nmols; ++m) {
+    const int s = b->molStart[m], len = b->molLen[m], kind = b->molKind[m];
+    double c = 0.0;
+    for (int i = 0; i < len; ++i) {
+      if (std::fabs(b->kindCharge[kind][i]) < 0.000000001) continue;
+      for (int j = i + 1; j < len; ++j) {
+        double dx = BoxHostMinImage(b->hx[s + i] - b->hx[s + j], 0, b);
+        double dy = BoxHostMinImage(b->hy[s + i] - b->hy[s + j], 1, b);
+        double dz = BoxHostMinImage(b->hz[s + i] - b->hz[s + j], 2, b);
+        double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+        c += (b->kindCharge[kind][i] * b->kindCharge[kind][j] * std::erf(b->alpha * dist) / dist);
+      }
+    }
+    corr += -1.0 * kQQFact * c * 1.0 * 1.0;
+  }
+  lrc = 0.0;
+  for (int i = 0; i < b->nkinds; ++i)
+    for (int j = 0; j < b->nkinds; ++j)
+      lrc += b->pairEnCorr[i * b->nkinds + j] * unsigned(b->kindCount[i]) *
+             unsigned(b->kindCount[j]) * b->volInv;
+}
+
+}  // namespace gomc
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *gomc_last_error(const gomc_box *b) { return b ? b->err.c_str() : "null box"; }
+
+static void FreeAll(gomc_box *b) {
+  void *ptrs[] = {b->x,        b->y,       b->z,        b->q,        b->type,   b->mol,
+                  b->tSigSq,   b->tEpsCn,  b->tNHalf,   b->tEpsCn6,  b->tNOver6, b->cellCount,
+                  b->cellCursor, b->cellStart, b->cellVec, b->atomCell, b->nbr,   b->kx,
+                  b->ky,       b->kz,      b->pref,     b->sumRref,  b->sumIref, b->sumRnew,
+                  b->sumInew,  b->partials, b->counter};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (b->hostRes) (void)hipHostFree(b->hostRes);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+}
+
+int gomc_box_destroy(gomc_box *b) {
+  if (!b) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  FreeAll(b);
+  delete b;
+  return GOMC_OK;
+}
+
+int gomc_box_info(const gomc_box *b, int64_t out[7]) {
+  if (!b) return GOMC_ERR_ARG;
+  out[0] = b->natoms;
+  out[1] = b->nmols;
+  out[2] = b->K;
+  out[3] = b->ncell;
+  out[4] = b->edge[0];
+  out[5] = b->edge[1];
+  out[6] = b->edge[2];
+  return GOMC_OK;
+}
+
+int gomc_set_coords(gomc_box *b, const double *x, const double *y, const double *z) {
+  if (!b || !x || !y || !z) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  std::copy(x, x + b->natoms, b->hx.begin());
+  std::copy(y, y + b->natoms, b->hy.begin());
+  std::copy(z, z + b->natoms, b->hz.begin());
+  const size_t bytes = sizeof(double) * b->natoms;
+  HIP_TRY(b, hipMemcpyAsync(b->x, x, bytes, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->y, y, bytes, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->z, z, bytes, hipMemcpyHostToDevice, b->stream));
+  b->pendingMol = -1;
+  return Sync(b);
+}
+
+int gomc_get_coords(const gomc_box *b, double *x, double *y, double *z) {
+  if (!b) return GOMC_ERR_ARG;
+  std::copy(b->hx.begin(), b->hx.end(), x);
+  std::copy(b->hy.begin(), b->hy.end(), y);
+  std::copy(b->hz.begin(), b->hz.end(), z);
+  return GOMC_OK;
+}
+
+int gomc_commit_mol(gomc_box *b, int m, const double *x, const double *y, const double *z) {
+  if (!b || m < 0 || m >= b->nmols) return GOMC_ERR_ARG;
+  const int s = b->molStart[m], len = b->molLen[m];
+  if (len > kMaxMolAtoms) return Fail(b, GOMC_ERR_ARG, "molecule has more than 16 atoms");
+  if (b->pendingMol >= 0 && b->pendingMol != m) {
+    // flush the earlier pending molecule first
+    (void)hipSetDevice(b->device);
+    const int ps = b->molStart[b->pendingMol], pl = b->molLen[b->pendingMol];
+    HIP_TRY(b, hipMemcpyAsync(b->x + ps, b->pend[0], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(b, hipMemcpyAsync(b->y + ps, b->pend[1], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(b, hipMemcpyAsync(b->z + ps, b->pend[2], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
+    int rc = Sync(b);
+    if (rc) return rc;
+  }
+  for (int p = 0; p < len; ++p) {
+    b->hx[s + p] = b->pend[0][p] = x[p];
+    b->hy[s + p] = b->pend[1][p] = y[p];
+    b->hz[s + p] = b->pend[2][p] = z[p];
+  }
+  b->pendingMol = m;  // written by the next launch that reads coordinates
+  return GOMC_OK;
+}
+
+}  // extern "C"
+
+namespace gomc {
+// Writes any pending accepted molecule to HBM before a kernel that reads
+// all coordinates without the trial's override.
+int FlushPending(gomc_box *b) {
+  if (b->pendingMol < 0) return GOMC_OK;
+  const int ps = b->molStart[b->pendingMol], pl = b->molLen[b->pendingMol];
+  HIP_TRY(b, hipMemcpyAsync(b->x + ps, b->pend[0], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->y + ps, b->pend[1], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->z + ps, b->pend[2], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
+  int rc = Sync(b);
+  if (rc) return rc;
+  b->pendingMol = -1;
+  return GOMC_OK;
+}
+}  // namespace gomc
+
+extern "C" {
+
+int gomc_build_cells(gomc_box *b, int32_t *cell_vector, int32_t *cell_start, int32_t *atom_cell) {
+  if (!b) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);
+  if (rc) return rc;
+  rc = BuildCells(b);
+  if (rc) return rc;
+  if (cell_vector)
+    HIP_TRY(b, hipMemcpyAsync(cell_vector, b->cellVec, sizeof(int) * b->natoms, hipMemcpyDeviceToHost, b->stream));
+  if (cell_start)
+    HIP_TRY(b, hipMemcpyAsync(cell_start, b->cellStart, sizeof(int) * (b->ncell + 1), hipMemcpyDeviceToHost, b->stream));
+  if (atom_cell)
+    HIP_TRY(b, hipMemcpyAsync(atom_cell, b->atomCell, sizeof(int) * b->natoms, hipMemcpyDeviceToHost, b->stream));
+  return Sync(b);
+}
+
+int gomc_box_inter(gomc_box *b, double *lj, double *real) {
+  if (!b || !lj || !real) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);
+  if (rc) return rc;
+  return BoxInter(b, lj, real);
+}
+
+int gomc_mol_inter(gomc_box *b, int m, const double *nx, const double *ny, const double *nz,
+                   double *dlj, double *dreal, int32_t *overlap) {
+  if (!b || m < 0 || m >= b->nmols) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  const int s = b->molStart[m];
+  double out[4];
+  int rc = LaunchTrial(b, m, &b->hx[s], &b->hy[s], &b->hz[s], nx, ny, nz, true, false, out);
+  if (rc) return rc;
+  *dlj = out[0];
+  *dreal = out[1];
+  *overlap = out[3] > 0.0 ? 1 : 0;
+  return GOMC_OK;
+}
+
+int gomc_mol_recip(gomc_box *b, int m, const double *nx, const double *ny, const double *nz,
+                   double *enew) {
+  if (!b || m < 0 || m >= b->nmols) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  const int s = b->molStart[m];
+  double out[4];
+  int rc = LaunchTrial(b, m, &b->hx[s], &b->hy[s], &b->hz[s], nx, ny, nz, false, true, out);
+  if (rc) return rc;
+  *enew = out[2];
+  return GOMC_OK;
+}
+
+int gomc_mol_trial(gomc_box *b, int m, const double *nx, const double *ny, const double *nz,
+                   double out[4]) {
+  if (!b || m < 0 || m >= b->nmols) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  const int s = b->molStart[m];
+  return LaunchTrial(b, m, &b->hx[s], &b->hy[s], &b->hz[s], nx, ny, nz, true, true, out);
+}
+
+int gomc_swap_recip(gomc_box *b, int kind, int insert, const double *x, const double *y,
+                    const double *z, double *enew) {
+  if (!b || kind < 0 || kind >= b->nkinds) return GOMC_ERR_ARG;
+  if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+  const int n = int(b->kindType[kind].size());
+  if (n > kMaxMolAtoms) return Fail(b, GOMC_ERR_ARG, "molecule has more than 16 atoms");
+  (void)hipSetDevice(b->device);
+  SwapArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.K = b->K;
+  a.n = n;
+  a.sign = insert ? 1.0 : -1.0;
+  for (int p = 0; p < n; ++p) {
+    a.px[p] = x[p];
+    a.py[p] = y[p];
+    a.pz[p] = z[p];
+    a.pq[p] = b->kindCharge[kind][p];
+  }
+  a.kx = b->kx;
+  a.ky = b->ky;
+  a.kz = b->kz;
+  a.pref = b->pref;
+  a.sumRref = b->sumRref;
+  a.sumIref = b->sumIref;
+  a.sumRnew = b->sumRnew;
+  a.sumInew = b->sumInew;
+  const int nblocks = std::max(1, (b->K + kBlock - 1) / kBlock);
+  int rc = EnsurePartials(b, nblocks);
+  if (rc) return rc;
+  a.partials = b->partials;
+  a.counter = b->counter;
+  a.result = b->devRes;
+  hipLaunchKernelGGL(SwapRecipKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  rc = Sync(b);
+  if (rc) return rc;
+  *enew = b->hostRes[2];
+  return GOMC_OK;
+}
+
+int gomc_particle_inter(gomc_box *b, int atom_type, double charge, int skip_mol, int ntrials,
+                        const double *tx, const double *ty, const double *tz, double *en,
+                        double *real, int32_t *overlap) {
+  if (!b || ntrials <= 0 || atom_type < 0 || atom_type >= b->ntypes) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);
+  if (rc) return rc;
+  double *buf = nullptr;
+  HIP_TRY(b, hipMalloc(&buf, sizeof(double) * 6 * ntrials));
+  const size_t nb = sizeof(double) * ntrials;
+  hipError_t e = hipMemcpyAsync(buf, tx, nb, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(buf + ntrials, ty, nb, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(buf + 2 * ntrials, tz, nb, hipMemcpyHostToDevice, b->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(buf);
+    return HipFail(b, e, "particle_inter upload");
+  }
+  ParticleArgs a;
+  a.g = b->geom;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.q = b->q;
+  a.type = b->type;
+  a.mol = b->mol;
+  a.sigSq = b->tSigSq;
+  a.epsCn = b->tEpsCn;
+  a.nHalf = b->tNHalf;
+  a.ntypes = b->ntypes;
+  a.natoms = b->natoms;
+  a.atype = atom_type;
+  a.skipMol = skip_mol;
+  a.charge = charge;
+  a.tx = buf;
+  a.ty = buf + ntrials;
+  a.tz = buf + 2 * ntrials;
+  a.en = buf + 3 * ntrials;
+  a.real = buf + 4 * ntrials;
+  a.overlap = buf + 5 * ntrials;
+  hipLaunchKernelGGL(ParticleInterKernel, dim3(ntrials), dim3(kBlock), 0, b->stream, a);
+  std::vector<double> host(3 * ntrials);
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host.data(), buf + 3 * ntrials, 3 * nb, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  (void)hipFree(buf);
+  if (e != hipSuccess) return HipFail(b, e, "particle_inter");
+  for (int t = 0; t < ntrials; ++t) {
+    en[t] = host[t];
+    real[t] = host[ntrials + t];
+    overlap[t] = host[2 * ntrials + t] > 0.0 ? 1 : 0;
+  }
+  return GOMC_OK;
+}
+
+int gomc_recip_sums(gomc_box *b, double *erecip) {
+  if (!b || !erecip) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);
+  if (rc) return rc;
+  return RecipSums(b, erecip);
+}
+
+int gomc_recip_accept(gomc_box *b) {
+  if (!b) return GOMC_ERR_ARG;
+  std::swap(b->sumRnew, b->sumRref);
+  std::swap(b->sumInew, b->sumIref);
+  return GOMC_OK;
+}
+
+int gomc_recip_copy_ref_to_new(gomc_box *b) {
+  if (!b) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  HIP_TRY(b, hipMemcpyAsync(b->sumRnew, b->sumRref, sizeof(double) * b->K, hipMemcpyDeviceToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->sumInew, b->sumIref, sizeof(double) * b->K, hipMemcpyDeviceToDevice, b->stream));
+  return Sync(b);
+}
+
+int gomc_recip_get(const gomc_box *b_, double *kx, double *ky, double *kz, double *prefact,
+                   double *sum_r, double *sum_i) {
+  gomc_box *b = const_cast<gomc_box *>(b_);
+  if (!b) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  const size_t n = sizeof(double) * b->K;
+  if (kx) HIP_TRY(b, hipMemcpyAsync(kx, b->kx, n, hipMemcpyDeviceToHost, b->stream));
+  if (ky) HIP_TRY(b, hipMemcpyAsync(ky, b->ky, n, hipMemcpyDeviceToHost, b->stream));
+  if (kz) HIP_TRY(b, hipMemcpyAsync(kz, b->kz, n, hipMemcpyDeviceToHost, b->stream));
+  if (prefact) HIP_TRY(b, hipMemcpyAsync(prefact, b->pref, n, hipMemcpyDeviceToHost, b->stream));
+  if (sum_r) HIP_TRY(b, hipMemcpyAsync(sum_r, b->sumRref, n, hipMemcpyDeviceToHost, b->stream));
+  if (sum_i) HIP_TRY(b, hipMemcpyAsync(sum_i, b->sumIref, n, hipMemcpyDeviceToHost, b->stream));
+  return Sync(b);
+}
+
+}  // extern "C"

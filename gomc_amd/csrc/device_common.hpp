@@ -1,0 +1,188 @@
+// Device helpers shared by the gfx950 kernels: minimum image, the Mie/LJ
+// and erfc-Coulomb pair terms, Philox4x64-10, and wave64/workgroup
+// reductions in a fixed order (so a sum does not depend on timing).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gomc {
+
+constexpr double kQQFact = 167103.208067979;  // lib/NumLib.h:22
+constexpr int kWave = 64;
+
+// BoxDimensions::MinImageSigned (src/BoxDimensions.h:169-175).
+__device__ __forceinline__ double MinImage(double d, double ax, double half) {
+  if (d > half)
+    d -= ax;
+  else if (d < -half)
+    d += ax;
+  return d;
+}
+
+// BoxDimensions::WrapPBC (src/BoxDimensions.cpp:261-296).
+__host__ __device__ __forceinline__ double Wrap(double v, double ax) {
+  if (v >= ax)
+    v -= ax;
+  else if (v < 0)
+    v += ax;
+  return v;
+}
+
+// BoxDimensions::UnwrapPBC (src/BoxDimensions.cpp:298-320).
+__host__ __device__ __forceinline__ double Unwrap(double v, double ref, double ax,
+                                                  double half) {
+  if (fabs(ref - v) > half) {
+    if (ref < half)
+      v -= ax;
+    else
+      v += ax;
+  }
+  return v;
+}
+
+// Orthogonal box geometry and cutoffs passed by value to every kernel.
+struct BoxGeom {
+  double L[3], half[3];
+  double rCutSq;      // box cutoff max(Rcut, RcutCoulomb) squared
+  double rCutLJSq;    // Rcut squared (FFParticle::CalcEn gate)
+  double rCutCoulSq;  // RcutCoulomb squared (FFParticle::CalcCoulomb gate)
+  double rCutLowSq;   // RcutLow squared (overlap test)
+  double alpha;       // Ewald alpha
+  int ewald, elect;
+};
+
+// Squared minimum-image distance in the reference's operation order
+// (InRcut, src/BoxDimensions.h:231-261): each component min-imaged, then
+// x*x + y*y + z*z left to right.  Files are built without FMA contraction.
+__device__ __forceinline__ double DistSq(const BoxGeom &g, double ax, double ay, double az,
+                                         double bx, double by, double bz) {
+  double dx = MinImage(ax - bx, g.L[0], g.half[0]);
+  double dy = MinImage(ay - by, g.L[1], g.half[1]);
+  double dz = MinImage(az - bz, g.L[2], g.half[2]);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// FFParticle::CalcEn (src/FFParticle.cpp:295-325, lambda = 1):
+// eps_cn * ((sigma^2/r^2)^(n/2) - (sigma^2/r^2)^3).  For n = 12 the
+// repulsion is the attraction squared (pow(x, 6) within an ulp or two).
+__device__ __forceinline__ double LJEnergy(double distSq, double sigmaSq, double epsCn,
+                                           double nHalf) {
+  double rRat2 = sigmaSq / distSq;
+  double rRat4 = rRat2 * rRat2;
+  double attract = rRat4 * rRat2;
+  double repulse = (nHalf == 6.0) ? attract * attract : pow(rRat2, nHalf);
+  return epsCn * (repulse - attract);
+}
+
+// FFParticle::CalcVir (src/FFParticle.cpp:327-359): F.r / r^2.
+__device__ __forceinline__ double LJVirial(double distSq, double sigmaSq, double epsCn6,
+                                           double nOver6, double nHalf) {
+  double rNeg2 = 1.0 / distSq;
+  double rRat2 = rNeg2 * sigmaSq;
+  double rRat4 = rRat2 * rRat2;
+  double attract = rRat4 * rRat2;
+  double repulse = (nHalf == 6.0) ? attract * attract : pow(rRat2, nHalf);
+  return epsCn6 * (nOver6 * repulse - attract) * rNeg2;
+}
+
+// FFParticle::CalcCoulomb (src/FFParticle.cpp:387-399).
+__device__ __forceinline__ double CoulombEnergy(const BoxGeom &g, double distSq,
+                                                double qiqj) {
+  double dist = sqrt(distSq);
+  if (g.ewald) return qiqj * erfc(g.alpha * dist) / dist;
+  return qiqj / dist;
+}
+
+// FFParticle::CalcCoulombVir (src/FFParticle.cpp:432-446).
+__device__ __forceinline__ double CoulombVirial(const BoxGeom &g, double distSq,
+                                                double qiqj) {
+  double dist = sqrt(distSq);
+  if (g.ewald) {
+    double constValue = g.alpha * M_2_SQRTPI;
+    double expConst = exp(-1.0 * (g.alpha * g.alpha) * distSq);
+    double temp = 1.0 - erf(g.alpha * dist);
+    return qiqj * (temp / dist + constValue * expConst) / distSq;
+  }
+  return qiqj / (distSq * dist);
+}
+
+// ---------------------------------------------------------- Philox -------
+// Philox4x64-10 (Salmon et al. 2011; the generator the reference vendors as
+// lib/Random123/philox.h): 10 rounds of two 64x64->128 multiplies with the
+// published multipliers, key bumped by the Weyl constants between rounds.
+struct U64x4 {
+  uint64_t v[4];
+};
+
+__host__ __device__ __forceinline__ void MulHiLo(uint64_t a, uint64_t b, uint64_t &hi,
+                                                 uint64_t &lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  lo = a * b;
+  hi = __umul64hi(a, b);
+#else
+  unsigned __int128 p = (unsigned __int128)a * b;
+  hi = uint64_t(p >> 64);
+  lo = uint64_t(p);
+#endif
+}
+
+__host__ __device__ __forceinline__ U64x4 Philox(uint64_t c0, uint64_t c1, uint64_t c2,
+                                                 uint64_t c3, uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ull, M1 = 0xCA5A826395121157ull;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ull, W1 = 0xBB67AE8584CAA73Bull;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) {
+      k0 += W0;
+      k1 += W1;
+    }
+    uint64_t hi0, lo0, hi1, lo1;
+    MulHiLo(M0, c0, hi0, lo0);
+    MulHiLo(M1, c2, hi1, lo1);
+    uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  U64x4 o;
+  o.v[0] = c0;
+  o.v[1] = c1;
+  o.v[2] = c2;
+  o.v[3] = c3;
+  return o;
+}
+
+// r123::u01<double> for a 64-bit word (lib/Random123/uniform.hpp:174-183):
+// x * 2^-64 + 2^-65, the product exact, one rounding in the add.
+__host__ __device__ __forceinline__ double U01(uint64_t v) {
+  return double(v) * 0x1p-64 + 0x1p-65;
+}
+
+// ------------------------------------------------------- reductions -------
+template <typename T>
+__device__ __forceinline__ T WaveSum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;  // valid in lane 0
+}
+
+// Sum over the workgroup; the result is valid in thread 0.  `lds` holds one
+// slot per wave.  Order: lanes by tree within a wave, then waves in index
+// order.  Contains __syncthreads().
+template <typename T>
+__device__ __forceinline__ T BlockSum(T v, T *lds) {
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int nwaves = (blockDim.x + kWave - 1) / kWave;
+  v = WaveSum(v);
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  T r = T(0);
+  if (threadIdx.x == 0)
+    for (int w = 0; w < nwaves; ++w) r += lds[w];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace gomc

@@ -1,0 +1,203 @@
+// Single-box MC driver over the plain device path: the reference's move loop
+// (Simulation::RunSimulation -> MoveSettings::AdjustMoves ->
+// System::ChooseAndRunMove, src/Simulation.cpp:94-96, src/System.cpp:262-302)
+// for Translate and Rotate, with each trial's energies from one fused launch
+// (gomc_mol_trial) and the accepted state kept in HBM.
+#include <algorithm>
+#include <cmath>
+
+#include "box.hpp"
+
+using namespace gomc;
+
+namespace {
+
+constexpr int kDisplace = 0, kRotate = 1;
+
+// MoveSettings::AdjustMoves / Adjust (src/MoveSettings.cpp:148-162,205-258).
+void AdjustMoves(gomc_box *b, uint64_t step) {
+  if ((step + 1) % b->perAdjust != 0) return;
+  const double lmin = std::min(b->geom.L[0], std::min(b->geom.L[1], b->geom.L[2]));
+  for (int mv = 0; mv < 2; ++mv)
+    for (int k = 0; k < b->nkinds; ++k) {
+      double &sc = b->moves.scale[mv][k];
+      if (b->moves.tempTries[mv][k] > 0) {
+        const double acc = double(b->moves.tempAccepted[mv][k]) / double(b->moves.tempTries[mv][k]);
+        const double fract = acc / 0.5;
+        sc *= (fract > 0.0) ? fract : 0.5;
+      }
+      const double lo = (mv == kDisplace) ? 1.0e-10 : 1.0e-6;
+      const double hi = (mv == kDisplace) ? (lmin * 0.5) - 1.0e-7 : M_PI - 1.0e-7;
+      if (sc < lo) sc = lo;
+      if (sc > hi) sc = hi;
+      b->moves.tempAccepted[mv][k] = 0;
+      b->moves.tempTries[mv][k] = 0;
+    }
+}
+
+struct Record {
+  int kind = -1, mol = -1, acc = 0, ovl = 0;
+  double dlj = 0, dreal = 0, drecip = 0;
+};
+
+// One move: PickArbDist -> PickMolAndBox -> TranslateRand/RotateRand ->
+// CalcEn -> Accept, drawing from MT19937 exactly where the reference does.
+int OneMove(gomc_box *b, Record &r) {
+  // PRNG::PickArbDist over the nine NVT move kinds (src/PRNG.h:153-163).
+  const double w[9] = {b->movePerc[0], b->movePerc[1], b->movePerc[2], 0, 0, 0, 0, 0, 0};
+  const double draw0 = b->mt.Closed() * 1.0;
+  double sum = 0.0, prev = 0.0;
+  uint32_t pick = 0;
+  for (pick = 0; pick < 9 && sum < draw0; ++pick) {
+    prev = sum;
+    sum += w[pick];
+  }
+  if (pick != 0) --pick;
+  double subDraw = draw0 - prev;
+  if (pick != kDisplace && pick != kRotate) return Fail(b, GOMC_ERR_STATE, "move kind not served by this driver");
+  // PRNG::PickMolAndBox with one box, then PickMol (src/PRNG.h:267-299).
+  const double boxDiv = b->movePerc[pick] / 1;
+  uint32_t box = uint32_t(subDraw / boxDiv);
+  if (box >= 1) box--;
+  subDraw -= box * boxDiv;
+  std::vector<int> canMove;
+  for (int m = 0; m < b->nmols; ++m)
+    if (std::find(canMove.begin(), canMove.end(), b->molKind[m]) == canMove.end())
+      canMove.push_back(b->molKind[m]);
+  const uint32_t mkTot = uint32_t(canMove.size());
+  uint32_t k = uint32_t(subDraw / (boxDiv / mkTot));
+  if (k == mkTot) k = mkTot - 1;
+  const int mk = canMove[k];
+  int mv = int(pick);
+  if (b->kindCount[mk] == 0) {
+    b->moves.tries[mv][mk]++;
+    b->moves.tempTries[mv][mk]++;
+    r.kind = mv;
+    return 0;
+  }
+  const uint32_t off = b->mt.Below(uint32_t(b->kindCount[mk]) - 1);
+  int m = -1;
+  for (int i = 0, seen = 0; i < b->nmols; ++i)
+    if (b->molKind[i] == mk && seen++ == int(off)) {
+      m = i;
+      break;
+    }
+  if (mv == kRotate && b->kindType[mk].size() <= 1) mv = kDisplace;  // Rotate::Prep
+  const int s = b->molStart[m], len = b->molLen[m];
+  const double *L = b->geom.L, *H = b->geom.half;
+  double nx[kMaxMolAtoms], ny[kMaxMolAtoms], nz[kMaxMolAtoms];
+  double ncx = b->comx[m], ncy = b->comy[m], ncz = b->comz[m];
+  if (mv == kDisplace) {
+    // Coordinates::TranslateRand (src/Coordinates.cpp:83-100).
+    const Vec3 sh = SymXYZ(b->mt, b->moves.scale[kDisplace][mk]);
+    for (int p = 0; p < len; ++p) {
+      nx[p] = Wrap(b->hx[s + p] + sh.x, L[0]);
+      ny[p] = Wrap(b->hy[s + p] + sh.y, L[1]);
+      nz[p] = Wrap(b->hz[s + p] + sh.z, L[2]);
+    }
+    ncx = Wrap(ncx + sh.x, L[0]);
+    ncy = Wrap(ncy + sh.y, L[1]);
+    ncz = Wrap(ncz + sh.z, L[2]);
+  } else {
+    // Coordinates::RotateRand (src/Coordinates.cpp:103-128).
+    const double theta = Sym(b->mt, b->moves.scale[kRotate][mk]);
+    const Vec3 axis = OnUnitSphere(b->mt);
+    const Mat3 R = AxisAngle(theta, axis);
+    const double cx = b->comx[m], cy = b->comy[m], cz = b->comz[m];
+    for (int p = 0; p < len; ++p) {
+      Vec3 v{Unwrap(b->hx[s + p], cx, L[0], H[0]) + (-cx), Unwrap(b->hy[s + p], cy, L[1], H[1]) + (-cy),
+             Unwrap(b->hz[s + p], cz, L[2], H[2]) + (-cz)};
+      v = R.Apply(v);
+      nx[p] = Wrap(v.x + cx, L[0]);
+      ny[p] = Wrap(v.y + cy, L[1]);
+      nz[p] = Wrap(v.z + cz, L[2]);
+    }
+  }
+  const double pr = b->mt.Closed();  // Translate/Rotate::Accept draw
+  double out[4];
+  int rc = LaunchTrial(b, m, &b->hx[s], &b->hy[s], &b->hz[s], nx, ny, nz, true, true, out);
+  if (rc) return rc;
+  const bool overlap = out[3] > 0.0;
+  const double dlj = out[0], dreal = out[1];
+  const double drecip = (!overlap && b->ewald) ? out[2] - b->pot.recip : 0.0;
+  const bool res = pr < std::exp(-b->beta * (dlj + dreal + drecip));
+  const bool result = res && !overlap;
+  if (result) {
+    b->pot.inter += dlj;
+    b->pot.real += dreal;
+    b->pot.recip += drecip;
+    rc = gomc_commit_mol(b, m, nx, ny, nz);
+    if (rc) return rc;
+    if (mv == kDisplace) {
+      b->comx[m] = ncx;
+      b->comy[m] = ncy;
+      b->comz[m] = ncz;
+    }
+    if (b->ewald) gomc_recip_accept(b);
+    b->pot.Total();
+  }
+  b->moves.tries[mv][mk]++;
+  b->moves.tempTries[mv][mk]++;
+  if (result) {
+    b->moves.tempAccepted[mv][mk]++;
+    b->moves.accepted[mv][mk]++;
+  }
+  r.kind = mv;
+  r.mol = m;
+  r.acc = result;
+  r.ovl = overlap;
+  r.dlj = dlj;
+  r.dreal = dreal;
+  r.drecip = drecip;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t gomc_run_moves(gomc_box *b, int64_t start_step, int64_t nsteps, int32_t *kind,
+                       int32_t *mol, int32_t *acc, int32_t *ovl, double *dlj, double *dreal,
+                       double *drecip) {
+  if (!b || nsteps < 0) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  int64_t accepted = 0;
+  for (int64_t i = 0; i < nsteps; ++i) {
+    AdjustMoves(b, uint64_t(start_step + i));
+    Record r;
+    int rc = OneMove(b, r);
+    if (rc) return rc;
+    accepted += r.acc;
+    if (kind) kind[i] = r.kind;
+    if (mol) mol[i] = r.mol;
+    if (acc) acc[i] = r.acc;
+    if (ovl) ovl[i] = r.ovl;
+    if (dlj) dlj[i] = r.dlj;
+    if (dreal) dreal[i] = r.dreal;
+    if (drecip) drecip[i] = r.drecip;
+  }
+  return accepted;
+}
+
+int gomc_energies(const gomc_box *b, double out[7]) {
+  if (!b || !out) return GOMC_ERR_ARG;
+  out[0] = b->pot.inter;
+  out[1] = b->pot.tc;
+  out[2] = b->pot.real;
+  out[3] = b->pot.recip;
+  out[4] = b->pot.self;
+  out[5] = b->pot.corr;
+  out[6] = b->pot.total;
+  return GOMC_OK;
+}
+
+int gomc_get_scales(const gomc_box *b, double *disp, double *rot) {
+  if (!b) return GOMC_ERR_ARG;
+  for (int k = 0; k < b->nkinds; ++k) {
+    if (disp) disp[k] = b->moves.scale[0][k];
+    if (rot) rot[k] = b->moves.scale[1][k];
+  }
+  return GOMC_OK;
+}
+
+}  // extern "C"

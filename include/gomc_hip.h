@@ -1,0 +1,216 @@
+/* gomc_hip.h -- C ABI of the MI355X (gfx950) energy hot path for GOMC.
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success and
+ * a negative code on failure (gomc_last_error() gives the text), and never
+ * exits the process.  The reference's GPU seam is a set of host free
+ * functions taking VariablesCUDA* (the src/GPU headers); each entry point below
+ * names the reference function it replaces.  One gomc_box holds one
+ * simulation box resident in HBM (coordinates, force-field tables, cell
+ * list, Ewald k-vectors and structure factors) across calls, so no call
+ * re-uploads the box.
+ */
+#ifndef GOMC_HIP_H
+#define GOMC_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GOMC_OK 0
+#define GOMC_ERR_ARG -1     /* bad argument / size */
+#define GOMC_ERR_HIP -2     /* HIP runtime error */
+#define GOMC_ERR_STATE -3   /* call out of order (e.g. no k-vectors yet) */
+#define GOMC_ERR_NODEV -4   /* no usable gfx950 device */
+
+/* One box: force field, molecule kinds, molecules, coordinates, moves.
+ * Field meaning follows the reference's in.conf keywords:
+ *   box        -- CellBasisVector1..3 lengths (orthogonal box), A
+ *   temperature-- Temperature, K
+ *   rcut, rcut_low, rcut_coulomb, tolerance -- Rcut, RcutLow,
+ *                 RcutCoulomb, Tolerance (src/ConfigSetup.cpp:671-701)
+ *   ewald, electrostatic, lrc -- Ewald, ElectroStatic, LRC
+ *   sigma/epsilon/n_rep -- Mie parameters per atom type (A, K, exponent)
+ *   kind_* -- molecule kinds: atom types and charges (e)
+ *   mol_kind -- kind of each molecule; atoms of molecule m are contiguous
+ *   x,y,z  -- wrapped coordinates of all atoms, A
+ *   seed   -- Random_Seed; move_* -- DisFreq, RotFreq, MultiParticleFreq
+ *   adj_steps -- AdjSteps
+ * The same layout is used by the CPU oracle's orc_system_desc. */
+typedef struct gomc_system_desc {
+  double box[3];
+  double temperature;
+  double rcut, rcut_low, rcut_coulomb, tolerance;
+  int32_t ewald, electrostatic, lrc, threads;
+  int32_t n_atom_types, n_kinds;
+  const double *sigma, *epsilon, *n_rep;
+  const int32_t *kind_natoms;
+  const int32_t *kind_atom_type;
+  const double *kind_charge;
+  int32_t n_mols, _pad0;
+  const int32_t *mol_kind;
+  const double *x, *y, *z;
+  uint64_t seed;
+  double move_disp, move_rot, move_mp;
+  int64_t adj_steps;
+} gomc_system_desc;
+
+typedef struct gomc_box gomc_box;
+
+/* ---- lifetime -------------------------------------------------------- */
+/* Replaces InitGPUForceField + InitCoordinatesCUDA + InitEwaldVariablesCUDA
+ * (src/GPU/ConstantDefinitionsCUDAKernel.cuh:22-46), called from
+ * FFParticle::Init (src/FFParticle.cpp:54-93) and
+ * CalculateEnergy::Init (src/CalculateEnergy.cpp:60-81). */
+int gomc_box_create(const gomc_system_desc *desc, int device, gomc_box **out);
+/* Replaces DestroyCUDAVars (src/GPU/ConstantDefinitionsCUDAKernel.cuh:44). */
+int gomc_box_destroy(gomc_box *b);
+const char *gomc_last_error(const gomc_box *b);
+/* Sizes: [n_atoms, n_mols, n_kvec, n_cells, cells_per_edge_x, y, z]. */
+int gomc_box_info(const gomc_box *b, int64_t out[7]);
+
+/* ---- coordinates ----------------------------------------------------- */
+/* Replaces the per-call coordinate uploads inside every Call*GPU
+ * (e.g. src/GPU/CalculateEnergyCUDAKernel.cu:70-87). */
+int gomc_set_coords(gomc_box *b, const double *x, const double *y,
+                    const double *z);
+int gomc_get_coords(const gomc_box *b, double *x, double *y, double *z);
+/* Writes the coordinates of one molecule (XYZArray::CopyRange at accept,
+ * src/moves/Translate.h:114). */
+int gomc_commit_mol(gomc_box *b, int mol, const double *x, const double *y,
+                    const double *z);
+
+/* ---- cell list (src/CellList.cpp:220-285) ---------------------------- */
+/* GridAll + GetCellListNeighbor on the device: counting sort of atoms into
+ * cells, each cell's atoms in ascending index order (the CSR form the
+ * reference builds on the host every BoxInter call).  Outputs may be NULL;
+ * cell_vector[n_atoms], cell_start[n_cells+1], atom_cell[n_atoms]. */
+int gomc_build_cells(gomc_box *b, int32_t *cell_vector, int32_t *cell_start,
+                     int32_t *atom_cell);
+
+/* ---- real-space pair energies ---------------------------------------- */
+/* CalculateEnergy::BoxInter (src/CalculateEnergy.cpp:157-266); replaces
+ * CallBoxInterGPU (src/GPU/CalculateEnergyCUDAKernel.cuh:17-26).  Rebuilds
+ * the cell list first. */
+int gomc_box_inter(gomc_box *b, double *lj, double *real);
+/* CalculateEnergy::MoleculeInter (src/CalculateEnergy.cpp:581-686): the
+ * change of LJ and real-space Coulomb energy when molecule `mol` moves to
+ * (nx,ny,nz); *overlap = 1 if a new pair is closer than RcutLow.  No GPU
+ * counterpart exists in the reference (host OpenMP only). */
+int gomc_mol_inter(gomc_box *b, int mol, const double *nx, const double *ny,
+                   const double *nz, double *dlj, double *dreal,
+                   int32_t *overlap);
+/* CalculateEnergy::ParticleInter (src/CalculateEnergy.cpp:727-785): LJ and
+ * real energies of `ntrials` trial positions of one atom (type, charge)
+ * against every atom of the box; en/real/overlap are written (not added).
+ * Atoms of molecule `skip_mol` (>=0) are left out. */
+int gomc_particle_inter(gomc_box *b, int atom_type, double charge,
+                        int skip_mol, int ntrials, const double *tx,
+                        const double *ty, const double *tz, double *en,
+                        double *real, int32_t *overlap);
+
+/* ---- Ewald reciprocal space (src/Ewald.cpp) -------------------------- */
+/* RecipInitOrth + BoxReciprocalSetup + SetRecipRef (Ewald.cpp:847-903,
+ * 193-274, 1021-1053); replaces CallBoxReciprocalSetupGPU
+ * (src/GPU/CalculateEwaldCUDAKernel.cuh:16-24).  *erecip = BoxReciprocal. */
+int gomc_recip_init(gomc_box *b, double *erecip);
+/* BoxReciprocalSums + BoxReciprocal (Ewald.cpp:281-406); replaces
+ * CallBoxReciprocalSumsGPU.  Fills the "new" sums from the current
+ * coordinates; gomc_recip_accept makes them the reference sums. */
+int gomc_recip_sums(gomc_box *b, double *erecip);
+/* Ewald::MolReciprocal (Ewald.cpp:409-473); replaces CallMolReciprocalGPU
+ * (src/GPU/CalculateEwaldCUDAKernel.cuh:31-36).  *enew = reciprocal energy
+ * with molecule `mol` at (nx,ny,nz); the caller's delta is
+ * enew - (current reciprocal energy).  No structure factors leave HBM. */
+int gomc_mol_recip(gomc_box *b, int mol, const double *nx, const double *ny,
+                   const double *nz, double *enew);
+/* SwapDestRecip / SwapSourceRecip (Ewald.cpp:478-531, 657-710); replaces
+ * CallSwapReciprocalGPU.  insert=1 adds, 0 removes the molecule of `kind`
+ * at (x,y,z); *enew as above. */
+int gomc_swap_recip(gomc_box *b, int kind, int insert, const double *x,
+                    const double *y, const double *z, double *enew);
+/* UpdateRecip (Ewald.cpp:1420-1434): the new sums become the reference
+ * sums (a pointer swap); replaces UpdateRecipCUDA. */
+int gomc_recip_accept(gomc_box *b);
+/* CopyRecip (Ewald.cpp:1441-1460); replaces CopyRefToNewCUDA. */
+int gomc_recip_copy_ref_to_new(gomc_box *b);
+/* Reference k-vectors and sums for inspection: arrays of n_kvec; any NULL. */
+int gomc_recip_get(const gomc_box *b, double *kx, double *ky, double *kz,
+                   double *prefact, double *sum_r, double *sum_i);
+
+/* ---- fused single-molecule trial ------------------------------------- */
+/* MoleculeInter + MolReciprocal of one displacement/rotation trial in one
+ * launch (the pair blocks and the k-vector blocks share the grid), one
+ * host read-back of out[4] = {dLJ, dReal, recip_new, overlap}. */
+int gomc_mol_trial(gomc_box *b, int mol, const double *nx, const double *ny,
+                   const double *nz, double out[4]);
+
+/* ---- whole-box totals ------------------------------------------------ */
+/* CalculateEnergy::SystemTotal for one box (src/CalculateEnergy.cpp:83-138)
+ * without intramolecular bonded terms (rigid molecules):
+ * out = {inter, tailCorrection, real, recip, self, correction, total}.
+ * Recomputes the reciprocal sums from the coordinates. */
+int gomc_system_total(gomc_box *b, double out[7]);
+
+/* ---- plain single-box MC driver (src/System.cpp:262-302) ------------- */
+/* Runs `nsteps` GOMC moves (Translate / Rotate, MT19937 draws in the
+ * reference's order, Metropolis acceptance, MoveSettings adaptation) with
+ * every energy evaluated by gomc_mol_trial.  Per-move records may be NULL.
+ * Returns the number of accepted moves (>= 0) or an error code. */
+int64_t gomc_run_moves(gomc_box *b, int64_t start_step, int64_t nsteps,
+                       int32_t *kind, int32_t *mol, int32_t *acc,
+                       int32_t *ovl, double *dlj, double *dreal,
+                       double *drecip);
+/* Running energies of the driver (same order as gomc_system_total). */
+int gomc_energies(const gomc_box *b, double out[7]);
+/* Current displacement / rotation scales per molecule kind. */
+int gomc_get_scales(const gomc_box *b, double *disp, double *rot);
+
+/* ---- batched persistent MC (many independent boxes, one launch) ------ */
+typedef struct gomc_batch gomc_batch;
+/* Boxes hold one molecule kind each (rigid, up to 8 atoms).  Each box runs
+ * its own Philox4x64-10 stream (key {seed, tag}, counter {move, block}).
+ * precision: 0 = FP64 everywhere; 1 = FP32 pair terms and phases with FP64
+ * accumulation, running totals and Metropolis exponent. */
+int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device,
+                      int precision, gomc_batch **out);
+int gomc_batch_destroy(gomc_batch *bt);
+const char *gomc_batch_last_error(const gomc_batch *bt);
+/* Advance box i by moves[i] moves (NULL: by its molecule count, one MC
+ * cycle) in one persistent launch. */
+int gomc_batch_run(gomc_batch *bt, const int64_t *moves);
+/* Per-box running energies [nboxes][7], accepted counts [nboxes], moves
+ * made [nboxes]; any NULL. */
+int gomc_batch_stats(const gomc_batch *bt, double *energies,
+                     int64_t *accepted, int64_t *moves_done);
+int gomc_batch_get_coords(const gomc_batch *bt, int box, double *x, double *y,
+                          double *z);
+/* Per-move trace of box `box` over the last gomc_batch_run (first `cap`
+ * moves): accepted flag and the three energy deltas. */
+int gomc_batch_set_trace(gomc_batch *bt, int box, int64_t cap);
+int gomc_batch_get_trace(const gomc_batch *bt, int32_t *acc, double *dlj,
+                         double *dreal, double *drecip, int64_t *n);
+/* Kernel-only time of the last gomc_batch_run in ms (HIP events on the
+ * launch stream). */
+int gomc_batch_last_kernel_ms(const gomc_batch *bt, double *ms);
+
+/* ---- synthetic inputs (host only) ------------------------------------ */
+/* Random sequential insertion of rigid molecules into a cubic box of edge
+ * `box`: uniform centre, uniform orientation (splitmix64 stream from
+ * `seed`), rejected if the first atoms of two molecules are closer than
+ * min_heavy or any other inter-molecular pair closer than min_any.
+ * kind_geom holds body-frame xyz triples of every kind's atoms in order.
+ * No counterpart in the reference: it stands in for the PDB coordinates
+ * the reference reads at start-up.  GOMC_ERR_STATE if a molecule cannot be
+ * placed within max_tries. */
+int gomc_place_molecules(int n_kinds, const int32_t *kind_natoms,
+                         const double *kind_geom, int n_mols,
+                         const int32_t *mol_kind, double box, uint64_t seed,
+                         double min_heavy, double min_any, int max_tries,
+                         double *x, double *y, double *z);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GOMC_HIP_H */
