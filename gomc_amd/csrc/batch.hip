@@ -87,17 +87,9 @@ __device__ __forceinline__ void KDelta(const Cplx (*tab)[3][kBatchMaxN + 1], con
   dI = sIN - sIO;
 }
 
-// Real-space Coulomb term qq erfc(alpha r) / r (FFParticle::CalcCoulomb,
-// src/FFParticle.cpp:387-399), out of line to keep the FP64 erfc's
-// registers out of the pair loop's budget.
-__device__ __noinline__ double CoulombTerm(double d2, double qiqj, double alpha) {
-  const double dist = sqrt(d2);
-  return qiqj * erfc(alpha * dist) / dist;
-}
-
 // One phase-table entry: e^{i (c_d n) r} (kept out of line: the FP64
 // sincos argument reduction is register-hungry and runs once per entry).
-__device__ __noinline__ Cplx PhaseEntry(double kc, int n, double r) {
+__device__ __forceinline__ Cplx PhaseEntry(double kc, int n, double r) {
   const double ang = (kc * double(n)) * r;
   double s, c;
   sincos(ang, &s, &c);
@@ -174,6 +166,189 @@ __device__ __noinline__ void MoveDraws(uint64_t seed, uint64_t step, double *out
   out[5] = U01(r1.v[1]);
 }
 
+// Phase tables e^{i (c_d n) r_d}, n = 0..nmax_d, for the moving molecule's
+// old (pos < len) and new positions; one entry per thread-iteration.
+__device__ __noinline__ void BuildTables(const BatchBox &sb, const MoveShared &ms, int len, int nTab,
+                                         Cplx (*tab)[3][kBatchMaxN + 1]) {
+  const int per = (sb.nmax[0] + 1) + (sb.nmax[1] + 1) + (sb.nmax[2] + 1);
+  for (int e = threadIdx.x; e < nTab; e += kBatchThreads) {
+    const int pos = e / per;
+    int rem = e - pos * per;
+    int d = 0;
+    while (rem > sb.nmax[d]) {
+      rem -= sb.nmax[d] + 1;
+      ++d;
+    }
+    const double r = pos < len ? ms.oldp[d][pos] : ms.newp[d][pos - len];
+    tab[pos][d][rem] = PhaseEntry(sb.kc[d], rem, r);
+  }
+}
+
+// Per-wave queue of in-cutoff pairs: squared distance and a code
+// (pair-table index | new-position flag << 8).
+struct PairQueue {
+  double d2[2 * kWave];
+  int code[2 * kWave];
+};
+
+// Energy terms of one queued pair (FFParticle::CalcEn / CalcCoulomb,
+// src/FFParticle.cpp:295-399), signed: + new position, - old position.
+__device__ __forceinline__ void PairTerm(const BatchBox &sb, double d2, int code, double &lj,
+                                         double &real) {
+  const int pq = code & 0xff;
+  const double sign = (code >> 8) ? 1.0 : -1.0;
+  const double qiqj = sb.qq[pq], eps = sb.epsCn[pq];
+  if (sb.elect && qiqj != 0.0 && !(sb.rCutCoulSq < d2)) {
+    const double dist = sqrt(d2);
+    const double ec = sb.ewald ? qiqj * erfc(sb.alpha * dist) / dist : qiqj / dist;
+    real += sign * ec;
+  }
+  if (eps != 0.0 && !(sb.rCutLJSq < d2)) {
+    // n = 12 (checked at setup): repulsion = attraction^2
+    const double rRat2 = sb.sigSq[pq] / d2;
+    const double att = rRat2 * rRat2 * rRat2;
+    lj += sign * (eps * (att * att - att));
+  }
+}
+
+// Evaluates `n` queued pairs (one per lane) and adds them to lj/real.
+__device__ __noinline__ void DrainQueue(const BatchBox &sb, const PairQueue &q, int n,
+                                        double *lj, double *real) {
+  const int lane = threadIdx.x & (kWave - 1);
+  if (lane < n) PairTerm(sb, q.d2[lane], q.code[lane], *lj, *real);
+}
+
+// Pair-energy change of the move (MoleculeInter, src/CalculateEnergy.cpp:
+// 581-686).  Work items are the atoms j of the box (the moving molecule's
+// own atoms skipped); each item is tested against the old and new
+// positions of all len moving atoms.  Distances inside the cutoff are
+// compacted into a per-wave queue (ballot + prefix count) and their
+// energies evaluated 64 at a time, so the erfc path runs with full waves
+// instead of being masked through for the few in-cutoff lanes.  X/Y/Z may
+// point to LDS or HBM.  acc = {lj, real, -, overlap} partial sums.
+__device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms, const double *X,
+                                       const double *Y, const double *Z, int m, int len,
+                                       int natoms, PairQueue *queues, double *acc4) {
+  BoxGeom g;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    g.L[d] = sb.L[d];
+    g.half[d] = sb.half[d];
+  }
+  const int lane = threadIdx.x & (kWave - 1);
+  PairQueue &q = queues[threadIdx.x / kWave];
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int count = 0;  // wave-uniform
+  double lj = 0.0, real = 0.0, ov = 0.0;
+  const int mlo = m * len, mhi = mlo + len;
+  const int nIter = (natoms + kBatchThreads - 1) / kBatchThreads;
+  for (int it = 0; it < nIter; ++it) {
+    const int j = it * kBatchThreads + int(threadIdx.x);
+    const bool valid = j < natoms && (j < mlo || j >= mhi);
+    const int jj = valid ? j : 0;
+    const double xj = X[jj], yj = Y[jj], zj = Z[jj];
+    const uint32_t mj = uint32_t((uint64_t(uint32_t(jj)) * sb.magicLen) >> 32);
+    const int qa = jj - int(mj) * len;
+    for (int p = 0; p < len; ++p) {
+      const int pq = p * kBatchMaxAtoms + qa;
+      const bool live = valid && (sb.qq[pq] != 0.0 || sb.epsCn[pq] != 0.0);
+#pragma unroll
+      for (int isNew = 0; isNew < 2; ++isNew) {
+        const double px = isNew ? ms.newp[0][p] : ms.oldp[0][p];
+        const double py = isNew ? ms.newp[1][p] : ms.oldp[1][p];
+        const double pz = isNew ? ms.newp[2][p] : ms.oldp[2][p];
+        const double d2 = DistSq(g, px, py, pz, xj, yj, zj);
+        const bool inCut = valid && sb.rCutSq > d2;
+        if (isNew && inCut && d2 < sb.rCutLowSq) ov = 1.0;
+        const bool push = live && inCut;
+        const uint64_t mask = __ballot(push);
+        if (push) {
+          const int slot = count + __popcll(mask & below);
+          q.d2[slot] = d2;
+          q.code[slot] = pq | (isNew << 8);
+        }
+        count += __popcll(mask);
+        if (count >= kWave) {  // drain one full wave of pairs
+          __builtin_amdgcn_wave_barrier();
+          DrainQueue(sb, q, kWave, &lj, &real);
+          const double d2n = q.d2[kWave + lane];
+          const int cn = q.code[kWave + lane];
+          __builtin_amdgcn_wave_barrier();
+          q.d2[lane] = d2n;
+          q.code[lane] = cn;
+          count -= kWave;
+          __builtin_amdgcn_wave_barrier();
+        }
+      }
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  DrainQueue(sb, q, count, &lj, &real);
+  acc4[0] = lj;
+  acc4[1] = real;
+  acc4[3] = ov;
+}
+
+// Reciprocal-space energy change of the move (MolReciprocal,
+// src/Ewald.cpp:409-473): per k-vector the change (dR, dI) of the
+// structure factor and prefact * ((R+dR)^2 + (I+dI)^2 - R^2 - I^2),
+// written as prefact * (dR (2R + dR) + dI (2I + dI)) so no large terms
+// cancel.  (dR, dI) are kept for the commit.  Returns this thread's sum.
+__device__ __noinline__ double RecipPhase(const BatchBox &sb, const Cplx (*tab)[3][kBatchMaxN + 1],
+                                          int len, int K, const int32_t *kn, const double *pref,
+                                          const double *sumR, const double *sumI, double *dRs,
+                                          double *dIs) {
+  double e = 0.0;
+  for (int k = threadIdx.x; k < K; k += kBatchThreads) {
+    double dR, dI;
+    KDelta(tab, sb.charge, len, kn[k], dR, dI);
+    const double r0 = sumR[k], i0 = sumI[k];
+    dRs[k] = dR;
+    dIs[k] = dI;
+    e += pref[k] * (dR * (2.0 * r0 + dR) + dI * (2.0 * i0 + dI));
+  }
+  return e;
+}
+
+// Accepted move: S(k) += (dR, dI) (Ewald::UpdateRecip, src/Ewald.cpp:1420).
+__device__ __noinline__ void RecipCommit(int K, double *sumR, double *sumI, const double *dRs,
+                                         const double *dIs) {
+  for (int k = threadIdx.x; k < K; k += kBatchThreads) {
+    sumR[k] += dRs[k];
+    sumI[k] += dIs[k];
+  }
+}
+
+// Metropolis test and bookkeeping (Translate/Rotate::Accept,
+// src/moves/Translate.h:97-138; MoveSettings::Update,
+// src/MoveSettings.cpp:94-146), run by thread 0.
+__device__ __noinline__ void Metropolis(const BatchBox &sb, BatchState &st, MoveShared &ms,
+                                        const double *acc4, double pr, int mtype, int64_t traceIdx,
+                                        const BatchTrace &trace) {
+  const bool overlap = acc4[3] > 0.0;
+  const double dE = acc4[0] + acc4[1] + acc4[2];
+  const bool res = pr < exp(-sb.beta * dE);
+  const bool accept = res && !overlap;
+  st.tries[mtype]++;
+  st.tempTries[mtype]++;
+  if (accept) {
+    st.accepted[mtype]++;
+    st.tempAccepted[mtype]++;
+    st.energy[0] += acc4[0];
+    st.energy[2] += acc4[1];
+    st.energy[3] += acc4[2];
+    st.energy[6] = st.energy[0] + st.energy[1] + st.energy[2] + st.energy[3] + st.energy[4] +
+                   st.energy[5];
+  }
+  if (traceIdx >= 0 && traceIdx < trace.cap) {
+    trace.acc[traceIdx] = accept ? 1 : 0;
+    trace.dlj[traceIdx] = acc4[0];
+    trace.dreal[traceIdx] = acc4[1];
+    trace.drecip[traceIdx] = overlap ? 0.0 : acc4[2];
+  }
+  ms.accept = accept ? 1 : 0;
+}
+
 template <int KPT>
 __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a) {
   __shared__ BatchBox sb;
@@ -182,6 +357,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   __shared__ MoveShared ms;
   __shared__ double red[kBatchThreads / kWave][4];
   __shared__ BatchState st;  // running energies and move settings
+  __shared__ PairQueue queues[kBatchThreads / kWave];
 
   const int b = blockIdx.x, tid = threadIdx.x;
   {
@@ -193,19 +369,23 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   const int len = sb.molLen, nmols = sb.nmols, K = sb.K;
   const int64_t aoff = sb.atomOff, moff = sb.molOff, koff = sb.kOff;
   double *X = a.x + aoff, *Y = a.y + aoff, *Z = a.z + aoff;
-  const int32_t *kn = a.kn + koff;
-  const double *pref = a.pref + koff;
-
-  // S(k) of this thread's k-vectors: k = tid + i * kBatchThreads.
-  double R[KPT > 0 ? KPT : 1], I[KPT > 0 ? KPT : 1];
-  if (KPT > 0) {
-#pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-      const int k = tid + i * kBatchThreads;
-      R[i] = (k < K) ? a.sumR[koff + k] : 0.0;
-      I[i] = (k < K) ? a.sumI[koff + k] : 0.0;
+  // The box's coordinates live in LDS for the launch when they fit
+  // (a.ldsAtoms per workgroup); Xr/Yr/Zr are where the move reads them.
+  extern __shared__ double ldsCoords[];
+  const bool inLds = sb.natoms <= a.ldsAtoms;
+  double *Xr = X, *Yr = Y, *Zr = Z;
+  if (inLds) {
+    Xr = ldsCoords;
+    Yr = ldsCoords + a.ldsAtoms;
+    Zr = ldsCoords + 2 * a.ldsAtoms;
+    for (int i = tid; i < sb.natoms; i += kBatchThreads) {
+      Xr[i] = X[i];
+      Yr[i] = Y[i];
+      Zr[i] = Z[i];
     }
   }
+  const int32_t *kn = a.kn + koff;
+  const double *pref = a.pref + koff;
 
   if (tid == 0) st = a.state[b];
   int64_t step = a.state[b].step;
@@ -226,7 +406,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     const int m = ms.mol, mtype = ms.type;
     if (tid < len) {
       const int64_t ai = int64_t(m) * len + tid;
-      const double o[3] = {X[ai], Y[ai], Z[ai]};
+      const double o[3] = {Xr[ai], Yr[ai], Zr[ai]};
       double nw[3];
       if (mtype == 0) {
 #pragma unroll
@@ -250,151 +430,35 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     __syncthreads();
 
     // ---- phase tables for the 2*len positions ----
-    if (sb.ewald) {
-      for (int e = tid; e < nTab; e += kBatchThreads) {
-        int rem = e;
-        const int pos = rem / ((sb.nmax[0] + 1) + (sb.nmax[1] + 1) + (sb.nmax[2] + 1));
-        rem -= pos * ((sb.nmax[0] + 1) + (sb.nmax[1] + 1) + (sb.nmax[2] + 1));
-        int d = 0;
-        while (rem > sb.nmax[d]) {
-          rem -= sb.nmax[d] + 1;
-          ++d;
-        }
-        const double r = pos < len ? ms.oldp[d][pos] : ms.newp[d][pos - len];
-        tab[pos][d][rem] = PhaseEntry(sb.kc[d], rem, r);
-      }
-    }
+    if (sb.ewald) BuildTables(sb, ms, len, nTab, tab);
 
-    // ---- pair phase: every other molecule, old and new positions ----
+    // ---- pair phase ----
     double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
-    {
-      BoxGeom g;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        g.L[d] = sb.L[d];
-        g.half[d] = sb.half[d];
-      }
-      g.rCutSq = sb.rCutSq;
-      g.alpha = sb.alpha;
-      g.ewald = sb.ewald;
-      for (int j = tid; j < nmols; j += kBatchThreads) {
-        if (j == m) continue;
-        for (int qa = 0; qa < len; ++qa) {
-          const int64_t ai = int64_t(j) * len + qa;
-          const double xj = X[ai], yj = Y[ai], zj = Z[ai];
-          // set 0: old positions (subtracted), set 1: new positions (added)
-          for (int ps = 0; ps < 2 * len; ++ps) {
-            const int p = ps < len ? ps : ps - len;
-            const bool isNew = ps >= len;
-            const int pq = p * kBatchMaxAtoms + qa;
-            const double px = isNew ? ms.newp[0][p] : ms.oldp[0][p];
-            const double py = isNew ? ms.newp[1][p] : ms.oldp[1][p];
-            const double pz = isNew ? ms.newp[2][p] : ms.oldp[2][p];
-            const double d2 = DistSq(g, px, py, pz, xj, yj, zj);
-            if (sb.rCutSq > d2) {
-              if (isNew && d2 < sb.rCutLowSq) acc4[3] = 1.0;
-              const double qiqj = sb.qq[pq];
-              double ec = 0.0, el = 0.0;
-              if (sb.elect && qiqj != 0.0 && !(sb.rCutCoulSq < d2)) ec = sb.ewald ? CoulombTerm(d2, qiqj, sb.alpha) : qiqj / sqrt(d2);
-              const double eps = sb.epsCn[pq];
-              if (eps != 0.0 && !(sb.rCutLJSq < d2)) {
-                // n = 12 (checked at setup): repulsion = attraction^2
-                const double rRat2 = sb.sigSq[pq] / d2;
-                const double att = rRat2 * rRat2 * rRat2;
-                el = eps * (att * att - att);
-              }
-              if (isNew) {
-                acc4[1] += ec;
-                acc4[0] += el;
-              } else {
-                acc4[1] -= ec;
-                acc4[0] -= el;
-              }
-            }
-          }
-        }
-      }
-    }
+    PairPhase(sb, ms, Xr, Yr, Zr, m, len, sb.natoms, queues, acc4);
     __syncthreads();  // tables complete
 
     // ---- reciprocal phase: energy change per k-vector ----
-    if (sb.ewald) {
-      if (KPT > 0) {
-#pragma unroll
-        for (int i = 0; i < KPT; ++i) {
-          const int k = tid + i * kBatchThreads;
-          if (k < K) {
-            double dR, dI;
-            KDelta(tab, sb.charge, len, kn[k], dR, dI);
-            acc4[2] += pref[k] * (dR * (2.0 * R[i] + dR) + dI * (2.0 * I[i] + dI));
-          }
-        }
-      } else {
-        for (int k = tid; k < K; k += kBatchThreads) {
-          double dR, dI;
-          KDelta(tab, sb.charge, len, kn[k], dR, dI);
-          const double r0 = a.sumR[koff + k], i0 = a.sumI[koff + k];
-          acc4[2] += pref[k] * (dR * (2.0 * r0 + dR) + dI * (2.0 * i0 + dI));
-        }
-      }
-    }
+    if (sb.ewald) acc4[2] = RecipPhase(sb, tab, len, K, kn, pref, a.sumR + koff, a.sumI + koff,
+                                       a.dR + koff, a.dI + koff);
     BlockSum4(acc4, red);
 
     // ---- Metropolis (thread 0) ----
-    if (tid == 0) {
-      const bool overlap = acc4[3] > 0.0;
-      const double dE = acc4[0] + acc4[1] + acc4[2];
-      const bool res = u[5] < exp(-sb.beta * dE);
-      const bool accept = res && !overlap;
-      st.tries[mtype]++;
-      st.tempTries[mtype]++;
-      if (accept) {
-        st.accepted[mtype]++;
-        st.tempAccepted[mtype]++;
-        st.energy[0] += acc4[0];
-        st.energy[2] += acc4[1];
-        st.energy[3] += acc4[2];
-        st.energy[6] = st.energy[0] + st.energy[1] + st.energy[2] + st.energy[3] + st.energy[4] +
-                       st.energy[5];
-      }
-      if (b == a.trace.box && mv < a.trace.cap) {
-        a.trace.acc[mv] = accept ? 1 : 0;
-        a.trace.dlj[mv] = acc4[0];
-        a.trace.dreal[mv] = acc4[1];
-        a.trace.drecip[mv] = overlap ? 0.0 : acc4[2];
-      }
-      ms.accept = accept ? 1 : 0;
-    }
+    if (tid == 0) Metropolis(sb, st, ms, acc4, u[5], mtype, (b == a.trace.box) ? mv : -1, a.trace);
     __syncthreads();
 
     // ---- commit ----
     if (ms.accept) {
-      if (sb.ewald) {
-        if (KPT > 0) {
-#pragma unroll
-          for (int i = 0; i < KPT; ++i) {
-            const int k = tid + i * kBatchThreads;
-            if (k < K) {
-              double dR, dI;
-              KDelta(tab, sb.charge, len, kn[k], dR, dI);
-              R[i] += dR;
-              I[i] += dI;
-            }
-          }
-        } else {
-          for (int k = tid; k < K; k += kBatchThreads) {
-            double dR, dI;
-            KDelta(tab, sb.charge, len, kn[k], dR, dI);
-            a.sumR[koff + k] += dR;
-            a.sumI[koff + k] += dI;
-          }
-        }
-      }
+      if (sb.ewald) RecipCommit(K, a.sumR + koff, a.sumI + koff, a.dR + koff, a.dI + koff);
       if (tid < len) {
         const int64_t ai = int64_t(m) * len + tid;
         X[ai] = ms.newp[0][tid];
         Y[ai] = ms.newp[1][tid];
         Z[ai] = ms.newp[2][tid];
+        if (inLds) {
+          Xr[ai] = ms.newp[0][tid];
+          Yr[ai] = ms.newp[1][tid];
+          Zr[ai] = ms.newp[2][tid];
+        }
       }
       if (tid == 0 && mtype == 0) {
         a.cx[moff + m] = ms.newCom[0];
@@ -405,16 +469,6 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     __syncthreads();
   }
 
-  if (KPT > 0) {
-#pragma unroll
-    for (int i = 0; i < KPT; ++i) {
-      const int k = tid + i * kBatchThreads;
-      if (k < K) {
-        a.sumR[koff + k] = R[i];
-        a.sumI[koff + k] = I[i];
-      }
-    }
-  }
   if (tid == 0) {
     st.step = step;
     a.state[b] = st;
@@ -423,26 +477,31 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
 
 }  // namespace
 
-static const int kKpts[] = {1, 2, 4};
-
-int BatchMaxKpt() { return 4; }
+int BatchMaxKpt() { return 0; }
 
 int BatchRoundKpt(int kpt) {
-  for (int v : kKpts)
-    if (kpt <= v) return v;
-  return 0;  // S(k) in global memory
+  (void)kpt;
+  return 0;  // S(k) stays in HBM (L2-resident per box)
+}
+
+template <int KPT>
+static hipError_t LaunchOne(int nboxes, const BatchArgs &a, hipStream_t s) {
+  const size_t dyn = sizeof(double) * 3 * size_t(a.ldsAtoms);
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&McBatchKernel<KPT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn));
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(McBatchKernel<KPT>, dim3(nboxes), dim3(kBatchThreads), dyn, s, a);
+  return hipGetLastError();
+}
+
+int BatchLdsAtoms(int maxAtoms) {
+  return maxAtoms <= kBatchLdsAtomCap ? maxAtoms : 0;
 }
 
 hipError_t LaunchBatch(int kpt, int precision, int nboxes, const BatchArgs &a, hipStream_t s) {
   (void)precision;
-  const dim3 grid(nboxes), block(kBatchThreads);
-  switch (kpt) {
-    case 1: hipLaunchKernelGGL(McBatchKernel<1>, grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL(McBatchKernel<2>, grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL(McBatchKernel<4>, grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL(McBatchKernel<0>, grid, block, 0, s, a); break;
-  }
-  return hipGetLastError();
+  (void)kpt;
+  return LaunchOne<0>(nboxes, a, s);
 }
 
 }  // namespace gomc

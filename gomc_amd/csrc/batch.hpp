@@ -19,6 +19,9 @@ constexpr int kBatchMaxTypes = 4;    // LJ atom types per box
 constexpr int kBatchMaxN = 32;       // largest |n| of a k-vector component
 constexpr int kDrawChunk = 64;       // moves whose Philox draws are made at once
 constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of every box stream
+// Atoms whose FP64 coordinates fit in the LDS left beside the static
+// tables and pair queues (160 KiB per CU on gfx950, ~56 KiB static).
+constexpr int kBatchLdsAtomCap = 4300;
 
 // Read-only description of one box (host-filled, 8-byte aligned fields).
 struct BatchBox {
@@ -34,6 +37,7 @@ struct BatchBox {
   int64_t atomOff, molOff, kOff;      // offsets into the flat arrays
   int64_t perAdjust;
   uint64_t seed;
+  uint64_t magicLen, magicLen2;       // ceil(2^32 / len), ceil(2^32 / len^2)
   int natoms, nmols, molLen, K;
   int nmax[3], ewald, elect, pad;
 };
@@ -64,7 +68,9 @@ struct BatchArgs {
   const int32_t *kn;          // [sum K] packed (nx, ny+128, nz+128) bytes
   const double *pref;         // [sum K]
   double *sumR, *sumI;        // [sum K] reference structure factors
+  double *dR, *dI;            // [sum K] structure-factor change of the trial
   const int64_t *moves;       // [nboxes] moves to make in this launch
+  int ldsAtoms;               // per-box coordinate capacity of dynamic LDS
   BatchTrace trace;
 };
 
@@ -84,9 +90,10 @@ struct gomc_batch {
   gomc::BatchState *state = nullptr;
   double *x = nullptr, *y = nullptr, *z = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
   int32_t *kn = nullptr;
-  double *pref = nullptr, *sumR = nullptr, *sumI = nullptr;
+  double *pref = nullptr, *sumR = nullptr, *sumI = nullptr, *dR = nullptr, *dI = nullptr;
   int64_t *moves = nullptr;
   int64_t totalAtoms = 0, totalMols = 0, totalK = 0;
+  int ldsAtoms = 0;
   gomc::BatchTrace trace{-1, 0, nullptr, nullptr, nullptr, nullptr};
   int64_t traceLen = 0;
 };
@@ -97,6 +104,9 @@ hipError_t LaunchBatch(int kpt, int precision, int nboxes, const BatchArgs &a, h
 // Largest k-vectors-per-thread any compiled instance serves.
 int BatchMaxKpt();
 int BatchRoundKpt(int kpt);
+// LDS coordinate capacity for a batch whose largest box has maxAtoms atoms
+// (0: boxes read coordinates from HBM).
+int BatchLdsAtoms(int maxAtoms);
 void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double volume,
                   std::vector<double> &kx, std::vector<double> &ky, std::vector<double> &kz,
                   std::vector<double> &pref, std::vector<int32_t> *packed);

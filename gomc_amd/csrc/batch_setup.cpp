@@ -26,7 +26,7 @@ int BFail(gomc_batch *bt, int code, const std::string &msg) {
 
 void FreeBatch(gomc_batch *bt) {
   void *ptrs[] = {bt->boxes, bt->state, bt->x,    bt->y,    bt->z,    bt->cx,         bt->cy,
-                  bt->cz,    bt->kn,    bt->pref, bt->sumR, bt->sumI, bt->moves,      bt->trace.acc,
+                  bt->cz,    bt->kn,    bt->pref, bt->sumR, bt->sumI, bt->dR, bt->dI, bt->moves,      bt->trace.acc,
                   bt->trace.dlj, bt->trace.dreal, bt->trace.drecip};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -119,6 +119,13 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
     P.natoms = b->natoms;
     P.nmols = b->nmols;
     P.molLen = len;
+    // exact floor division by len and len^2 for item indices < 2^32 / len^2
+    P.magicLen = ((uint64_t(1) << 32) + uint64_t(len) - 1) / uint64_t(len);
+    P.magicLen2 = ((uint64_t(1) << 32) + uint64_t(len * len) - 1) / uint64_t(len * len);
+    if (uint64_t(b->nmols) * len * len >= (uint64_t(1) << 32) / uint64_t(len * len)) {
+      gomc_box_destroy(b);
+      return fail(GOMC_ERR_ARG, "box too large for the batched item indexing");
+    }
     P.ewald = b->ewald;
     P.elect = b->elect;
     hx.insert(hx.end(), b->hx.begin(), b->hx.end());
@@ -157,6 +164,11 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
     S.scale[1] = b->moves.scale[1][0];
     gomc_box_destroy(b);
   }
+  int maxAtoms = 0;
+  for (const BatchBox &P : bt->hboxes) maxAtoms = std::max(maxAtoms, P.natoms);
+  // Boxes up to the cap keep coordinates in LDS; a batch whose largest box
+  // exceeds it sizes LDS for the boxes that fit.
+  bt->ldsAtoms = BatchLdsAtoms(std::min(maxAtoms, kBatchLdsAtomCap));
   bt->totalAtoms = int64_t(hx.size());
   bt->totalMols = int64_t(hcx.size());
   bt->totalK = int64_t(hpref.size());
@@ -185,6 +197,8 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
   ok &= alloc((void **)&bt->pref, sizeof(double) * hpref.size());
   ok &= alloc((void **)&bt->sumR, sizeof(double) * hsr.size());
   ok &= alloc((void **)&bt->sumI, sizeof(double) * hsi.size());
+  ok &= alloc((void **)&bt->dR, sizeof(double) * hsi.size());
+  ok &= alloc((void **)&bt->dI, sizeof(double) * hsi.size());
   if (!ok) return fail(GOMC_ERR_HIP, "batch allocation failed");
   auto up = [&](void *dst, const void *src, size_t bytes) {
     return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -229,7 +243,10 @@ int gomc_batch_run(gomc_batch *bt, const int64_t *moves) {
   a.pref = bt->pref;
   a.sumR = bt->sumR;
   a.sumI = bt->sumI;
+  a.dR = bt->dR;
+  a.dI = bt->dI;
   a.moves = bt->moves;
+  a.ldsAtoms = bt->ldsAtoms;
   a.trace = bt->trace;
   BHIP(bt, hipEventRecord(bt->ev0, bt->stream));
   BHIP(bt, LaunchBatch(bt->kpt, bt->precision, bt->nboxes, a, bt->stream));
