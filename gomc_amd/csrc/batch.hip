@@ -19,13 +19,29 @@
 namespace gomc {
 namespace {
 
-struct Cplx {
-  double re, im;
+// Phase-table entries: FP64 in the FP64 mode, FP32 in the mixed mode.
+template <typename T>
+struct CplxT {
+  T re, im;
 };
+using Cplx = CplxT<double>;
 
-__device__ __forceinline__ Cplx CMul(Cplx a, Cplx b) {
+template <typename T>
+__device__ __forceinline__ CplxT<T> CMul(CplxT<T> a, CplxT<T> b) {
   return {fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
 }
+
+// Precision traits: P = 0 FP64 everywhere; P = 1 FP32 pair terms and phase
+// products (distances, accumulation, S(k), totals and the Metropolis
+// exponent stay FP64).
+template <int P>
+struct Prec {
+  using T = double;
+};
+template <>
+struct Prec<1> {
+  using T = float;
+};
 
 struct alignas(16) MoveShared {
   double oldp[3][kBatchMaxAtoms], newp[3][kBatchMaxAtoms];
@@ -55,45 +71,57 @@ __device__ __forceinline__ void BlockSum4(double v[4], double (*lds)[4]) {
 
 // Structure-factor change of one k-vector for the moving molecule: sums of
 // q e^{i k.r} over its new and old positions, from the phase tables.
-__device__ __forceinline__ void KDelta(const Cplx (*tab)[3][kBatchMaxN + 1], const double *charge,
+template <typename T>
+__device__ __forceinline__ void KDelta(const CplxT<T> (*tab)[3][kBatchMaxN + 1], const double *charge,
                                        int len, int packed, double &dR, double &dI) {
   const int nx = packed & 0xff;
   const int ny = ((packed >> 8) & 0xff) - 128;
   const int nz = ((packed >> 16) & 0xff) - 128;
   const int ay = ny < 0 ? -ny : ny, az = nz < 0 ? -nz : nz;
-  const double sy = ny < 0 ? -1.0 : 1.0, sz = nz < 0 ? -1.0 : 1.0;
-  double sRN = 0.0, sIN = 0.0, sRO = 0.0, sIO = 0.0;
+  const T sy = ny < 0 ? T(-1) : T(1), sz = nz < 0 ? T(-1) : T(1);
+  T sRN = 0, sIN = 0, sRO = 0, sIO = 0;
   for (int p = 0; p < len; ++p) {
-    const double q = charge[p];
-    if (fabs(q) < 0.000000001) continue;
+    const double qd = charge[p];
+    if (fabs(qd) < 0.000000001) continue;
+    const T q = T(qd);
     {
-      Cplx ex = tab[len + p][0][nx], ey = tab[len + p][1][ay], ez = tab[len + p][2][az];
+      CplxT<T> ex = tab[len + p][0][nx], ey = tab[len + p][1][ay], ez = tab[len + p][2][az];
       ey.im *= sy;
       ez.im *= sz;
-      const Cplx e = CMul(CMul(ex, ey), ez);
+      const CplxT<T> e = CMul(CMul(ex, ey), ez);
       sRN += q * e.re;
       sIN += q * e.im;
     }
     {
-      Cplx ex = tab[p][0][nx], ey = tab[p][1][ay], ez = tab[p][2][az];
+      CplxT<T> ex = tab[p][0][nx], ey = tab[p][1][ay], ez = tab[p][2][az];
       ey.im *= sy;
       ez.im *= sz;
-      const Cplx e = CMul(CMul(ex, ey), ez);
+      const CplxT<T> e = CMul(CMul(ex, ey), ez);
       sRO += q * e.re;
       sIO += q * e.im;
     }
   }
-  dR = sRN - sRO;
-  dI = sIN - sIO;
+  dR = double(sRN - sRO);
+  dI = double(sIN - sIO);
 }
 
 // One phase-table entry: e^{i (c_d n) r} (kept out of line: the FP64
 // sincos argument reduction is register-hungry and runs once per entry).
-__device__ __forceinline__ Cplx PhaseEntry(double kc, int n, double r) {
+template <typename T>
+__device__ __forceinline__ CplxT<T> PhaseEntry(double kc, int n, double r) {
   const double ang = (kc * double(n)) * r;
-  double s, c;
-  sincos(ang, &s, &c);
-  return {c, s};
+  if constexpr (sizeof(T) == 8) {
+    double sn, c;
+    sincos(ang, &sn, &c);
+    return {c, sn};
+  } else {
+    // FP64 reduction to [-pi, pi), FP32 sincos of the remainder.
+    const double turns = ang * (0.5 * M_1_PI);
+    const float red = float((turns - rint(turns)) * (2.0 * M_PI));
+    float sn, c;
+    sincosf(red, &sn, &c);
+    return {c, sn};
+  }
 }
 
 // Thread 0's proposal: move settings, move kind, molecule, and the shift or
@@ -168,8 +196,9 @@ __device__ __noinline__ void MoveDraws(uint64_t seed, uint64_t step, double *out
 
 // Phase tables e^{i (c_d n) r_d}, n = 0..nmax_d, for the moving molecule's
 // old (pos < len) and new positions; one entry per thread-iteration.
+template <typename T>
 __device__ __noinline__ void BuildTables(const BatchBox &sb, const MoveShared &ms, int len, int nTab,
-                                         Cplx (*tab)[3][kBatchMaxN + 1]) {
+                                         CplxT<T> (*tab)[3][kBatchMaxN + 1]) {
   const int per = (sb.nmax[0] + 1) + (sb.nmax[1] + 1) + (sb.nmax[2] + 1);
   for (int e = threadIdx.x; e < nTab; e += kBatchThreads) {
     const int pos = e / per;
@@ -180,7 +209,7 @@ __device__ __noinline__ void BuildTables(const BatchBox &sb, const MoveShared &m
       ++d;
     }
     const double r = pos < len ? ms.oldp[d][pos] : ms.newp[d][pos - len];
-    tab[pos][d][rem] = PhaseEntry(sb.kc[d], rem, r);
+    tab[pos][d][rem] = PhaseEntry<T>(sb.kc[d], rem, r);
   }
 }
 
@@ -193,29 +222,31 @@ struct PairQueue {
 
 // Energy terms of one queued pair (FFParticle::CalcEn / CalcCoulomb,
 // src/FFParticle.cpp:295-399), signed: + new position, - old position.
+template <typename T>
 __device__ __forceinline__ void PairTerm(const BatchBox &sb, double d2, int code, double &lj,
                                          double &real) {
   const int pq = code & 0xff;
   const double sign = (code >> 8) ? 1.0 : -1.0;
   const double qiqj = sb.qq[pq], eps = sb.epsCn[pq];
   if (sb.elect && qiqj != 0.0 && !(sb.rCutCoulSq < d2)) {
-    const double dist = sqrt(d2);
-    const double ec = sb.ewald ? qiqj * erfc(sb.alpha * dist) / dist : qiqj / dist;
-    real += sign * ec;
+    const T dist = sqrt(T(d2));
+    const T ec = sb.ewald ? T(qiqj) * erfc(T(sb.alpha) * dist) / dist : T(qiqj) / dist;
+    real += sign * double(ec);
   }
   if (eps != 0.0 && !(sb.rCutLJSq < d2)) {
     // n = 12 (checked at setup): repulsion = attraction^2
-    const double rRat2 = sb.sigSq[pq] / d2;
-    const double att = rRat2 * rRat2 * rRat2;
-    lj += sign * (eps * (att * att - att));
+    const T rRat2 = T(sb.sigSq[pq]) / T(d2);
+    const T att = rRat2 * rRat2 * rRat2;
+    lj += sign * double(T(eps) * (att * att - att));
   }
 }
 
 // Evaluates `n` queued pairs (one per lane) and adds them to lj/real.
+template <typename T>
 __device__ __noinline__ void DrainQueue(const BatchBox &sb, const PairQueue &q, int n,
                                         double *lj, double *real) {
   const int lane = threadIdx.x & (kWave - 1);
-  if (lane < n) PairTerm(sb, q.d2[lane], q.code[lane], *lj, *real);
+  if (lane < n) PairTerm<T>(sb, q.d2[lane], q.code[lane], *lj, *real);
 }
 
 // Pair-energy change of the move (MoleculeInter, src/CalculateEnergy.cpp:
@@ -226,6 +257,7 @@ __device__ __noinline__ void DrainQueue(const BatchBox &sb, const PairQueue &q, 
 // energies evaluated 64 at a time, so the erfc path runs with full waves
 // instead of being masked through for the few in-cutoff lanes.  X/Y/Z may
 // point to LDS or HBM.  acc = {lj, real, -, overlap} partial sums.
+template <typename T>
 __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms, const double *X,
                                        const double *Y, const double *Z, int m, int len,
                                        int natoms, PairQueue *queues, double *acc4) {
@@ -270,7 +302,7 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
         count += __popcll(mask);
         if (count >= kWave) {  // drain one full wave of pairs
           __builtin_amdgcn_wave_barrier();
-          DrainQueue(sb, q, kWave, &lj, &real);
+          DrainQueue<T>(sb, q, kWave, &lj, &real);
           const double d2n = q.d2[kWave + lane];
           const int cn = q.code[kWave + lane];
           __builtin_amdgcn_wave_barrier();
@@ -283,7 +315,7 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
     }
   }
   __builtin_amdgcn_wave_barrier();
-  DrainQueue(sb, q, count, &lj, &real);
+  DrainQueue<T>(sb, q, count, &lj, &real);
   acc4[0] = lj;
   acc4[1] = real;
   acc4[3] = ov;
@@ -294,7 +326,8 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
 // structure factor and prefact * ((R+dR)^2 + (I+dI)^2 - R^2 - I^2),
 // written as prefact * (dR (2R + dR) + dI (2I + dI)) so no large terms
 // cancel.  (dR, dI) are kept for the commit.  Returns this thread's sum.
-__device__ __noinline__ double RecipPhase(const BatchBox &sb, const Cplx (*tab)[3][kBatchMaxN + 1],
+template <typename T>
+__device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*tab)[3][kBatchMaxN + 1],
                                           int len, int K, const int32_t *kn, const double *pref,
                                           const double *sumR, const double *sumI, double *dRs,
                                           double *dIs) {
@@ -349,11 +382,12 @@ __device__ __noinline__ void Metropolis(const BatchBox &sb, BatchState &st, Move
   ms.accept = accept ? 1 : 0;
 }
 
-template <int KPT>
+template <int PREC>
 __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a) {
+  using T = typename Prec<PREC>::T;
   __shared__ BatchBox sb;
   __shared__ double draws[kDrawChunk][6];
-  __shared__ Cplx tab[2 * kBatchMaxAtoms][3][kBatchMaxN + 1];
+  __shared__ CplxT<T> tab[2 * kBatchMaxAtoms][3][kBatchMaxN + 1];
   __shared__ MoveShared ms;
   __shared__ double red[kBatchThreads / kWave][4];
   __shared__ BatchState st;  // running energies and move settings
@@ -430,15 +464,15 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     __syncthreads();
 
     // ---- phase tables for the 2*len positions ----
-    if (sb.ewald) BuildTables(sb, ms, len, nTab, tab);
+    if (sb.ewald) BuildTables<T>(sb, ms, len, nTab, tab);
 
     // ---- pair phase ----
     double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
-    PairPhase(sb, ms, Xr, Yr, Zr, m, len, sb.natoms, queues, acc4);
+    PairPhase<T>(sb, ms, Xr, Yr, Zr, m, len, sb.natoms, queues, acc4);
     __syncthreads();  // tables complete
 
     // ---- reciprocal phase: energy change per k-vector ----
-    if (sb.ewald) acc4[2] = RecipPhase(sb, tab, len, K, kn, pref, a.sumR + koff, a.sumI + koff,
+    if (sb.ewald) acc4[2] = RecipPhase<T>(sb, tab, len, K, kn, pref, a.sumR + koff, a.sumI + koff,
                                        a.dR + koff, a.dI + koff);
     BlockSum4(acc4, red);
 
@@ -484,13 +518,13 @@ int BatchRoundKpt(int kpt) {
   return 0;  // S(k) stays in HBM (L2-resident per box)
 }
 
-template <int KPT>
+template <int PREC>
 static hipError_t LaunchOne(int nboxes, const BatchArgs &a, hipStream_t s) {
   const size_t dyn = sizeof(double) * 3 * size_t(a.ldsAtoms);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&McBatchKernel<KPT>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&McBatchKernel<PREC>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn));
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(McBatchKernel<KPT>, dim3(nboxes), dim3(kBatchThreads), dyn, s, a);
+  hipLaunchKernelGGL(McBatchKernel<PREC>, dim3(nboxes), dim3(kBatchThreads), dyn, s, a);
   return hipGetLastError();
 }
 
@@ -499,9 +533,8 @@ int BatchLdsAtoms(int maxAtoms) {
 }
 
 hipError_t LaunchBatch(int kpt, int precision, int nboxes, const BatchArgs &a, hipStream_t s) {
-  (void)precision;
   (void)kpt;
-  return LaunchOne<0>(nboxes, a, s);
+  return precision == 1 ? LaunchOne<1>(nboxes, a, s) : LaunchOne<0>(nboxes, a, s);
 }
 
 }  // namespace gomc

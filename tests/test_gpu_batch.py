@@ -73,3 +73,31 @@ def test_batch_running_energy_matches_recompute():
         s2 = systems.System(**{**s.__dict__, "xyz": bt.coords(i), "_keep": []})
         ref = Box(s2).system_total()
         assert np.allclose(e[i], ref, rtol=1e-10, atol=1e-6), (e[i], ref)
+
+
+def test_mixed_precision_error_against_fp64():
+    """Mixed mode (FP32 pair terms and phase products, FP64 distances,
+    accumulation, S(k), totals and Metropolis exponent): while the two runs
+    have made the same decisions their configurations are identical, so the
+    per-move energy changes can be compared directly.  Bar: |dE_mixed -
+    dE_fp64| <= 1e-6 x max(|dE_fp64|, |E_total of the box|) (relative error
+    at the box's energy scale; overlapping trials with |dE| >> |E_total| at
+    their own scale), and the decisions agree for the first 300 moves."""
+    s = systems.water_box(400, seed=41, adj_steps=500)
+    n = 3000
+    runs = {}
+    for prec in (0, 1):
+        bt = Batch([s], precision=prec)
+        bt.set_trace(0, n)
+        bt.run([n])
+        runs[prec] = bt.trace()
+    o = Oracle(s, threads=4)
+    etot = abs(o.energies()[6])
+    a64, a32 = runs[0]["acc"], runs[1]["acc"]
+    diff = np.nonzero(a64 != a32)[0]
+    first = int(diff[0]) if len(diff) else n
+    assert first >= 300, first
+    for k in ("dlj", "dreal", "drecip"):
+        ref = runs[0][k][:first]
+        err = np.abs(runs[1][k][:first] - ref)
+        assert np.all(err <= 1e-6 * np.maximum(np.abs(ref), etot)), (k, err.max(), etot)
