@@ -42,6 +42,12 @@ SIGNATURES = {
     "gomc_recip_get": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "gomc_mol_trial": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "gomc_system_total": (C.c_int, [_vp, _dp]),
+    "gomc_box_force": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "gomc_box_force_recip": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "gomc_torque": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "gomc_mp_trial": (C.c_int, [_vp, C.c_int, C.c_uint64, C.c_double, _dp]),
+    "gomc_mp_accept": (C.c_int, [_vp, C.c_int, _dp]),
+    "gomc_mp_settings": (C.c_int, [_vp, _dp]),
     "gomc_run_moves": (C.c_int64, [_vp, C.c_int64, C.c_int64, _ip, _ip, _ip, _ip, _dp, _dp, _dp]),
     "gomc_energies": (C.c_int, [_vp, _dp]),
     "gomc_get_scales": (C.c_int, [_vp, _dp, _dp]),

@@ -149,6 +149,40 @@ class Box:
         self._check(self.lib.gomc_build_cells(self.h, ptr(cv), ptr(cs), ptr(ac)))
         return cv, cs, ac
 
+    # ---- forces and the MultiParticle move ----
+    def box_force(self):
+        """(lj, real, atom forces (n_atoms, 3), molecule forces (n_mols, 3))."""
+        lj, real = C.c_double(), C.c_double()
+        a = [np.zeros(self.natoms) for _ in range(3)] + [np.zeros(self.nmols) for _ in range(3)]
+        self._check(self.lib.gomc_box_force(self.h, C.byref(lj), C.byref(real),
+                                            *(ptr(v) for v in a)))
+        return lj.value, real.value, np.stack(a[:3], 1), np.stack(a[3:], 1)
+
+    def box_force_recip(self, copy_ref_to_new: bool = True):
+        if copy_ref_to_new:
+            self._check(self.lib.gomc_recip_copy_ref_to_new(self.h))
+        a = [np.zeros(self.natoms) for _ in range(3)] + [np.zeros(self.nmols) for _ in range(3)]
+        self._check(self.lib.gomc_box_force_recip(self.h, *(ptr(v) for v in a)))
+        return np.stack(a[:3], 1), np.stack(a[3:], 1)
+
+    def torque(self) -> np.ndarray:
+        t = [np.zeros(self.nmols) for _ in range(3)]
+        self._check(self.lib.gomc_torque(self.h, *(ptr(v) for v in t)))
+        return np.stack(t, 1)
+
+    def mp_trial(self, move_type: int, step: int, max_step: float) -> np.ndarray:
+        out = np.zeros(5)
+        self._check(self.lib.gomc_mp_trial(self.h, move_type, step, max_step, ptr(out)))
+        return out
+
+    def mp_accept(self, accept: bool, out: np.ndarray):
+        self._check(self.lib.gomc_mp_accept(self.h, int(accept), ptr(out)))
+
+    def mp_settings(self) -> np.ndarray:
+        out = np.zeros(2)
+        self._check(self.lib.gomc_mp_settings(self.h, ptr(out)))
+        return out
+
     # ---- MC ----
     def run(self, nsteps: int, start_step: int = 0, record: bool = True):
         if not record:

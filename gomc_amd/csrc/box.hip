@@ -24,46 +24,7 @@ int HipFail(gomc_box *b, hipError_t e, const char *what) {
 
 }  // namespace gomc
 
-#define HIP_TRY(b, call)                                   \
-  do {                                                     \
-    hipError_t e_ = (call);                                \
-    if (e_ != hipSuccess) return gomc::HipFail(b, e_, #call); \
-  } while (0)
-
 namespace {
-
-// ----------------------------------------------------------------------
-// Completion of a multi-block reduction: each block publishes its partial,
-// the last block to arrive sums all partials in block order and writes
-// `nout` values to the host-mapped result, then re-arms the counter.
-__device__ void FinishPartials(const Partial4 &mine, Partial4 *partials, unsigned *counter,
-                               double *result) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = mine;
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    volatile Partial4 *vp = partials;
-    for (unsigned i = 0; i < gridDim.x; ++i) {
-      s0 += vp[i].lj;
-      s1 += vp[i].real;
-      s2 += vp[i].recip;
-      s3 += vp[i].overlap;
-    }
-    result[0] = s0;
-    result[1] = s1;
-    result[2] = s2;
-    result[3] = s3;
-    *counter = 0u;
-    __threadfence_system();
-  }
-}
 
 // ---------------------------------------------------------------- trial --
 struct TrialArgs {
@@ -450,7 +411,7 @@ __global__ void __launch_bounds__(kBlock) ParticleInterKernel(const ParticleArgs
 
 namespace gomc {
 
-static int EnsurePartials(gomc_box *b, int nblocks) {
+int EnsurePartials(gomc_box *b, int nblocks) {
   if (nblocks <= b->maxPartials) return GOMC_OK;
   if (b->partials) (void)hipFree(b->partials);
   b->partials = nullptr;
@@ -460,7 +421,7 @@ static int EnsurePartials(gomc_box *b, int nblocks) {
   return GOMC_OK;
 }
 
-static int Sync(gomc_box *b) {
+int Sync(gomc_box *b) {
   HIP_TRY(b, hipStreamSynchronize(b->stream));
   HIP_TRY(b, hipGetLastError());
   return GOMC_OK;
@@ -543,7 +504,7 @@ int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const do
   return GOMC_OK;
 }
 
-static int BuildCells(gomc_box *b) {
+int BuildCells(gomc_box *b) {
   CellArgs a;
   a.x = b->x;
   a.y = b->y;
@@ -570,6 +531,7 @@ static int BuildCells(gomc_box *b) {
                             hipMemcpyDeviceToHost, b->stream));
   int rc = Sync(b);
   if (rc) return rc;
+  b->hCellCount = counts;
   int mx = *std::max_element(counts.begin(), counts.end());
   if (mx * sizeof(int) > 60 * 1024) return Fail(b, GOMC_ERR_ARG, "cell too full for device sort");
   hipLaunchKernelGGL(CellSortKernel, dim3(b->ncell), dim3(kBlock), std::max(mx, 1) * sizeof(int),
@@ -698,6 +660,7 @@ static void FreeAll(gomc_box *b) {
                   b->sumInew,  b->partials, b->counter};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  MPFree(b);
   if (b->hostRes) (void)hipHostFree(b->hostRes);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }

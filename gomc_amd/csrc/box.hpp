@@ -11,6 +11,12 @@
 #include "device_common.hpp"
 #include "host_rng.hpp"
 
+#define HIP_TRY(b, call)                                      \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return gomc::HipFail(b, e_, #call); \
+  } while (0)
+
 namespace gomc {
 
 constexpr int kMaxMolAtoms = 16;   // atoms per molecule in a trial launch
@@ -19,6 +25,59 @@ constexpr int kBlock = 256;        // threads per workgroup, plain kernels
 // Per-block partial of a fused trial: LJ, real, reciprocal, overlap.
 struct Partial4 {
   double lj, real, recip, overlap;
+};
+
+// Completion of a multi-block reduction: each block publishes its partial,
+// the last block to arrive sums all partials in block order and writes the
+// four sums to the host-mapped result, then re-arms the counter.
+__device__ inline void FinishPartials(const Partial4 &mine, Partial4 *partials,
+                                      unsigned *counter, double *result) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = mine;
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    volatile Partial4 *vp = partials;
+    for (unsigned i = 0; i < gridDim.x; ++i) {
+      s0 += vp[i].lj;
+      s1 += vp[i].real;
+      s2 += vp[i].recip;
+      s3 += vp[i].overlap;
+    }
+    result[0] = s0;
+    result[1] = s1;
+    result[2] = s2;
+    result[3] = s3;
+    *counter = 0u;
+    __threadfence_system();
+  }
+}
+
+// MultiParticle state (src/moves/MultiParticle.h:37-58 and the
+// MultiParticle members of src/MoveSettings.h): step sizes, interval
+// counters, and the device force sets [ref|trial].
+struct MPState {
+  bool singleAccepted = true;  // MoveSettings::isSingleMoveAccepted
+  double tMax = 0.02, rMax = 0.01 * M_PI;
+  uint64_t tries[2] = {0, 0}, acc[2] = {0, 0}, itries[2] = {0, 0}, iacc[2] = {0, 0};
+  int ref = 0;                 // index of the reference force set
+  bool ready = false;
+  double *xn = nullptr, *yn = nullptr, *zn = nullptr;  // trial coordinates
+  double *com[2][3] = {};                               // [current|trial][xyz]
+  double *aF[2][3] = {}, *aFR[2][3] = {};               // atom forces [set][xyz]
+  double *mF[2][3] = {}, *mFR[2][3] = {}, *tq[2][3] = {};  // molecule terms
+  double *tk[3] = {};                                   // t_k or r_k
+  int *inRange = nullptr, *fail = nullptr, *molStart = nullptr;
+  int *chunks = nullptr;                                // force work list
+  int nchunks = 0, maxChunks = 0;
+  double *wpart = nullptr;                              // per-block products
+  unsigned *wcounter = nullptr;
 };
 
 // MoveSettings state of the driver (src/MoveSettings.cpp:19-258).
@@ -83,6 +142,10 @@ struct gomc_box {
   int pendingMol = -1;
   double pend[3][gomc::kMaxMolAtoms];
 
+  std::vector<int> hCellCount;  // atoms per cell at the last build
+  uint64_t seed = 0;            // Random_Seed (MT and Random123)
+  gomc::MPState mp;
+
   // ---- single-box MC driver state ----
   gomc::MTRand mt{5489u};
   gomc::DriverMoves moves;
@@ -101,4 +164,16 @@ int LaunchTrial(gomc_box *b, int mol, const double *ox, const double *oy, const 
 void HostCorrectionSelfLRC(gomc_box *b, double &self, double &corr, double &lrc);
 int RecipSums(gomc_box *b, double *erecip);
 int BoxInter(gomc_box *b, double *lj, double *real);
+int EnsurePartials(gomc_box *b, int nblocks);
+int Sync(gomc_box *b);
+int BuildCells(gomc_box *b);
+int FlushPending(gomc_box *b);
+// MultiParticle pieces (mp.hip).
+int MPEnsure(gomc_box *b);
+void MPFree(gomc_box *b);
+int MPUploadCom(gomc_box *b);
+int MPPairForce(gomc_box *b, int set, double *lj, double *real);
+int MPRecipForce(gomc_box *b, int set);
+int MPMolTerms(gomc_box *b, int set, int comSet);
+int MPMove(gomc_box *b, uint64_t step, int &moveType, int &accepted, int &failed, double dE[3]);
 }  // namespace gomc

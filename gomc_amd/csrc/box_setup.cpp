@@ -357,6 +357,7 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
     b->moves.tempAccepted[mv].assign(b->nkinds, 0);
   }
   b->mt.Seed(uint32_t(d->seed));
+  b->seed = d->seed;
   *out = b;
   return GOMC_OK;
 }

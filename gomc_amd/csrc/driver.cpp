@@ -1,8 +1,9 @@
 // Single-box MC driver over the plain device path: the reference's move loop
 // (Simulation::RunSimulation -> MoveSettings::AdjustMoves ->
 // System::ChooseAndRunMove, src/Simulation.cpp:94-96, src/System.cpp:262-302)
-// for Translate and Rotate, with each trial's energies from one fused launch
-// (gomc_mol_trial) and the accepted state kept in HBM.
+// for Translate, Rotate and MultiParticle, with each single-molecule trial's
+// energies from one fused launch (gomc_mol_trial), the MultiParticle move
+// from the force/transform kernels, and the accepted state kept in HBM.
 #include <algorithm>
 #include <cmath>
 
@@ -12,7 +13,7 @@ using namespace gomc;
 
 namespace {
 
-constexpr int kDisplace = 0, kRotate = 1;
+constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2;
 
 // MoveSettings::AdjustMoves / Adjust (src/MoveSettings.cpp:148-162,205-258).
 void AdjustMoves(gomc_box *b, uint64_t step) {
@@ -33,6 +34,27 @@ void AdjustMoves(gomc_box *b, uint64_t step) {
       b->moves.tempAccepted[mv][k] = 0;
       b->moves.tempTries[mv][k] = 0;
     }
+  // MoveSettings::AdjustMultiParticle (src/MoveSettings.cpp:164-200): the
+  // MultiParticle target is mp::TARGET_ACCEPT_FRACT = 0.3 (MoveSettings.h:22),
+  // the tolerance 0.1 and the averaging weight 0.2.
+  MPState &mp = b->mp;
+  for (int t = 0; t < 2; ++t) {
+    if (mp.itries[t] == 0) continue;
+    mp.tries[t] += mp.itries[t];
+    mp.acc[t] += mp.iacc[t];
+    const double total = (double(mp.acc[t]) / double(mp.tries[t])) / 0.3;
+    const double interval = (double(mp.iacc[t]) / double(mp.itries[t])) / 0.3;
+    double &mx = t == 0 ? mp.tMax : mp.rMax;
+    if (interval == 0.0)
+      mx *= 0.5;
+    else if (std::fabs(interval - 0.3) > 0.1)
+      mx *= ((1.0 - 0.2) * total + 0.2 * interval);
+    const double lo = 0.001, hi = t == 0 ? (lmin * 0.5) - 0.001 : M_PI - 0.001;
+    if (mx < lo) mx = lo;
+    if (mx > hi) mx = hi;
+    mp.iacc[t] = 0;
+    mp.itries[t] = 0;
+  }
 }
 
 struct Record {
@@ -42,7 +64,7 @@ struct Record {
 
 // One move: PickArbDist -> PickMolAndBox -> TranslateRand/RotateRand ->
 // CalcEn -> Accept, drawing from MT19937 exactly where the reference does.
-int OneMove(gomc_box *b, Record &r) {
+int OneMove(gomc_box *b, uint64_t step, Record &r) {
   // PRNG::PickArbDist over the nine NVT move kinds (src/PRNG.h:153-163).
   const double w[9] = {b->movePerc[0], b->movePerc[1], b->movePerc[2], 0, 0, 0, 0, 0, 0};
   const double draw0 = b->mt.Closed() * 1.0;
@@ -54,6 +76,21 @@ int OneMove(gomc_box *b, Record &r) {
   }
   if (pick != 0) --pick;
   double subDraw = draw0 - prev;
+  if (pick == kMultiParticle) {
+    // MultiParticle::Prep picks the box without a draw (one box).
+    int type = 0, acc = 0, failed = 0;
+    double dE[3];
+    int rc = MPMove(b, step, type, acc, failed, dE);
+    if (rc) return rc;
+    r.kind = kMultiParticle;
+    r.mol = type;
+    r.acc = acc;
+    r.ovl = failed;
+    r.dlj = dE[0];
+    r.dreal = dE[1];
+    r.drecip = dE[2];
+    return 0;
+  }
   if (pick != kDisplace && pick != kRotate) return Fail(b, GOMC_ERR_STATE, "move kind not served by this driver");
   // PRNG::PickMolAndBox with one box, then PickMol (src/PRNG.h:267-299).
   const double boxDiv = b->movePerc[pick] / 1;
@@ -135,6 +172,7 @@ int OneMove(gomc_box *b, Record &r) {
     }
     if (b->ewald) gomc_recip_accept(b);
     b->pot.Total();
+    b->mp.singleAccepted = true;  // MoveSettings::Update (MoveSettings.cpp:102-104)
   }
   b->moves.tries[mv][mk]++;
   b->moves.tempTries[mv][mk]++;
@@ -165,7 +203,7 @@ int64_t gomc_run_moves(gomc_box *b, int64_t start_step, int64_t nsteps, int32_t 
   for (int64_t i = 0; i < nsteps; ++i) {
     AdjustMoves(b, uint64_t(start_step + i));
     Record r;
-    int rc = OneMove(b, r);
+    int rc = OneMove(b, uint64_t(start_step + i), r);
     if (rc) return rc;
     accepted += r.acc;
     if (kind) kind[i] = r.kind;

@@ -153,11 +153,49 @@ int gomc_mol_trial(gomc_box *b, int mol, const double *nx, const double *ny,
  * Recomputes the reciprocal sums from the coordinates. */
 int gomc_system_total(gomc_box *b, double out[7]);
 
+/* ---- forces and the MultiParticle move (src/moves/MultiParticle.h) --- */
+/* CalculateEnergy::BoxForce (src/CalculateEnergy.cpp:268-406); replaces
+ * CallBoxForceGPU (src/GPU/CalculateForceCUDAKernel.cuh:17-30).  Pair
+ * energies and the atom / molecule forces of the current coordinates into
+ * the device's reference force set; the six host arrays (n_atoms, n_mols
+ * doubles) receive a copy and may each be NULL. */
+int gomc_box_force(gomc_box *b, double *lj, double *real, double *afx,
+                   double *afy, double *afz, double *mfx, double *mfy,
+                   double *mfz);
+/* Ewald::BoxForceReciprocal (src/Ewald.cpp:1496-1596); replaces
+ * CallBoxForceReciprocalGPU (src/GPU/CalculateEwaldCUDAKernel.cuh:50-58).
+ * Uses the current "new" sums: call gomc_recip_copy_ref_to_new first, as
+ * MultiParticle::Prep does (MultiParticle.h:211-217). */
+int gomc_box_force_recip(gomc_box *b, double *afx, double *afy, double *afz,
+                         double *mfx, double *mfy, double *mfz);
+/* CalculateEnergy::CalculateTorque (src/CalculateEnergy.cpp:1365-1406) of
+ * the reference force set about the molecule centres; arrays of n_mols. */
+int gomc_torque(gomc_box *b, double *tx, double *ty, double *tz);
+/* MultiParticle Transform + CalcEn + GetCoeff (MultiParticle.h:332-511);
+ * replaces CallTranslateParticlesGPU / CallRotateParticlesGPU
+ * (src/GPU/TransformParticlesCUDAKernel.cuh:20-35) and the host force and
+ * coefficient work.  move_type 0 displaces, 1 rotates every molecule with
+ * the force-biased transform from the reference force set, Random123 key
+ * {step, seed}; `max` is t_max or r_max.  out = {inter, real, recip of the
+ * trial, w_ratio, fail (1 when the transform check of MultiParticle.h:366-406
+ * fails)}.  Until gomc_mp_accept the trial coordinates are the box's. */
+int gomc_mp_trial(gomc_box *b, int move_type, uint64_t step, double max,
+                  double out[5]);
+/* MultiParticle::Accept's state change (MultiParticle.h:522-538): accept
+ * keeps the trial coordinates, energies (from `out` of gomc_mp_trial),
+ * forces and sums; reject restores the reference state. */
+int gomc_mp_accept(gomc_box *b, int accept, const double out[5]);
+/* The driver's MultiParticle step sizes {t_max, r_max}. */
+int gomc_mp_settings(const gomc_box *b, double out[2]);
+
 /* ---- plain single-box MC driver (src/System.cpp:262-302) ------------- */
-/* Runs `nsteps` GOMC moves (Translate / Rotate, MT19937 draws in the
- * reference's order, Metropolis acceptance, MoveSettings adaptation) with
- * every energy evaluated by gomc_mol_trial.  Per-move records may be NULL.
- * Returns the number of accepted moves (>= 0) or an error code. */
+/* Runs `nsteps` GOMC moves (Translate / Rotate / MultiParticle, MT19937
+ * draws in the reference's order and Random123 draws for MultiParticle,
+ * Metropolis acceptance, MoveSettings adaptation) with every
+ * single-molecule energy evaluated by gomc_mol_trial.  Per-move records may
+ * be NULL; a MultiParticle record has kind 2, mol = its move type and ovl =
+ * its transform failure.  Returns the number of accepted moves (>= 0) or
+ * an error code. */
 int64_t gomc_run_moves(gomc_box *b, int64_t start_step, int64_t nsteps,
                        int32_t *kind, int32_t *mol, int32_t *acc,
                        int32_t *ovl, double *dlj, double *dreal,

@@ -71,6 +71,7 @@ class Random123Wrapper {
   void SetKey(uint32_t k) { key2_ = k; }
   void Raw(uint32_t counter, uint64_t out[4]) const;
   double GetRandomNumber(uint32_t counter) const;
+  XYZ GetRandomCoords(uint32_t counter) const;
   XYZ GetSymRandomCoords(uint32_t counter, double bound) const;
   XYZ GetRandomCoordsOnSphere(uint32_t counter) const;
   double GetSymRandom(uint32_t counter, double bound) const;
@@ -313,6 +314,16 @@ class System {
   int RunStepMT(uint64_t step, MoveRecord *rec);           // System.cpp:262-302
   int RunStepPhilox(uint64_t step, MoveRecord *rec);       // batched variant
   void AdjustMoves(uint64_t step);                         // MoveSettings.cpp:148-258
+  // MultiParticle (src/moves/MultiParticle.h:164-712), NVT, one box.
+  int RunMultiParticle(uint64_t step, MoveRecord *rec);
+  void GridAll();                                          // CellList.cpp:220-240
+
+  // Forces of the MultiParticle move (MultiParticle.h:37-58): the
+  // reference set and the trial set.
+  struct MPForces {
+    std::vector<XYZ> aF, aFR, mF, mFR, torque;
+  };
+  MPForces mpRef, mpNew;
 
   XYZ Pos(int a) const { return {x[a], y[a], z[a]}; }
   int NumAtoms() const { return int(x.size()); }

@@ -229,6 +229,32 @@ void orc_box_force_recip(void *h, double *afx, double *afy, double *afz, double 
   }
 }
 
+// CalculateEnergy::CalculateTorque (CalculateEnergy.cpp:1365-1406) from the
+// pair forces and the reciprocal forces of the current coordinates.
+void orc_torque(void *h, double *tx, double *ty, double *tz) {
+  System *s = S(h);
+  std::vector<XYZ> aF, mF, aFR, mFR, t;
+  double lj = 0, real = 0;
+  s->BoxForce(lj, real, aF, mF);
+  if (s->ff.ewald) {
+    s->ewald.CopyRecip();
+    s->ewald.BoxForceReciprocal(s->x, s->y, s->z, aFR, mFR);
+  } else {
+    aFR.assign(s->NumAtoms(), XYZ());
+  }
+  s->CalculateTorque(aF, aFR, t);
+  for (int m = 0; m < s->NumMols(); ++m) {
+    tx[m] = t[m].x;
+    ty[m] = t[m].y;
+    tz[m] = t[m].z;
+  }
+}
+
+void orc_mp_settings(void *h, double out[2]) {
+  out[0] = S(h)->ms.mp_t_max;
+  out[1] = S(h)->ms.mp_r_max;
+}
+
 double orc_batch_run(int nboxes, void *const *handles, const int64_t *moves, int threads) {
   for (int b = 0; b < nboxes; ++b) S(handles[b])->threads = 1;
   auto t0 = std::chrono::steady_clock::now();

@@ -67,6 +67,9 @@ void orc_box_force(void *h, double *lj, double *real, double *afx,
                    double *mfz);
 void orc_box_force_recip(void *h, double *afx, double *afy, double *afz,
                          double *mfx, double *mfy, double *mfz);
+void orc_torque(void *h, double *tx, double *ty, double *tz);
+/* MultiParticle step sizes: {t_max, r_max}. */
+void orc_mp_settings(void *h, double out[2]);
 
 /* Many boxes, Philox streams, one box per OpenMP thread.  Returns elapsed
  * seconds of the move loop only; moves[b] moves are made in box b. */

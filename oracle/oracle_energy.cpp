@@ -3,6 +3,9 @@
 #include <algorithm>
 #include <cstring>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "oracle.hpp"
 
 namespace orc {
@@ -399,40 +402,67 @@ double System::MoleculeTailChange(int kind, bool add) const {
 void System::BoxForce(double &lj, double &real, std::vector<XYZ> &aF,
                       std::vector<XYZ> &mF) {
   const int na = NumAtoms(), nm = NumMols();
-  aF.assign(na, XYZ());
-  mF.assign(nm, XYZ());
   std::vector<int> cellVector, cellStart, map;
   cells.GetCellListNeighbor(na, cellVector, cellStart, map);
-  double tempREn = 0.0, tempLJEn = 0.0;
-  for (int ci = 0; ci < int(cellVector.size()); ci++) {
-    int cur = cellVector[ci];
-    int curCell = map[cur];
-    for (int nc = 0; nc < kNeighborCells; nc++) {
-      int nb = cells.neighbors[curCell][nc];
-      for (int ni = cellStart[nb]; ni < cellStart[nb + 1]; ni++) {
-        int n = cellVector[ni];
-        if (cur < n && particleMol[cur] != particleMol[n]) {
-          double distSq;
-          XYZ vc, fLJ, fReal;
-          if (box.InRcut(distSq, vc, Pos(cur), Pos(n))) {
-            if (ff.electrostatic) {
-              double qiqj = particleCharge[cur] * particleCharge[n] * kQQFact;
-              if (qiqj != 0.0) {
-                tempREn += ff.CalcCoulomb(distSq, qiqj);
-                fReal = vc * ff.CalcCoulombVir(distSq, qiqj);
+  const int nv = int(cellVector.size());
+  // The reference reduces OpenMP thread copies of the force arrays
+  // (CalculateEnergy.cpp:320-327); here each thread owns a contiguous
+  // static slice and the copies are summed in thread order.
+  const int nt = std::max(1, threads);
+  std::vector<std::vector<XYZ>> tA(nt, std::vector<XYZ>(na)), tM(nt, std::vector<XYZ>(nm));
+  std::vector<double> tR(nt, 0.0), tL(nt, 0.0);
+#pragma omp parallel num_threads(nt)
+  {
+#ifdef _OPENMP
+    const int t = omp_get_thread_num(), T = omp_get_num_threads();
+#else
+    const int t = 0, T = 1;
+#endif
+    const int lo = int(int64_t(nv) * t / T), hi = int(int64_t(nv) * (t + 1) / T);
+    std::vector<XYZ> &A = tA[t];
+    std::vector<XYZ> &M = tM[t];
+    double tempREn = 0.0, tempLJEn = 0.0;
+    for (int ci = lo; ci < hi; ci++) {
+      int cur = cellVector[ci];
+      int curCell = map[cur];
+      for (int nc = 0; nc < kNeighborCells; nc++) {
+        int nb = cells.neighbors[curCell][nc];
+        for (int ni = cellStart[nb]; ni < cellStart[nb + 1]; ni++) {
+          int n = cellVector[ni];
+          if (cur < n && particleMol[cur] != particleMol[n]) {
+            double distSq;
+            XYZ vc, fLJ, fReal;
+            if (box.InRcut(distSq, vc, Pos(cur), Pos(n))) {
+              if (ff.electrostatic) {
+                double qiqj = particleCharge[cur] * particleCharge[n] * kQQFact;
+                if (qiqj != 0.0) {
+                  tempREn += ff.CalcCoulomb(distSq, qiqj);
+                  fReal = vc * ff.CalcCoulombVir(distSq, qiqj);
+                }
               }
+              tempLJEn += ff.CalcEn(distSq, particleType[cur], particleType[n]);
+              fLJ = vc * ff.CalcVir(distSq, particleType[cur], particleType[n]);
+              XYZ f = {fLJ.x + fReal.x, fLJ.y + fReal.y, fLJ.z + fReal.z};
+              A[cur] = A[cur] + f;
+              A[n] = A[n] + (-f);
+              M[particleMol[cur]] = M[particleMol[cur]] + f;
+              M[particleMol[n]] = M[particleMol[n]] + (-f);
             }
-            tempLJEn += ff.CalcEn(distSq, particleType[cur], particleType[n]);
-            fLJ = vc * ff.CalcVir(distSq, particleType[cur], particleType[n]);
-            XYZ f = {fLJ.x + fReal.x, fLJ.y + fReal.y, fLJ.z + fReal.z};
-            aF[cur] = aF[cur] + f;
-            aF[n] = aF[n] + (-f);
-            mF[particleMol[cur]] = mF[particleMol[cur]] + f;
-            mF[particleMol[n]] = mF[particleMol[n]] + (-f);
           }
         }
       }
     }
+    tR[t] = tempREn;
+    tL[t] = tempLJEn;
+  }
+  aF.assign(na, XYZ());
+  mF.assign(nm, XYZ());
+  double tempREn = 0.0, tempLJEn = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    for (int i = 0; i < na; ++i) aF[i] = aF[i] + tA[t][i];
+    for (int m = 0; m < nm; ++m) mF[m] = mF[m] + tM[t][m];
+    tempREn += tR[t];
+    tempLJEn += tL[t];
   }
   lj = tempLJEn;
   real = tempREn;
@@ -532,13 +562,16 @@ void Ewald::BoxReciprocalSetup() {
 
 // src/Ewald.cpp:281-361: the same sums over the Ref k-vectors.
 void Ewald::BoxReciprocalSums() {
-  std::fill(sumRnew.begin(), sumRnew.begin() + imageSizeRef, 0.0);
-  std::fill(sumInew.begin(), sumInew.begin() + imageSizeRef, 0.0);
-  for (int m = 0; m < sys.NumMols(); ++m) {
-    const MolKind &kind = sys.kinds[sys.molKind[m]];
-    int start = sys.molStart[m];
-#pragma omp parallel for num_threads(sys.threads) schedule(static) if (imageSizeRef > 2048)
-    for (int i = 0; i < imageSizeRef; i++) {
+  // The reference loops molecules outside and k inside, adding each
+  // molecule's term to sumRnew[i]; looping k outside and molecules inside
+  // performs the same additions in the same order for every k.
+  const int nm = sys.NumMols();
+#pragma omp parallel for num_threads(sys.threads) schedule(static)
+  for (int i = 0; i < imageSizeRef; i++) {
+    double R = 0.0, I = 0.0;
+    for (int m = 0; m < nm; ++m) {
+      const MolKind &kind = sys.kinds[sys.molKind[m]];
+      int start = sys.molStart[m];
       double sumReal = 0.0, sumImaginary = 0.0;
       for (int j = 0; j < kind.NumAtoms(); j++) {
         int a = start + j;
@@ -547,9 +580,11 @@ void Ewald::BoxReciprocalSums() {
         sumReal += (kind.charge[j] * std::cos(dot));
         sumImaginary += (kind.charge[j] * std::sin(dot));
       }
-      sumRnew[i] += sumReal;
-      sumInew[i] += sumImaginary;
+      R += sumReal;
+      I += sumImaginary;
     }
+    sumRnew[i] = R;
+    sumInew[i] = I;
   }
 }
 
@@ -720,6 +755,8 @@ void Ewald::BoxForceReciprocal(const std::vector<double> &x, const std::vector<d
   double constValue = ff.alpha * M_2_SQRTPI;
   aFR.assign(sys.NumAtoms(), XYZ());
   mFR.assign(sys.NumMols(), XYZ());
+  // Molecules are independent; each atom's k-sum keeps the serial order.
+#pragma omp parallel for num_threads(sys.threads) schedule(dynamic, 4)
   for (int m = 0; m < sys.NumMols(); m++) {
     int start = sys.molStart[m], length = sys.molLen[m];
     const double lambdaCoef = 1.0;

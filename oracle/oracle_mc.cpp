@@ -173,6 +173,27 @@ void System::AdjustMoves(uint64_t step) {
       ms.tempAccepted[mv][k] = 0;
       ms.tempTries[mv][k] = 0;
     }
+  // MoveSettings::AdjustMultiParticle (MoveSettings.cpp:164-200); the
+  // MultiParticle target acceptance is mp::TARGET_ACCEPT_FRACT = 0.3
+  // (MoveSettings.h:22), the tolerance 0.1 and both alphas 0.2.
+  for (int t = 0; t < 2; ++t) {
+    if (ms.mp_int_tries[t] == 0) continue;
+    ms.mp_tries[t] += ms.mp_int_tries[t];
+    ms.mp_accepted[t] += ms.mp_int_acc[t];
+    double fractTotal = (double(ms.mp_accepted[t]) / double(ms.mp_tries[t])) / 0.3;
+    double fractInterval = (double(ms.mp_int_acc[t]) / double(ms.mp_int_tries[t])) / 0.3;
+    double &mx = (t == 0) ? ms.mp_t_max : ms.mp_r_max;
+    if (fractInterval == 0.0)
+      mx *= 0.5;
+    else if (std::fabs(fractInterval - 0.3) > 0.1)
+      mx *= ((1.0 - 0.2) * fractTotal + 0.2 * fractInterval);
+    if (t == 0)
+      Bound(mx, 0.001, (lmin * 0.5) - 0.001);
+    else
+      Bound(mx, 0.001, M_PI - 0.001);
+    ms.mp_int_acc[t] = 0;
+    ms.mp_int_tries[t] = 0;
+  }
 }
 
 namespace {
@@ -263,7 +284,8 @@ int System::RunStepMT(uint64_t step, MoveRecord *rec) {
   uint32_t kind = 0;
   double draw = 0;
   prng.PickArbDist(kind, draw, w, 1.0, 9);
-  if (kind != kDisplace && kind != kRotate) return -1;  // MP not in this loop
+  if (kind == kMultiParticle) return RunMultiParticle(step, rec);
+  if (kind != kDisplace && kind != kRotate) return -1;
   // PickMolAndBox with one box: boxDiv = movPerc, subDraw unchanged.
   double subPerc = movePerc[kind];
   double boxDiv = subPerc / 1;
@@ -308,6 +330,202 @@ int System::RunStepMT(uint64_t step, MoveRecord *rec) {
   RotateCoords(*this, m, theta, axis, newPos);
   double pr = prng();
   return EvaluateAndAccept(*this, kRotate, m, newPos, nullptr, pr, rec);
+}
+
+
+// CellList::GridAll (src/CellList.cpp:220-240): every molecule re-added in
+// molecule order.
+void System::GridAll() {
+  cells.Clear(NumAtoms());
+  for (int m = 0; m < NumMols(); ++m)
+    for (int p = molStart[m]; p < molStart[m] + molLen[m]; ++p) cells.AddAtom(p, Pos(p));
+}
+
+namespace {
+constexpr double kMinForce = 1E-12, kMaxForce = 30;  // MultiParticle.h:18-19
+constexpr double kMPLambda = 0.5;                     // MultiParticle.h:95
+
+// MultiParticle::CalcRandomTransform (MultiParticle.h:546-563).
+XYZ CalcRandomTransform(bool &inRange, const XYZ &lb, double max, uint32_t m,
+                        const Random123Wrapper &r) {
+  XYZ lbmax = lb * max;
+  XYZ val;
+  inRange = std::abs(lbmax.x) > kMinForce && std::abs(lbmax.x) < kMaxForce &&
+            std::abs(lbmax.y) > kMinForce && std::abs(lbmax.y) < kMaxForce &&
+            std::abs(lbmax.z) > kMinForce && std::abs(lbmax.z) < kMaxForce;
+  if (inRange) {
+    XYZ u = r.GetRandomCoords(m);
+    val.x = std::log(std::exp(-1.0 * lbmax.x) + 2.0 * u.x * std::sinh(lbmax.x)) / lb.x;
+    val.y = std::log(std::exp(-1.0 * lbmax.y) + 2.0 * u.y * std::sinh(lbmax.y)) / lb.y;
+    val.z = std::log(std::exp(-1.0 * lbmax.z) + 2.0 * u.z * std::sinh(lbmax.z)) / lb.z;
+  }
+  return val;
+}
+
+// MultiParticle::CalculateWRatio (MultiParticle.h:445-462).
+double CalculateWRatio(const XYZ &n, const XYZ &o, const XYZ &k, double max) {
+  double w = 1.0;
+  w *= n.x * std::exp(-n.x * k.x) / (2.0 * std::sinh(n.x * max));
+  w /= o.x * std::exp(o.x * k.x) / (2.0 * std::sinh(o.x * max));
+  w *= n.y * std::exp(-n.y * k.y) / (2.0 * std::sinh(n.y * max));
+  w /= o.y * std::exp(o.y * k.y) / (2.0 * std::sinh(o.y * max));
+  w *= n.z * std::exp(-n.z * k.z) / (2.0 * std::sinh(n.z * max));
+  w /= o.z * std::exp(o.z * k.z) / (2.0 * std::sinh(o.z * max));
+  return w;
+}
+}  // namespace
+
+// MultiParticle Prep -> Transform -> CalcEn -> Accept for the one box
+// (MultiParticle.h:164-235, 332-413, 415-544, 565-712), CPU branch.
+int System::RunMultiParticle(uint64_t step, MoveRecord *rec) {
+  const int nm = NumMols(), na = NumAtoms();
+  bool allTranslate = true;  // MultiParticle.h:99-107
+  for (const MolKind &k : kinds) allTranslate = allTranslate && k.NumAtoms() == 1;
+  const int moveType = allTranslate ? 0 : int(prng.randIntExc(2));
+  Random123Wrapper r;  // System.cpp:262-269: key {step, seed}
+  r.SetStep(step);
+  r.SetRandomSeed(philoxSeed);
+  // Prep: forces of the current state when a single-molecule move was
+  // accepted since the last MultiParticle move.
+  if (ms.singleMoveAccepted) {
+    if (ff.ewald) {
+      ewald.CopyRecip();
+      ewald.BoxForceReciprocal(x, y, z, mpRef.aFR, mpRef.mFR);
+    } else {
+      mpRef.aFR.assign(na, XYZ());
+      mpRef.mFR.assign(nm, XYZ());
+    }
+    double lj = 0, real = 0;
+    BoxForce(lj, real, mpRef.aF, mpRef.mF);
+    CalculateTorque(mpRef.aF, mpRef.aFR, mpRef.torque);
+    pot.Total();
+  }
+  // Transform: CalculateTrialDistRot (MultiParticle.h:565-611).
+  const double tmax = ms.mp_t_max, rmax = ms.mp_r_max;
+  std::vector<double> nx = x, ny = y, nz = z, ncx = comx, ncy = comy, ncz = comz;
+  std::vector<XYZ> tk(nm), rk(nm);
+  std::vector<char> inRange(nm, 0);
+  for (int m = 0; m < nm; ++m) {
+    const int st = molStart[m], len = molLen[m];
+    const XYZ com(comx[m], comy[m], comz[m]);
+    bool in = false;
+    if (moveType == 1) {
+      XYZ lbt = mpRef.torque[m] * kMPLambda * ff.beta;
+      XYZ val = CalcRandomTransform(in, lbt, rmax, uint32_t(m), r);
+      rk[m] = val;
+      RotationMatrix mat;
+      if (in) {  // RotateForceBiased (613-644)
+        double rotLen = std::sqrt(val.x * val.x + val.y * val.y + val.z * val.z);
+        XYZ axis = val * (1.0 / rotLen);
+        mat = RotationMatrix::FromAxisAngle(rotLen, axis);
+      } else {  // RotateRandom (667-692)
+        double symRand = r.GetSymRandom(uint32_t(m), rmax);
+        XYZ sph = r.GetRandomCoordsOnSphere(uint32_t(m));
+        mat = RotationMatrix::FromAxisAngle(symRand, sph);
+      }
+      for (int p = st; p < st + len; ++p) {
+        XYZ t = box.Unwrap(Pos(p), com);
+        t = t + (-com);
+        t = mat.Apply(t);
+        t = t + com;
+        t = box.Wrap(t);
+        nx[p] = t.x;
+        ny[p] = t.y;
+        nz[p] = t.z;
+      }
+    } else {
+      XYZ lbf = (mpRef.mF[m] + mpRef.mFR[m]) * kMPLambda * ff.beta;
+      XYZ val = CalcRandomTransform(in, lbf, tmax, uint32_t(m), r);
+      tk[m] = val;
+      // TranslateForceBiased (646-665) / TranslateRandom (694-712)
+      XYZ shift = in ? val : r.GetSymRandomCoords(uint32_t(m), tmax);
+      for (int p = st; p < st + len; ++p) {
+        XYZ t = box.Wrap(Pos(p) + shift);
+        nx[p] = t.x;
+        ny[p] = t.y;
+        nz[p] = t.z;
+      }
+      XYZ c = box.Wrap(com + shift);
+      ncx[m] = c.x;
+      ncy[m] = c.y;
+      ncz[m] = c.z;
+    }
+    inRange[m] = in;
+  }
+  // Transform's checks (MultiParticle.h:366-406): a displacement beyond
+  // half the box in any component, or a non-finite transform, fails.
+  bool fail = false;
+  for (int m = 0; m < nm && !fail; ++m) {
+    XYZ num = (moveType == 1) ? rk[m] : tk[m];
+    if (moveType == 0 && (num.x > box.halfAx.x || num.y > box.halfAx.y || num.z > box.halfAx.z))
+      fail = true;
+    else if (!std::isfinite(std::sqrt(num.x * num.x + num.y * num.y + num.z * num.z)))
+      fail = true;
+  }
+  Energy potNew = pot;
+  double wRatio = 1.0;
+  if (!fail) {
+    // CalcEn (415-443) on the trial coordinates.
+    std::swap(x, nx);
+    std::swap(y, ny);
+    std::swap(z, nz);
+    std::swap(comx, ncx);
+    std::swap(comy, ncy);
+    std::swap(comz, ncz);
+    GridAll();
+    if (ff.ewald) ewald.BoxReciprocalSums();
+    BoxForce(potNew.inter, potNew.real, mpNew.aF, mpNew.mF);
+    if (ff.ewald) {
+      potNew.recip = ewald.BoxReciprocal(false);
+      ewald.BoxForceReciprocal(x, y, z, mpNew.aFR, mpNew.mFR);
+    } else {
+      mpNew.aFR.assign(na, XYZ());
+      mpNew.mFR.assign(nm, XYZ());
+    }
+    CalculateTorque(mpNew.aF, mpNew.aFR, mpNew.torque);
+    // GetCoeff (464-511), molecules in index order.
+    const double lBeta = kMPLambda * ff.beta;
+    for (int m = 0; m < nm; ++m) {
+      if (!inRange[m]) continue;
+      if (moveType == 1)
+        wRatio *= CalculateWRatio(mpNew.torque[m] * lBeta, mpRef.torque[m] * lBeta, rk[m], rmax);
+      else
+        wRatio *= CalculateWRatio((mpNew.mF[m] + mpNew.mFR[m]) * lBeta,
+                                  (mpRef.mF[m] + mpRef.mFR[m]) * lBeta, tk[m], tmax);
+    }
+    if (!std::isfinite(wRatio)) wRatio = 0.0;
+  }
+  // Accept (513-544).
+  const double uBoltz = std::exp(-ff.beta * (potNew.Total() - pot.Total()));
+  const double accept = wRatio * uBoltz;
+  const double pr = r.GetRandomNumber(uint32_t(nm));
+  const bool result = !fail && pr < accept;
+  if (rec) {
+    rec->kind = kMultiParticle;
+    rec->mol = moveType;
+    rec->accepted = result;
+    rec->overlap = fail;
+    rec->dLJ = fail ? 0.0 : potNew.inter - pot.inter;
+    rec->dReal = fail ? 0.0 : potNew.real - pot.real;
+    rec->dRecip = fail ? 0.0 : potNew.recip - pot.recip;
+  }
+  if (result) {
+    pot = potNew;
+    std::swap(mpRef, mpNew);
+    if (ff.ewald) ewald.UpdateRecip();
+  } else if (!fail) {
+    std::swap(x, nx);
+    std::swap(y, ny);
+    std::swap(z, nz);
+    std::swap(comx, ncx);
+    std::swap(comy, ncy);
+    std::swap(comz, ncz);
+    GridAll();
+  }
+  ms.mp_int_tries[moveType]++;  // UpdateMoveSettingMultiParticle (202-208)
+  if (result) ms.mp_int_acc[moveType]++;
+  ms.singleMoveAccepted = false;  // MoveSettings::Update (MoveSettings.cpp:117-118)
+  return result;
 }
 
 // The batched streams: one Philox4x64-10 stream per box, two blocks per move

@@ -101,6 +101,13 @@ double Random123Wrapper::GetRandomNumber(uint32_t counter) const {
   return U01(r[0]);
 }
 
+// src/Random123Wrapper.cpp:46-53.
+XYZ Random123Wrapper::GetRandomCoords(uint32_t counter) const {
+  uint64_t r[4];
+  Raw(counter, r);
+  return {U01(r[0]), U01(r[1]), U01(r[2])};
+}
+
 XYZ Random123Wrapper::GetSymRandomCoords(uint32_t counter, double bound) const {
   uint64_t r[4];
   Raw(counter, r);

@@ -47,6 +47,8 @@ _SIG = {
     "orc_run": (C.c_int64, [_vp, C.c_int64, C.c_int64, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp]),
     "orc_box_force": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "orc_box_force_recip": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "orc_torque": (None, [_vp, _dp, _dp, _dp]),
+    "orc_mp_settings": (None, [_vp, _dp]),
     "orc_batch_run": (C.c_double, [C.c_int, C.POINTER(_vp), _lp, C.c_int]),
     "orc_mt_raw": (None, [C.c_uint32, C.c_int, _up]),
     "orc_mt_rand_int": (None, [C.c_uint32, C.c_int, C.c_uint32, _up]),
@@ -230,6 +232,16 @@ class Oracle:
         a = [np.zeros(self.natoms) for _ in range(3)] + [np.zeros(self.nmols) for _ in range(3)]
         self.L.orc_box_force_recip(self.h, *(_p(v) for v in a))
         return np.stack(a[:3], 1), np.stack(a[3:], 1)
+
+    def torque(self):
+        t = [np.zeros(self.nmols) for _ in range(3)]
+        self.L.orc_torque(self.h, *(_p(v) for v in t))
+        return np.stack(t, 1)
+
+    def mp_settings(self):
+        out = np.zeros(2)
+        self.L.orc_mp_settings(self.h, _p(out))
+        return out
 
 
 def batch_run(oracles: list[Oracle], moves, threads: int) -> float:
