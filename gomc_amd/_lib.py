@@ -50,6 +50,7 @@ SIGNATURES = {
     "gomc_mp_settings": (C.c_int, [_vp, _dp]),
     "gomc_run_moves": (C.c_int64, [_vp, C.c_int64, C.c_int64, _ip, _ip, _ip, _ip, _dp, _dp, _dp]),
     "gomc_energies": (C.c_int, [_vp, _dp]),
+    "gomc_mol_boxes": (C.c_int, [_vp, _ip]),
     "gomc_get_scales": (C.c_int, [_vp, _dp, _dp]),
     "gomc_batch_create": (C.c_int, [C.POINTER(SystemDesc), C.c_int, C.c_int, C.c_int,
                                     C.POINTER(_vp)]),

@@ -197,6 +197,12 @@ class Box:
                                                 ptr(rec["drecip"])))
         return n, rec
 
+    def mol_boxes(self):
+        """(molecules in box 0, box of every molecule: 0 box, 1 reservoir)."""
+        b = np.zeros(self.nmols, dtype=np.int32)
+        n = self._check(self.lib.gomc_mol_boxes(self.h, ptr(b)))
+        return n, b
+
     def scales(self):
         nk = len(self.system.kinds)
         d, r = np.zeros(nk), np.zeros(nk)

@@ -30,7 +30,7 @@ namespace {
 struct TrialArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
-  const int *type;
+  const int *type, *mol;
   const double *sigSq, *epsCn, *nHalf;
   int ntypes, natoms, mstart, mlen;
   int nPairBlocks, doPair, doRecip, K;
@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
   if (int(blockIdx.x) < a.nPairBlocks) {
     double lj = 0.0, real = 0.0, ov = 0.0;
     const int j = blockIdx.x * kBlock + tid;
-    if (a.doPair && j < a.natoms && (j < a.mstart || j >= a.mstart + a.mlen)) {
+    if (a.doPair && j < a.natoms && (j < a.mstart || j >= a.mstart + a.mlen) && a.mol[j] >= 0) {
       double xj, yj, zj;
       const int cj = j - a.cstart;
       if (cj >= 0 && cj < a.clen) {
@@ -147,6 +147,7 @@ __device__ __forceinline__ int PositionToCell(double px, double py, double pz, c
 
 struct CellArgs {
   const double *x, *y, *z;
+  const int *mol;  // < 0: reservoir atom, not gridded
   int natoms, ncell;
   double cs[3];
   int edge[3];
@@ -156,6 +157,10 @@ struct CellArgs {
 __global__ void __launch_bounds__(kBlock) CellCountKernel(const CellArgs a) {
   int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.natoms) return;
+  if (a.mol[i] < 0) {
+    a.atomCell[i] = -1;
+    return;
+  }
   int c = PositionToCell(a.x[i], a.y[i], a.z[i], a.cs, a.edge);
   a.atomCell[i] = c;
   atomicAdd(&a.cellCount[c], 1);
@@ -191,7 +196,7 @@ __global__ void __launch_bounds__(1024) CellScanKernel(const CellArgs a) {
 
 __global__ void __launch_bounds__(kBlock) CellScatterKernel(const CellArgs a) {
   int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.natoms) return;
+  if (i >= a.natoms || a.atomCell[i] < 0) return;
   int pos = atomicAdd(&a.cellCursor[a.atomCell[i]], 1);
   a.cellVec[pos] = i;
 }
@@ -278,6 +283,7 @@ __global__ void __launch_bounds__(kBlock) BoxInterKernel(const BoxInterArgs a) {
 // energy prefact (Re^2 + Im^2) (Ewald.cpp:375-406).
 struct SumsArgs {
   const double *x, *y, *z, *q;
+  const int *mol;
   int natoms, K;
   const double *kx, *ky, *kz, *pref;
   double *sumR, *sumI;
@@ -301,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) RecipSumsKernel(const SumsArgs a) {
       sx[threadIdx.x] = a.x[j];
       sy[threadIdx.x] = a.y[j];
       sz[threadIdx.x] = a.z[j];
-      sq[threadIdx.x] = a.q[j];
+      sq[threadIdx.x] = a.mol[j] >= 0 ? a.q[j] : 0.0;  // reservoir atoms add nothing
     }
     __syncthreads();
     if (active)
@@ -383,7 +389,7 @@ __global__ void __launch_bounds__(kBlock) ParticleInterKernel(const ParticleArgs
   const double px = a.tx[t], py = a.ty[t], pz = a.tz[t];
   double lj = 0.0, real = 0.0, ov = 0.0;
   for (int j = threadIdx.x; j < a.natoms; j += kBlock) {
-    if (a.mol[j] == a.skipMol) continue;
+    if (a.mol[j] == a.skipMol || a.mol[j] < 0) continue;
     double d2 = DistSq(a.g, px, py, pz, a.x[j], a.y[j], a.z[j]);
     if (a.g.rCutSq > d2) {
       if (d2 < a.g.rCutLowSq) ov = 1.0;
@@ -442,6 +448,7 @@ int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const do
   a.z = b->z;
   a.q = b->q;
   a.type = b->type;
+  a.mol = b->mol;
   a.sigSq = b->tSigSq;
   a.epsCn = b->tEpsCn;
   a.nHalf = b->tNHalf;
@@ -509,6 +516,7 @@ int BuildCells(gomc_box *b) {
   a.x = b->x;
   a.y = b->y;
   a.z = b->z;
+  a.mol = b->mol;
   a.natoms = b->natoms;
   a.ncell = b->ncell;
   for (int i = 0; i < 3; ++i) {
@@ -581,6 +589,7 @@ int RecipSums(gomc_box *b, double *erecip) {
   a.y = b->y;
   a.z = b->z;
   a.q = b->q;
+  a.mol = b->mol;
   a.natoms = b->natoms;
   a.K = b->K;
   a.kx = b->kx;
@@ -624,6 +633,7 @@ void HostCorrectionSelfLRC(gomc_box *b, double &self, double &corr, double &lrc)
   self *= -1.0 * b->alpha * kQQFact * M_2_SQRTPI * 0.5;
   corr = 0.0;
   for (int m = 0; m < b->nmols; ++m) {
+    if (!b->molBox.empty() && b->molBox[m] != 0) continue;
     const int s = b->molStart[m], len = b->molLen[m], kind = b->molKind[m];
     double c = 0.0;
     for (int i = 0; i < len; ++i) {
@@ -661,6 +671,7 @@ static void FreeAll(gomc_box *b) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   MPFree(b);
+  if (b->trialBuf) (void)hipFree(b->trialBuf);
   if (b->hostRes) (void)hipHostFree(b->hostRes);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
@@ -745,6 +756,69 @@ int FlushPending(gomc_box *b) {
   if (rc) return rc;
   b->pendingMol = -1;
   return GOMC_OK;
+}
+
+// Energies of `n` trial positions of one atom against every live atom
+// (CalculateEnergy::ParticleInter, src/CalculateEnergy.cpp:727-785); the
+// trial staging buffer is the box's and grows only when `n` does.
+int ParticleInterImpl(gomc_box *b, int atomType, double charge, int skipMol, int n,
+                      const double *tx, const double *ty, const double *tz, double *en,
+                      double *real, int32_t *overlap) {
+  if (n > b->trialCap) {
+    if (b->trialBuf) (void)hipFree(b->trialBuf);
+    b->trialBuf = nullptr;
+    b->trialCap = 0;
+    HIP_TRY(b, hipMalloc(&b->trialBuf, sizeof(double) * 6 * size_t(n)));
+    b->trialCap = n;
+  }
+  double *buf = b->trialBuf;
+  const size_t nb = sizeof(double) * n;
+  HIP_TRY(b, hipMemcpyAsync(buf, tx, nb, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(buf + n, ty, nb, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(buf + 2 * n, tz, nb, hipMemcpyHostToDevice, b->stream));
+  ParticleArgs a;
+  a.g = b->geom;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.q = b->q;
+  a.type = b->type;
+  a.mol = b->mol;
+  a.sigSq = b->tSigSq;
+  a.epsCn = b->tEpsCn;
+  a.nHalf = b->tNHalf;
+  a.ntypes = b->ntypes;
+  a.natoms = b->natoms;
+  a.atype = atomType;
+  a.skipMol = skipMol;
+  a.charge = charge;
+  a.tx = buf;
+  a.ty = buf + n;
+  a.tz = buf + 2 * n;
+  a.en = buf + 3 * n;
+  a.real = buf + 4 * n;
+  a.overlap = buf + 5 * n;
+  hipLaunchKernelGGL(ParticleInterKernel, dim3(n), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  std::vector<double> host(3 * size_t(n));
+  HIP_TRY(b, hipMemcpyAsync(host.data(), buf + 3 * n, 3 * nb, hipMemcpyDeviceToHost, b->stream));
+  int rc = Sync(b);
+  if (rc) return rc;
+  for (int t = 0; t < n; ++t) {
+    en[t] = host[t];
+    real[t] = host[n + t];
+    overlap[t] = host[2 * n + t] > 0.0 ? 1 : 0;
+  }
+  return GOMC_OK;
+}
+
+// Moves molecule m's atoms in or out of the interacting set on the device.
+int SetMolLive(gomc_box *b, int m, bool live) {
+  const int s = b->molStart[m], len = b->molLen[m];
+  std::vector<int> ids(len, live ? m : -1);
+  HIP_TRY(b, hipMemcpyAsync(b->mol + s, ids.data(), sizeof(int) * len, hipMemcpyHostToDevice,
+                            b->stream));
+  return Sync(b);
 }
 }  // namespace gomc
 
@@ -851,56 +925,14 @@ int gomc_swap_recip(gomc_box *b, int kind, int insert, const double *x, const do
 int gomc_particle_inter(gomc_box *b, int atom_type, double charge, int skip_mol, int ntrials,
                         const double *tx, const double *ty, const double *tz, double *en,
                         double *real, int32_t *overlap) {
-  if (!b || ntrials <= 0 || atom_type < 0 || atom_type >= b->ntypes) return GOMC_ERR_ARG;
+  if (!b || ntrials <= 0 || atom_type < 0 || atom_type >= b->ntypes || !tx || !ty || !tz ||
+      !en || !real || !overlap)
+    return GOMC_ERR_ARG;
   (void)hipSetDevice(b->device);
   int rc = FlushPending(b);
   if (rc) return rc;
-  double *buf = nullptr;
-  HIP_TRY(b, hipMalloc(&buf, sizeof(double) * 6 * ntrials));
-  const size_t nb = sizeof(double) * ntrials;
-  hipError_t e = hipMemcpyAsync(buf, tx, nb, hipMemcpyHostToDevice, b->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(buf + ntrials, ty, nb, hipMemcpyHostToDevice, b->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(buf + 2 * ntrials, tz, nb, hipMemcpyHostToDevice, b->stream);
-  if (e != hipSuccess) {
-    (void)hipFree(buf);
-    return HipFail(b, e, "particle_inter upload");
-  }
-  ParticleArgs a;
-  a.g = b->geom;
-  a.x = b->x;
-  a.y = b->y;
-  a.z = b->z;
-  a.q = b->q;
-  a.type = b->type;
-  a.mol = b->mol;
-  a.sigSq = b->tSigSq;
-  a.epsCn = b->tEpsCn;
-  a.nHalf = b->tNHalf;
-  a.ntypes = b->ntypes;
-  a.natoms = b->natoms;
-  a.atype = atom_type;
-  a.skipMol = skip_mol;
-  a.charge = charge;
-  a.tx = buf;
-  a.ty = buf + ntrials;
-  a.tz = buf + 2 * ntrials;
-  a.en = buf + 3 * ntrials;
-  a.real = buf + 4 * ntrials;
-  a.overlap = buf + 5 * ntrials;
-  hipLaunchKernelGGL(ParticleInterKernel, dim3(ntrials), dim3(kBlock), 0, b->stream, a);
-  std::vector<double> host(3 * ntrials);
-  e = hipGetLastError();
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(host.data(), buf + 3 * ntrials, 3 * nb, hipMemcpyDeviceToHost, b->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-  (void)hipFree(buf);
-  if (e != hipSuccess) return HipFail(b, e, "particle_inter");
-  for (int t = 0; t < ntrials; ++t) {
-    en[t] = host[t];
-    real[t] = host[ntrials + t];
-    overlap[t] = host[2 * ntrials + t] > 0.0 ? 1 : 0;
-  }
-  return GOMC_OK;
+  return ParticleInterImpl(b, atom_type, charge, skip_mol, ntrials, tx, ty, tz, en, real,
+                           overlap);
 }
 
 int gomc_recip_sums(gomc_box *b, double *erecip) {

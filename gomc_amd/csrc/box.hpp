@@ -143,6 +143,28 @@ struct gomc_box {
   double pend[3][gomc::kMaxMolAtoms];
 
   std::vector<int> hCellCount;  // atoms per cell at the last build
+  // staging for trial positions and their energies (ParticleInter)
+  double *trialBuf = nullptr;
+  int trialCap = 0;
+
+  // ---- GCMC (ensemble 1): box 0 interacts, box 1 is the reservoir ----
+  // Reservoir atoms carry molecule id -1 on the device and are skipped by
+  // every pair, cell and structure-factor kernel.
+  int ensemble = 0;
+  std::vector<int> molBox;
+  // MoleculeLookup (src/MoleculeLookup.cpp:24-119): molecule indices by
+  // [box][kind], section starts, swappable counts, kinds in order of first
+  // appearance.
+  std::vector<uint32_t> lookup, bkStart, swapCount;
+  std::vector<int> canSwapKind, canMoveKind;
+  std::vector<double> fugacity;  // K / A^3
+  int cbmcFirst = 16, cbmcNth = 8, cbmcAng = 100;
+  struct Hedron {  // rigid three-site kind grown from atom 0
+    double bPrev = 0, bBonded = 0, theta = 0;
+  };
+  std::vector<Hedron> hedron;
+  double moveW[13] = {0};  // StaticVals movePerc by move kind
+  std::vector<uint64_t> swapTries, swapAcc;  // [dest box][kind]
   uint64_t seed = 0;            // Random_Seed (MT and Random123)
   gomc::MPState mp;
 
@@ -176,4 +198,14 @@ int MPPairForce(gomc_box *b, int set, double *lj, double *real);
 int MPRecipForce(gomc_box *b, int set);
 int MPMolTerms(gomc_box *b, int set, int comSet);
 int MPMove(gomc_box *b, uint64_t step, int &moveType, int &accepted, int &failed, double dE[3]);
+// GCMC pieces (gcmc.cpp / box.hip).
+int ParticleInterImpl(gomc_box *b, int atomType, double charge, int skipMol, int n,
+                      const double *tx, const double *ty, const double *tz, double *en,
+                      double *real, int32_t *overlap);
+int SetMolLive(gomc_box *b, int m, bool live);
+void LookupInit(gomc_box *b);
+uint32_t NumKindInBox(const gomc_box *b, int k, int box);
+int GetMolNum(const gomc_box *b, uint32_t sub, int k, int box);
+int TransferMove(gomc_box *b, double subDraw, int &mol, int &accepted, int &overlap,
+                 double dE[3]);
 }  // namespace gomc

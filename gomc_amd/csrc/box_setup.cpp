@@ -213,13 +213,21 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   }
   b->nmols = d->n_mols;
   b->kindCount.assign(b->nkinds, 0);
+  b->ensemble = d->ensemble == 1 ? 1 : 0;
+  if (d->ensemble != 0 && d->ensemble != 1) return fail(GOMC_ERR_ARG, "ensemble must be 0 or 1");
+  b->molBox.assign(b->nmols, 0);
+  if (b->ensemble == 1 && d->mol_box)
+    for (int m = 0; m < b->nmols; ++m) {
+      if (d->mol_box[m] != 0 && d->mol_box[m] != 1) return fail(GOMC_ERR_ARG, "mol_box must be 0 or 1");
+      b->molBox[m] = d->mol_box[m];
+    }
   for (int m = 0; m < b->nmols; ++m) {
     const int k = d->mol_kind[m];
     if (k < 0 || k >= b->nkinds) return fail(GOMC_ERR_ARG, "molecule kind out of range");
     b->molKind.push_back(k);
     b->molStart.push_back(b->natoms);
     b->molLen.push_back(int(b->kindType[k].size()));
-    b->kindCount[k]++;
+    if (b->molBox[m] == 0) b->kindCount[k]++;
     for (size_t a = 0; a < b->kindType[k].size(); ++a) {
       b->atomType.push_back(b->kindType[k][a]);
       b->atomCharge.push_back(b->kindCharge[k][a]);
@@ -306,7 +314,10 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   cp &= hipMemcpy(b->z, b->hz.data(), nd, hipMemcpyHostToDevice) == hipSuccess;
   cp &= hipMemcpy(b->q, b->atomCharge.data(), nd, hipMemcpyHostToDevice) == hipSuccess;
   cp &= hipMemcpy(b->type, b->atomType.data(), ni, hipMemcpyHostToDevice) == hipSuccess;
-  cp &= hipMemcpy(b->mol, b->atomMol.data(), ni, hipMemcpyHostToDevice) == hipSuccess;
+  std::vector<int> liveMol(b->atomMol);
+  for (int i = 0; i < b->natoms; ++i)
+    if (b->molBox[b->atomMol[i]] != 0) liveMol[i] = -1;
+  cp &= hipMemcpy(b->mol, liveMol.data(), ni, hipMemcpyHostToDevice) == hipSuccess;
   cp &= hipMemcpy(b->tSigSq, b->sigSq.data(), sizeof(double) * T2, hipMemcpyHostToDevice) == hipSuccess;
   cp &= hipMemcpy(b->tEpsCn, b->epsCn.data(), sizeof(double) * T2, hipMemcpyHostToDevice) == hipSuccess;
   cp &= hipMemcpy(b->tNHalf, b->nHalf.data(), sizeof(double) * T2, hipMemcpyHostToDevice) == hipSuccess;
@@ -342,11 +353,61 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   b->pot.corr = b->ewald ? corr : 0.0;
   b->pot.Total();
   // ---- driver: move mix (StaticVals.cpp:37-96), MoveSettings::Init ----
-  const double tot = d->move_disp + d->move_rot + d->move_mp;
+  const double swapW = b->ensemble == 1 ? d->move_swap : 0.0;
+  const double tot = d->move_disp + d->move_rot + d->move_mp + swapW;
   if (!(tot > 0)) return fail(GOMC_ERR_ARG, "no moves enabled");
+  if (b->ensemble == 1 && d->move_mp > 0)
+    return fail(GOMC_ERR_ARG, "MultiParticle moves are not supported with GCMC");
   b->movePerc[0] = d->move_disp / tot;
   b->movePerc[1] = d->move_rot / tot;
   b->movePerc[2] = d->move_mp / tot;
+  for (double &w : b->moveW) w = 0.0;
+  b->moveW[0] = b->movePerc[0];
+  b->moveW[1] = b->movePerc[1];
+  b->moveW[2] = b->movePerc[2];
+  b->moveW[10] = swapW / tot;
+  // GCMC: fugacities (bar -> K/A^3, UnitConst.h:44), CBMC trial counts, the
+  // molecule lookup and the growth geometry of three-site kinds.
+  b->fugacity.assign(b->nkinds, 0.0);
+  if (d->kind_fugacity)
+    for (int k = 0; k < b->nkinds; ++k) b->fugacity[k] = d->kind_fugacity[k] * 0.007242971;
+  if (d->cbmc_first > 0) b->cbmcFirst = d->cbmc_first;
+  if (d->cbmc_nth > 0) b->cbmcNth = d->cbmc_nth;
+  if (d->cbmc_ang > 0) b->cbmcAng = d->cbmc_ang;
+  LookupInit(b);
+  b->hedron.assign(b->nkinds, gomc_box::Hedron());
+  for (int k = 0; k < b->nkinds; ++k) {
+    const int na = int(b->kindType[k].size());
+    if (b->ensemble == 1 && na != 1 && na != 3)
+      return fail(GOMC_ERR_ARG, "GCMC kinds must have one or three atoms");
+    if (b->ensemble == 1 && !(b->fugacity[k] > 0))
+      return fail(GOMC_ERR_ARG, "GCMC needs a positive fugacity for every kind");
+    if (na != 3) continue;
+    for (int m = 0; m < b->nmols; ++m) {
+      if (b->molKind[m] != k) continue;
+      const int st = b->molStart[m];
+      double a[3], c[3];
+      for (int i = 0; i < 3; ++i) {
+        const double *P = i == 0 ? b->hx.data() : i == 1 ? b->hy.data() : b->hz.data();
+        double u = P[st + 1] - P[st], v = P[st + 2] - P[st];
+        if (u > b->geom.half[i]) u -= b->geom.L[i];
+        else if (u < -b->geom.half[i]) u += b->geom.L[i];
+        if (v > b->geom.half[i]) v -= b->geom.L[i];
+        else if (v < -b->geom.half[i]) v += b->geom.L[i];
+        a[i] = u;
+        c[i] = v;
+      }
+      const double cx = a[1] * c[2] - a[2] * c[1], cy = a[2] * c[0] - a[0] * c[2],
+                   cz = a[0] * c[1] - a[1] * c[0];
+      b->hedron[k].bPrev = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      b->hedron[k].bBonded = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      b->hedron[k].theta = std::atan2(std::sqrt(cx * cx + cy * cy + cz * cz),
+                                      a[0] * c[0] + a[1] * c[1] + a[2] * c[2]);
+      break;
+    }
+  }
+  b->swapTries.assign(2 * b->nkinds, 0);
+  b->swapAcc.assign(2 * b->nkinds, 0);
   b->perAdjust = d->adj_steps > 0 ? uint64_t(d->adj_steps) : 1000;
   const double lmin = std::min(d->box[0], std::min(d->box[1], d->box[2]));
   for (int mv = 0; mv < 2; ++mv) {

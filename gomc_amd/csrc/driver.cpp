@@ -13,7 +13,7 @@ using namespace gomc;
 
 namespace {
 
-constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2;
+constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2, kMolTransfer = 10;
 
 // MoveSettings::AdjustMoves / Adjust (src/MoveSettings.cpp:148-162,205-258).
 void AdjustMoves(gomc_box *b, uint64_t step) {
@@ -65,12 +65,14 @@ struct Record {
 // One move: PickArbDist -> PickMolAndBox -> TranslateRand/RotateRand ->
 // CalcEn -> Accept, drawing from MT19937 exactly where the reference does.
 int OneMove(gomc_box *b, uint64_t step, Record &r) {
-  // PRNG::PickArbDist over the nine NVT move kinds (src/PRNG.h:153-163).
-  const double w[9] = {b->movePerc[0], b->movePerc[1], b->movePerc[2], 0, 0, 0, 0, 0, 0};
+  // PRNG::PickArbDist over the move kinds (src/PRNG.h:153-163): nine in
+  // NVT, thirteen in GCMC (src/MoveConst.h:31-40).
+  const double *w = b->moveW;
+  const uint32_t nKinds = b->ensemble == 1 ? 13 : 9;
   const double draw0 = b->mt.Closed() * 1.0;
   double sum = 0.0, prev = 0.0;
   uint32_t pick = 0;
-  for (pick = 0; pick < 9 && sum < draw0; ++pick) {
+  for (pick = 0; pick < nKinds && sum < draw0; ++pick) {
     prev = sum;
     sum += w[pick];
   }
@@ -91,34 +93,40 @@ int OneMove(gomc_box *b, uint64_t step, Record &r) {
     r.drecip = dE[2];
     return 0;
   }
+  if (pick == kMolTransfer && b->ensemble == 1) {
+    int m = -1, acc = 0, ovl = 0;
+    double dE[3];
+    int rc = TransferMove(b, subDraw, m, acc, ovl, dE);
+    if (rc) return rc;
+    r.kind = kMolTransfer;
+    r.mol = m;
+    r.acc = acc;
+    r.ovl = ovl;
+    r.dlj = dE[0];
+    r.dreal = dE[1];
+    r.drecip = dE[2];
+    return 0;
+  }
   if (pick != kDisplace && pick != kRotate) return Fail(b, GOMC_ERR_STATE, "move kind not served by this driver");
   // PRNG::PickMolAndBox with one box, then PickMol (src/PRNG.h:267-299).
   const double boxDiv = b->movePerc[pick] / 1;
   uint32_t box = uint32_t(subDraw / boxDiv);
   if (box >= 1) box--;
   subDraw -= box * boxDiv;
-  std::vector<int> canMove;
-  for (int m = 0; m < b->nmols; ++m)
-    if (std::find(canMove.begin(), canMove.end(), b->molKind[m]) == canMove.end())
-      canMove.push_back(b->molKind[m]);
-  const uint32_t mkTot = uint32_t(canMove.size());
+  // the molecule through the lookup of box 0 (PRNG.h:283-296)
+  const uint32_t mkTot = uint32_t(b->canMoveKind.size());
   uint32_t k = uint32_t(subDraw / (boxDiv / mkTot));
   if (k == mkTot) k = mkTot - 1;
-  const int mk = canMove[k];
+  const int mk = b->canMoveKind[k];
   int mv = int(pick);
-  if (b->kindCount[mk] == 0) {
+  if (NumKindInBox(b, mk, 0) == 0) {
     b->moves.tries[mv][mk]++;
     b->moves.tempTries[mv][mk]++;
     r.kind = mv;
     return 0;
   }
-  const uint32_t off = b->mt.Below(uint32_t(b->kindCount[mk]) - 1);
-  int m = -1;
-  for (int i = 0, seen = 0; i < b->nmols; ++i)
-    if (b->molKind[i] == mk && seen++ == int(off)) {
-      m = i;
-      break;
-    }
+  const uint32_t off = b->mt.Below(NumKindInBox(b, mk, 0) - 1);
+  const int m = GetMolNum(b, off, mk, 0);
   if (mv == kRotate && b->kindType[mk].size() <= 1) mv = kDisplace;  // Rotate::Prep
   const int s = b->molStart[m], len = b->molLen[m];
   const double *L = b->geom.L, *H = b->geom.half;

@@ -103,4 +103,30 @@ inline Mat3 AxisAngle(double theta, const Vec3 &a) {
   return r;
 }
 
+// TransformMatrix::UniformRandom (src/TransformMatrix.h:216-262): Arvo's
+// (1992) uniform rotation -- a rotation about the pole followed by an
+// inverted Householder reflection that moves the pole to a random point.
+inline Mat3 UniformRotation(double u1, double u2, double u3) {
+  u1 *= 2.0 * M_PI;
+  u1 -= M_PI;
+  u2 *= 2.0 * M_PI;
+  u3 *= 2.0;
+  const double r = std::sqrt(u3);
+  const double vx = std::sin(u2) * r, vy = std::cos(u2) * r, vz = std::sqrt(2.0 - u3);
+  const double sinTh = std::sin(u1), cosTh = std::cos(u1);
+  const double sx = vx * cosTh - vy * sinTh;
+  const double sy = vx * sinTh + vy * cosTh;
+  Mat3 m;
+  m.m[0][0] = vx * sx - cosTh;
+  m.m[0][1] = vx * sy - sinTh;
+  m.m[0][2] = vx * vz;
+  m.m[1][0] = vy * sx + sinTh;
+  m.m[1][1] = vy * sy - cosTh;
+  m.m[1][2] = vy * vz;
+  m.m[2][0] = vz * sx;
+  m.m[2][1] = vz * sy;
+  m.m[2][2] = 1.0 - u3;
+  return m;
+}
+
 }  // namespace gomc

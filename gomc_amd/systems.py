@@ -56,6 +56,13 @@ class SystemDesc(C.Structure):
         ("move_rot", C.c_double),
         ("move_mp", C.c_double),
         ("adj_steps", C.c_int64),
+        ("ensemble", C.c_int32),
+        ("cbmc_first", C.c_int32),
+        ("cbmc_nth", C.c_int32),
+        ("cbmc_ang", C.c_int32),
+        ("mol_box", _ip),
+        ("kind_fugacity", _dp),
+        ("move_swap", C.c_double),
     ]
 
 
@@ -92,6 +99,15 @@ class System:
     electrostatic: bool = True
     lrc: bool = True
     threads: int = 1
+    # GCMC: 1 selects the grand-canonical move set; mol_box 0 (box) / 1
+    # (reservoir) per molecule; fugacity per kind in bar.
+    ensemble: int = 0
+    mol_box: np.ndarray | None = None
+    fugacity: list[float] | None = None
+    move_swap: float = 0.0
+    cbmc_first: int = 16
+    cbmc_nth: int = 8
+    cbmc_ang: int = 100
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -125,7 +141,11 @@ class System:
         x = np.ascontiguousarray(self.xyz[:, 0], dtype=np.float64)
         y = np.ascontiguousarray(self.xyz[:, 1], dtype=np.float64)
         z = np.ascontiguousarray(self.xyz[:, 2], dtype=np.float64)
-        self._keep = [sig, eps, nrep, kn, kt, kq, mk, x, y, z]
+        mb = np.ascontiguousarray(self.mol_box if self.mol_box is not None
+                                  else np.zeros(len(mk)), dtype=np.int32)
+        fug = np.ascontiguousarray(self.fugacity if self.fugacity is not None
+                                   else np.zeros(len(self.kinds)), dtype=np.float64)
+        self._keep = [sig, eps, nrep, kn, kt, kq, mk, x, y, z, mb, fug]
         d.n_atom_types = len(sig)
         d.n_kinds = len(self.kinds)
         d.sigma, d.epsilon, d.n_rep = (a.ctypes.data_as(_dp) for a in (sig, eps, nrep))
@@ -138,6 +158,12 @@ class System:
         d.seed = int(self.seed)
         d.move_disp, d.move_rot, d.move_mp = self.move_disp, self.move_rot, self.move_mp
         d.adj_steps = int(self.adj_steps)
+        d.ensemble = int(self.ensemble)
+        d.cbmc_first, d.cbmc_nth, d.cbmc_ang = (int(self.cbmc_first), int(self.cbmc_nth),
+                                                int(self.cbmc_ang))
+        d.mol_box = mb.ctypes.data_as(_ip)
+        d.kind_fugacity = fug.ctypes.data_as(_dp)
+        d.move_swap = float(self.move_swap)
         return d
 
 
@@ -245,3 +271,47 @@ def mixture_box(n_ch4: int, n_co2: int, box: float, temperature: float = 300.0,
     return System(box=(box, box, box), temperature=temperature, rcut=rcut, rcut_low=1.0,
                   rcut_coulomb=rcut, tolerance=tolerance, kinds=kinds, mol_kind=mk, xyz=xyz,
                   seed=seed, **MIXTURE_TYPES, **kw)
+
+
+BAR_TO_K_PER_A3 = 0.007242971  # UnitConst.h:44
+
+
+def gcmc_mixture(n_ch4: int, n_co2: int, box: float, temperature: float = 300.0,
+                 fugacity_bar: tuple[float, float] = (10.0, 10.0), reservoir: int | None = None,
+                 rcut: float = 10.0, tolerance: float = 1e-5, seed: int = 12345,
+                 swap: float = 0.4, disp: float = 0.3, rot: float = 0.3, **kw) -> System:
+    """Grand-canonical methane / CO2 box: `n_ch4` + `n_co2` molecules placed in
+    the box, and a reservoir (GOMC's second GCMC box, no interactions) of
+    `reservoir` molecules of each kind to insert from.  Fugacities in bar."""
+    kinds = [methane_kind(0), co2_kind(1, 2)]
+    rng = np.random.default_rng(seed)
+    if reservoir is None:
+        reservoir = max(n_ch4, n_co2, 16)
+    mk_box = np.array([0] * n_ch4 + [1] * n_co2, dtype=np.int32)
+    rng.shuffle(mk_box)
+    mk_res = np.array([0] * reservoir + [1] * reservoir, dtype=np.int32)
+    rng.shuffle(mk_res)
+    parts, boxes = [], []
+    if len(mk_box):
+        parts.append(place_molecules(kinds, mk_box, box, seed, min_heavy=3.0, min_any=2.0))
+        boxes.append(np.zeros(len(mk_box), np.int32))
+    # reservoir molecules do not interact: any non-overlapping draw will do
+    parts.append(place_molecules(kinds, mk_res, box, seed + 7, min_heavy=0.5, min_any=0.3))
+    boxes.append(np.ones(len(mk_res), np.int32))
+    xyz = np.concatenate(parts)
+    mk = np.concatenate([mk_box, mk_res])
+    rc = min(rcut, box / 2.0)
+    return System(box=(box, box, box), temperature=temperature, rcut=rc, rcut_low=1.0,
+                  rcut_coulomb=rc, tolerance=tolerance, kinds=kinds, mol_kind=mk, xyz=xyz,
+                  seed=seed, ensemble=1, mol_box=np.concatenate(boxes),
+                  fugacity=list(fugacity_bar), move_swap=swap, move_disp=disp, move_rot=rot,
+                  **MIXTURE_TYPES, **kw)
+
+
+def config3(seed: int = 12345) -> System:
+    """GCMC methane + CO2, 40 A box, 300 K, about 2000 molecules in total at
+    the chosen fugacities; 40 % transfers with 16 first-atom trials, 30 %
+    translate, 30 % rotate, Ewald tolerance 1e-5."""
+    return gcmc_mixture(1000, 1000, 40.0, temperature=300.0, fugacity_bar=(60.0, 25.0),
+                        reservoir=1500, seed=seed, cbmc_first=16, cbmc_nth=8, cbmc_ang=100)
+

@@ -37,6 +37,8 @@ extern "C" {
  *   x,y,z  -- wrapped coordinates of all atoms, A
  *   seed   -- Random_Seed; move_* -- DisFreq, RotFreq, MultiParticleFreq
  *   adj_steps -- AdjSteps
+ *   ensemble, cbmc_*, mol_box, kind_fugacity, move_swap -- GCMC settings
+ *                 (Fugacity, CBMC_First/Nth/Ang, SwapFreq)
  * The same layout is used by the CPU oracle's orc_system_desc. */
 typedef struct gomc_system_desc {
   double box[3];
@@ -54,6 +56,14 @@ typedef struct gomc_system_desc {
   uint64_t seed;
   double move_disp, move_rot, move_mp;
   int64_t adj_steps;
+  /* GCMC (ensemble 1): molecules with mol_box[m] == 0 are in the simulation
+   * box; mol_box[m] == 1 puts a molecule in the non-interacting reservoir
+   * (GOMC's second GCMC box), given the same edge lengths. */
+  int32_t ensemble;                        /* 0 NVT, 1 GCMC */
+  int32_t cbmc_first, cbmc_nth, cbmc_ang;  /* CBMC_First, CBMC_Nth, CBMC_Ang */
+  const int32_t *mol_box;                  /* [n_mols]; NULL: all in box 0 */
+  const double *kind_fugacity;             /* [n_kinds] Fugacity, bar */
+  double move_swap;                        /* SwapFreq */
 } gomc_system_desc;
 
 typedef struct gomc_box gomc_box;
@@ -189,17 +199,23 @@ int gomc_mp_accept(gomc_box *b, int accept, const double out[5]);
 int gomc_mp_settings(const gomc_box *b, double out[2]);
 
 /* ---- plain single-box MC driver (src/System.cpp:262-302) ------------- */
-/* Runs `nsteps` GOMC moves (Translate / Rotate / MultiParticle, MT19937
+/* Runs `nsteps` GOMC moves (Translate / Rotate / MultiParticle, and in
+ * GCMC MoleculeTransfer with CBMC growth, src/moves/MoleculeTransfer.h; MT19937
  * draws in the reference's order and Random123 draws for MultiParticle,
  * Metropolis acceptance, MoveSettings adaptation) with every
  * single-molecule energy evaluated by gomc_mol_trial.  Per-move records may
  * be NULL; a MultiParticle record has kind 2, mol = its move type and ovl =
- * its transform failure.  Returns the number of accepted moves (>= 0) or
+ * its transform failure; a MoleculeTransfer record has kind 10, the
+ * molecule, and energy changes of box 0.  Returns the number of accepted moves (>= 0) or
  * an error code. */
 int64_t gomc_run_moves(gomc_box *b, int64_t start_step, int64_t nsteps,
                        int32_t *kind, int32_t *mol, int32_t *acc,
                        int32_t *ovl, double *dlj, double *dreal,
                        double *drecip);
+/* GCMC: box of every molecule (0 simulation box, 1 reservoir) into `box`
+ * (n_mols); returns the number of molecules in box 0 or an error code.
+ * MoleculeLookup::NumInBox / IsMoleculeInBox (src/MoleculeLookup.h:98-130). */
+int gomc_mol_boxes(const gomc_box *b, int32_t *box);
 /* Running energies of the driver (same order as gomc_system_total). */
 int gomc_energies(const gomc_box *b, double out[7]);
 /* Current displacement / rotation scales per molecule kind. */

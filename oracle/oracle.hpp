@@ -275,6 +275,7 @@ struct MoveSettings {  // src/MoveSettings.cpp:19-258 (displace/rotate)
   // [move][molecule kind]
   std::vector<double> scale[2];
   std::vector<uint64_t> tries[2], accepted[2], tempTries[2], tempAccepted[2];
+  std::vector<uint64_t> swapTries, swapAccepted;  // MOL_TRANSFER [dest box][kind]
   double mp_t_max = 0.02, mp_r_max = 0.01 * M_PI;
   uint64_t mp_tries[2] = {0, 0}, mp_accepted[2] = {0, 0};
   uint64_t mp_int_tries[2] = {0, 0}, mp_int_acc[2] = {0, 0};
@@ -347,6 +348,41 @@ class System {
   uint64_t philoxSeed = 0;
   MoveSettings ms;
   double movePerc[3] = {0.5, 0.5, 0.0};  // displace, rotate, multiparticle
+  // StaticVals movePerc over the reference's move kinds (MoveConst.h:22-35):
+  // DISPLACE 0, ROTATE 1, MULTIPARTICLE 2, MOL_TRANSFER 10 (GCMC).
+  double moveW[13] = {0};
+
+  // ---- GCMC (ensemble 1): box 0 interacts, box 1 is the reservoir ----
+  int ensemble = 0;
+  std::vector<int> molBox;
+  // MoleculeLookup (MoleculeLookup.cpp:24-119,165-198): molecule indices
+  // grouped by [box][kind], section starts, swappable counts, and the
+  // kinds in order of first appearance.
+  std::vector<uint32_t> lookup, bkStart, swapCount;
+  std::vector<int> canSwapKind, canMoveKind;
+  std::vector<double> fugacity;  // K / A^3 (bar * 0.007242971, UnitConst.h:44)
+  int cbmcFirst = 16, cbmcNth = 8, cbmcAng = 100;
+  // Rigid three-site kinds grow from atom 0 (bonded to atoms 1 and 2):
+  // bond lengths and the 1-0-2 angle of the kind.
+  struct Hedron {
+    double bPrev = 0, bBonded = 0, theta = 0;
+  };
+  std::vector<Hedron> hedron;
+  bool InBox0(int m) const { return molBox.empty() || molBox[m] == 0; }
+  uint32_t NumKindInBox(int k, int b) const {
+    const int K = int(kinds.size());
+    return bkStart[b * K + k + 1] - bkStart[b * K + k];
+  }
+  uint32_t NumInBox(int b) const {
+    const int K = int(kinds.size());
+    return bkStart[(b + 1) * K] - bkStart[b * K];
+  }
+  int GetMolNum(uint32_t sub, int k, int b) const {
+    return int(lookup[bkStart[b * int(kinds.size()) + k] + sub]);
+  }
+  void ShiftMolBox(int m, int from, int to, int k);  // MoleculeLookup.cpp:165-198
+  void SetNewCOM(int m);                              // COM.h:63-88
+  int RunMoleculeTransfer(double subDraw, MoveRecord *rec);  // MoleculeTransfer.h
   uint64_t perAdjust = 1000;
   int threads = 1;
 

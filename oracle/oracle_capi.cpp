@@ -182,6 +182,7 @@ int64_t orc_run(void *h, int64_t start_step, int64_t nsteps, int mode, int32_t *
     s->AdjustMoves(step);
     orc::MoveRecord rec{-1, -1, 0, 0, 0, 0, 0};
     int r = mode == 0 ? s->RunStepMT(step, &rec) : s->RunStepPhilox(step, &rec);
+    if (r <= -2) return r;  // the reference stops the run (empty reservoir)
     if (r > 0) accepted++;
     if (kind) kind[i] = rec.kind;
     if (mol) mol[i] = rec.mol;
@@ -248,6 +249,12 @@ void orc_torque(void *h, double *tx, double *ty, double *tz) {
     ty[m] = t[m].y;
     tz[m] = t[m].z;
   }
+}
+
+int orc_mol_boxes(void *h, int32_t *box) {
+  System *s = S(h);
+  for (int m = 0; m < s->NumMols(); ++m) box[m] = s->molBox[m];
+  return int(s->NumInBox(0));
 }
 
 void orc_mp_settings(void *h, double out[2]) {

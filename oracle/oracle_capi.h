@@ -27,6 +27,14 @@ typedef struct orc_system_desc {
   uint64_t seed;
   double move_disp, move_rot, move_mp;
   int64_t adj_steps;
+  /* GCMC (ensemble 1): molecules with mol_box[m] == 0 are in the simulation
+   * box; mol_box[m] == 1 puts a molecule in the non-interacting reservoir
+   * (GOMC's second GCMC box), given the same edge lengths. */
+  int32_t ensemble;                        /* 0 NVT, 1 GCMC */
+  int32_t cbmc_first, cbmc_nth, cbmc_ang;  /* CBMC_First, CBMC_Nth, CBMC_Ang */
+  const int32_t *mol_box;                  /* [n_mols]; NULL: all in box 0 */
+  const double *kind_fugacity;             /* [n_kinds] Fugacity, bar */
+  double move_swap;                        /* SwapFreq */
 } orc_system_desc;
 
 void *orc_create(const orc_system_desc *d);
@@ -68,6 +76,8 @@ void orc_box_force(void *h, double *lj, double *real, double *afx,
 void orc_box_force_recip(void *h, double *afx, double *afy, double *afz,
                          double *mfx, double *mfy, double *mfz);
 void orc_torque(void *h, double *tx, double *ty, double *tz);
+/* Box of every molecule (0 box, 1 reservoir); returns molecules in box 0. */
+int orc_mol_boxes(void *h, int32_t *box);
 /* MultiParticle step sizes: {t_max, r_max}. */
 void orc_mp_settings(void *h, double out[2]);
 

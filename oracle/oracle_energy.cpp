@@ -277,8 +277,9 @@ void System::BoxInterBrute(double &lj, double &real) const {
   const int na = NumAtoms();
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 64) reduction(+ : tempREn, tempLJEn)
   for (int i = 0; i < na; i++) {
+    if (!InBox0(particleMol[i])) continue;
     for (int j = i + 1; j < na; j++) {
-      if (particleMol[i] == particleMol[j]) continue;
+      if (particleMol[i] == particleMol[j] || !InBox0(particleMol[j])) continue;
       double distSq;
       XYZ d;
       if (box.InRcut(distSq, d, Pos(i), Pos(j))) {
@@ -541,6 +542,7 @@ void Ewald::BoxReciprocalSetup() {
   std::fill(sumRnew.begin(), sumRnew.end(), 0.0);
   std::fill(sumInew.begin(), sumInew.end(), 0.0);
   for (int m = 0; m < sys.NumMols(); ++m) {
+    if (!sys.InBox0(m)) continue;
     const MolKind &kind = sys.kinds[sys.molKind[m]];
     const double lambdaCoef = 1.0;
     int start = sys.molStart[m];
@@ -570,6 +572,7 @@ void Ewald::BoxReciprocalSums() {
   for (int i = 0; i < imageSizeRef; i++) {
     double R = 0.0, I = 0.0;
     for (int m = 0; m < nm; ++m) {
+      if (!sys.InBox0(m)) continue;
       const MolKind &kind = sys.kinds[sys.molKind[m]];
       int start = sys.molStart[m];
       double sumReal = 0.0, sumImaginary = 0.0;
@@ -758,6 +761,7 @@ void Ewald::BoxForceReciprocal(const std::vector<double> &x, const std::vector<d
   // Molecules are independent; each atom's k-sum keeps the serial order.
 #pragma omp parallel for num_threads(sys.threads) schedule(dynamic, 4)
   for (int m = 0; m < sys.NumMols(); m++) {
+    if (!sys.InBox0(m)) continue;
     int start = sys.molStart[m], length = sys.molLen[m];
     const double lambdaCoef = 1.0;
     for (int p = start; p < start + length; p++) {

@@ -8,7 +8,7 @@
 namespace orc {
 
 namespace {
-constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2;
+constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2, kMolTransfer = 10;
 constexpr double kTinyAmount = 1.0e-7;      // MoveSettings.cpp:14
 constexpr double kTargetAccept = 0.5;       // MoveSettings.cpp:13
 constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of the batched streams
@@ -45,12 +45,16 @@ void System::Build(const SystemDesc &d) {
   particleCharge.clear();
   particleHasNoCharge.clear();
   kindCount.assign(kinds.size(), 0);
+  ensemble = d.ensemble;
+  molBox.assign(d.n_mols, 0);
+  if (d.mol_box)
+    for (int m = 0; m < d.n_mols; ++m) molBox[m] = d.mol_box[m] ? 1 : 0;
   int atom = 0;
   for (int m = 0; m < d.n_mols; ++m) {
     const MolKind &mk = kinds[molKind[m]];
     molStart.push_back(atom);
     molLen.push_back(mk.NumAtoms());
-    kindCount[molKind[m]]++;
+    if (molBox[m] == 0) kindCount[molKind[m]]++;
     for (int a = 0; a < mk.NumAtoms(); ++a) {
       particleType.push_back(mk.atomType[a]);
       particleMol.push_back(m);
@@ -75,19 +79,75 @@ void System::Build(const SystemDesc &d) {
         pairEnCorrections[j * K + i] = pairEnCorrections[i * K + j];
       }
   }
+  // MoleculeLookup::Init (MoleculeLookup.cpp:24-119): [box][kind] lists in
+  // molecule order; kinds in order of first appearance.
+  {
+    const int K = int(kinds.size());
+    std::vector<std::vector<uint32_t>> idx(2 * K);
+    canSwapKind.clear();
+    canMoveKind.clear();
+    swapCount.assign(2 * K, 0);
+    for (int m = 0; m < d.n_mols; ++m) {
+      int b = molBox[m], k = molKind[m];
+      idx[b * K + k].push_back(uint32_t(m));
+      if (std::find(canSwapKind.begin(), canSwapKind.end(), k) == canSwapKind.end())
+        canSwapKind.push_back(k);
+      if (std::find(canMoveKind.begin(), canMoveKind.end(), k) == canMoveKind.end())
+        canMoveKind.push_back(k);
+      swapCount[b * K + k]++;
+    }
+    lookup.clear();
+    bkStart.assign(2 * K + 1, 0);
+    for (int s = 0; s < 2 * K; ++s) {
+      bkStart[s] = uint32_t(lookup.size());
+      lookup.insert(lookup.end(), idx[s].begin(), idx[s].end());
+    }
+    bkStart[2 * K] = uint32_t(d.n_mols);
+  }
+  fugacity.assign(kinds.size(), 0.0);
+  if (d.kind_fugacity)
+    for (size_t k = 0; k < kinds.size(); ++k) fugacity[k] = d.kind_fugacity[k] * 0.007242971;
+  if (d.cbmc_first > 0) cbmcFirst = d.cbmc_first;
+  if (d.cbmc_nth > 0) cbmcNth = d.cbmc_nth;
+  if (d.cbmc_ang > 0) cbmcAng = d.cbmc_ang;
   // System.cpp:250-259: COM, then cell list, then Ewald, then totals.
   CalcCOM();
+  hedron.assign(kinds.size(), Hedron());
+  for (size_t k = 0; k < kinds.size(); ++k) {
+    if (kinds[k].NumAtoms() != 3) continue;
+    for (int m = 0; m < NumMols(); ++m) {
+      if (molKind[m] != int(k)) continue;
+      int st = molStart[m];
+      XYZ a = box.MinImage(Pos(st + 1) - Pos(st)), b = box.MinImage(Pos(st + 2) - Pos(st));
+      XYZ c = Cross(a, b);
+      hedron[k].bPrev = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+      hedron[k].bBonded = std::sqrt(b.x * b.x + b.y * b.y + b.z * b.z);
+      hedron[k].theta = std::atan2(std::sqrt(c.x * c.x + c.y * c.y + c.z * c.z),
+                                   a.x * b.x + a.y * b.y + a.z * b.z);
+      break;
+    }
+  }
   cells.Resize(box, box.rCut);
   cells.Clear(atom);
   for (int m = 0; m < NumMols(); ++m)
-    for (int p = molStart[m]; p < molStart[m] + molLen[m]; ++p) cells.AddAtom(p, Pos(p));
+    if (InBox0(m))
+      for (int p = molStart[m]; p < molStart[m] + molLen[m]; ++p) cells.AddAtom(p, Pos(p));
   ewald.Init();
   pot = SystemTotal();
-  // StaticVals.cpp:37-96: move percentages normalised to one.
-  double tot = d.move_disp + d.move_rot + d.move_mp;
+  // StaticVals.cpp:37-96: move percentages normalised to one, summed in
+  // move-kind order.
+  const double swapW = d.ensemble == 1 ? d.move_swap : 0.0;
+  double tot = d.move_disp + d.move_rot + d.move_mp + swapW;
   movePerc[0] = d.move_disp / tot;
   movePerc[1] = d.move_rot / tot;
   movePerc[2] = d.move_mp / tot;
+  for (double &w : moveW) w = 0.0;
+  moveW[0] = movePerc[0];
+  moveW[1] = movePerc[1];
+  moveW[2] = movePerc[2];
+  moveW[10] = swapW / tot;
+  ms.swapTries.assign(2 * K, 0);
+  ms.swapAccepted.assign(2 * K, 0);
   perAdjust = d.adj_steps > 0 ? uint64_t(d.adj_steps) : 1000;
   // MoveSettings.cpp:42-56: initial displacement L_min/4, rotation pi/4.
   double lmin = std::min(box.axis.x, std::min(box.axis.y, box.axis.z));
@@ -145,7 +205,8 @@ Energy System::SystemTotal() {
   e.recip = ff.ewald ? ewald.BoxReciprocal(false) : 0.0;
   double correction = 0.0;
   if (ff.ewald)
-    for (int m = 0; m < NumMols(); ++m) correction += ewald.MolCorrection(m);
+    for (int m = 0; m < NumMols(); ++m)
+      if (InBox0(m)) correction += ewald.MolCorrection(m);
   e.correction = correction;
   e.self = ff.ewald ? ewald.BoxSelf() : 0.0;
   e.Total();
@@ -280,11 +341,12 @@ static int EvaluateAndAccept(System &s, int mv, int m, const std::vector<XYZ> &n
 // NVT displace/rotate moves, MT19937 draws in the reference's order.
 int System::RunStepMT(uint64_t step, MoveRecord *rec) {
   (void)step;
-  const double w[9] = {movePerc[0], movePerc[1], movePerc[2], 0, 0, 0, 0, 0, 0};
   uint32_t kind = 0;
   double draw = 0;
-  prng.PickArbDist(kind, draw, w, 1.0, 9);
-  if (kind == kMultiParticle) return RunMultiParticle(step, rec);
+  // MOVE_KINDS_TOTAL: 9 kinds in NVT, 13 in GCMC (MoveConst.h:31-40).
+  prng.PickArbDist(kind, draw, moveW, 1.0, ensemble == 1 ? 13 : 9);
+  if (kind == kMultiParticle) return ensemble == 1 ? -1 : RunMultiParticle(step, rec);
+  if (kind == kMolTransfer) return RunMoleculeTransfer(draw, rec);
   if (kind != kDisplace && kind != kRotate) return -1;
   // PickMolAndBox with one box: boxDiv = movPerc, subDraw unchanged.
   double subPerc = movePerc[kind];
@@ -292,29 +354,24 @@ int System::RunStepMT(uint64_t step, MoveRecord *rec) {
   uint32_t b = uint32_t(draw / boxDiv);
   if (b >= 1) b--;
   draw -= b * boxDiv;
-  // PickMol: kinds in order of first appearance with molecules.
-  std::vector<int> canMove;
-  for (int m = 0; m < NumMols(); ++m)
-    if (std::find(canMove.begin(), canMove.end(), molKind[m]) == canMove.end())
-      canMove.push_back(molKind[m]);
-  uint32_t mkTot = uint32_t(canMove.size());
+  // PickMol (PRNG.h:267-299) in box 0 through the molecule lookup.
+  uint32_t mkTot = uint32_t(canMoveKind.size());
   double molDiv = boxDiv / mkTot;
   uint32_t k = uint32_t(draw / molDiv);
   if (k == mkTot) k = mkTot - 1;
-  int mk = canMove[k];
-  if (kindCount[mk] == 0) {
+  int mk = canMoveKind[k];
+  if (NumKindInBox(mk, 0) == 0) {
     // NO_MOL_OF_KIND_IN_BOX: Accept draws nothing and counts a failed try.
     ms.tries[kind][mk]++;
     ms.tempTries[kind][mk]++;
+    if (rec) {
+      rec->kind = int(kind);
+      rec->mol = -1;
+    }
     return 0;
   }
-  uint32_t mOff = prng.randIntExc(uint32_t(kindCount[mk]));
-  int m = -1, seen = 0;
-  for (int i = 0; i < NumMols(); ++i)
-    if (molKind[i] == mk && seen++ == int(mOff)) {
-      m = i;
-      break;
-    }
+  uint32_t mOff = prng.randIntExc(NumKindInBox(mk, 0));
+  int m = GetMolNum(mOff, mk, 0);
   int mv = int(kind);
   if (mv == kRotate && kinds[mk].NumAtoms() <= 1) mv = kDisplace;  // Rotate::Prep
   std::vector<XYZ> newPos;
@@ -338,7 +395,8 @@ int System::RunStepMT(uint64_t step, MoveRecord *rec) {
 void System::GridAll() {
   cells.Clear(NumAtoms());
   for (int m = 0; m < NumMols(); ++m)
-    for (int p = molStart[m]; p < molStart[m] + molLen[m]; ++p) cells.AddAtom(p, Pos(p));
+    if (InBox0(m))
+      for (int p = molStart[m]; p < molStart[m] + molLen[m]; ++p) cells.AddAtom(p, Pos(p));
 }
 
 namespace {

@@ -49,6 +49,7 @@ _SIG = {
     "orc_box_force_recip": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "orc_torque": (None, [_vp, _dp, _dp, _dp]),
     "orc_mp_settings": (None, [_vp, _dp]),
+    "orc_mol_boxes": (C.c_int, [_vp, _ip]),
     "orc_batch_run": (C.c_double, [C.c_int, C.POINTER(_vp), _lp, C.c_int]),
     "orc_mt_raw": (None, [C.c_uint32, C.c_int, _up]),
     "orc_mt_rand_int": (None, [C.c_uint32, C.c_int, C.c_uint32, _up]),
@@ -237,6 +238,11 @@ class Oracle:
         t = [np.zeros(self.nmols) for _ in range(3)]
         self.L.orc_torque(self.h, *(_p(v) for v in t))
         return np.stack(t, 1)
+
+    def mol_boxes(self):
+        b = np.zeros(self.nmols, np.int32)
+        n = self.L.orc_mol_boxes(self.h, _p(b))
+        return n, b
 
     def mp_settings(self):
         out = np.zeros(2)

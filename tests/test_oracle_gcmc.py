@@ -1,0 +1,57 @@
+"""The oracle's GCMC move (MoleculeTransfer with CBMC growth) against what
+statistical mechanics fixes, and its bookkeeping against recomputation.
+
+* Ideal gas: with no interactions the grand-canonical mean occupation of
+  each kind is <N_k> = beta f_k V for fugacity f_k.  A dropped volume or
+  beta factor in the acceptance coefficient (MoleculeTransfer.h:143-165), or
+  Rosenbluth weights of the new and old molecules normalised differently,
+  moves <N> far outside the tolerance.
+* Interacting mixture: after thousands of insertions, deletions,
+  translations and rotations the running energies equal a from-scratch
+  recomputation of the final state.
+"""
+import numpy as np
+
+from gomc_amd import systems
+from oracle.pyoracle import Oracle
+
+
+def test_ideal_gas_mean_occupation():
+    T, L = 300.0, 25.0
+    target = np.array([40.0, 25.0])
+    fug_bar = target / ((1.0 / T) * L**3) / systems.BAR_TO_K_PER_A3
+    s = systems.gcmc_mixture(40, 25, L, temperature=T, fugacity_bar=tuple(fug_bar),
+                             reservoir=120, seed=9, swap=0.6, disp=0.2, rot=0.2)
+    s.epsilon = [0.0, 0.0, 0.0]
+    s.electrostatic = False
+    s.ewald = False
+    s.rcut_low = 1e-3
+    o = Oracle(s, threads=4)
+    counts = []
+    mk = s.mol_kind
+    for _ in range(400):
+        o.run(500, mode=0, record=False)
+        n, b = o.mol_boxes()
+        counts.append([np.sum((b == 0) & (mk == 0)), np.sum((b == 0) & (mk == 1))])
+    c = np.array(counts[20:], dtype=float)
+    mean, var = c.mean(0), c.var(0)
+    # blocks of 500 moves are close to independent: standard error of the
+    # mean about sqrt(<N> / samples) (0.3 here); 4.5 of them is the bar.  A
+    # dropped beta, volume or fugacity factor moves <N> by orders of
+    # magnitude; an (N + 1) -> N slip moves it by one molecule, about three
+    # standard errors, so this check alone does not rule that slip out.
+    sem = np.sqrt(target / len(c))
+    assert np.all(np.abs(mean - target) < 4.5 * sem), (mean, target, sem)
+    # and the occupation is Poisson: variance equals the mean
+    assert np.all(np.abs(var / target - 1.0) < 0.3), (var, target)
+
+
+def test_running_energy_equals_recompute_after_transfers():
+    s = systems.gcmc_mixture(60, 60, 25.0, fugacity_bar=(200.0, 60.0), reservoir=120, seed=3)
+    o = Oracle(s, threads=4)
+    n, rec = o.run(4000, mode=0)
+    k = rec["kind"]
+    assert (k == 10).sum() > 1000
+    assert rec["acc"][k == 10].sum() > 100
+    e, tot = o.energies(), o.system_total()
+    assert np.allclose(e, tot, rtol=1e-10, atol=1e-6), (e, tot)

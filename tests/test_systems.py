@@ -51,4 +51,7 @@ def test_config4_sizes():
 def test_desc_layout():
     # field offsets used by the C side (include/gomc_hip.h)
     assert systems.SystemDesc.seed.offset == 176
-    assert ctypes.sizeof(systems.SystemDesc) == 216
+    assert systems.SystemDesc.ensemble.offset == 216
+    assert systems.SystemDesc.mol_box.offset == 232
+    assert systems.SystemDesc.move_swap.offset == 248
+    assert ctypes.sizeof(systems.SystemDesc) == 256
