@@ -61,6 +61,14 @@ SIGNATURES = {
     "gomc_batch_get_coords": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "gomc_batch_set_trace": (C.c_int, [_vp, C.c_int, C.c_int64]),
     "gomc_batch_get_trace": (C.c_int, [_vp, _ip, _dp, _dp, _dp, _lp]),
+    "gomc_server_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double,
+                                     C.POINTER(_vp)]),
+    "gomc_server_destroy": (C.c_int, [_vp]),
+    "gomc_server_last_error": (C.c_char_p, [_vp]),
+    "gomc_server_submit": (C.c_int, [_vp, C.POINTER(SystemDesc), C.c_int64, _lp]),
+    "gomc_server_poll": (C.c_int, [_vp, C.c_int64, _ip, _dp, _dp, _lp, _lp]),
+    "gomc_server_drain": (C.c_int, [_vp, C.c_double]),
+    "gomc_server_stats": (C.c_int, [_vp, _dp]),
     "gomc_batch_last_kernel_ms": (C.c_int, [_vp, _dp]),
     "gomc_place_molecules": (C.c_int, [C.c_int, _ip, _dp, C.c_int, _ip, C.c_double, C.c_uint64,
                                        C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]),

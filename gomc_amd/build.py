@@ -36,7 +36,7 @@ FLAGS = [
     "-Wno-unused-function",
     "-Wno-unused-variable",
 ]
-SOURCES = ["box.hip", "box_setup.cpp", "driver.cpp", "mp.hip", "gcmc.cpp", "batch.hip", "batch_setup.cpp",
+SOURCES = ["box.hip", "box_setup.cpp", "driver.cpp", "mp.hip", "gcmc.cpp", "batch.hip", "batch_setup.cpp", "server.cpp",
            "placement.cpp"]
 
 

@@ -511,13 +511,6 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
 
 }  // namespace
 
-int BatchMaxKpt() { return 0; }
-
-int BatchRoundKpt(int kpt) {
-  (void)kpt;
-  return 0;  // S(k) stays in HBM (L2-resident per box)
-}
-
 template <int PREC>
 static hipError_t LaunchOne(int nboxes, const BatchArgs &a, hipStream_t s) {
   const size_t dyn = sizeof(double) * 3 * size_t(a.ldsAtoms);
@@ -532,8 +525,7 @@ int BatchLdsAtoms(int maxAtoms) {
   return maxAtoms <= kBatchLdsAtomCap ? maxAtoms : 0;
 }
 
-hipError_t LaunchBatch(int kpt, int precision, int nboxes, const BatchArgs &a, hipStream_t s) {
-  (void)kpt;
+hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s) {
   return precision == 1 ? LaunchOne<1>(nboxes, a, s) : LaunchOne<0>(nboxes, a, s);
 }
 

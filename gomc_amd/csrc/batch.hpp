@@ -80,7 +80,6 @@ struct gomc_batch {
   int device = 0;
   int precision = 0;
   int nboxes = 0;
-  int kpt = 0;
   std::string err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -99,15 +98,30 @@ struct gomc_batch {
 };
 
 namespace gomc {
-// Launch the MC kernel instance for `kpt` k-vectors per thread.
-hipError_t LaunchBatch(int kpt, int precision, int nboxes, const BatchArgs &a, hipStream_t s);
-// Largest k-vectors-per-thread any compiled instance serves.
-int BatchMaxKpt();
-int BatchRoundKpt(int kpt);
+// Launch the MC kernel at the given precision (0 FP64, 1 FP32 pair terms).
+hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s);
 // LDS coordinate capacity for a batch whose largest box has maxAtoms atoms
 // (0: boxes read coordinates from HBM).
 int BatchLdsAtoms(int maxAtoms);
 void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double volume,
                   std::vector<double> &kx, std::vector<double> &ky, std::vector<double> &kz,
                   std::vector<double> &pref, std::vector<int32_t> *packed);
+
+// One box packed for the batched kernel: its parameters, initial state and
+// the contents of its flat-array extents.
+struct PackedBox {
+  BatchBox P;
+  BatchState S;
+  std::vector<double> x, y, z, cx, cy, cz, pref, sr, si;
+  std::vector<int32_t> kn;
+};
+// Builds the box once through the plain path (initial energies, structure
+// factors, centres) and packs it at the given offsets; errors go to bt->err.
+int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t molOff,
+            int64_t kOff, PackedBox &out);
+// Device buffers for `nboxes` boxes and the given flat-array lengths.
+int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t kvec);
+// Uploads a packed box: parameters and state at index `slot`, arrays at
+// the box's offsets.
+int UploadBox(gomc_batch *bt, int slot, const PackedBox &pb);
 }  // namespace gomc

@@ -49,6 +49,160 @@ int gomc_batch_destroy(gomc_batch *bt) {
   return GOMC_OK;
 }
 
+}  // extern "C"
+
+namespace gomc {
+
+int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t molOff,
+            int64_t kOff, PackedBox &out) {
+  if (d.n_kinds != 1) return BFail(bt, GOMC_ERR_ARG, "batched boxes hold one molecule kind");
+  if (d.ensemble != 0) return BFail(bt, GOMC_ERR_ARG, "batched boxes are NVT");
+  const int len = d.kind_natoms[0];
+  if (len <= 0 || len > kBatchMaxAtoms) return BFail(bt, GOMC_ERR_ARG, "molecule size out of range");
+  if (d.n_atom_types > kBatchMaxTypes) return BFail(bt, GOMC_ERR_ARG, "too many atom types");
+  gomc_box *b = nullptr;
+  int rc = gomc_box_create(&d, bt->device, &b);
+  if (rc) return BFail(bt, rc, "box setup failed");
+  struct Guard {
+    gomc_box *b;
+    ~Guard() { gomc_box_destroy(b); }
+  } guard{b};
+  BatchBox &P = out.P;
+  std::memset(&P, 0, sizeof(P));
+  for (int a = 0; a < 3; ++a) {
+    P.L[a] = b->geom.L[a];
+    P.half[a] = b->geom.half[a];
+    P.kc[a] = (1.0 / b->geom.L[a]) * (2.0 * M_PI);
+    P.nmax[a] = int(b->recipRcut * b->geom.L[a] / (2.0 * M_PI)) + 1;
+    if (P.nmax[a] > kBatchMaxN) return BFail(bt, GOMC_ERR_ARG, "box too large for the batched phase tables");
+  }
+  P.rCutSq = b->geom.rCutSq;
+  P.rCutLJSq = b->geom.rCutLJSq;
+  P.rCutCoulSq = b->geom.rCutCoulSq;
+  P.rCutLowSq = b->geom.rCutLowSq;
+  P.alpha = b->alpha;
+  P.beta = b->beta;
+  P.pDisp = b->movePerc[0] / (b->movePerc[0] + b->movePerc[1]);
+  for (int p = 0; p < len; ++p) {
+    P.charge[p] = b->kindCharge[0][p];
+    for (int q = 0; q < len; ++q) {
+      const int idx = b->kindType[0][p] + b->kindType[0][q] * b->ntypes;
+      const int pq = p * kBatchMaxAtoms + q;
+      P.sigSq[pq] = b->sigSq[idx];
+      P.epsCn[pq] = b->epsCn[idx];
+      P.nHalf[pq] = b->nHalf[idx];
+      if (b->epsCn[idx] != 0.0 && b->nHalf[idx] != 6.0)
+        return BFail(bt, GOMC_ERR_ARG, "batched boxes use the 12-6 Lennard-Jones form");
+      P.qq[pq] = b->kindCharge[0][p] * b->kindCharge[0][q] * kQQFact;
+    }
+  }
+  P.atomOff = atomOff;
+  P.molOff = molOff;
+  P.kOff = kOff;
+  P.perAdjust = int64_t(b->perAdjust);
+  P.seed = d.seed;
+  P.natoms = b->natoms;
+  P.nmols = b->nmols;
+  P.molLen = len;
+  // exact floor division by len and len^2 for item indices < 2^32 / len^2
+  P.magicLen = ((uint64_t(1) << 32) + uint64_t(len) - 1) / uint64_t(len);
+  P.magicLen2 = ((uint64_t(1) << 32) + uint64_t(len * len) - 1) / uint64_t(len * len);
+  if (uint64_t(b->nmols) * len * len >= (uint64_t(1) << 32) / uint64_t(len * len))
+    return BFail(bt, GOMC_ERR_ARG, "box too large for the batched item indexing");
+  P.ewald = b->ewald;
+  P.elect = b->elect;
+  out.x = b->hx;
+  out.y = b->hy;
+  out.z = b->hz;
+  out.cx = b->comx;
+  out.cy = b->comy;
+  out.cz = b->comz;
+  out.pref.clear();
+  out.kn.clear();
+  out.sr.clear();
+  out.si.clear();
+  if (b->ewald) {
+    std::vector<double> kx, ky, kz, pref;
+    std::vector<int32_t> packed;
+    KVectorsOrth(b->geom.L, b->alpha * b->alpha, b->recipRcut, b->volume, kx, ky, kz, pref, &packed);
+    std::vector<double> sr(b->K), si(b->K);
+    rc = gomc_recip_get(b, nullptr, nullptr, nullptr, nullptr, sr.data(), si.data());
+    if (rc || int(pref.size()) != b->K) return BFail(bt, GOMC_ERR_STATE, "k-vector setup mismatch");
+    P.K = b->K;
+    out.pref = pref;
+    out.kn = packed;
+    out.sr = sr;
+    out.si = si;
+  }
+  BatchState &S = out.S;
+  std::memset(&S, 0, sizeof(S));
+  S.energy[0] = b->pot.inter;
+  S.energy[1] = b->pot.tc;
+  S.energy[2] = b->pot.real;
+  S.energy[3] = b->pot.recip;
+  S.energy[4] = b->pot.self;
+  S.energy[5] = b->pot.corr;
+  S.energy[6] = b->pot.total;
+  S.scale[0] = b->moves.scale[0][0];
+  S.scale[1] = b->moves.scale[1][0];
+  return GOMC_OK;
+}
+
+int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t kvec) {
+  BHIP(bt, hipSetDevice(bt->device));
+  auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess; };
+  bool ok = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking) == hipSuccess;
+  ok &= hipEventCreate(&bt->ev0) == hipSuccess;
+  ok &= hipEventCreate(&bt->ev1) == hipSuccess;
+  ok &= alloc((void **)&bt->boxes, sizeof(BatchBox) * nboxes);
+  ok &= alloc((void **)&bt->state, sizeof(BatchState) * nboxes);
+  ok &= alloc((void **)&bt->moves, sizeof(int64_t) * nboxes);
+  ok &= alloc((void **)&bt->x, sizeof(double) * atoms);
+  ok &= alloc((void **)&bt->y, sizeof(double) * atoms);
+  ok &= alloc((void **)&bt->z, sizeof(double) * atoms);
+  ok &= alloc((void **)&bt->cx, sizeof(double) * mols);
+  ok &= alloc((void **)&bt->cy, sizeof(double) * mols);
+  ok &= alloc((void **)&bt->cz, sizeof(double) * mols);
+  ok &= alloc((void **)&bt->kn, sizeof(int32_t) * kvec);
+  ok &= alloc((void **)&bt->pref, sizeof(double) * kvec);
+  ok &= alloc((void **)&bt->sumR, sizeof(double) * kvec);
+  ok &= alloc((void **)&bt->sumI, sizeof(double) * kvec);
+  ok &= alloc((void **)&bt->dR, sizeof(double) * kvec);
+  ok &= alloc((void **)&bt->dI, sizeof(double) * kvec);
+  if (!ok) return BFail(bt, GOMC_ERR_HIP, "batch allocation failed");
+  bt->totalAtoms = atoms;
+  bt->totalMols = mols;
+  bt->totalK = kvec;
+  return GOMC_OK;
+}
+
+int UploadBox(gomc_batch *bt, int slot, const PackedBox &pb) {
+  const BatchBox &P = pb.P;
+  hipStream_t s = bt->stream;
+  auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
+    return bytes ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s) : hipSuccess;
+  };
+  BHIP(bt, up(bt->boxes + slot, &pb.P, sizeof(BatchBox)));
+  BHIP(bt, up(bt->state + slot, &pb.S, sizeof(BatchState)));
+  BHIP(bt, up(bt->x + P.atomOff, pb.x.data(), sizeof(double) * pb.x.size()));
+  BHIP(bt, up(bt->y + P.atomOff, pb.y.data(), sizeof(double) * pb.y.size()));
+  BHIP(bt, up(bt->z + P.atomOff, pb.z.data(), sizeof(double) * pb.z.size()));
+  BHIP(bt, up(bt->cx + P.molOff, pb.cx.data(), sizeof(double) * pb.cx.size()));
+  BHIP(bt, up(bt->cy + P.molOff, pb.cy.data(), sizeof(double) * pb.cy.size()));
+  BHIP(bt, up(bt->cz + P.molOff, pb.cz.data(), sizeof(double) * pb.cz.size()));
+  BHIP(bt, up(bt->kn + P.kOff, pb.kn.data(), sizeof(int32_t) * pb.kn.size()));
+  BHIP(bt, up(bt->pref + P.kOff, pb.pref.data(), sizeof(double) * pb.pref.size()));
+  BHIP(bt, up(bt->sumR + P.kOff, pb.sr.data(), sizeof(double) * pb.sr.size()));
+  BHIP(bt, up(bt->sumI + P.kOff, pb.si.data(), sizeof(double) * pb.si.size()));
+  BHIP(bt, hipStreamSynchronize(s));
+  bt->hboxes[slot] = pb.P;
+  return GOMC_OK;
+}
+
+}  // namespace gomc
+
+extern "C" {
+
 int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int precision,
                       gomc_batch **out) {
   if (!descs || nboxes <= 0 || !out || precision < 0 || precision > 1) return GOMC_ERR_ARG;
@@ -57,165 +211,32 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
   bt->device = device;
   bt->precision = precision;
   bt->nboxes = nboxes;
-  auto fail = [&](int code, const std::string &msg) {
-    std::string m = msg;
+  auto fail = [&](int code) {
     gomc_batch_destroy(bt);
-    (void)m;
     return code;
   };
-  std::vector<double> hx, hy, hz, hcx, hcy, hcz, hpref, hsr, hsi;
-  std::vector<int32_t> hkn;
-  std::vector<BatchState> hstate(nboxes);
-  bt->hboxes.resize(nboxes);
-  int kmax = 0;
-  for (int i = 0; i < nboxes; ++i) {
-    const gomc_system_desc &d = descs[i];
-    if (d.n_kinds != 1) return fail(GOMC_ERR_ARG, "batched boxes hold one molecule kind");
-    const int len = d.kind_natoms[0];
-    if (len <= 0 || len > kBatchMaxAtoms) return fail(GOMC_ERR_ARG, "molecule size out of range");
-    if (d.n_atom_types > kBatchMaxTypes) return fail(GOMC_ERR_ARG, "too many atom types");
-    gomc_box *b = nullptr;
-    int rc = gomc_box_create(&d, device, &b);
-    if (rc) return fail(rc, "box setup failed");
-    BatchBox &P = bt->hboxes[i];
-    std::memset(&P, 0, sizeof(P));
-    for (int a = 0; a < 3; ++a) {
-      P.L[a] = b->geom.L[a];
-      P.half[a] = b->geom.half[a];
-      P.kc[a] = (1.0 / b->geom.L[a]) * (2.0 * M_PI);
-      P.nmax[a] = int(b->recipRcut * b->geom.L[a] / (2.0 * M_PI)) + 1;
-      if (P.nmax[a] > kBatchMaxN) {
-        gomc_box_destroy(b);
-        return fail(GOMC_ERR_ARG, "box too large for the batched phase tables");
-      }
-    }
-    P.rCutSq = b->geom.rCutSq;
-    P.rCutLJSq = b->geom.rCutLJSq;
-    P.rCutCoulSq = b->geom.rCutCoulSq;
-    P.rCutLowSq = b->geom.rCutLowSq;
-    P.alpha = b->alpha;
-    P.beta = b->beta;
-    P.pDisp = b->movePerc[0] / (b->movePerc[0] + b->movePerc[1]);
-    for (int p = 0; p < len; ++p) {
-      P.charge[p] = b->kindCharge[0][p];
-      for (int q = 0; q < len; ++q) {
-        const int idx = b->kindType[0][p] + b->kindType[0][q] * b->ntypes;
-        const int pq = p * kBatchMaxAtoms + q;
-        P.sigSq[pq] = b->sigSq[idx];
-        P.epsCn[pq] = b->epsCn[idx];
-        P.nHalf[pq] = b->nHalf[idx];
-        if (b->epsCn[idx] != 0.0 && b->nHalf[idx] != 6.0) {
-          gomc_box_destroy(b);
-          return fail(GOMC_ERR_ARG, "batched boxes use the 12-6 Lennard-Jones form");
-        }
-        P.qq[pq] = b->kindCharge[0][p] * b->kindCharge[0][q] * kQQFact;
-      }
-    }
-    P.atomOff = int64_t(hx.size());
-    P.molOff = int64_t(hcx.size());
-    P.kOff = int64_t(hpref.size());
-    P.perAdjust = int64_t(b->perAdjust);
-    P.seed = d.seed;
-    P.natoms = b->natoms;
-    P.nmols = b->nmols;
-    P.molLen = len;
-    // exact floor division by len and len^2 for item indices < 2^32 / len^2
-    P.magicLen = ((uint64_t(1) << 32) + uint64_t(len) - 1) / uint64_t(len);
-    P.magicLen2 = ((uint64_t(1) << 32) + uint64_t(len * len) - 1) / uint64_t(len * len);
-    if (uint64_t(b->nmols) * len * len >= (uint64_t(1) << 32) / uint64_t(len * len)) {
-      gomc_box_destroy(b);
-      return fail(GOMC_ERR_ARG, "box too large for the batched item indexing");
-    }
-    P.ewald = b->ewald;
-    P.elect = b->elect;
-    hx.insert(hx.end(), b->hx.begin(), b->hx.end());
-    hy.insert(hy.end(), b->hy.begin(), b->hy.end());
-    hz.insert(hz.end(), b->hz.begin(), b->hz.end());
-    hcx.insert(hcx.end(), b->comx.begin(), b->comx.end());
-    hcy.insert(hcy.end(), b->comy.begin(), b->comy.end());
-    hcz.insert(hcz.end(), b->comz.begin(), b->comz.end());
-    if (b->ewald) {
-      std::vector<double> kx, ky, kz, pref;
-      std::vector<int32_t> packed;
-      KVectorsOrth(b->geom.L, b->alpha * b->alpha, b->recipRcut, b->volume, kx, ky, kz, pref, &packed);
-      std::vector<double> sr(b->K), si(b->K);
-      rc = gomc_recip_get(b, nullptr, nullptr, nullptr, nullptr, sr.data(), si.data());
-      if (rc || int(pref.size()) != b->K) {
-        gomc_box_destroy(b);
-        return fail(GOMC_ERR_STATE, "k-vector setup mismatch");
-      }
-      P.K = b->K;
-      kmax = std::max(kmax, b->K);
-      hpref.insert(hpref.end(), pref.begin(), pref.end());
-      hkn.insert(hkn.end(), packed.begin(), packed.end());
-      hsr.insert(hsr.end(), sr.begin(), sr.end());
-      hsi.insert(hsi.end(), si.begin(), si.end());
-    }
-    BatchState &S = hstate[i];
-    std::memset(&S, 0, sizeof(S));
-    S.energy[0] = b->pot.inter;
-    S.energy[1] = b->pot.tc;
-    S.energy[2] = b->pot.real;
-    S.energy[3] = b->pot.recip;
-    S.energy[4] = b->pot.self;
-    S.energy[5] = b->pot.corr;
-    S.energy[6] = b->pot.total;
-    S.scale[0] = b->moves.scale[0][0];
-    S.scale[1] = b->moves.scale[1][0];
-    gomc_box_destroy(b);
-  }
+  if (hipSetDevice(device) != hipSuccess) return fail(GOMC_ERR_NODEV);
+  std::vector<PackedBox> packed(nboxes);
+  int64_t atoms = 0, mols = 0, kvec = 0;
   int maxAtoms = 0;
-  for (const BatchBox &P : bt->hboxes) maxAtoms = std::max(maxAtoms, P.natoms);
+  for (int i = 0; i < nboxes; ++i) {
+    int rc = PackBox(bt, descs[i], atoms, mols, kvec, packed[i]);
+    if (rc) return fail(rc);
+    atoms += packed[i].P.natoms;
+    mols += packed[i].P.nmols;
+    kvec += packed[i].P.K;
+    maxAtoms = std::max(maxAtoms, packed[i].P.natoms);
+  }
   // Boxes up to the cap keep coordinates in LDS; a batch whose largest box
   // exceeds it sizes LDS for the boxes that fit.
   bt->ldsAtoms = BatchLdsAtoms(std::min(maxAtoms, kBatchLdsAtomCap));
-  bt->totalAtoms = int64_t(hx.size());
-  bt->totalMols = int64_t(hcx.size());
-  bt->totalK = int64_t(hpref.size());
-  const int kptNeed = (kmax + kBatchThreads - 1) / kBatchThreads;
-  bt->kpt = BatchRoundKpt(std::max(kptNeed, 1));
-  if (const char *f = std::getenv("GOMC_BATCH_KPT")) {
-    // 0 keeps S(k) in HBM; otherwise the register-resident instance, if it fits.
-    const int want = std::atoi(f);
-    if (want == 0 || (BatchRoundKpt(want) == want && want >= kptNeed)) bt->kpt = want;
+  int rc = AllocBatch(bt, nboxes, atoms, mols, kvec);
+  if (rc) return fail(rc);
+  bt->hboxes.resize(nboxes);
+  for (int i = 0; i < nboxes; ++i) {
+    rc = UploadBox(bt, i, packed[i]);
+    if (rc) return fail(rc);
   }
-  if (hipSetDevice(device) != hipSuccess) return fail(GOMC_ERR_NODEV, "hipSetDevice");
-  auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess; };
-  bool ok = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking) == hipSuccess;
-  ok &= hipEventCreate(&bt->ev0) == hipSuccess;
-  ok &= hipEventCreate(&bt->ev1) == hipSuccess;
-  ok &= alloc((void **)&bt->boxes, sizeof(BatchBox) * nboxes);
-  ok &= alloc((void **)&bt->state, sizeof(BatchState) * nboxes);
-  ok &= alloc((void **)&bt->moves, sizeof(int64_t) * nboxes);
-  ok &= alloc((void **)&bt->x, sizeof(double) * hx.size());
-  ok &= alloc((void **)&bt->y, sizeof(double) * hx.size());
-  ok &= alloc((void **)&bt->z, sizeof(double) * hx.size());
-  ok &= alloc((void **)&bt->cx, sizeof(double) * hcx.size());
-  ok &= alloc((void **)&bt->cy, sizeof(double) * hcx.size());
-  ok &= alloc((void **)&bt->cz, sizeof(double) * hcx.size());
-  ok &= alloc((void **)&bt->kn, sizeof(int32_t) * hkn.size());
-  ok &= alloc((void **)&bt->pref, sizeof(double) * hpref.size());
-  ok &= alloc((void **)&bt->sumR, sizeof(double) * hsr.size());
-  ok &= alloc((void **)&bt->sumI, sizeof(double) * hsi.size());
-  ok &= alloc((void **)&bt->dR, sizeof(double) * hsi.size());
-  ok &= alloc((void **)&bt->dI, sizeof(double) * hsi.size());
-  if (!ok) return fail(GOMC_ERR_HIP, "batch allocation failed");
-  auto up = [&](void *dst, const void *src, size_t bytes) {
-    return bytes == 0 || hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
-  };
-  ok &= up(bt->boxes, bt->hboxes.data(), sizeof(BatchBox) * nboxes);
-  ok &= up(bt->state, hstate.data(), sizeof(BatchState) * nboxes);
-  ok &= up(bt->x, hx.data(), sizeof(double) * hx.size());
-  ok &= up(bt->y, hy.data(), sizeof(double) * hy.size());
-  ok &= up(bt->z, hz.data(), sizeof(double) * hz.size());
-  ok &= up(bt->cx, hcx.data(), sizeof(double) * hcx.size());
-  ok &= up(bt->cy, hcy.data(), sizeof(double) * hcy.size());
-  ok &= up(bt->cz, hcz.data(), sizeof(double) * hcz.size());
-  ok &= up(bt->kn, hkn.data(), sizeof(int32_t) * hkn.size());
-  ok &= up(bt->pref, hpref.data(), sizeof(double) * hpref.size());
-  ok &= up(bt->sumR, hsr.data(), sizeof(double) * hsr.size());
-  ok &= up(bt->sumI, hsi.data(), sizeof(double) * hsi.size());
-  if (!ok) return fail(GOMC_ERR_HIP, "batch upload failed");
   *out = bt;
   return GOMC_OK;
 }
@@ -249,7 +270,7 @@ int gomc_batch_run(gomc_batch *bt, const int64_t *moves) {
   a.ldsAtoms = bt->ldsAtoms;
   a.trace = bt->trace;
   BHIP(bt, hipEventRecord(bt->ev0, bt->stream));
-  BHIP(bt, LaunchBatch(bt->kpt, bt->precision, bt->nboxes, a, bt->stream));
+  BHIP(bt, LaunchBatch(bt->precision, bt->nboxes, a, bt->stream));
   BHIP(bt, hipEventRecord(bt->ev1, bt->stream));
   BHIP(bt, hipStreamSynchronize(bt->stream));
   float ms = 0.0f;

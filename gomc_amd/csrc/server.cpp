@@ -1,0 +1,403 @@
+// Serving engine: a request loop that admits state-point jobs into one
+// resident batch (the persistent MC kernel) as slots and device memory
+// free up, advances every resident job in each launch, and retires jobs as
+// they finish -- continuous batching.  A job's coordinates, centres, k
+// vectors and structure factors stay in HBM from admission to completion.
+//
+// Memory: the flat atom / molecule / k-vector arrays of the batch are
+// arenas carved out of the byte budget; a job is admitted (in arrival
+// order) when a slot is free and first-fit extents of all three fit.
+// Time: each launch gives every resident job a number of moves that costs
+// about `tick_ms` at the measured per-move cost of its size, so one launch
+// keeps every slot busy for the same time and small jobs are not held
+// behind large ones.
+//
+// A job's moves draw from its own Philox stream indexed by its move
+// number, so its result does not depend on which other jobs share the
+// launches or how its moves are split between launches.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+#include "batch.hpp"
+#include "box.hpp"
+
+using namespace gomc;
+
+namespace {
+
+// First-fit extents over [0, cap) with coalescing on release.
+class Arena {
+ public:
+  explicit Arena(int64_t cap = 0) {
+    if (cap > 0) free_[0] = cap;
+  }
+  int64_t Alloc(int64_t n) {
+    if (n <= 0) return 0;
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= n) {
+        const int64_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > n) free_[off + n] = len - n;
+        return off;
+      }
+    return -1;
+  }
+  void Release(int64_t off, int64_t n) {
+    if (n <= 0) return;
+    auto it = free_.emplace(off, n).first;
+    auto next = std::next(it);
+    if (next != free_.end() && it->first + it->second == next->first) {
+      it->second += next->second;
+      free_.erase(next);
+    }
+    if (it != free_.begin()) {
+      auto prev = std::prev(it);
+      if (prev->first + prev->second == it->first) {
+        prev->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<int64_t, int64_t> free_;
+};
+
+constexpr int64_t kAtomBytes = 3 * sizeof(double);                          // x, y, z
+constexpr int64_t kMolBytes = 3 * sizeof(double);                           // centres
+constexpr int64_t kKBytes = sizeof(int32_t) + 5 * sizeof(double);           // kn, pref, S, dS
+
+struct Job {
+  int64_t id = 0, totalMoves = 0, done = 0;
+  int status = 0;  // 0 queued, 1 running, 2 done, 3 failed
+  int slot = -1;
+  int64_t atomOff = -1, molOff = -1, kOff = -1;
+  double tSubmit = 0, tAdmit = 0, tDone = 0;
+  double energy[7] = {0};
+  int64_t accepted = 0;
+  double costUs = 1.0;  // per-move cost estimate
+  PackedBox pb;
+};
+
+double Now(const std::chrono::steady_clock::time_point &t0) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct gomc_server {
+  int device = 0, precision = 0, nslots = 0;
+  double tickMs = 20.0;
+  int64_t budget = 0, atomCap = 0, molCap = 0, kCap = 0, bytesUsed = 0, bytesPeak = 0;
+  gomc_batch *bt = nullptr;
+  Arena atoms, mols, kvec;
+  std::mutex mu;
+  std::condition_variable cv, doneCv;
+  std::deque<Job *> queue;
+  std::map<int64_t, std::unique_ptr<Job>> jobs;
+  std::vector<Job *> slotJob;
+  int64_t nextId = 1, movesDone = 0, ticks = 0, outstanding = 0;
+  double kernelSec = 0.0;
+  bool stop = false;
+  std::string err;
+  std::thread loop;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+};
+
+namespace {
+
+int64_t JobBytes(const Job &j) {
+  return j.pb.P.natoms * kAtomBytes + j.pb.P.nmols * kMolBytes + int64_t(j.pb.P.K) * kKBytes;
+}
+
+// Admits queued jobs in arrival order while a slot and memory are free.
+// Called with the lock held; uploads run with it released.
+void Admit(gomc_server *sv, std::unique_lock<std::mutex> &lk) {
+  while (!sv->queue.empty()) {
+    int slot = -1;
+    for (int s = 0; s < sv->nslots; ++s)
+      if (!sv->slotJob[s]) {
+        slot = s;
+        break;
+      }
+    if (slot < 0) return;
+    Job *j = sv->queue.front();
+    const int64_t a = sv->atoms.Alloc(j->pb.P.natoms);
+    const int64_t m = a < 0 ? -1 : sv->mols.Alloc(j->pb.P.nmols);
+    const int64_t k = m < 0 ? -1 : sv->kvec.Alloc(j->pb.P.K);
+    if (k < 0) {  // does not fit yet: give back what was taken, wait for a retire
+      if (a >= 0) sv->atoms.Release(a, j->pb.P.natoms);
+      if (m >= 0) sv->mols.Release(m, j->pb.P.nmols);
+      return;
+    }
+    sv->queue.pop_front();
+    j->atomOff = a;
+    j->molOff = m;
+    j->kOff = k;
+    j->slot = slot;
+    j->pb.P.atomOff = a;
+    j->pb.P.molOff = m;
+    j->pb.P.kOff = k;
+    sv->slotJob[slot] = j;
+    sv->bytesUsed += JobBytes(*j);
+    sv->bytesPeak = std::max(sv->bytesPeak, sv->bytesUsed);
+    lk.unlock();
+    int rc = UploadBox(sv->bt, slot, j->pb);
+    lk.lock();
+    if (rc) {
+      j->status = 3;
+      sv->err = sv->bt->err;
+      sv->slotJob[slot] = nullptr;
+      --sv->outstanding;
+      sv->doneCv.notify_all();
+      continue;
+    }
+    j->status = 1;
+    j->tAdmit = Now(sv->t0);
+    // the host copies are on the device now
+    j->pb.x.clear();
+    j->pb.y.clear();
+    j->pb.z.clear();
+    j->pb.cx.clear();
+    j->pb.cy.clear();
+    j->pb.cz.clear();
+    j->pb.pref.clear();
+    j->pb.kn.clear();
+    j->pb.sr.clear();
+    j->pb.si.clear();
+  }
+}
+
+void Loop(gomc_server *sv) {
+  (void)hipSetDevice(sv->device);
+  std::vector<int64_t> mv(sv->nslots, 0);
+  std::vector<BatchState> st(sv->nslots);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (true) {
+    Admit(sv, lk);
+    int running = 0;
+    for (Job *j : sv->slotJob) running += j != nullptr;
+    if (running == 0) {
+      if (sv->stop) break;
+      sv->cv.wait_for(lk, std::chrono::milliseconds(2));
+      continue;
+    }
+    for (int s = 0; s < sv->nslots; ++s) {
+      Job *j = sv->slotJob[s];
+      if (!j) {
+        mv[s] = 0;
+        continue;
+      }
+      const int64_t quota = std::max<int64_t>(1, int64_t(sv->tickMs * 1000.0 / j->costUs));
+      mv[s] = std::min(quota, j->totalMoves - j->done);
+    }
+    lk.unlock();
+    int rc = gomc_batch_run(sv->bt, mv.data());
+    double ms = sv->bt->lastKernelMs;
+    hipError_t e = rc ? hipSuccess
+                      : hipMemcpy(st.data(), sv->bt->state, sizeof(BatchState) * sv->nslots,
+                                  hipMemcpyDeviceToHost);
+    lk.lock();
+    if (rc || e != hipSuccess) {
+      sv->err = rc ? sv->bt->err : std::string("state copy: ") + hipGetErrorString(e);
+      for (int s = 0; s < sv->nslots; ++s)
+        if (Job *j = sv->slotJob[s]) {
+          j->status = 3;
+          sv->atoms.Release(j->atomOff, j->pb.P.natoms);
+          sv->mols.Release(j->molOff, j->pb.P.nmols);
+          sv->kvec.Release(j->kOff, j->pb.P.K);
+          sv->bytesUsed -= JobBytes(*j);
+          sv->slotJob[s] = nullptr;
+          --sv->outstanding;
+        }
+      sv->doneCv.notify_all();
+      continue;
+    }
+    sv->kernelSec += ms * 1e-3;
+    ++sv->ticks;
+    const double now = Now(sv->t0);
+    for (int s = 0; s < sv->nslots; ++s) {
+      Job *j = sv->slotJob[s];
+      if (!j) continue;
+      j->done += mv[s];
+      sv->movesDone += mv[s];
+      if (j->done < j->totalMoves) continue;
+      for (int i = 0; i < 7; ++i) j->energy[i] = st[s].energy[i];
+      j->accepted = st[s].accepted[0] + st[s].accepted[1];
+      j->status = 2;
+      j->tDone = now;
+      sv->atoms.Release(j->atomOff, j->pb.P.natoms);
+      sv->mols.Release(j->molOff, j->pb.P.nmols);
+      sv->kvec.Release(j->kOff, j->pb.P.K);
+      sv->bytesUsed -= JobBytes(*j);
+      sv->slotJob[s] = nullptr;
+      // an empty slot makes no moves and reads nothing
+      BatchBox empty;
+      std::memset(&empty, 0, sizeof(empty));
+      sv->bt->hboxes[s] = empty;
+      (void)hipMemcpy(sv->bt->boxes + s, &empty, sizeof(empty), hipMemcpyHostToDevice);
+      --sv->outstanding;
+    }
+    sv->doneCv.notify_all();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gomc_server_last_error(const gomc_server *sv) {
+  return sv ? sv->err.c_str() : "null server";
+}
+
+int gomc_server_create(int device, int precision, int slots, int64_t budget_bytes, double tick_ms,
+                       gomc_server **out) {
+  if (!out || slots <= 0 || budget_bytes <= 0 || !(tick_ms > 0) || precision < 0 || precision > 1)
+    return GOMC_ERR_ARG;
+  *out = nullptr;
+  auto sv = std::make_unique<gomc_server>();
+  sv->device = device;
+  sv->precision = precision;
+  sv->nslots = slots;
+  sv->tickMs = tick_ms;
+  sv->budget = budget_bytes;
+  // split the budget in the proportions of a water box (3 atoms and about
+  // 3.1 k-vectors per molecule)
+  sv->atomCap = int64_t(0.31 * budget_bytes) / kAtomBytes;
+  sv->molCap = int64_t(0.104 * budget_bytes) / kMolBytes;
+  sv->kCap = int64_t(0.586 * budget_bytes) / kKBytes;
+  sv->atoms = Arena(sv->atomCap);
+  sv->mols = Arena(sv->molCap);
+  sv->kvec = Arena(sv->kCap);
+  if (hipSetDevice(device) != hipSuccess) return GOMC_ERR_NODEV;
+  gomc_batch *bt = new gomc_batch();
+  bt->device = device;
+  bt->precision = precision;
+  bt->nboxes = slots;
+  bt->hboxes.assign(slots, BatchBox());
+  std::memset(bt->hboxes.data(), 0, sizeof(BatchBox) * slots);
+  int rc = AllocBatch(bt, slots, sv->atomCap, sv->molCap, sv->kCap);
+  if (!rc) {
+    std::vector<BatchState> zs(slots);
+    std::memset(zs.data(), 0, sizeof(BatchState) * slots);
+    if (hipMemcpy(bt->boxes, bt->hboxes.data(), sizeof(BatchBox) * slots, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(bt->state, zs.data(), sizeof(BatchState) * slots, hipMemcpyHostToDevice) != hipSuccess)
+      rc = GOMC_ERR_HIP;
+  }
+  if (rc) {
+    gomc_batch_destroy(bt);
+    return rc;
+  }
+  bt->ldsAtoms = BatchLdsAtoms(kBatchLdsAtomCap);
+  sv->bt = bt;
+  sv->slotJob.assign(slots, nullptr);
+  gomc_server *raw = sv.release();
+  raw->loop = std::thread(Loop, raw);
+  *out = raw;
+  return GOMC_OK;
+}
+
+int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d, int64_t moves, int64_t *job_id) {
+  if (!sv || !d || moves < 0 || !job_id) return GOMC_ERR_ARG;
+  auto j = std::make_unique<Job>();
+  j->totalMoves = moves;
+  // build and pack the job (initial energies, structure factors) now, on
+  // the caller's thread, while the loop keeps the device busy
+  gomc_batch scratch;  // holds the packing error, not shared with the loop
+  scratch.device = sv->device;
+  int rc = PackBox(&scratch, *d, 0, 0, 0, j->pb);
+  if (rc) {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->err = scratch.err;
+    return rc;
+  }
+  if (j->pb.P.natoms > sv->atomCap || j->pb.P.nmols > sv->molCap || j->pb.P.K > sv->kCap) {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->err = "job larger than the memory budget";
+    return GOMC_ERR_ARG;
+  }
+  // per-move cost of the batched kernel at this size, FP64, microseconds
+  j->costUs = 16.0 + 0.0125 * double(j->pb.P.nmols);
+  std::lock_guard<std::mutex> g(sv->mu);
+  j->id = sv->nextId++;
+  j->tSubmit = Now(sv->t0);
+  *job_id = j->id;
+  Job *raw = j.get();
+  sv->jobs[j->id] = std::move(j);
+  if (raw->totalMoves == 0) {
+    raw->status = 2;
+    raw->tAdmit = raw->tDone = raw->tSubmit;
+    for (int i = 0; i < 7; ++i) raw->energy[i] = raw->pb.S.energy[i];
+    return GOMC_OK;
+  }
+  ++sv->outstanding;
+  sv->queue.push_back(raw);
+  sv->cv.notify_all();
+  return GOMC_OK;
+}
+
+int gomc_server_poll(gomc_server *sv, int64_t job_id, int32_t *status, double times[3],
+                     double energies[7], int64_t *accepted, int64_t *moves_done) {
+  if (!sv) return GOMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->jobs.find(job_id);
+  if (it == sv->jobs.end()) return GOMC_ERR_ARG;
+  const Job &j = *it->second;
+  if (status) *status = j.status;
+  if (times) {
+    times[0] = j.tSubmit;
+    times[1] = j.tAdmit;
+    times[2] = j.tDone;
+  }
+  if (energies)
+    for (int i = 0; i < 7; ++i) energies[i] = j.energy[i];
+  if (accepted) *accepted = j.accepted;
+  if (moves_done) *moves_done = j.done;
+  return GOMC_OK;
+}
+
+int gomc_server_drain(gomc_server *sv, double timeout_s) {
+  if (!sv) return GOMC_ERR_ARG;
+  std::unique_lock<std::mutex> lk(sv->mu);
+  const bool ok = sv->doneCv.wait_for(lk, std::chrono::duration<double>(timeout_s),
+                                      [&] { return sv->outstanding == 0; });
+  return ok ? GOMC_OK : GOMC_ERR_STATE;
+}
+
+int gomc_server_stats(gomc_server *sv, double out[8]) {
+  if (!sv || !out) return GOMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(sv->mu);
+  out[0] = double(sv->movesDone);
+  out[1] = sv->kernelSec;
+  out[2] = double(sv->ticks);
+  out[3] = double(sv->bytesPeak);
+  out[4] = double(sv->budget);
+  out[5] = Now(sv->t0);
+  out[6] = double(sv->queue.size());
+  out[7] = double(sv->outstanding);
+  return GOMC_OK;
+}
+
+int gomc_server_destroy(gomc_server *sv) {
+  if (!sv) return GOMC_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    sv->stop = true;
+    // queued jobs are dropped; running ones finish their current launch
+    sv->queue.clear();
+    for (int s = 0; s < sv->nslots; ++s) sv->slotJob[s] = nullptr;
+    sv->cv.notify_all();
+  }
+  if (sv->loop.joinable()) sv->loop.join();
+  if (sv->bt) gomc_batch_destroy(sv->bt);
+  delete sv;
+  return GOMC_OK;
+}
+
+}  // extern "C"

@@ -249,6 +249,36 @@ int gomc_batch_get_trace(const gomc_batch *bt, int32_t *acc, double *dlj,
  * launch stream). */
 int gomc_batch_last_kernel_ms(const gomc_batch *bt, double *ms);
 
+/* ---- serving: continuous batching of state-point jobs ---------------- */
+/* No counterpart in the reference (one simulation per process there).  A
+ * request loop on its own thread keeps one resident batch of `slots`
+ * workgroup slots; submitted jobs (NVT boxes as for gomc_batch_create) are
+ * admitted in arrival order when a slot is free and their coordinates,
+ * centres, k-vectors and structure factors fit the `budget_bytes` device
+ * arenas; every launch advances each resident job by the moves that cost
+ * about `tick_ms`; finished jobs release their slot and memory.  A job's
+ * result does not depend on the other jobs in the batch. */
+typedef struct gomc_server gomc_server;
+int gomc_server_create(int device, int precision, int slots, int64_t budget_bytes,
+                       double tick_ms, gomc_server **out);
+int gomc_server_destroy(gomc_server *sv);
+const char *gomc_server_last_error(const gomc_server *sv);
+/* Builds the job's initial state (on the caller's thread) and queues it for
+ * `moves` MC moves; *job_id identifies it. */
+int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d,
+                       int64_t moves, int64_t *job_id);
+/* status 0 queued, 1 running, 2 done, 3 failed; times = {submitted,
+ * admitted, done} in seconds since the server started; energies as
+ * gomc_energies; any output may be NULL. */
+int gomc_server_poll(gomc_server *sv, int64_t job_id, int32_t *status,
+                     double times[3], double energies[7], int64_t *accepted,
+                     int64_t *moves_done);
+/* Waits until every submitted job has finished (GOMC_ERR_STATE on timeout). */
+int gomc_server_drain(gomc_server *sv, double timeout_s);
+/* {moves done, kernel seconds, launches, peak bytes in use, budget bytes,
+ *  seconds since start, jobs queued, jobs outstanding}. */
+int gomc_server_stats(gomc_server *sv, double out[8]);
+
 /* ---- synthetic inputs (host only) ------------------------------------ */
 /* Random sequential insertion of rigid molecules into a cubic box of edge
  * `box`: uniform centre, uniform orientation (splitmix64 stream from
