@@ -69,6 +69,7 @@ SIGNATURES = {
     "gomc_server_poll": (C.c_int, [_vp, C.c_int64, _ip, _dp, _dp, _lp, _lp]),
     "gomc_server_drain": (C.c_int, [_vp, C.c_double]),
     "gomc_server_stats": (C.c_int, [_vp, _dp]),
+    "gomc_batch_run_cycles": (C.c_int, [_vp, C.c_int, C.c_int]),
     "gomc_batch_last_kernel_ms": (C.c_int, [_vp, _dp]),
     "gomc_place_molecules": (C.c_int, [C.c_int, _ip, _dp, C.c_int, _ip, C.c_double, C.c_uint64,
                                        C.c_double, C.c_double, C.c_int, _dp, _dp, _dp]),

@@ -247,6 +247,10 @@ class Batch:
         mv = None if moves is None else np.ascontiguousarray(moves, dtype=np.int64)
         self._check(self.lib.gomc_batch_run(self.h, ptr(mv) if mv is not None else None))
 
+    def run_cycles(self, ncycles: int, graph: bool = True):
+        """`ncycles` MC cycles, replaying one captured HIP graph per cycle."""
+        self._check(self.lib.gomc_batch_run_cycles(self.h, ncycles, int(graph)))
+
     def stats(self):
         e = np.zeros((self.nboxes, 7))
         acc = np.zeros(self.nboxes, dtype=np.int64)

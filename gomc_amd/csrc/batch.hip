@@ -163,6 +163,8 @@ __device__ __noinline__ void Propose(const BatchBox &sb, BatchState &st, MoveSha
 #pragma unroll
     for (int d = 0; d < 3; ++d) ms.newCom[d] = Wrap(ms.com[d] + ms.shift[d], sb.L[d]);
   } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ms.newCom[d] = ms.com[d];  // a rotation keeps the centre
     const double bb = st.scale[1];
     const double theta = u[2] * (2.0 * bb) - bb;
     const double cu = u[3] * 2.0 - 1.0;
@@ -250,69 +252,98 @@ __device__ __noinline__ void DrainQueue(const BatchBox &sb, const PairQueue &q, 
 }
 
 // Pair-energy change of the move (MoleculeInter, src/CalculateEnergy.cpp:
-// 581-686).  Work items are the atoms j of the box (the moving molecule's
-// own atoms skipped); each item is tested against the old and new
-// positions of all len moving atoms.  Distances inside the cutoff are
-// compacted into a per-wave queue (ballot + prefix count) and their
-// energies evaluated 64 at a time, so the erfc path runs with full waves
-// instead of being masked through for the few in-cutoff lanes.  X/Y/Z may
-// point to LDS or HBM.  acc = {lj, real, -, overlap} partial sums.
+// 581-686) in two levels.  Level 1: one molecule per thread; a molecule
+// whose centre is at least rc + 2 r_max from both the old and the new centre
+// of the moving molecule (minimum image) holds no atom pair inside the
+// cutoff (triangle inequality), so only the others ("near") are kept, in a
+// workgroup-wide list.  Level 2: one atom of a near molecule per work item,
+// tested against the old and new positions of all len moving atoms;
+// distances inside the cutoff are compacted into a per-wave queue (ballot +
+// prefix count) and their energies evaluated 64 at a time, so the erfc
+// path runs with full waves.  CX/CY/CZ may point to LDS or HBM.
+// acc = {lj, real, -, overlap} partial sums.
 template <typename T>
 __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms, const double *X,
-                                       const double *Y, const double *Z, int m, int len,
-                                       int natoms, PairQueue *queues, double *acc4) {
+                                       const double *Y, const double *Z, const double *CX,
+                                       const double *CY, const double *CZ, int m, int len,
+                                       int nmols, PairQueue *queues, int *nearList, int *waveCnt,
+                                       double *acc4) {
   BoxGeom g;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     g.L[d] = sb.L[d];
     g.half[d] = sb.half[d];
   }
-  const int lane = threadIdx.x & (kWave - 1);
-  PairQueue &q = queues[threadIdx.x / kWave];
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  PairQueue &q = queues[wave];
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int count = 0;  // wave-uniform
   double lj = 0.0, real = 0.0, ov = 0.0;
-  const int mlo = m * len, mhi = mlo + len;
-  const int nIter = (natoms + kBatchThreads - 1) / kBatchThreads;
-  for (int it = 0; it < nIter; ++it) {
-    const int j = it * kBatchThreads + int(threadIdx.x);
-    const bool valid = j < natoms && (j < mlo || j >= mhi);
-    const int jj = valid ? j : 0;
-    const double xj = X[jj], yj = Y[jj], zj = Z[jj];
-    const uint32_t mj = uint32_t((uint64_t(uint32_t(jj)) * sb.magicLen) >> 32);
-    const int qa = jj - int(mj) * len;
-    for (int p = 0; p < len; ++p) {
-      const int pq = p * kBatchMaxAtoms + qa;
-      const bool live = valid && (sb.qq[pq] != 0.0 || sb.epsCn[pq] != 0.0);
+  for (int base = 0; base < nmols; base += kBatchThreads) {
+    // ---- level 1: near molecules of this chunk, compacted in index order
+    const int j = base + int(threadIdx.x);
+    bool near = false;
+    if (j < nmols && j != m) {
+      const double cx = CX[j], cy = CY[j], cz = CZ[j];
+      const double d2o = DistSq(g, ms.com[0], ms.com[1], ms.com[2], cx, cy, cz);
+      const double d2n = DistSq(g, ms.newCom[0], ms.newCom[1], ms.newCom[2], cx, cy, cz);
+      near = d2o < sb.nearCutSq || d2n < sb.nearCutSq;
+    }
+    const uint64_t nmask = __ballot(near);
+    if (lane == 0) waveCnt[wave] = __popcll(nmask);
+    __syncthreads();
+    int off = 0, total = 0;
+    for (int w = 0; w < kBatchThreads / kWave; ++w) {
+      const int c = waveCnt[w];
+      off += (w < wave) ? c : 0;
+      total += c;
+    }
+    if (near) nearList[off + __popcll(nmask & below)] = j;
+    __syncthreads();
+    // ---- level 2: atoms of the near molecules against the moving atoms
+    const int items = total * len;
+    for (int ib = 0; ib < items; ib += kBatchThreads) {
+      const int it = ib + int(threadIdx.x);
+      const bool valid = it < items;
+      const int itc = valid ? it : 0;
+      const uint32_t k = uint32_t((uint64_t(uint32_t(itc)) * sb.magicLen) >> 32);  // itc / len
+      const int qa = itc - int(k) * len;
+      const int jj = nearList[k] * len + qa;
+      const double xj = X[jj], yj = Y[jj], zj = Z[jj];
+      for (int p = 0; p < len; ++p) {
+        const int pq = p * kBatchMaxAtoms + qa;
+        const bool live = valid && (sb.qq[pq] != 0.0 || sb.epsCn[pq] != 0.0);
 #pragma unroll
-      for (int isNew = 0; isNew < 2; ++isNew) {
-        const double px = isNew ? ms.newp[0][p] : ms.oldp[0][p];
-        const double py = isNew ? ms.newp[1][p] : ms.oldp[1][p];
-        const double pz = isNew ? ms.newp[2][p] : ms.oldp[2][p];
-        const double d2 = DistSq(g, px, py, pz, xj, yj, zj);
-        const bool inCut = valid && sb.rCutSq > d2;
-        if (isNew && inCut && d2 < sb.rCutLowSq) ov = 1.0;
-        const bool push = live && inCut;
-        const uint64_t mask = __ballot(push);
-        if (push) {
-          const int slot = count + __popcll(mask & below);
-          q.d2[slot] = d2;
-          q.code[slot] = pq | (isNew << 8);
-        }
-        count += __popcll(mask);
-        if (count >= kWave) {  // drain one full wave of pairs
-          __builtin_amdgcn_wave_barrier();
-          DrainQueue<T>(sb, q, kWave, &lj, &real);
-          const double d2n = q.d2[kWave + lane];
-          const int cn = q.code[kWave + lane];
-          __builtin_amdgcn_wave_barrier();
-          q.d2[lane] = d2n;
-          q.code[lane] = cn;
-          count -= kWave;
-          __builtin_amdgcn_wave_barrier();
+        for (int isNew = 0; isNew < 2; ++isNew) {
+          const double px = isNew ? ms.newp[0][p] : ms.oldp[0][p];
+          const double py = isNew ? ms.newp[1][p] : ms.oldp[1][p];
+          const double pz = isNew ? ms.newp[2][p] : ms.oldp[2][p];
+          const double d2 = DistSq(g, px, py, pz, xj, yj, zj);
+          const bool inCut = valid && sb.rCutSq > d2;
+          if (isNew && inCut && d2 < sb.rCutLowSq) ov = 1.0;
+          const bool push = live && inCut;
+          const uint64_t mask = __ballot(push);
+          if (push) {
+            const int slot = count + __popcll(mask & below);
+            q.d2[slot] = d2;
+            q.code[slot] = pq | (isNew << 8);
+          }
+          count += __popcll(mask);
+          if (count >= kWave) {  // drain one full wave of pairs
+            __builtin_amdgcn_wave_barrier();
+            DrainQueue<T>(sb, q, kWave, &lj, &real);
+            const double d2n = q.d2[kWave + lane];
+            const int cn = q.code[kWave + lane];
+            __builtin_amdgcn_wave_barrier();
+            q.d2[lane] = d2n;
+            q.code[lane] = cn;
+            count -= kWave;
+            __builtin_amdgcn_wave_barrier();
+          }
         }
       }
     }
+    __syncthreads();  // the near list is rebuilt for the next chunk
   }
   __builtin_amdgcn_wave_barrier();
   DrainQueue<T>(sb, q, count, &lj, &real);
@@ -392,6 +423,8 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   __shared__ double red[kBatchThreads / kWave][4];
   __shared__ BatchState st;  // running energies and move settings
   __shared__ PairQueue queues[kBatchThreads / kWave];
+  __shared__ int nearList[kBatchThreads];
+  __shared__ int waveCnt[kBatchThreads / kWave];
 
   const int b = blockIdx.x, tid = threadIdx.x;
   {
@@ -403,19 +436,19 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   const int len = sb.molLen, nmols = sb.nmols, K = sb.K;
   const int64_t aoff = sb.atomOff, moff = sb.molOff, koff = sb.kOff;
   double *X = a.x + aoff, *Y = a.y + aoff, *Z = a.z + aoff;
-  // The box's coordinates live in LDS for the launch when they fit
-  // (a.ldsAtoms per workgroup); Xr/Yr/Zr are where the move reads them.
-  extern __shared__ double ldsCoords[];
-  const bool inLds = sb.natoms <= a.ldsAtoms;
-  double *Xr = X, *Yr = Y, *Zr = Z;
+  // The box's molecule centres live in LDS for the launch when they fit
+  // (a.ldsMols per workgroup); CXr/CYr/CZr are where the pair phase reads them.
+  extern __shared__ double ldsCom[];
+  const bool inLds = sb.nmols <= a.ldsMols;
+  double *CXr = a.cx + moff, *CYr = a.cy + moff, *CZr = a.cz + moff;
   if (inLds) {
-    Xr = ldsCoords;
-    Yr = ldsCoords + a.ldsAtoms;
-    Zr = ldsCoords + 2 * a.ldsAtoms;
-    for (int i = tid; i < sb.natoms; i += kBatchThreads) {
-      Xr[i] = X[i];
-      Yr[i] = Y[i];
-      Zr[i] = Z[i];
+    CXr = ldsCom;
+    CYr = ldsCom + a.ldsMols;
+    CZr = ldsCom + 2 * a.ldsMols;
+    for (int i = tid; i < sb.nmols; i += kBatchThreads) {
+      CXr[i] = a.cx[moff + i];
+      CYr[i] = a.cy[moff + i];
+      CZr[i] = a.cz[moff + i];
     }
   }
   const int32_t *kn = a.kn + koff;
@@ -440,7 +473,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     const int m = ms.mol, mtype = ms.type;
     if (tid < len) {
       const int64_t ai = int64_t(m) * len + tid;
-      const double o[3] = {Xr[ai], Yr[ai], Zr[ai]};
+      const double o[3] = {X[ai], Y[ai], Z[ai]};
       double nw[3];
       if (mtype == 0) {
 #pragma unroll
@@ -468,7 +501,8 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
 
     // ---- pair phase ----
     double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
-    PairPhase<T>(sb, ms, Xr, Yr, Zr, m, len, sb.natoms, queues, acc4);
+    PairPhase<T>(sb, ms, X, Y, Z, CXr, CYr, CZr, m, len, sb.nmols, queues, nearList, waveCnt,
+                 acc4);
     __syncthreads();  // tables complete
 
     // ---- reciprocal phase: energy change per k-vector ----
@@ -488,16 +522,16 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
         X[ai] = ms.newp[0][tid];
         Y[ai] = ms.newp[1][tid];
         Z[ai] = ms.newp[2][tid];
-        if (inLds) {
-          Xr[ai] = ms.newp[0][tid];
-          Yr[ai] = ms.newp[1][tid];
-          Zr[ai] = ms.newp[2][tid];
-        }
       }
       if (tid == 0 && mtype == 0) {
         a.cx[moff + m] = ms.newCom[0];
         a.cy[moff + m] = ms.newCom[1];
         a.cz[moff + m] = ms.newCom[2];
+        if (inLds) {
+          CXr[m] = ms.newCom[0];
+          CYr[m] = ms.newCom[1];
+          CZr[m] = ms.newCom[2];
+        }
       }
     }
     __syncthreads();
@@ -512,21 +546,31 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
 }  // namespace
 
 template <int PREC>
-static hipError_t LaunchOne(int nboxes, const BatchArgs &a, hipStream_t s) {
-  const size_t dyn = sizeof(double) * 3 * size_t(a.ldsAtoms);
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&McBatchKernel<PREC>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn));
-  if (e != hipSuccess) return e;
+static hipError_t SetLds(int ldsMols) {
+  const size_t dyn = sizeof(double) * 3 * size_t(ldsMols);
+  return hipFuncSetAttribute(reinterpret_cast<const void *>(&McBatchKernel<PREC>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(dyn));
+}
+
+template <int PREC>
+static hipError_t LaunchOne(int nboxes, const BatchArgs &a, hipStream_t s, bool setAttr) {
+  const size_t dyn = sizeof(double) * 3 * size_t(a.ldsMols);
+  if (setAttr) {
+    hipError_t e = SetLds<PREC>(a.ldsMols);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(McBatchKernel<PREC>, dim3(nboxes), dim3(kBatchThreads), dyn, s, a);
   return hipGetLastError();
 }
 
-int BatchLdsAtoms(int maxAtoms) {
-  return maxAtoms <= kBatchLdsAtomCap ? maxAtoms : 0;
+hipError_t PrepareBatch(int precision, int ldsMols) {
+  return precision == 1 ? SetLds<1>(ldsMols) : SetLds<0>(ldsMols);
 }
 
-hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s) {
-  return precision == 1 ? LaunchOne<1>(nboxes, a, s) : LaunchOne<0>(nboxes, a, s);
+int BatchLdsMols(int maxMols) { return maxMols < kBatchLdsMolCap ? maxMols : kBatchLdsMolCap; }
+
+hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s, bool setAttr) {
+  return precision == 1 ? LaunchOne<1>(nboxes, a, s, setAttr) : LaunchOne<0>(nboxes, a, s, setAttr);
 }
 
 }  // namespace gomc

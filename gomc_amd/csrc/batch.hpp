@@ -19,9 +19,10 @@ constexpr int kBatchMaxTypes = 4;    // LJ atom types per box
 constexpr int kBatchMaxN = 32;       // largest |n| of a k-vector component
 constexpr int kDrawChunk = 64;       // moves whose Philox draws are made at once
 constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of every box stream
-// Atoms whose FP64 coordinates fit in the LDS left beside the static
-// tables and pair queues (160 KiB per CU on gfx950, ~56 KiB static).
-constexpr int kBatchLdsAtomCap = 4300;
+// Molecules whose FP64 centres fit in the LDS left beside the static
+// tables, pair queues and near list (160 KiB per CU on gfx950, ~60 KiB
+// static): 4096 x 24 B.
+constexpr int kBatchLdsMolCap = 4096;
 
 // Read-only description of one box (host-filled, 8-byte aligned fields).
 struct BatchBox {
@@ -29,6 +30,8 @@ struct BatchBox {
   double rCutSq, rCutLJSq, rCutCoulSq, rCutLowSq;
   double alpha, beta, pDisp;
   double kc[3];                       // (1/L) * 2pi, the k-vector spacing
+  double nearCutSq;                   // (rc + 2 r_max + 1e-6)^2: centres farther apart
+                                      // than this hold no pair inside the cutoff
   double sigSq[kBatchMaxAtoms * kBatchMaxAtoms];  // per (atom p, atom q) of the kind
   double epsCn[kBatchMaxAtoms * kBatchMaxAtoms];
   double nHalf[kBatchMaxAtoms * kBatchMaxAtoms];
@@ -70,7 +73,7 @@ struct BatchArgs {
   double *sumR, *sumI;        // [sum K] reference structure factors
   double *dR, *dI;            // [sum K] structure-factor change of the trial
   const int64_t *moves;       // [nboxes] moves to make in this launch
-  int ldsAtoms;               // per-box coordinate capacity of dynamic LDS
+  int ldsMols;                // per-box centre capacity of dynamic LDS
   BatchTrace trace;
 };
 
@@ -92,17 +95,24 @@ struct gomc_batch {
   double *pref = nullptr, *sumR = nullptr, *sumI = nullptr, *dR = nullptr, *dI = nullptr;
   int64_t *moves = nullptr;
   int64_t totalAtoms = 0, totalMols = 0, totalK = 0;
-  int ldsAtoms = 0;
+  int ldsMols = 0;
   gomc::BatchTrace trace{-1, 0, nullptr, nullptr, nullptr, nullptr};
   int64_t traceLen = 0;
+  // one MC cycle (N_i moves per box) captured as a graph, replayed per cycle
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
 };
 
 namespace gomc {
-// Launch the MC kernel at the given precision (0 FP64, 1 FP32 pair terms).
-hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s);
-// LDS coordinate capacity for a batch whose largest box has maxAtoms atoms
-// (0: boxes read coordinates from HBM).
-int BatchLdsAtoms(int maxAtoms);
+// Launch the MC kernel at the given precision (0 FP64, 1 FP32 pair terms);
+// setAttr = false when PrepareBatch already sized its dynamic LDS (inside a
+// stream capture).
+hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s,
+                       bool setAttr = true);
+hipError_t PrepareBatch(int precision, int ldsMols);
+// LDS centre capacity for a batch whose largest box has maxMols molecules
+// (boxes beyond it read centres from HBM).
+int BatchLdsMols(int maxMols);
 void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double volume,
                   std::vector<double> &kx, std::vector<double> &ky, std::vector<double> &kz,
                   std::vector<double> &pref, std::vector<int32_t> *packed);

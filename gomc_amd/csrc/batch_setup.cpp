@@ -30,6 +30,8 @@ void FreeBatch(gomc_batch *bt) {
                   bt->trace.dlj, bt->trace.dreal, bt->trace.drecip};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (bt->graphExec) (void)hipGraphExecDestroy(bt->graphExec);
+  if (bt->graph) (void)hipGraphDestroy(bt->graph);
   if (bt->ev0) (void)hipEventDestroy(bt->ev0);
   if (bt->ev1) (void)hipEventDestroy(bt->ev1);
   if (bt->stream) (void)hipStreamDestroy(bt->stream);
@@ -76,6 +78,23 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     P.nmax[a] = int(b->recipRcut * b->geom.L[a] / (2.0 * M_PI)) + 1;
     if (P.nmax[a] > kBatchMaxN) return BFail(bt, GOMC_ERR_ARG, "box too large for the batched phase tables");
   }
+  // largest atom-to-centre distance of the kind (its first molecule), for
+  // the centre cull of the pair phase
+  double rmax = 0.0;
+  for (int p = 0; p < len; ++p) {
+    double d2 = 0.0;
+    const double *P3[3] = {b->hx.data(), b->hy.data(), b->hz.data()};
+    const double c3[3] = {b->comx[0], b->comy[0], b->comz[0]};
+    for (int a = 0; a < 3; ++a) {
+      double v = P3[a][p] - c3[a];
+      if (v > b->geom.half[a]) v -= b->geom.L[a];
+      else if (v < -b->geom.half[a]) v += b->geom.L[a];
+      d2 += v * v;
+    }
+    rmax = std::max(rmax, std::sqrt(d2));
+  }
+  const double nearCut = std::sqrt(b->geom.rCutSq) + 2.0 * rmax + 1e-6;
+  P.nearCutSq = nearCut * nearCut;
   P.rCutSq = b->geom.rCutSq;
   P.rCutLJSq = b->geom.rCutLJSq;
   P.rCutCoulSq = b->geom.rCutCoulSq;
@@ -148,6 +167,28 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
   return GOMC_OK;
 }
 
+BatchArgs MakeArgs(const gomc_batch *bt) {
+  BatchArgs a;
+  a.boxes = bt->boxes;
+  a.state = bt->state;
+  a.x = bt->x;
+  a.y = bt->y;
+  a.z = bt->z;
+  a.cx = bt->cx;
+  a.cy = bt->cy;
+  a.cz = bt->cz;
+  a.kn = bt->kn;
+  a.pref = bt->pref;
+  a.sumR = bt->sumR;
+  a.sumI = bt->sumI;
+  a.dR = bt->dR;
+  a.dI = bt->dI;
+  a.moves = bt->moves;
+  a.ldsMols = bt->ldsMols;
+  a.trace = bt->trace;
+  return a;
+}
+
 int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t kvec) {
   BHIP(bt, hipSetDevice(bt->device));
   auto alloc = [&](void **p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 8)) == hipSuccess; };
@@ -218,18 +259,17 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
   if (hipSetDevice(device) != hipSuccess) return fail(GOMC_ERR_NODEV);
   std::vector<PackedBox> packed(nboxes);
   int64_t atoms = 0, mols = 0, kvec = 0;
-  int maxAtoms = 0;
+  int maxMols = 0;
   for (int i = 0; i < nboxes; ++i) {
     int rc = PackBox(bt, descs[i], atoms, mols, kvec, packed[i]);
     if (rc) return fail(rc);
     atoms += packed[i].P.natoms;
     mols += packed[i].P.nmols;
     kvec += packed[i].P.K;
-    maxAtoms = std::max(maxAtoms, packed[i].P.natoms);
+    maxMols = std::max(maxMols, packed[i].P.nmols);
   }
-  // Boxes up to the cap keep coordinates in LDS; a batch whose largest box
-  // exceeds it sizes LDS for the boxes that fit.
-  bt->ldsAtoms = BatchLdsAtoms(std::min(maxAtoms, kBatchLdsAtomCap));
+  // Boxes up to the cap keep their molecule centres in LDS.
+  bt->ldsMols = BatchLdsMols(maxMols);
   int rc = AllocBatch(bt, nboxes, atoms, mols, kvec);
   if (rc) return fail(rc);
   bt->hboxes.resize(nboxes);
@@ -251,24 +291,7 @@ int gomc_batch_run(gomc_batch *bt, const int64_t *moves) {
   }
   BHIP(bt, hipMemcpyAsync(bt->moves, mv.data(), sizeof(int64_t) * bt->nboxes, hipMemcpyHostToDevice,
                           bt->stream));
-  BatchArgs a;
-  a.boxes = bt->boxes;
-  a.state = bt->state;
-  a.x = bt->x;
-  a.y = bt->y;
-  a.z = bt->z;
-  a.cx = bt->cx;
-  a.cy = bt->cy;
-  a.cz = bt->cz;
-  a.kn = bt->kn;
-  a.pref = bt->pref;
-  a.sumR = bt->sumR;
-  a.sumI = bt->sumI;
-  a.dR = bt->dR;
-  a.dI = bt->dI;
-  a.moves = bt->moves;
-  a.ldsAtoms = bt->ldsAtoms;
-  a.trace = bt->trace;
+  const BatchArgs a = MakeArgs(bt);
   BHIP(bt, hipEventRecord(bt->ev0, bt->stream));
   BHIP(bt, LaunchBatch(bt->precision, bt->nboxes, a, bt->stream));
   BHIP(bt, hipEventRecord(bt->ev1, bt->stream));
@@ -277,6 +300,45 @@ int gomc_batch_run(gomc_batch *bt, const int64_t *moves) {
   BHIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
   bt->lastKernelMs = ms;
   if (bt->trace.box >= 0) bt->traceLen = std::min<int64_t>(bt->trace.cap, mv[bt->trace.box]);
+  return GOMC_OK;
+}
+
+int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph) {
+  if (!bt || ncycles < 0) return GOMC_ERR_ARG;
+  BHIP(bt, hipSetDevice(bt->device));
+  std::vector<int64_t> mv(bt->nboxes);
+  for (int i = 0; i < bt->nboxes; ++i) mv[i] = int64_t(bt->hboxes[i].nmols);
+  BHIP(bt, hipMemcpyAsync(bt->moves, mv.data(), sizeof(int64_t) * bt->nboxes, hipMemcpyHostToDevice,
+                          bt->stream));
+  BHIP(bt, hipStreamSynchronize(bt->stream));
+  const BatchArgs a = MakeArgs(bt);
+  if (use_graph && !bt->graphExec) {
+    // one cycle = one kernel node; the move counts are read from device
+    // memory when a replay runs, so the graph stays valid while they hold
+    BHIP(bt, PrepareBatch(bt->precision, bt->ldsMols));
+    BHIP(bt, hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = LaunchBatch(bt->precision, bt->nboxes, a, bt->stream, false);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(bt->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return BFail(bt, GOMC_ERR_HIP, "batch graph capture failed");
+    }
+    bt->graph = g;
+    BHIP(bt, hipGraphInstantiate(&bt->graphExec, bt->graph, nullptr, nullptr, 0));
+  }
+  BHIP(bt, hipEventRecord(bt->ev0, bt->stream));
+  for (int c = 0; c < ncycles; ++c) {
+    if (use_graph)
+      BHIP(bt, hipGraphLaunch(bt->graphExec, bt->stream));
+    else
+      BHIP(bt, LaunchBatch(bt->precision, bt->nboxes, a, bt->stream));
+  }
+  BHIP(bt, hipEventRecord(bt->ev1, bt->stream));
+  BHIP(bt, hipStreamSynchronize(bt->stream));
+  float ms = 0.0f;
+  BHIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
+  bt->lastKernelMs = ms;
   return GOMC_OK;
 }
 
@@ -315,6 +377,11 @@ int gomc_batch_set_trace(gomc_batch *bt, int box, int64_t cap) {
     if (p) (void)hipFree(p);
   bt->trace = BatchTrace{-1, 0, nullptr, nullptr, nullptr, nullptr};
   bt->traceLen = 0;
+  // a captured cycle holds the old trace pointers
+  if (bt->graphExec) (void)hipGraphExecDestroy(bt->graphExec);
+  if (bt->graph) (void)hipGraphDestroy(bt->graph);
+  bt->graphExec = nullptr;
+  bt->graph = nullptr;
   if (box < 0 || cap == 0) return GOMC_OK;
   BHIP(bt, hipMalloc(&bt->trace.acc, sizeof(int32_t) * cap));
   BHIP(bt, hipMalloc(&bt->trace.dlj, sizeof(double) * cap));

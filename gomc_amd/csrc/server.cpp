@@ -294,7 +294,7 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
     gomc_batch_destroy(bt);
     return rc;
   }
-  bt->ldsAtoms = BatchLdsAtoms(kBatchLdsAtomCap);
+  bt->ldsMols = BatchLdsMols(kBatchLdsMolCap);
   sv->bt = bt;
   sv->slotJob.assign(slots, nullptr);
   gomc_server *raw = sv.release();

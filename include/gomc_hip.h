@@ -234,6 +234,10 @@ const char *gomc_batch_last_error(const gomc_batch *bt);
 /* Advance box i by moves[i] moves (NULL: by its molecule count, one MC
  * cycle) in one persistent launch. */
 int gomc_batch_run(gomc_batch *bt, const int64_t *moves);
+/* `ncycles` MC cycles of every box (N_i moves each, one launch per cycle).
+ * With use_graph the cycle is captured once as a HIP graph and replayed;
+ * gomc_batch_last_kernel_ms then covers all cycles. */
+int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph);
 /* Per-box running energies [nboxes][7], accepted counts [nboxes], moves
  * made [nboxes]; any NULL. */
 int gomc_batch_stats(const gomc_batch *bt, double *energies,
