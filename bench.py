@@ -1,12 +1,16 @@
 """Benchmark: MC trial moves/s of the batched persistent MC kernel on one
 MI355X, on 256 independent NVT water boxes (N log-uniform in [200, 4000]
 molecules at 0.997 g/cm^3, T uniform in [280, 360] K, one Philox stream per
-box).  One step = one MC cycle of the whole batch: box i makes N_i moves.
+box).  One step = one MC cycle of the whole batch (box i makes N_i moves),
+replayed from one captured HIP graph.  `value` is the FP64 batched rate.
 
-Reported beside it, on the same boxes:
+Reported beside it:
+  * the same cycles with FP32 pair terms / FP64 accumulation;
   * the plain FP64 GPU path (one fused launch per move, host Metropolis,
     MT19937 draws -- the reference's call pattern with state kept in HBM),
-    measured on a bounded sample of moves per box;
+    on a bounded sample of moves per box;
+  * a bounded serving run: a Poisson stream of water state-point jobs through
+    the continuous-batching server (moves/s, p50/p99 latency);
   * the CPU oracle (C++/OpenMP, one box per thread) on a bounded sample.
 
 Prints one JSON line.
@@ -35,12 +39,16 @@ W_DIST, W_COUL, W_LJ, W_SINCOS, W_KENERGY = 12, 30, 15, 24, 6
 
 
 def move_flops(s, nkvec: int) -> float:
-    """Algorithmic FLOPs of one single-molecule trial in box `s`."""
+    """Algorithmic FLOPs of one single-molecule trial in box `s`: the
+    distance checks the reference makes over the 27 neighbour cells of its
+    cell list (cells per edge max(floor(L / rc), 3), CellList.cpp:138-163),
+    in-cutoff Coulomb and LJ terms, 2 a K sincos and K energy terms."""
     a = 3
     n_atoms = s.natoms
     L = s.box[0]
     frac = min(1.0, (4.0 / 3.0) * math.pi * s.rcut ** 3 / L ** 3)
-    D = 2 * a * (n_atoms - a)                    # distance checks
+    edge = max(int(L / max(s.rcut, s.rcut_coulomb)), 3)
+    D = 2 * a * (n_atoms - a) * min(1.0, 27.0 / edge ** 3)   # distance checks
     C = 2 * a * ((n_atoms - a) * frac)           # charged pairs in cutoff
     Lj = 2 * 1 * ((s.nmols - 1) * frac)          # O-O pairs in cutoff
     S = 2 * a * nkvec                            # sincos (old + new)
@@ -59,6 +67,8 @@ def main() -> None:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=60, help="0 skips the serving run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -91,11 +101,11 @@ def main() -> None:
     print(f"# batch ready ({time.time() - t0:.1f}s); warmup {args.warmup}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
-        bt.run(nmol)
+        bt.run_cycles(1, graph=True)   # the first call captures the cycle
     kernel_ms = []
     t_start = time.perf_counter()
     for i in range(args.steps):
-        bt.run(nmol)
+        bt.run_cycles(1, graph=True)
         kernel_ms.append(bt.last_kernel_ms())
         print(f"# step {i}: kernel {kernel_ms[-1]:.1f} ms", file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t_start
@@ -105,6 +115,41 @@ def main() -> None:
     achieved_tflops = flops_step / (avg_ms * 1e-3) / 1e12
     e, acc, done = bt.stats()
     accept_rate = float(acc.sum() / done.sum())
+
+    mixed = None
+    if not args.no_mixed and args.precision == 0:
+        bm = Batch(boxes, precision=1)
+        for _ in range(args.warmup):
+            bm.run_cycles(1, graph=True)
+        mk = []
+        tm = time.perf_counter()
+        for i in range(args.steps):
+            bm.run_cycles(1, graph=True)
+            mk.append(bm.last_kernel_ms())
+        tm = time.perf_counter() - tm
+        mixed = dict(moves_per_s=float(nmol.sum() * args.steps / tm),
+                     kernel_ms_per_step=float(np.mean(mk)),
+                     dtype="f32 pair terms and phase products, f64 accumulation")
+        bm.close()
+        print(f"# mixed precision {mixed['moves_per_s']:.0f} moves/s", file=sys.stderr, flush=True)
+    bt.close()
+
+    served = None
+    if args.serve_jobs > 0:
+        from gomc_amd.serve import Server, poisson_stream, run_stream
+        jobs = poisson_stream(args.serve_jobs, rate=20.0, seed=11, nmin=200, nmax=2000,
+                              cycles=20)
+        sv = Server(slots=256, budget_bytes=8 << 30, tick_ms=20.0)
+        rep = run_stream(sv, jobs, timeout=300.0)
+        sv.close()
+        served = dict(workload=f"Poisson(20/s) stream of {rep['jobs']} NVT water state-point "
+                               "jobs, N log-uniform 200-2000, T 250-350 K, 20 cycles each",
+                      moves_per_s=rep["moves_per_s"], p50_s=rep["p50_s"], p99_s=rep["p99_s"],
+                      mean_queue_s=rep["mean_queue_s"], wall_s=rep["wall_s"],
+                      moves=rep["moves"], budget_bytes=rep["stats"]["budget_bytes"],
+                      peak_bytes=rep["stats"]["peak_bytes"])
+        print(f"# served {served['moves_per_s']:.0f} moves/s p50 {served['p50_s']:.2f} s "
+              f"p99 {served['p99_s']:.2f} s", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain:
@@ -118,14 +163,9 @@ def main() -> None:
             tot_time += time.perf_counter() - t
             tot_moves += args.plain_moves
             bx.close()
-        # The plain path advances boxes one after another: its aggregate rate
-        # over the batch is moves / (sum of per-box times), weighted like the
-        # batch (box i contributes N_i moves per cycle).
-        per_move = {}
-        for i in idx:
-            per_move[i] = None
+        # the plain path advances one box at a time: its rate is moves over
+        # the summed per-box times
         plain_rate = tot_moves / tot_time
-        # weight the sample by each box's share of a cycle
         plain = dict(moves_per_s=plain_rate, sample=f"{len(idx)} boxes x {args.plain_moves} moves")
         print(f"# plain path {plain_rate:.0f} moves/s", file=sys.stderr, flush=True)
 
@@ -171,8 +211,13 @@ def main() -> None:
         },
         "kernel_ms_per_step": avg_ms,
         "accept_rate": accept_rate,
+        "graph": "one HIP graph per MC cycle, replayed",
+        "mixed_precision": mixed,
+        "served": served,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["moves_per_s"]) if plain else None,
+        "mixed_speedup_vs_plain": (mixed["moves_per_s"] / plain["moves_per_s"])
+        if (plain and mixed) else None,
         "roofline": {
             "bound": "valu",
             "achieved": achieved_tflops,
