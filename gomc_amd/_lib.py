@@ -36,7 +36,8 @@ SIGNATURES = {
     "gomc_recip_init": (C.c_int, [_vp, _dp]),
     "gomc_recip_sums": (C.c_int, [_vp, _dp]),
     "gomc_mol_recip": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
-    "gomc_swap_recip": (C.c_int, [_vp, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "gomc_swap_recip": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),
+    "gomc_recip_cache": (C.c_int, [_vp, C.c_int]),
     "gomc_recip_accept": (C.c_int, [_vp]),
     "gomc_recip_copy_ref_to_new": (C.c_int, [_vp]),
     "gomc_recip_get": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),

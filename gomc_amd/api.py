@@ -104,12 +104,16 @@ class Box:
         self._check(self.lib.gomc_mol_trial(self.h, m, ptr(x), ptr(y), ptr(z), ptr(out)))
         return out
 
-    def swap_recip(self, kind: int, insert: bool, xyz) -> float:
+    def swap_recip(self, kind: int, insert: bool, xyz, mol: int = -1) -> float:
         x, y, z = self._xyz3(xyz)
         e = C.c_double()
-        self._check(self.lib.gomc_swap_recip(self.h, kind, int(insert), ptr(x), ptr(y), ptr(z),
-                                             C.byref(e)))
+        self._check(self.lib.gomc_swap_recip(self.h, kind, mol, int(insert), ptr(x), ptr(y),
+                                             ptr(z), C.byref(e)))
         return e.value
+
+    def recip_cache(self, enable: bool = True):
+        """EwaldCached mode: per-molecule structure factors kept in HBM."""
+        self._check(self.lib.gomc_recip_cache(self.h, int(enable)))
 
     def particle_inter(self, atom_type: int, charge: float, trials, skip_mol: int = -1):
         x, y, z = self._xyz3(trials)

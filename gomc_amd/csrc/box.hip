@@ -40,6 +40,10 @@ struct TrialArgs {
   int mtype[kMaxMolAtoms];
   const double *kx, *ky, *kz, *pref, *sumRref, *sumIref;
   double *sumRnew, *sumInew;
+  // EwaldCached: the moving molecule's cached old terms, and where its new
+  // terms are staged (both null without the cache)
+  const double *oldC, *oldS;
+  double *stC, *stS;
   // accepted coordinates of molecule [cstart, cstart+clen) not yet in HBM
   int cstart, clen;
   double cx[kMaxMolAtoms], cy[kMaxMolAtoms], cz[kMaxMolAtoms];
@@ -106,17 +110,32 @@ __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
     if (a.doRecip && k < a.K) {
       const double kxk = a.kx[k], kyk = a.ky[k], kzk = a.kz[k];
       double sRN = 0.0, sIN = 0.0, sRO = 0.0, sIO = 0.0;
-      for (int p = 0; p < a.mlen; ++p) {
-        if (fabs(a.mq[p]) < 0.000000001) continue;
-        double dNew = a.nx[p] * kxk + a.ny[p] * kyk + a.nz[p] * kzk;
-        double dOld = a.ox[p] * kxk + a.oy[p] * kyk + a.oz[p] * kzk;
-        double sn, cn, so, co;
-        sincos(dNew, &sn, &cn);
-        sincos(dOld, &so, &co);
-        sRN += a.mq[p] * cn;
-        sIN += a.mq[p] * sn;
-        sRO += a.mq[p] * co;
-        sIO += a.mq[p] * so;
+      if (a.oldC) {  // EwaldCached::MolReciprocal (EwaldCached.cpp:244-295)
+        for (int p = 0; p < a.mlen; ++p) {
+          if (fabs(a.mq[p]) < 0.000000001) continue;
+          double dNew = a.nx[p] * kxk + a.ny[p] * kyk + a.nz[p] * kzk;
+          double sn, cn;
+          sincos(dNew, &sn, &cn);
+          sRN += a.mq[p] * cn;
+          sIN += a.mq[p] * sn;
+        }
+        sRO = a.oldC[k];
+        sIO = a.oldS[k];
+        a.stC[k] = sRN;
+        a.stS[k] = sIN;
+      } else {
+        for (int p = 0; p < a.mlen; ++p) {
+          if (fabs(a.mq[p]) < 0.000000001) continue;
+          double dNew = a.nx[p] * kxk + a.ny[p] * kyk + a.nz[p] * kzk;
+          double dOld = a.ox[p] * kxk + a.oy[p] * kyk + a.oz[p] * kzk;
+          double sn, cn, so, co;
+          sincos(dNew, &sn, &cn);
+          sincos(dOld, &so, &co);
+          sRN += a.mq[p] * cn;
+          sIN += a.mq[p] * sn;
+          sRO += a.mq[p] * co;
+          sIO += a.mq[p] * so;
+        }
       }
       double R = a.sumRref[k] + (sRN - sRO);
       double I = a.sumIref[k] + (sIN - sIO);
@@ -340,6 +359,7 @@ struct SwapArgs {
   double px[kMaxMolAtoms], py[kMaxMolAtoms], pz[kMaxMolAtoms], pq[kMaxMolAtoms];
   const double *kx, *ky, *kz, *pref, *sumRref, *sumIref;
   double *sumRnew, *sumInew;
+  double *stC, *stS;  // EwaldCached insertion: the new molecule's terms
   Partial4 *partials;
   unsigned *counter;
   double *result;
@@ -359,6 +379,10 @@ __global__ void __launch_bounds__(kBlock) SwapRecipKernel(const SwapArgs a) {
       sR += a.pq[p] * c;
       sI += a.pq[p] * s;
     }
+    if (a.stC) {
+      a.stC[k] = sR;
+      a.stS[k] = sI;
+    }
     double R = (a.sign > 0) ? a.sumRref[k] + sR : a.sumRref[k] - sR;
     double I = (a.sign > 0) ? a.sumIref[k] + sI : a.sumIref[k] - sI;
     a.sumRnew[k] = R;
@@ -368,6 +392,39 @@ __global__ void __launch_bounds__(kBlock) SwapRecipKernel(const SwapArgs a) {
   Partial4 part{0, 0, 0, 0};
   part.recip = BlockSum(e, red);
   FinishPartials(part, a.partials, a.counter, a.result);
+}
+
+// EwaldCached::BoxReciprocalSetup's per-molecule cache (EwaldCached.cpp:
+// 92-141): for each k (one per thread) and each molecule in the box,
+// sum q (cos, sin)(k . r) over its charged atoms in atom order -- the same
+// arithmetic the trial kernel uses for a molecule's old terms.
+struct CacheArgs {
+  const double *x, *y, *z, *q;
+  const int *mol, *molStart;
+  int nmols, K;
+  const double *kx, *ky, *kz;
+  double *cacheC, *cacheS;
+};
+
+__global__ void __launch_bounds__(kBlock) CacheBuildKernel(const CacheArgs a) {
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= a.K) return;
+  const double kxk = a.kx[k], kyk = a.ky[k], kzk = a.kz[k];
+  for (int m = 0; m < a.nmols; ++m) {
+    const int s = a.molStart[m], e = a.molStart[m + 1];
+    double c = 0.0, sn = 0.0;
+    if (a.mol[s] >= 0)
+      for (int p = s; p < e; ++p) {
+        if (fabs(a.q[p]) < 0.000000001) continue;
+        double dot = a.x[p] * kxk + a.y[p] * kyk + a.z[p] * kzk;
+        double sv, cv;
+        sincos(dot, &sv, &cv);
+        c += a.q[p] * cv;
+        sn += a.q[p] * sv;
+      }
+    a.cacheC[int64_t(m) * a.K + k] = c;
+    a.cacheS[int64_t(m) * a.K + k] = sn;
+  }
 }
 
 // Trial positions of one atom against all atoms (CalculateEnergy.cpp:741-783):
@@ -480,6 +537,14 @@ int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const do
   a.sumIref = b->sumIref;
   a.sumRnew = b->sumRnew;
   a.sumInew = b->sumInew;
+  b->stageMol = -1;
+  if (b->cached && a.doRecip) {
+    a.oldC = b->cacheC + int64_t(m) * b->K;
+    a.oldS = b->cacheS + int64_t(m) * b->K;
+    a.stC = b->stageC;
+    a.stS = b->stageS;
+    b->stageMol = m;
+  }
   a.cstart = 0;
   a.clen = 0;
   if (b->pendingMol >= 0) {
@@ -663,7 +728,7 @@ extern "C" {
 const char *gomc_last_error(const gomc_box *b) { return b ? b->err.c_str() : "null box"; }
 
 static void FreeAll(gomc_box *b) {
-  void *ptrs[] = {b->x,        b->y,       b->z,        b->q,        b->type,   b->mol,
+  void *ptrs[] = {b->x,        b->y,       b->z,        b->q,        b->type,   b->mol,   b->molStartDev,
                   b->tSigSq,   b->tEpsCn,  b->tNHalf,   b->tEpsCn6,  b->tNOver6, b->cellCount,
                   b->cellCursor, b->cellStart, b->cellVec, b->atomCell, b->nbr,   b->kx,
                   b->ky,       b->kz,      b->pref,     b->sumRref,  b->sumIref, b->sumRnew,
@@ -672,6 +737,8 @@ static void FreeAll(gomc_box *b) {
     if (p) (void)hipFree(p);
   MPFree(b);
   if (b->trialBuf) (void)hipFree(b->trialBuf);
+  for (void *p : {(void *)b->cacheC, (void *)b->cacheS, (void *)b->stageC, (void *)b->stageS})
+    if (p) (void)hipFree(p);
   if (b->hostRes) (void)hipHostFree(b->hostRes);
   if (b->stream) (void)hipStreamDestroy(b->stream);
 }
@@ -707,7 +774,9 @@ int gomc_set_coords(gomc_box *b, const double *x, const double *y, const double 
   HIP_TRY(b, hipMemcpyAsync(b->y, y, bytes, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->z, z, bytes, hipMemcpyHostToDevice, b->stream));
   b->pendingMol = -1;
-  return Sync(b);
+  int rc = Sync(b);
+  if (!rc && b->cached) rc = CacheBuild(b);
+  return rc;
 }
 
 int gomc_get_coords(const gomc_box *b, double *x, double *y, double *z) {
@@ -812,6 +881,78 @@ int ParticleInterImpl(gomc_box *b, int atomType, double charge, int skipMol, int
   return GOMC_OK;
 }
 
+// Ewald::SwapDestRecip / SwapSourceRecip (src/Ewald.cpp:478-531,657-710)
+// and, with the molecule cache, EwaldCached::SwapDestRecip's cache
+// staging (src/EwaldCached.cpp:300-352).
+int SwapRecipImpl(gomc_box *b, int kind, int insert, const double *x, const double *y,
+                  const double *z, int mol, double *enew) {
+  if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+  const int n = int(b->kindType[kind].size());
+  if (n > kMaxMolAtoms) return Fail(b, GOMC_ERR_ARG, "molecule has more than 16 atoms");
+  SwapArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.K = b->K;
+  a.n = n;
+  a.sign = insert ? 1.0 : -1.0;
+  for (int p = 0; p < n; ++p) {
+    a.px[p] = x[p];
+    a.py[p] = y[p];
+    a.pz[p] = z[p];
+    a.pq[p] = b->kindCharge[kind][p];
+  }
+  a.kx = b->kx;
+  a.ky = b->ky;
+  a.kz = b->kz;
+  a.pref = b->pref;
+  a.sumRref = b->sumRref;
+  a.sumIref = b->sumIref;
+  a.sumRnew = b->sumRnew;
+  a.sumInew = b->sumInew;
+  b->stageMol = -1;
+  if (b->cached && insert && mol >= 0) {
+    a.stC = b->stageC;
+    a.stS = b->stageS;
+    b->stageMol = mol;
+  }
+  const int nblocks = std::max(1, (b->K + kBlock - 1) / kBlock);
+  int rc = EnsurePartials(b, nblocks);
+  if (rc) return rc;
+  a.partials = b->partials;
+  a.counter = b->counter;
+  a.result = b->devRes;
+  hipLaunchKernelGGL(SwapRecipKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  rc = Sync(b);
+  if (rc) return rc;
+  *enew = b->hostRes[2];
+  return GOMC_OK;
+}
+
+// EwaldCached::BoxReciprocalSetup's per-molecule terms for the current
+// coordinates.
+int CacheBuild(gomc_box *b) {
+  if (!b->cached) return GOMC_OK;
+  CacheArgs a;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.q = b->q;
+  a.mol = b->mol;
+  a.molStart = b->molStartDev;
+  a.nmols = b->nmols;
+  a.K = b->K;
+  a.kx = b->kx;
+  a.ky = b->ky;
+  a.kz = b->kz;
+  a.cacheC = b->cacheC;
+  a.cacheS = b->cacheS;
+  const int g = std::max(1, (b->K + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(CacheBuildKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
+  b->stageMol = -1;
+  return Sync(b);
+}
+
 // Moves molecule m's atoms in or out of the interacting set on the device.
 int SetMolLive(gomc_box *b, int m, bool live) {
   const int s = b->molStart[m], len = b->molLen[m];
@@ -882,44 +1023,38 @@ int gomc_mol_trial(gomc_box *b, int m, const double *nx, const double *ny, const
   return LaunchTrial(b, m, &b->hx[s], &b->hy[s], &b->hz[s], nx, ny, nz, true, true, out);
 }
 
-int gomc_swap_recip(gomc_box *b, int kind, int insert, const double *x, const double *y,
+int gomc_swap_recip(gomc_box *b, int kind, int mol, int insert, const double *x, const double *y,
                     const double *z, double *enew) {
-  if (!b || kind < 0 || kind >= b->nkinds) return GOMC_ERR_ARG;
-  if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
-  const int n = int(b->kindType[kind].size());
-  if (n > kMaxMolAtoms) return Fail(b, GOMC_ERR_ARG, "molecule has more than 16 atoms");
+  if (!b || kind < 0 || kind >= b->nkinds || mol >= b->nmols || !x || !y || !z || !enew)
+    return GOMC_ERR_ARG;
   (void)hipSetDevice(b->device);
-  SwapArgs a;
-  std::memset(&a, 0, sizeof(a));
-  a.K = b->K;
-  a.n = n;
-  a.sign = insert ? 1.0 : -1.0;
-  for (int p = 0; p < n; ++p) {
-    a.px[p] = x[p];
-    a.py[p] = y[p];
-    a.pz[p] = z[p];
-    a.pq[p] = b->kindCharge[kind][p];
+  return SwapRecipImpl(b, kind, insert, x, y, z, mol, enew);
+}
+
+int gomc_recip_cache(gomc_box *b, int enable) {
+  if (!b) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  if (!enable) {
+    void *ptrs[] = {b->cacheC, b->cacheS, b->stageC, b->stageS};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+    b->cacheC = b->cacheS = b->stageC = b->stageS = nullptr;
+    b->cached = false;
+    b->stageMol = -1;
+    return GOMC_OK;
   }
-  a.kx = b->kx;
-  a.ky = b->ky;
-  a.kz = b->kz;
-  a.pref = b->pref;
-  a.sumRref = b->sumRref;
-  a.sumIref = b->sumIref;
-  a.sumRnew = b->sumRnew;
-  a.sumInew = b->sumInew;
-  const int nblocks = std::max(1, (b->K + kBlock - 1) / kBlock);
-  int rc = EnsurePartials(b, nblocks);
+  if (!b->ewald || !b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+  if (!b->cached) {
+    const size_t row = sizeof(double) * size_t(std::max(b->K, 1));
+    HIP_TRY(b, hipMalloc(&b->cacheC, row * b->nmols));
+    HIP_TRY(b, hipMalloc(&b->cacheS, row * b->nmols));
+    HIP_TRY(b, hipMalloc(&b->stageC, row));
+    HIP_TRY(b, hipMalloc(&b->stageS, row));
+    b->cached = true;
+  }
+  int rc = FlushPending(b);
   if (rc) return rc;
-  a.partials = b->partials;
-  a.counter = b->counter;
-  a.result = b->devRes;
-  hipLaunchKernelGGL(SwapRecipKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
-  HIP_TRY(b, hipGetLastError());
-  rc = Sync(b);
-  if (rc) return rc;
-  *enew = b->hostRes[2];
-  return GOMC_OK;
+  return CacheBuild(b);
 }
 
 int gomc_particle_inter(gomc_box *b, int atom_type, double charge, int skip_mol, int ntrials,
@@ -947,6 +1082,15 @@ int gomc_recip_accept(gomc_box *b) {
   if (!b) return GOMC_ERR_ARG;
   std::swap(b->sumRnew, b->sumRref);
   std::swap(b->sumInew, b->sumIref);
+  if (b->cached && b->stageMol >= 0) {  // the trial molecule's terms become its cache
+    (void)hipSetDevice(b->device);
+    const size_t row = sizeof(double) * b->K;
+    HIP_TRY(b, hipMemcpyAsync(b->cacheC + int64_t(b->stageMol) * b->K, b->stageC, row,
+                              hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(b, hipMemcpyAsync(b->cacheS + int64_t(b->stageMol) * b->K, b->stageS, row,
+                              hipMemcpyDeviceToDevice, b->stream));
+    b->stageMol = -1;
+  }
   return GOMC_OK;
 }
 

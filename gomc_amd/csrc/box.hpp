@@ -121,6 +121,7 @@ struct gomc_box {
   // ---- device state (HBM resident) ----
   double *x = nullptr, *y = nullptr, *z = nullptr, *q = nullptr;
   int *type = nullptr, *mol = nullptr;
+  int *molStartDev = nullptr;  // [nmols + 1] first atom of each molecule
   double *tSigSq = nullptr, *tEpsCn = nullptr, *tNHalf = nullptr, *tEpsCn6 = nullptr,
          *tNOver6 = nullptr;
   // cell list
@@ -143,6 +144,12 @@ struct gomc_box {
   double pend[3][gomc::kMaxMolAtoms];
 
   std::vector<int> hCellCount;  // atoms per cell at the last build
+  // EwaldCached mode (src/EwaldCached.cpp): per-molecule structure factors
+  // cacheC/S [nmols][K] of the current positions, and the trial molecule's
+  // new terms staged until the move is accepted.
+  bool cached = false;
+  double *cacheC = nullptr, *cacheS = nullptr, *stageC = nullptr, *stageS = nullptr;
+  int stageMol = -1;
   // staging for trial positions and their energies (ParticleInter)
   double *trialBuf = nullptr;
   int trialCap = 0;
@@ -208,4 +215,10 @@ uint32_t NumKindInBox(const gomc_box *b, int k, int box);
 int GetMolNum(const gomc_box *b, uint32_t sub, int k, int box);
 int TransferMove(gomc_box *b, double subDraw, int &mol, int &accepted, int &overlap,
                  double dE[3]);
+// Ewald swap delta; with the molecule cache on, an insertion of molecule
+// `mol` stages its new terms.
+int SwapRecipImpl(gomc_box *b, int kind, int insert, const double *x, const double *y,
+                  const double *z, int mol, double *enew);
+// Rebuilds the per-molecule cache from the current coordinates.
+int CacheBuild(gomc_box *b);
 }  // namespace gomc

@@ -292,6 +292,7 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   ok &= hipMalloc(&b->q, nd) == hipSuccess;
   ok &= hipMalloc(&b->type, ni) == hipSuccess;
   ok &= hipMalloc(&b->mol, ni) == hipSuccess;
+  ok &= hipMalloc(&b->molStartDev, sizeof(int) * (b->nmols + 1)) == hipSuccess;
   ok &= hipMalloc(&b->tSigSq, sizeof(double) * T2) == hipSuccess;
   ok &= hipMalloc(&b->tEpsCn, sizeof(double) * T2) == hipSuccess;
   ok &= hipMalloc(&b->tNHalf, sizeof(double) * T2) == hipSuccess;
@@ -318,6 +319,10 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   for (int i = 0; i < b->natoms; ++i)
     if (b->molBox[b->atomMol[i]] != 0) liveMol[i] = -1;
   cp &= hipMemcpy(b->mol, liveMol.data(), ni, hipMemcpyHostToDevice) == hipSuccess;
+  std::vector<int> msd(b->molStart);
+  msd.push_back(b->natoms);
+  cp &= hipMemcpy(b->molStartDev, msd.data(), sizeof(int) * msd.size(), hipMemcpyHostToDevice) ==
+        hipSuccess;
   cp &= hipMemcpy(b->tSigSq, b->sigSq.data(), sizeof(double) * T2, hipMemcpyHostToDevice) == hipSuccess;
   cp &= hipMemcpy(b->tEpsCn, b->epsCn.data(), sizeof(double) * T2, hipMemcpyHostToDevice) == hipSuccess;
   cp &= hipMemcpy(b->tNHalf, b->nHalf.data(), sizeof(double) * T2, hipMemcpyHostToDevice) == hipSuccess;

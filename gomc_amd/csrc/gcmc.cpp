@@ -429,7 +429,7 @@ int TransferMove(gomc_box *b, double subDraw, int &mol, int &accepted, int &over
       z[p] = t.pos[p].z;
     }
     double enew = 0.0;
-    rc = gomc_swap_recip(b, kind, dst == 0 ? 1 : 0, x.data(), y.data(), z.data(), &enew);
+    rc = SwapRecipImpl(b, kind, dst == 0 ? 1 : 0, x.data(), y.data(), z.data(), m, &enew);
     if (rc) return rc;
     if (dst == 0)
       recipGain = enew - b->pot.recip;

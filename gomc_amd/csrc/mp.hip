@@ -407,8 +407,8 @@ namespace gomc {
 
 void MPFree(gomc_box *b) {
   MPState &mp = b->mp;
-  std::vector<void *> ptrs = {mp.xn, mp.yn, mp.zn, mp.inRange, mp.fail, mp.molStart,
-                              mp.chunks, mp.wpart, mp.wcounter};
+  std::vector<void *> ptrs = {mp.xn, mp.yn, mp.zn, mp.inRange, mp.fail, mp.chunks, mp.wpart,
+                              mp.wcounter};
   for (int s = 0; s < 2; ++s)
     for (int d = 0; d < 3; ++d) {
       ptrs.push_back(mp.com[s][d]);
@@ -445,17 +445,14 @@ int MPEnsure(gomc_box *b) {
   for (int d = 0; d < 3; ++d) ok &= hipMalloc(&mp.tk[d], nm) == hipSuccess;
   ok &= hipMalloc(&mp.inRange, sizeof(int) * std::max(b->nmols, 1)) == hipSuccess;
   ok &= hipMalloc(&mp.fail, sizeof(int)) == hipSuccess;
-  ok &= hipMalloc(&mp.molStart, sizeof(int) * (b->nmols + 1)) == hipSuccess;
+
   ok &= hipMalloc(&mp.wpart, sizeof(double) * ((b->nmols + kBlock - 1) / kBlock + 1)) == hipSuccess;
   ok &= hipMalloc(&mp.wcounter, sizeof(unsigned)) == hipSuccess;
   if (!ok) {
     MPFree(b);
     return Fail(b, GOMC_ERR_HIP, "MultiParticle allocation failed");
   }
-  std::vector<int> ms(b->nmols + 1);
-  for (int m = 0; m < b->nmols; ++m) ms[m] = b->molStart[m];
-  ms[b->nmols] = b->natoms;
-  HIP_TRY(b, hipMemcpy(mp.molStart, ms.data(), sizeof(int) * ms.size(), hipMemcpyHostToDevice));
+  mp.molStart = b->molStartDev;  // owned by the box
   HIP_TRY(b, hipMemset(mp.wcounter, 0, sizeof(unsigned)));
   mp.ready = true;
   return GOMC_OK;
@@ -601,6 +598,7 @@ int MPMolTerms(gomc_box *b, int set, int comSet) { return MolTerms(b, set, comSe
 static int MPTrial(gomc_box *b, int rotate, uint64_t step, double max, double out[5]) {
   MPState &mp = b->mp;
   const int R = mp.ref, N = 1 - mp.ref;
+  b->stageMol = -1;
   HIP_TRY(b, hipMemsetAsync(mp.fail, 0, sizeof(int), b->stream));
   TransformArgs t;
   t.g = b->geom;
@@ -701,7 +699,10 @@ static int MPAccept(gomc_box *b, bool result, bool evaluated, const double e[3])
     HIP_TRY(b, hipMemcpyAsync(b->comx.data(), mp.com[0][0], nm, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(b, hipMemcpyAsync(b->comy.data(), mp.com[0][1], nm, hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(b, hipMemcpyAsync(b->comz.data(), mp.com[0][2], nm, hipMemcpyDeviceToHost, b->stream));
-    return Sync(b);
+    int rc = Sync(b);
+    // every molecule moved: EwaldCached::BoxReciprocalSums refilled the cache
+    if (!rc && b->cached) rc = CacheBuild(b);
+    return rc;
   }
   if (evaluated) {
     std::swap(b->x, mp.xn);

@@ -137,9 +137,18 @@ int gomc_mol_recip(gomc_box *b, int mol, const double *nx, const double *ny,
                    const double *nz, double *enew);
 /* SwapDestRecip / SwapSourceRecip (Ewald.cpp:478-531, 657-710); replaces
  * CallSwapReciprocalGPU.  insert=1 adds, 0 removes the molecule of `kind`
- * at (x,y,z); *enew as above. */
-int gomc_swap_recip(gomc_box *b, int kind, int insert, const double *x,
+ * at (x,y,z); *enew as above.  `mol` is the molecule's index (-1 if none):
+ * with the molecule cache on, an inserted molecule's terms become its cache
+ * entry when the move is accepted (EwaldCached::SwapDestRecip,
+ * src/EwaldCached.cpp:300-352). */
+int gomc_swap_recip(gomc_box *b, int kind, int mol, int insert, const double *x,
                     const double *y, const double *z, double *enew);
+/* EwaldCached mode (src/EwaldCached.cpp, chosen by CachedFourier,
+ * src/System.cpp:132-148): enable=1 keeps each molecule's structure-factor
+ * terms [n_mols][n_kvec] in HBM, so a translate/rotate trial computes only
+ * the new positions' terms; the cache follows accepted moves.  enable=0
+ * frees it.  Energies are the same as without the cache. */
+int gomc_recip_cache(gomc_box *b, int enable);
 /* UpdateRecip (Ewald.cpp:1420-1434): the new sums become the reference
  * sums (a pointer swap); replaces UpdateRecipCUDA. */
 int gomc_recip_accept(gomc_box *b);
