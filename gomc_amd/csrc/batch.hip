@@ -252,16 +252,17 @@ __device__ __noinline__ void DrainQueue(const BatchBox &sb, const PairQueue &q, 
 }
 
 // Pair-energy change of the move (MoleculeInter, src/CalculateEnergy.cpp:
-// 581-686) in two levels.  Level 1: one molecule per thread; a molecule
-// whose centre is at least rc + 2 r_max from both the old and the new centre
-// of the moving molecule (minimum image) holds no atom pair inside the
-// cutoff (triangle inequality), so only the others ("near") are kept, in a
-// workgroup-wide list.  Level 2: one atom of a near molecule per work item,
-// tested against the old and new positions of all len moving atoms;
-// distances inside the cutoff are compacted into a per-wave queue (ballot +
-// prefix count) and their energies evaluated 64 at a time, so the erfc
-// path runs with full waves.  CX/CY/CZ may point to LDS or HBM.
-// acc = {lj, real, -, overlap} partial sums.
+// 581-686) in two levels.  Level 1 (boxes with sb.cull): a molecule whose
+// centre is at least rc + 2 r_max from both the old and the new centre of
+// the moving molecule (minimum image) holds no atom pair inside the cutoff
+// (triangle inequality), so only the others ("near") are listed: a count
+// pass, a workgroup prefix over the waves, a store pass -- a fixed order and
+// two barriers per move.  Level 2: one atom of a near molecule (of every
+// other molecule without the cull) per work item, tested against the old and
+// new positions of all len moving atoms; distances inside the cutoff are
+// compacted into a per-wave queue (ballot + prefix count) and their energies
+// evaluated 64 at a time, so the erfc path runs with full waves.
+// CX/CY/CZ may point to LDS or HBM.  acc = {lj, real, -, overlap}.
 template <typename T>
 __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms, const double *X,
                                        const double *Y, const double *Z, const double *CX,
@@ -279,71 +280,78 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int count = 0;  // wave-uniform
   double lj = 0.0, real = 0.0, ov = 0.0;
-  for (int base = 0; base < nmols; base += kBatchThreads) {
-    // ---- level 1: near molecules of this chunk, compacted in index order
-    const int j = base + int(threadIdx.x);
-    bool near = false;
-    if (j < nmols && j != m) {
-      const double cx = CX[j], cy = CY[j], cz = CZ[j];
-      const double d2o = DistSq(g, ms.com[0], ms.com[1], ms.com[2], cx, cy, cz);
-      const double d2n = DistSq(g, ms.newCom[0], ms.newCom[1], ms.newCom[2], cx, cy, cz);
-      near = d2o < sb.nearCutSq || d2n < sb.nearCutSq;
-    }
-    const uint64_t nmask = __ballot(near);
-    if (lane == 0) waveCnt[wave] = __popcll(nmask);
+  auto isNear = [&](int j) {
+    if (j >= nmols || j == m) return false;
+    const double cx = CX[j], cy = CY[j], cz = CZ[j];
+    const double d2o = DistSq(g, ms.com[0], ms.com[1], ms.com[2], cx, cy, cz);
+    const double d2n = DistSq(g, ms.newCom[0], ms.newCom[1], ms.newCom[2], cx, cy, cz);
+    return d2o < sb.nearCutSq || d2n < sb.nearCutSq;
+  };
+  int total = nmols - 1;
+  if (sb.cull) {
+    int wc = 0;
+    for (int base = 0; base < nmols; base += kBatchThreads)
+      wc += __popcll(__ballot(isNear(base + int(threadIdx.x))));
+    if (lane == 0) waveCnt[wave] = wc;
     __syncthreads();
-    int off = 0, total = 0;
+    int pos = 0;
+    total = 0;
     for (int w = 0; w < kBatchThreads / kWave; ++w) {
       const int c = waveCnt[w];
-      off += (w < wave) ? c : 0;
+      pos += (w < wave) ? c : 0;
       total += c;
     }
-    if (near) nearList[off + __popcll(nmask & below)] = j;
+    for (int base = 0; base < nmols; base += kBatchThreads) {
+      const int j = base + int(threadIdx.x);
+      const bool near = isNear(j);
+      const uint64_t nmask = __ballot(near);
+      if (near) nearList[pos + __popcll(nmask & below)] = j;
+      pos += __popcll(nmask);
+    }
     __syncthreads();
-    // ---- level 2: atoms of the near molecules against the moving atoms
-    const int items = total * len;
-    for (int ib = 0; ib < items; ib += kBatchThreads) {
-      const int it = ib + int(threadIdx.x);
-      const bool valid = it < items;
-      const int itc = valid ? it : 0;
-      const uint32_t k = uint32_t((uint64_t(uint32_t(itc)) * sb.magicLen) >> 32);  // itc / len
-      const int qa = itc - int(k) * len;
-      const int jj = nearList[k] * len + qa;
-      const double xj = X[jj], yj = Y[jj], zj = Z[jj];
-      for (int p = 0; p < len; ++p) {
-        const int pq = p * kBatchMaxAtoms + qa;
-        const bool live = valid && (sb.qq[pq] != 0.0 || sb.epsCn[pq] != 0.0);
+  }
+  const int items = total * len;
+  for (int ib = 0; ib < items; ib += kBatchThreads) {
+    const int it = ib + int(threadIdx.x);
+    const bool valid = it < items;
+    const int itc = valid ? it : 0;
+    const uint32_t k = uint32_t((uint64_t(uint32_t(itc)) * sb.magicLen) >> 32);  // itc / len
+    const int qa = itc - int(k) * len;
+    const int jm = sb.cull ? nearList[k] : (int(k) < m ? int(k) : int(k) + 1);
+    const int jj = jm * len + qa;
+    const double xj = X[jj], yj = Y[jj], zj = Z[jj];
+    for (int p = 0; p < len; ++p) {
+      const int pq = p * kBatchMaxAtoms + qa;
+      const bool live = valid && (sb.qq[pq] != 0.0 || sb.epsCn[pq] != 0.0);
 #pragma unroll
-        for (int isNew = 0; isNew < 2; ++isNew) {
-          const double px = isNew ? ms.newp[0][p] : ms.oldp[0][p];
-          const double py = isNew ? ms.newp[1][p] : ms.oldp[1][p];
-          const double pz = isNew ? ms.newp[2][p] : ms.oldp[2][p];
-          const double d2 = DistSq(g, px, py, pz, xj, yj, zj);
-          const bool inCut = valid && sb.rCutSq > d2;
-          if (isNew && inCut && d2 < sb.rCutLowSq) ov = 1.0;
-          const bool push = live && inCut;
-          const uint64_t mask = __ballot(push);
-          if (push) {
-            const int slot = count + __popcll(mask & below);
-            q.d2[slot] = d2;
-            q.code[slot] = pq | (isNew << 8);
-          }
-          count += __popcll(mask);
-          if (count >= kWave) {  // drain one full wave of pairs
-            __builtin_amdgcn_wave_barrier();
-            DrainQueue<T>(sb, q, kWave, &lj, &real);
-            const double d2n = q.d2[kWave + lane];
-            const int cn = q.code[kWave + lane];
-            __builtin_amdgcn_wave_barrier();
-            q.d2[lane] = d2n;
-            q.code[lane] = cn;
-            count -= kWave;
-            __builtin_amdgcn_wave_barrier();
-          }
+      for (int isNew = 0; isNew < 2; ++isNew) {
+        const double px = isNew ? ms.newp[0][p] : ms.oldp[0][p];
+        const double py = isNew ? ms.newp[1][p] : ms.oldp[1][p];
+        const double pz = isNew ? ms.newp[2][p] : ms.oldp[2][p];
+        const double d2 = DistSq(g, px, py, pz, xj, yj, zj);
+        const bool inCut = valid && sb.rCutSq > d2;
+        if (isNew && inCut && d2 < sb.rCutLowSq) ov = 1.0;
+        const bool push = live && inCut;
+        const uint64_t mask = __ballot(push);
+        if (push) {
+          const int slot = count + __popcll(mask & below);
+          q.d2[slot] = d2;
+          q.code[slot] = pq | (isNew << 8);
+        }
+        count += __popcll(mask);
+        if (count >= kWave) {  // drain one full wave of pairs
+          __builtin_amdgcn_wave_barrier();
+          DrainQueue<T>(sb, q, kWave, &lj, &real);
+          const double d2n = q.d2[kWave + lane];
+          const int cn = q.code[kWave + lane];
+          __builtin_amdgcn_wave_barrier();
+          q.d2[lane] = d2n;
+          q.code[lane] = cn;
+          count -= kWave;
+          __builtin_amdgcn_wave_barrier();
         }
       }
     }
-    __syncthreads();  // the near list is rebuilt for the next chunk
   }
   __builtin_amdgcn_wave_barrier();
   DrainQueue<T>(sb, q, count, &lj, &real);
@@ -423,7 +431,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   __shared__ double red[kBatchThreads / kWave][4];
   __shared__ BatchState st;  // running energies and move settings
   __shared__ PairQueue queues[kBatchThreads / kWave];
-  __shared__ int nearList[kBatchThreads];
+  __shared__ int nearList[kBatchNearCap];
   __shared__ int waveCnt[kBatchThreads / kWave];
 
   const int b = blockIdx.x, tid = threadIdx.x;

@@ -14,7 +14,7 @@
 namespace gomc {
 
 constexpr int kBatchThreads = 1024;  // one workgroup (16 waves) per box
-constexpr int kBatchMaxAtoms = 8;    // atoms per molecule
+constexpr int kBatchMaxAtoms = 4;    // atoms per molecule (water, CO2, TIP4P)
 constexpr int kBatchMaxTypes = 4;    // LJ atom types per box
 constexpr int kBatchMaxN = 32;       // largest |n| of a k-vector component
 constexpr int kDrawChunk = 64;       // moves whose Philox draws are made at once
@@ -23,6 +23,9 @@ constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of every box strea
 // tables, pair queues and near list (160 KiB per CU on gfx950, ~60 KiB
 // static): 4096 x 24 B.
 constexpr int kBatchLdsMolCap = 4096;
+// Capacity of the per-move list of molecules near the moving one; boxes of
+// more molecules skip the centre cull.
+constexpr int kBatchNearCap = 4096;
 
 // Read-only description of one box (host-filled, 8-byte aligned fields).
 struct BatchBox {
@@ -42,7 +45,8 @@ struct BatchBox {
   uint64_t seed;
   uint64_t magicLen, magicLen2;       // ceil(2^32 / len), ceil(2^32 / len^2)
   int natoms, nmols, molLen, K;
-  int nmax[3], ewald, elect, pad;
+  int nmax[3], ewald, elect;
+  int cull;  // 1: cull far molecules by their centres before atom pairs
 };
 
 static_assert(sizeof(BatchBox) % 8 == 0, "BatchBox is copied as 64-bit words");

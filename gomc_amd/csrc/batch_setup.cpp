@@ -95,6 +95,9 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
   }
   const double nearCut = std::sqrt(b->geom.rCutSq) + 2.0 * rmax + 1e-6;
   P.nearCutSq = nearCut * nearCut;
+  // the cull pays when the near sphere is well inside the box
+  P.cull = (b->nmols <= kBatchNearCap &&
+            (4.0 / 3.0) * M_PI * nearCut * nearCut * nearCut < 0.5 * b->volume) ? 1 : 0;
   P.rCutSq = b->geom.rCutSq;
   P.rCutLJSq = b->geom.rCutLJSq;
   P.rCutCoulSq = b->geom.rCutCoulSq;

@@ -24,11 +24,13 @@ def dE_close(a, b, total, rtol=1e-10):
 pytestmark = pytest.mark.gpu
 
 
-def test_batch_box_matches_oracle_philox():
-    s = systems.water_box(300, seed=31, adj_steps=500)
+@pytest.mark.parametrize("nmol,n", [(300, 6000), (1500, 4000)])
+def test_batch_box_matches_oracle_philox(nmol, n):
+    # 300 molecules: every other molecule is paired; 1500: the centre cull
+    # lists only molecules within rc + 2 r_max of the moving one
+    s = systems.water_box(nmol, seed=31, adj_steps=500)
     o = Oracle(s, threads=8)
     e0 = o.energies()
-    n = 6000
     bt = Batch([s])
     bt.set_trace(0, n)
     bt.run([n])
