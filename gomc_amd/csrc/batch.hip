@@ -6,9 +6,7 @@
 // src/CalculateEnergy.cpp:581-686), the structure-factor change per
 // k-vector (MolReciprocal, src/Ewald.cpp:409-473), the Metropolis test
 // (src/moves/Translate.h:97-138) and, on acceptance, the update of
-// coordinates and S(k).  S(k) of the box's k-vectors stays in registers for
-// the whole launch (KPT k-vectors per thread), so a move reads only the
-// read-only k data and the coordinates.
+// coordinates and S(k).
 //
 // Phases e^{i k.r} come from per-dimension tables e^{i (c_d n) r_d},
 // n = 0..nmax, built each move for the moving molecule's old and new
@@ -18,6 +16,21 @@
 
 namespace gomc {
 namespace {
+
+// Pointers into global memory (HBM) and LDS.  The phases are out-of-line
+// functions; a plain pointer parameter there is a generic (flat) address,
+// and every flat access also counts against the LDS wait counter, so a
+// wait for an LDS read would stall on outstanding HBM loads.
+#define GMEM __attribute__((address_space(1)))
+#define LMEM __attribute__((address_space(3)))
+template <typename T>
+__device__ __forceinline__ GMEM T *Gp(T *p) {
+  return (GMEM T *)p;
+}
+template <typename T>
+__device__ __forceinline__ LMEM T *Lp(T *p) {
+  return (LMEM T *)p;
+}
 
 // Phase-table entries: FP64 in the FP64 mode, FP32 in the mixed mode.
 template <typename T>
@@ -69,42 +82,6 @@ __device__ __forceinline__ void BlockSum4(double v[4], double (*lds)[4]) {
   }
 }
 
-// Structure-factor change of one k-vector for the moving molecule: sums of
-// q e^{i k.r} over its new and old positions, from the phase tables.
-template <typename T>
-__device__ __forceinline__ void KDelta(const CplxT<T> (*tab)[3][kBatchMaxN + 1], const double *charge,
-                                       int len, int packed, double &dR, double &dI) {
-  const int nx = packed & 0xff;
-  const int ny = ((packed >> 8) & 0xff) - 128;
-  const int nz = ((packed >> 16) & 0xff) - 128;
-  const int ay = ny < 0 ? -ny : ny, az = nz < 0 ? -nz : nz;
-  const T sy = ny < 0 ? T(-1) : T(1), sz = nz < 0 ? T(-1) : T(1);
-  T sRN = 0, sIN = 0, sRO = 0, sIO = 0;
-  for (int p = 0; p < len; ++p) {
-    const double qd = charge[p];
-    if (fabs(qd) < 0.000000001) continue;
-    const T q = T(qd);
-    {
-      CplxT<T> ex = tab[len + p][0][nx], ey = tab[len + p][1][ay], ez = tab[len + p][2][az];
-      ey.im *= sy;
-      ez.im *= sz;
-      const CplxT<T> e = CMul(CMul(ex, ey), ez);
-      sRN += q * e.re;
-      sIN += q * e.im;
-    }
-    {
-      CplxT<T> ex = tab[p][0][nx], ey = tab[p][1][ay], ez = tab[p][2][az];
-      ey.im *= sy;
-      ez.im *= sz;
-      const CplxT<T> e = CMul(CMul(ex, ey), ez);
-      sRO += q * e.re;
-      sIO += q * e.im;
-    }
-  }
-  dR = double(sRN - sRO);
-  dI = double(sIN - sIO);
-}
-
 // One phase-table entry: e^{i (c_d n) r} (kept out of line: the FP64
 // sincos argument reduction is register-hungry and runs once per entry).
 template <typename T>
@@ -128,7 +105,7 @@ __device__ __forceinline__ CplxT<T> PhaseEntry(double kc, int n, double r) {
 // rotation matrix (out of line; it runs on one lane).
 __device__ __noinline__ void Propose(const BatchBox &sb, BatchState &st, MoveShared &ms,
                                      const double *u, int64_t step, const double *cx,
-                                     const double *cy, const double *cz, int64_t moff) {
+                                     const double *cy, const double *cz) {
   const double lmin = fmin(sb.L[0], fmin(sb.L[1], sb.L[2]));
   // MoveSettings::AdjustMoves (src/MoveSettings.cpp:148-258).
   if ((uint64_t(step) + 1) % uint64_t(sb.perAdjust) == 0) {
@@ -152,9 +129,9 @@ __device__ __noinline__ void Propose(const BatchBox &sb, BatchState &st, MoveSha
   if (sb.molLen <= 1) t = 0;
   ms.mol = m;
   ms.type = t;
-  ms.com[0] = cx[moff + m];
-  ms.com[1] = cy[moff + m];
-  ms.com[2] = cz[moff + m];
+  ms.com[0] = cx[m];
+  ms.com[1] = cy[m];
+  ms.com[2] = cz[m];
   if (t == 0) {
     const double bb = st.scale[0], b2 = 2.0 * bb;
     ms.shift[0] = u[2] * b2 - bb;
@@ -243,12 +220,17 @@ __device__ __forceinline__ void PairTerm(const BatchBox &sb, double d2, int code
   }
 }
 
-// Evaluates `n` queued pairs (one per lane) and adds them to lj/real.
+struct LjReal {
+  double lj, real, ov;
+};
+
+// Evaluates `n` queued pairs (one per lane) and adds them to acc.
 template <typename T>
-__device__ __noinline__ void DrainQueue(const BatchBox &sb, const PairQueue &q, int n,
-                                        double *lj, double *real) {
+__device__ __noinline__ LjReal DrainQueue(const BatchBox &sb, const LMEM PairQueue *q, int n,
+                                          LjReal acc) {
   const int lane = threadIdx.x & (kWave - 1);
-  if (lane < n) PairTerm<T>(sb, q.d2[lane], q.code[lane], *lj, *real);
+  if (lane < n) PairTerm<T>(sb, q->d2[lane], q->code[lane], acc.lj, acc.real);
+  return acc;
 }
 
 // Pair-energy change of the move (MoleculeInter, src/CalculateEnergy.cpp:
@@ -262,13 +244,14 @@ __device__ __noinline__ void DrainQueue(const BatchBox &sb, const PairQueue &q, 
 // new positions of all len moving atoms; distances inside the cutoff are
 // compacted into a per-wave queue (ballot + prefix count) and their energies
 // evaluated 64 at a time, so the erfc path runs with full waves.
-// CX/CY/CZ may point to LDS or HBM.  acc = {lj, real, -, overlap}.
-template <typename T>
-__device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms, const double *X,
-                                       const double *Y, const double *Z, const double *CX,
-                                       const double *CY, const double *CZ, int m, int len,
-                                       int nmols, PairQueue *queues, int *nearList, int *waveCnt,
-                                       double *acc4) {
+// CX/CY/CZ point to LDS or HBM (ComPtr).  Returns this thread's {lj, real,
+// overlap}: overlap = 1 when a new position lies inside rCutLow of an atom.
+template <typename T, typename ComPtr>
+__device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &ms,
+                                         const GMEM double *X, const GMEM double *Y,
+                                         const GMEM double *Z, ComPtr CX, ComPtr CY, ComPtr CZ,
+                                         int m, int len, int nmols, PairQueue *queues,
+                                         int *nearList, int *waveCnt) {
   BoxGeom g;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -279,7 +262,7 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
   PairQueue &q = queues[wave];
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int count = 0;  // wave-uniform
-  double lj = 0.0, real = 0.0, ov = 0.0;
+  LjReal acc{0.0, 0.0, 0.0};
   auto isNear = [&](int j) {
     if (j >= nmols || j == m) return false;
     const double cx = CX[j], cy = CY[j], cz = CZ[j];
@@ -330,7 +313,7 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
         const double pz = isNew ? ms.newp[2][p] : ms.oldp[2][p];
         const double d2 = DistSq(g, px, py, pz, xj, yj, zj);
         const bool inCut = valid && sb.rCutSq > d2;
-        if (isNew && inCut && d2 < sb.rCutLowSq) ov = 1.0;
+        if (isNew && inCut && d2 < sb.rCutLowSq) acc.ov = 1.0;
         const bool push = live && inCut;
         const uint64_t mask = __ballot(push);
         if (push) {
@@ -341,7 +324,7 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
         count += __popcll(mask);
         if (count >= kWave) {  // drain one full wave of pairs
           __builtin_amdgcn_wave_barrier();
-          DrainQueue<T>(sb, q, kWave, &lj, &real);
+          acc = DrainQueue<T>(sb, Lp(&q), kWave, acc);
           const double d2n = q.d2[kWave + lane];
           const int cn = q.code[kWave + lane];
           __builtin_amdgcn_wave_barrier();
@@ -354,40 +337,186 @@ __device__ __noinline__ void PairPhase(const BatchBox &sb, const MoveShared &ms,
     }
   }
   __builtin_amdgcn_wave_barrier();
-  DrainQueue<T>(sb, q, count, &lj, &real);
-  acc4[0] = lj;
-  acc4[1] = real;
-  acc4[3] = ov;
+  return DrainQueue<T>(sb, Lp(&q), count, acc);
 }
 
 // Reciprocal-space energy change of the move (MolReciprocal,
 // src/Ewald.cpp:409-473): per k-vector the change (dR, dI) of the
 // structure factor and prefact * ((R+dR)^2 + (I+dI)^2 - R^2 - I^2),
 // written as prefact * (dR (2R + dR) + dI (2I + dI)) so no large terms
-// cancel.  (dR, dI) are kept for the commit.  Returns this thread's sum.
+// cancel.  (dR, dI) are kept in dRs/dIs for the commit.
+//
+// A thread takes one segment of S k-vectors per round: consecutive nz at
+// fixed (nx, ny).  The phase of each of the 2 LEN positions is formed from
+// the tables at the segment start, (ex * ey) * ez, and advanced along nz by
+// one complex product with e^{i c_z z} per step, so a k-vector costs no
+// table reads after the first of its segment.  The loads of two (FP64) or
+// four (mixed mode) steps are issued before they are used.
+// pend: the previous move was accepted and its (dR, dI), still in the
+// thread's slots, are added to S(k) here (Ewald::UpdateRecip,
+// src/Ewald.cpp:1420) before the slots are reused.  Returns this thread's
+// energy sum.  Unused slots (kn -1, pref 0, S(k) 0) add zero and are not
+// written.
+constexpr int kZeroCode = (128 << 8) | (128 << 16);  // n = (0, 0, 0)
+
 template <typename T>
+__device__ __forceinline__ CplxT<T> TablePhase(const CplxT<T> (*row)[kBatchMaxN + 1], int code) {
+  const int nx = code & 0xff;
+  const int ny = ((code >> 8) & 0xff) - 128;
+  const int nz = ((code >> 16) & 0xff) - 128;
+  CplxT<T> ey = row[1][ny < 0 ? -ny : ny], ez = row[2][nz < 0 ? -nz : nz];
+  if (ny < 0) ey.im = -ey.im;
+  if (nz < 0) ez.im = -ez.im;
+  return CMul(CMul(row[0][nx], ey), ez);
+}
+
+// A value every lane holds alike, moved to scalar registers.
+__device__ __forceinline__ double Uniform(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(u));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(u >> 32));
+  return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+__device__ __forceinline__ float Uniform(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, v)));
+}
+
+template <typename T, int S, int LEN>
 __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*tab)[3][kBatchMaxN + 1],
-                                          int len, int K, const int32_t *kn, const double *pref,
-                                          const double *sumR, const double *sumI, double *dRs,
-                                          double *dIs) {
+                                          const GMEM int32_t *__restrict__ kn,
+                                          const GMEM double *__restrict__ pref,
+                                          GMEM double *__restrict__ sumR, GMEM double *__restrict__ sumI,
+                                          GMEM double *__restrict__ dRs, GMEM double *__restrict__ dIs,
+                                          bool pend) {
+  // steps whose loads are issued together (FP64 phases leave room for two)
+  constexpr int GMax = sizeof(T) == 8 ? 2 : 4;
+  constexpr int G = S < GMax ? S : GMax;
+  T q[LEN];
+  CplxT<T> stepO[LEN], stepN[LEN];
+#pragma unroll
+  for (int p = 0; p < LEN; ++p) {  // the same for every lane: kept in scalar registers
+    q[p] = Uniform(fabs(sb.charge[p]) < 0.000000001 ? T(0) : T(sb.charge[p]));
+    const CplxT<T> so = tab[p][2][1], sn = tab[LEN + p][2][1];
+    stepO[p] = {Uniform(so.re), Uniform(so.im)};
+    stepN[p] = {Uniform(sn.re), Uniform(sn.im)};
+  }
   double e = 0.0;
-  for (int k = threadIdx.x; k < K; k += kBatchThreads) {
-    double dR, dI;
-    KDelta(tab, sb.charge, len, kn[k], dR, dI);
-    const double r0 = sumR[k], i0 = sumI[k];
-    dRs[k] = dR;
-    dIs[k] = dI;
-    e += pref[k] * (dR * (2.0 * r0 + dR) + dI * (2.0 * i0 + dI));
+  for (int r = 0; r < sb.kChunk; ++r) {
+    CplxT<T> eO[LEN], eN[LEN];
+#pragma unroll 1
+    for (int jb = 0; jb < S; jb += G) {
+      int code[G];
+      double r0[G], i0[G], pf[G];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int s = ((r * S) + jb + g) * kBatchThreads + int(threadIdx.x);
+        code[g] = kn[s];
+        r0[g] = sumR[s];
+        i0[g] = sumI[s];
+        pf[g] = pref[s];
+      }
+      if (pend) {
+        double ar[G], ai[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          const int s = ((r * S) + jb + g) * kBatchThreads + int(threadIdx.x);
+          ar[g] = dRs[s];
+          ai[g] = dIs[s];
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          // unused slots hold no change (their dRs/dIs are never written)
+          if (code[g] >= 0) {
+            const int s = ((r * S) + jb + g) * kBatchThreads + int(threadIdx.x);
+            r0[g] += ar[g];
+            i0[g] += ai[g];
+            sumR[s] = r0[g];
+            sumI[s] = i0[g];
+          }
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int j = jb + g;
+        const bool used = code[g] >= 0;
+        if (j == 0) {
+          const int c0 = used ? code[g] : kZeroCode;
+          // one position at a time: the scheduler would otherwise hoist all
+          // 6 LEN table reads and run out of registers
+#pragma unroll
+          for (int p = 0; p < LEN; ++p) {
+            eO[p] = TablePhase<T>(tab[p], c0);
+            __builtin_amdgcn_sched_barrier(0);
+            eN[p] = TablePhase<T>(tab[LEN + p], c0);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        } else {
+#pragma unroll
+          for (int p = 0; p < LEN; ++p) {
+            eO[p] = CMul(eO[p], stepO[p]);
+            eN[p] = CMul(eN[p], stepN[p]);
+          }
+        }
+        T sRN = 0, sIN = 0, sRO = 0, sIO = 0;
+#pragma unroll
+        for (int p = 0; p < LEN; ++p) {
+          sRN += q[p] * eN[p].re;
+          sIN += q[p] * eN[p].im;
+          sRO += q[p] * eO[p].re;
+          sIO += q[p] * eO[p].im;
+        }
+        const double dR = used ? double(sRN - sRO) : 0.0;
+        const double dI = used ? double(sIN - sIO) : 0.0;
+        if (used) {
+          const int s = ((r * S) + j) * kBatchThreads + int(threadIdx.x);
+          dRs[s] = dR;
+          dIs[s] = dI;
+        }
+        e += pf[g] * (dR * (2.0 * r0[g] + dR) + dI * (2.0 * i0[g] + dI));
+      }
+    }
   }
   return e;
 }
 
-// Accepted move: S(k) += (dR, dI) (Ewald::UpdateRecip, src/Ewald.cpp:1420).
-__device__ __noinline__ void RecipCommit(int K, double *sumR, double *sumI, const double *dRs,
-                                         const double *dIs) {
-  for (int k = threadIdx.x; k < K; k += kBatchThreads) {
-    sumR[k] += dRs[k];
-    sumI[k] += dIs[k];
+template <typename T, int S>
+__device__ __forceinline__ double RecipPhaseLen(const BatchBox &sb,
+                                                const CplxT<T> (*tab)[3][kBatchMaxN + 1], int len,
+                                                const GMEM int32_t *kn, const GMEM double *pref,
+                                                GMEM double *sumR, GMEM double *sumI, GMEM double *dRs,
+                                                GMEM double *dIs, bool pend) {
+  switch (len) {
+    case 1:
+      return RecipPhase<T, S, 1>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+    case 2:
+      return RecipPhase<T, S, 2>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+    case 3:
+      return RecipPhase<T, S, 3>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+    default:
+      return RecipPhase<T, S, 4>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ double RecipDispatch(const BatchBox &sb,
+                                                const CplxT<T> (*tab)[3][kBatchMaxN + 1], int len,
+                                                const GMEM int32_t *kn, const GMEM double *pref,
+                                                GMEM double *sumR, GMEM double *sumI, GMEM double *dRs,
+                                                GMEM double *dIs, bool pend) {
+  if (sb.kSeg == 8) return RecipPhaseLen<T, 8>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, pend);
+  if (sb.kSeg == 4) return RecipPhaseLen<T, 4>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, pend);
+  return RecipPhaseLen<T, 1>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, pend);
+}
+
+// Adds a pending accepted change to S(k) at the end of a launch.
+__device__ __noinline__ void RecipFold(int nslots, const GMEM int32_t *kn, GMEM double *sumR,
+                                       GMEM double *sumI, const GMEM double *dRs,
+                                       const GMEM double *dIs) {
+  for (int s = int(threadIdx.x); s < nslots; s += kBatchThreads) {
+    if (kn[s] >= 0) {
+      sumR[s] += dRs[s];
+      sumI[s] += dIs[s];
+    }
   }
 }
 
@@ -441,7 +570,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     for (int i = tid; i < int(sizeof(BatchBox) / 8); i += kBatchThreads) dst[i] = src[i];
   }
   __syncthreads();
-  const int len = sb.molLen, nmols = sb.nmols, K = sb.K;
+  const int len = sb.molLen;
   const int64_t aoff = sb.atomOff, moff = sb.molOff, koff = sb.kOff;
   double *X = a.x + aoff, *Y = a.y + aoff, *Z = a.z + aoff;
   // The box's molecule centres live in LDS for the launch when they fit
@@ -467,6 +596,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   const int64_t nMoves = a.moves[b];
   const int nTab = 2 * len * ((sb.nmax[0] + 1) + (sb.nmax[1] + 1) + (sb.nmax[2] + 1));
 
+  bool pend = false;  // the last move was accepted and S(k) awaits its change
   for (int64_t mv = 0; mv < nMoves; ++mv, ++step) {
     // ---- Philox draws for the next kDrawChunk moves ----
     if ((mv % kDrawChunk) == 0) {
@@ -476,7 +606,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     const double *u = draws[mv % kDrawChunk];
 
     // ---- proposal (thread 0): move settings, move kind, molecule ----
-    if (tid == 0) Propose(sb, st, ms, u, step, a.cx, a.cy, a.cz, moff);
+    if (tid == 0) Propose(sb, st, ms, u, step, CXr, CYr, CZr);
     __syncthreads();
     const int m = ms.mol, mtype = ms.type;
     if (tid < len) {
@@ -509,22 +639,31 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
 
     // ---- pair phase ----
     double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
-    PairPhase<T>(sb, ms, X, Y, Z, CXr, CYr, CZr, m, len, sb.nmols, queues, nearList, waveCnt,
-                 acc4);
+    LjReal pr;
+    if (inLds)
+      pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Lp(CXr), Lp(CYr), Lp(CZr), m, len, sb.nmols,
+                        queues, nearList, waveCnt);
+    else
+      pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Gp(CXr), Gp(CYr), Gp(CZr), m, len, sb.nmols,
+                        queues, nearList, waveCnt);
+    acc4[0] = pr.lj;
+    acc4[1] = pr.real;
+    acc4[3] = pr.ov;
     __syncthreads();  // tables complete
 
     // ---- reciprocal phase: energy change per k-vector ----
-    if (sb.ewald) acc4[2] = RecipPhase<T>(sb, tab, len, K, kn, pref, a.sumR + koff, a.sumI + koff,
-                                       a.dR + koff, a.dI + koff);
+    if (sb.ewald)
+      acc4[2] = RecipDispatch<T>(sb, tab, len, Gp(kn), Gp(pref), Gp(a.sumR + koff),
+                                 Gp(a.sumI + koff), Gp(a.dR + koff), Gp(a.dI + koff), pend);
     BlockSum4(acc4, red);
 
     // ---- Metropolis (thread 0) ----
     if (tid == 0) Metropolis(sb, st, ms, acc4, u[5], mtype, (b == a.trace.box) ? mv : -1, a.trace);
     __syncthreads();
 
-    // ---- commit ----
+    // ---- commit (S(k) in the next move's reciprocal phase) ----
+    pend = ms.accept != 0;
     if (ms.accept) {
-      if (sb.ewald) RecipCommit(K, a.sumR + koff, a.sumI + koff, a.dR + koff, a.dI + koff);
       if (tid < len) {
         const int64_t ai = int64_t(m) * len + tid;
         X[ai] = ms.newp[0][tid];
@@ -545,6 +684,9 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     __syncthreads();
   }
 
+  if (pend && sb.ewald)
+    RecipFold(sb.K, Gp(kn), Gp(a.sumR + koff), Gp(a.sumI + koff), Gp(a.dR + koff),
+              Gp(a.dI + koff));
   if (tid == 0) {
     st.step = step;
     a.state[b] = st;

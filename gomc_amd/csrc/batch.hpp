@@ -44,7 +44,10 @@ struct BatchBox {
   int64_t perAdjust;
   uint64_t seed;
   uint64_t magicLen, magicLen2;       // ceil(2^32 / len), ceil(2^32 / len^2)
-  int natoms, nmols, molLen, K;
+  int natoms, nmols, molLen;
+  int K;       // k-vector slots of the box: kChunk * kSeg * kBatchThreads
+  int kChunk;  // rounds: segments per thread
+  int kSeg;    // S: k-vectors per segment (1, 4 or 8)
   int nmax[3], ewald, elect;
   int cull;  // 1: cull far molecules by their centres before atom pairs
 };
@@ -72,10 +75,12 @@ struct BatchArgs {
   BatchState *state;
   double *x, *y, *z;          // [sum atoms]
   double *cx, *cy, *cz;       // [sum mols] centres of mass
-  const int32_t *kn;          // [sum K] packed (nx, ny+128, nz+128) bytes
-  const double *pref;         // [sum K]
-  double *sumR, *sumI;        // [sum K] reference structure factors
-  double *dR, *dI;            // [sum K] structure-factor change of the trial
+  // k-vector arrays, [sum K] slots; per box, segments of S consecutive nz
+  // at fixed (nx, ny): thread t, round r, step j at slot (r*S + j)*1024 + t
+  const int32_t *kn;          // packed (nx, ny+128, nz+128) bytes, -1 unused slot
+  const double *pref;
+  double *sumR, *sumI;        // reference structure factors
+  double *dR, *dI;            // structure-factor change of the last trial
   const int64_t *moves;       // [nboxes] moves to make in this launch
   int ldsMols;                // per-box centre capacity of dynamic LDS
   BatchTrace trace;

@@ -150,11 +150,60 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     std::vector<double> sr(b->K), si(b->K);
     rc = gomc_recip_get(b, nullptr, nullptr, nullptr, nullptr, sr.data(), si.data());
     if (rc || int(pref.size()) != b->K) return BFail(bt, GOMC_ERR_STATE, "k-vector setup mismatch");
-    P.K = b->K;
-    out.pref = pref;
-    out.kn = packed;
-    out.sr = sr;
-    out.si = si;
+    // Segments: each run of consecutive nz at fixed (nx, ny) (contiguous
+    // in the reference order) is cut into pieces of at most S k-vectors;
+    // thread t takes segment r * kBatchThreads + t in round r, its step j
+    // at slot (r * S + j) * kBatchThreads + t.  Unused slots carry kn -1.
+    std::vector<int> runStart, runLen;
+    for (int k = 0; k < b->K; ++k) {
+      if (k > 0 && packed[k] == packed[k - 1] + (1 << 16)) {
+        ++runLen.back();
+      } else {
+        runStart.push_back(k);
+        runLen.push_back(1);
+      }
+    }
+    // per round a thread pays about three table products to start a
+    // segment and one complex product per step
+    int bestS = 1;
+    int64_t bestCost = -1;
+    for (int S : {1, 4, 8}) {
+      int64_t segs = 0;
+      for (int L : runLen) segs += (L + S - 1) / S;
+      const int64_t rounds = (segs + kBatchThreads - 1) / kBatchThreads;
+      const int64_t cost = rounds * (S + 3);
+      if (bestCost < 0 || cost < bestCost) {
+        bestCost = cost;
+        bestS = S;
+      }
+    }
+    const int S = bestS;
+    int64_t segs = 0;
+    for (int L : runLen) segs += (L + S - 1) / S;
+    const int64_t rounds = (segs + kBatchThreads - 1) / kBatchThreads;
+    if (rounds * S * kBatchThreads > (int64_t(1) << 30))
+      return BFail(bt, GOMC_ERR_ARG, "too many k-vectors for the batched layout");
+    P.kSeg = S;
+    P.kChunk = int(rounds);
+    P.K = int(rounds * S * kBatchThreads);
+    out.pref.assign(P.K, 0.0);
+    out.kn.assign(P.K, -1);
+    out.sr.assign(P.K, 0.0);
+    out.si.assign(P.K, 0.0);
+    int64_t seg = 0;
+    for (size_t r = 0; r < runLen.size(); ++r) {
+      for (int off = 0; off < runLen[r]; off += S, ++seg) {
+        const int64_t round = seg / kBatchThreads, t = seg % kBatchThreads;
+        for (int j = 0; j < S && off + j < runLen[r]; ++j) {
+          const int k = runStart[r] + off + j;
+          const int64_t slot = (round * S + j) * kBatchThreads + t;
+          out.pref[slot] = pref[k];
+          out.kn[slot] = packed[k];
+          out.sr[slot] = sr[k];
+          out.si[slot] = si[k];
+        }
+      }
+    }
   }
   BatchState &S = out.S;
   std::memset(&S, 0, sizeof(S));
@@ -214,6 +263,11 @@ int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t 
   ok &= alloc((void **)&bt->dR, sizeof(double) * kvec);
   ok &= alloc((void **)&bt->dI, sizeof(double) * kvec);
   if (!ok) return BFail(bt, GOMC_ERR_HIP, "batch allocation failed");
+  // the structure-factor changes start defined (unused k slots are read)
+  if (kvec > 0) {
+    BHIP(bt, hipMemsetAsync(bt->dR, 0, sizeof(double) * kvec, bt->stream));
+    BHIP(bt, hipMemsetAsync(bt->dI, 0, sizeof(double) * kvec, bt->stream));
+  }
   bt->totalAtoms = atoms;
   bt->totalMols = mols;
   bt->totalK = kvec;

@@ -267,11 +267,12 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
   sv->nslots = slots;
   sv->tickMs = tick_ms;
   sv->budget = budget_bytes;
-  // split the budget in the proportions of a water box (3 atoms and about
-  // 3.1 k-vectors per molecule)
-  sv->atomCap = int64_t(0.31 * budget_bytes) / kAtomBytes;
-  sv->molCap = int64_t(0.104 * budget_bytes) / kMolBytes;
-  sv->kCap = int64_t(0.586 * budget_bytes) / kKBytes;
+  // split the budget in the proportions of a water box: 3 atoms and about
+  // 3.1 k-vectors per molecule, k-vector slots rounded up to whole chunks
+  // of 1024 (about 1.4 slots per k-vector over 200-2000 molecules)
+  sv->atomCap = int64_t(0.25 * budget_bytes) / kAtomBytes;
+  sv->molCap = int64_t(0.085 * budget_bytes) / kMolBytes;
+  sv->kCap = int64_t(0.665 * budget_bytes) / kKBytes;
   sv->atoms = Arena(sv->atomCap);
   sv->mols = Arena(sv->molCap);
   sv->kvec = Arena(sv->kCap);

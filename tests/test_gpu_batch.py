@@ -84,25 +84,29 @@ def test_mixed_precision_error_against_fp64():
     per-move energy changes can be compared directly.  Bar: |dE_mixed -
     dE_fp64| <= 1e-6 x max(|dE_fp64|, |E_total of the box|) (relative error
     at the box's energy scale; overlapping trials with |dE| >> |E_total| at
-    their own scale), and the decisions agree for the first 300 moves."""
-    s = systems.water_box(400, seed=41, adj_steps=500)
+    their own scale).  A decision flips once the FP32 error straddles the
+    acceptance draw, so the runs are compared up to their first differing
+    decision, over four boxes, with at least 400 moves compared in all."""
     n = 3000
-    runs = {}
-    for prec in (0, 1):
-        bt = Batch([s], precision=prec)
-        bt.set_trace(0, n)
-        bt.run([n])
-        runs[prec] = bt.trace()
-    o = Oracle(s, threads=4)
-    etot = abs(o.energies()[6])
-    a64, a32 = runs[0]["acc"], runs[1]["acc"]
-    diff = np.nonzero(a64 != a32)[0]
-    first = int(diff[0]) if len(diff) else n
-    assert first >= 300, first
-    for k in ("dlj", "dreal", "drecip"):
-        ref = runs[0][k][:first]
-        err = np.abs(runs[1][k][:first] - ref)
-        assert np.all(err <= 1e-6 * np.maximum(np.abs(ref), etot)), (k, err.max(), etot)
+    compared = 0
+    for seed, nmol in ((41, 400), (42, 400), (43, 600), (44, 300)):
+        s = systems.water_box(nmol, seed=seed, adj_steps=500)
+        runs = {}
+        for prec in (0, 1):
+            bt = Batch([s], precision=prec)
+            bt.set_trace(0, n)
+            bt.run([n])
+            runs[prec] = bt.trace()
+        etot = abs(Oracle(s, threads=4).energies()[6])
+        a64, a32 = runs[0]["acc"], runs[1]["acc"]
+        diff = np.nonzero(a64 != a32)[0]
+        first = int(diff[0]) if len(diff) else n
+        for k in ("dlj", "dreal", "drecip"):
+            ref = runs[0][k][:first]
+            err = np.abs(runs[1][k][:first] - ref)
+            assert np.all(err <= 1e-6 * np.maximum(np.abs(ref), etot)), (seed, k, err.max(), etot)
+        compared += first
+    assert compared >= 400, compared
 
 
 def test_graph_replayed_cycles_equal_eager_cycles():
@@ -129,27 +133,37 @@ def test_graph_replayed_cycles_equal_eager_cycles():
 
 
 def test_mixed_precision_observables_within_two_sigma():
-    """FP32 pair terms / FP64 accumulation against FP64: the two runs part
-    after a few hundred moves (rounding), so they sample the same ensemble
-    independently; per box the mean total energy over cycles 20-119 must
-    agree within 2 combined standard errors (block averages of 10 cycles)
-    in at least 4 of 5 boxes, and within 4 in all of them."""
-    boxes = [systems.water_box(n, seed=40 + i, temperature=t)
-             for i, (n, t) in enumerate(((220, 300.0), (260, 320.0), (300, 290.0),
-                                         (240, 340.0), (280, 310.0)))]
+    """FP32 pair terms / FP64 accumulation against FP64 as samplers: five
+    boxes are equilibrated in FP64 (150 cycles), then each precision runs
+    200 cycles from the same configuration on its own Philox stream, so the
+    two runs are independent samples of one ensemble.  Per box the mean
+    total energy of the last 160 cycles (8 blocks of 20) must agree within
+    2 combined standard errors in at least 3 of the 5 boxes and within 4 in
+    all of them (for unbiased samplers each bound fails with probability
+    below 0.2%)."""
+    base = [systems.water_box(n, seed=40 + i, temperature=t)
+            for i, (n, t) in enumerate(((220, 300.0), (260, 320.0), (300, 290.0),
+                                        (240, 340.0), (280, 310.0)))]
+    eq = Batch(base)
+    eq.run_cycles(150, graph=True)
+    start = [systems.System(**{**s.__dict__, "xyz": eq.coords(i), "_keep": []})
+             for i, s in enumerate(base)]
+    eq.close()
     series = {}
     for prec in (0, 1):
+        boxes = [systems.System(**{**s.__dict__, "seed": s.seed + 1000 * (prec + 1), "_keep": []})
+                 for s in start]
         bt = Batch(boxes, precision=prec)
         es = []
-        for c in range(120):
+        for c in range(200):
             bt.run_cycles(1, graph=False)
             e, _, _ = bt.stats()
             es.append(e[:, 6].copy())
-        series[prec] = np.array(es[20:])
+        series[prec] = np.array(es[40:])
     z = []
-    for b in range(len(boxes)):
-        blocks = [series[p][:, b].reshape(10, -1).mean(1) for p in (0, 1)]
+    for b in range(len(base)):
+        blocks = [series[p][:, b].reshape(8, -1).mean(1) for p in (0, 1)]
         se = np.sqrt(sum(x.var(ddof=1) / len(x) for x in blocks))
         z.append(abs(blocks[0].mean() - blocks[1].mean()) / se)
     z = np.array(z)
-    assert np.sum(z < 2.0) >= 4 and np.all(z < 4.0), z
+    assert np.sum(z < 2.0) >= 3 and np.all(z < 4.0), z

@@ -59,8 +59,10 @@ def test_job_same_alone_in_mix_and_batched():
 
 def test_admission_respects_memory_budget():
     jobs = [_job(400, 20 + i) for i in range(5)]
-    # each job needs about 400 * 232 B; the budget holds two at a time
-    budget = 2 * 400 * 232 + 4000
+    # each job needs 1200 atoms x 24 B, 400 centres x 24 B and 2048
+    # k-vector slots x 44 B; the k-vector share (0.665 of the budget) of
+    # 280 kB holds two jobs at a time
+    budget = 280_000
     sv = Server(slots=8, budget_bytes=budget, tick_ms=2.0)
     ids = [sv.submit(j, 800) for j in jobs]
     sv.drain(300)
