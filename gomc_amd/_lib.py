@@ -60,6 +60,7 @@ SIGNATURES = {
     "gomc_batch_run": (C.c_int, [_vp, _lp]),
     "gomc_batch_stats": (C.c_int, [_vp, _dp, _lp, _lp]),
     "gomc_batch_get_coords": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
+    "gomc_batch_box_layout": (C.c_int, [_vp, C.c_int, _ip]),
     "gomc_batch_set_trace": (C.c_int, [_vp, C.c_int, C.c_int64]),
     "gomc_batch_get_trace": (C.c_int, [_vp, _ip, _dp, _dp, _dp, _lp]),
     "gomc_server_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double,

@@ -268,6 +268,13 @@ class Batch:
         self._check(self.lib.gomc_batch_get_coords(self.h, box, ptr(x), ptr(y), ptr(z)))
         return np.stack([x, y, z], axis=1)
 
+    def layout(self, box: int) -> dict:
+        """Device layout of a box: team size, k-vectors per segment, segment
+        rounds, centre cull."""
+        out = np.zeros(4, dtype=np.int32)
+        self._check(self.lib.gomc_batch_box_layout(self.h, box, ptr(out)))
+        return dict(team=int(out[0]), kseg=int(out[1]), rounds=int(out[2]), cull=bool(out[3]))
+
     def set_trace(self, box: int, cap: int):
         self._check(self.lib.gomc_batch_set_trace(self.h, box, cap))
 

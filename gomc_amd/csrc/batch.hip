@@ -1,6 +1,8 @@
-// Batched persistent MC on gfx950: one 1024-thread workgroup owns one box
-// and makes all of that box's moves of a launch without leaving the
-// kernel.  Per move: Philox draws (made 64 moves ahead, one lane each),
+// Batched persistent MC on gfx950: a 1024-thread workgroup runs the boxes
+// of its task list one after another, making all of a box's moves of a
+// launch without leaving the kernel; a large box is run by a team of 2, 4
+// or 8 workgroups that split its pair items and k-vectors and exchange
+// partial sums once per move.  Per move: Philox draws (made 64 moves ahead, one lane each),
 // the trial coordinates (Translate/Rotate, src/Coordinates.cpp:83-128), the
 // pair-energy change over every other molecule (MoleculeInter,
 // src/CalculateEnergy.cpp:581-686), the structure-factor change per
@@ -10,7 +12,7 @@
 //
 // Phases e^{i k.r} come from per-dimension tables e^{i (c_d n) r_d},
 // n = 0..nmax, built each move for the moving molecule's old and new
-// positions (one sincos per table entry), and multiplied per k-vector.
+// positions (one sincos per table entry), and advanced along nz.
 #include "batch.hpp"
 #include "device_common.hpp"
 
@@ -59,7 +61,8 @@ struct Prec<1> {
 struct alignas(16) MoveShared {
   double oldp[3][kBatchMaxAtoms], newp[3][kBatchMaxAtoms];
   double shift[3], rot[9], com[3], newCom[3];
-  int mol, type, accept, pad;
+  int mol, type, accept;
+  int abort;  // a team exchange ran out of time: the box stops
 };
 
 // Reduction of four doubles over the workgroup in a fixed order; result in
@@ -243,15 +246,16 @@ __device__ __noinline__ LjReal DrainQueue(const BatchBox &sb, const LMEM PairQue
 // other molecule without the cull) per work item, tested against the old and
 // new positions of all len moving atoms; distances inside the cutoff are
 // compacted into a per-wave queue (ballot + prefix count) and their energies
-// evaluated 64 at a time, so the erfc path runs with full waves.
+// evaluated 64 at a time, so the erfc path runs with full waves.  A team
+// member evaluates its share of the level-2 items.
 // CX/CY/CZ point to LDS or HBM (ComPtr).  Returns this thread's {lj, real,
 // overlap}: overlap = 1 when a new position lies inside rCutLow of an atom.
 template <typename T, typename ComPtr>
 __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &ms,
                                          const GMEM double *X, const GMEM double *Y,
                                          const GMEM double *Z, ComPtr CX, ComPtr CY, ComPtr CZ,
-                                         int m, int len, int nmols, PairQueue *queues,
-                                         int *nearList, int *waveCnt) {
+                                         int m, int len, int nmols, int rank, int team,
+                                         PairQueue *queues, int *nearList, int *waveCnt) {
   BoxGeom g;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -293,10 +297,13 @@ __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &m
     }
     __syncthreads();
   }
+  // a team member takes its share [lo, hi) of the items
   const int items = total * len;
-  for (int ib = 0; ib < items; ib += kBatchThreads) {
+  const int per = (items + team - 1) / team;
+  const int lo = rank * per, hi = min(items, lo + per);
+  for (int ib = lo; ib < hi; ib += kBatchThreads) {
     const int it = ib + int(threadIdx.x);
-    const bool valid = it < items;
+    const bool valid = it < hi;
     const int itc = valid ? it : 0;
     const uint32_t k = uint32_t((uint64_t(uint32_t(itc)) * sb.magicLen) >> 32);  // itc / len
     const int qa = itc - int(k) * len;
@@ -381,13 +388,20 @@ __device__ __forceinline__ float Uniform(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(uint32_t, v)));
 }
 
+__device__ __forceinline__ int64_t UniformI64(int64_t v) {
+  const uint64_t u = uint64_t(v);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(u));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(u >> 32));
+  return int64_t((uint64_t(hi) << 32) | lo);
+}
+
 template <typename T, int S, int LEN>
 __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*tab)[3][kBatchMaxN + 1],
                                           const GMEM int32_t *__restrict__ kn,
                                           const GMEM double *__restrict__ pref,
                                           GMEM double *__restrict__ sumR, GMEM double *__restrict__ sumI,
                                           GMEM double *__restrict__ dRs, GMEM double *__restrict__ dIs,
-                                          bool pend) {
+                                          int rank, int team, bool pend) {
   // steps whose loads are issued together (FP64 phases leave room for two)
   constexpr int GMax = sizeof(T) == 8 ? 2 : 4;
   constexpr int G = S < GMax ? S : GMax;
@@ -401,7 +415,7 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
     stepN[p] = {Uniform(sn.re), Uniform(sn.im)};
   }
   double e = 0.0;
-  for (int r = 0; r < sb.kChunk; ++r) {
+  for (int r = rank; r < sb.kChunk; r += team) {  // a team member's rounds
     CplxT<T> eO[LEN], eN[LEN];
 #pragma unroll 1
     for (int jb = 0; jb < S; jb += G) {
@@ -484,16 +498,16 @@ __device__ __forceinline__ double RecipPhaseLen(const BatchBox &sb,
                                                 const CplxT<T> (*tab)[3][kBatchMaxN + 1], int len,
                                                 const GMEM int32_t *kn, const GMEM double *pref,
                                                 GMEM double *sumR, GMEM double *sumI, GMEM double *dRs,
-                                                GMEM double *dIs, bool pend) {
+                                                GMEM double *dIs, int rank, int team, bool pend) {
   switch (len) {
     case 1:
-      return RecipPhase<T, S, 1>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+      return RecipPhase<T, S, 1>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
     case 2:
-      return RecipPhase<T, S, 2>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+      return RecipPhase<T, S, 2>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
     case 3:
-      return RecipPhase<T, S, 3>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+      return RecipPhase<T, S, 3>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
     default:
-      return RecipPhase<T, S, 4>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, pend);
+      return RecipPhase<T, S, 4>(sb, tab, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
   }
 }
 
@@ -502,32 +516,95 @@ __device__ __forceinline__ double RecipDispatch(const BatchBox &sb,
                                                 const CplxT<T> (*tab)[3][kBatchMaxN + 1], int len,
                                                 const GMEM int32_t *kn, const GMEM double *pref,
                                                 GMEM double *sumR, GMEM double *sumI, GMEM double *dRs,
-                                                GMEM double *dIs, bool pend) {
-  if (sb.kSeg == 8) return RecipPhaseLen<T, 8>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, pend);
-  if (sb.kSeg == 4) return RecipPhaseLen<T, 4>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, pend);
-  return RecipPhaseLen<T, 1>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, pend);
+                                                GMEM double *dIs, int rank, int team, bool pend) {
+  if (sb.kSeg == 8) return RecipPhaseLen<T, 8>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
+  if (sb.kSeg == 4) return RecipPhaseLen<T, 4>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
+  return RecipPhaseLen<T, 1>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
 }
 
-// Adds a pending accepted change to S(k) at the end of a launch.
-__device__ __noinline__ void RecipFold(int nslots, const GMEM int32_t *kn, GMEM double *sumR,
-                                       GMEM double *sumI, const GMEM double *dRs,
+// Adds a pending accepted change to S(k) at the end of a launch (the
+// member's own rounds).
+__device__ __noinline__ void RecipFold(const BatchBox &sb, int rank, int team, const GMEM int32_t *kn,
+                                       GMEM double *sumR, GMEM double *sumI, const GMEM double *dRs,
                                        const GMEM double *dIs) {
-  for (int s = int(threadIdx.x); s < nslots; s += kBatchThreads) {
-    if (kn[s] >= 0) {
-      sumR[s] += dRs[s];
-      sumI[s] += dIs[s];
+  for (int r = rank; r < sb.kChunk; r += team)
+    for (int j = 0; j < sb.kSeg; ++j) {
+      const int s = (r * sb.kSeg + j) * kBatchThreads + int(threadIdx.x);
+      if (kn[s] >= 0) {
+        sumR[s] += dRs[s];
+        sumI[s] += dIs[s];
+      }
+    }
+}
+
+// Team exchange, run by wave 0 of each member (acc valid in lane 0): lane 0
+// publishes this member's partial sums {lj, real, recip, overlap} for move
+// `seq`; lane q (q < team, q != rank) waits for member q's counter and
+// loads its partials, so all waits and loads are in flight together; every
+// lane then forms the team total in member order, the same bits in every
+// member.  Words go through device-coherent atomics and the counter is
+// written after the partials have landed.  A wait that outlasts
+// kTeamSpinLimit polls sets *err and ok = false.
+struct Sum4 {
+  double v[4];
+  bool ok;
+};
+
+__device__ __noinline__ Sum4 TeamSum(GMEM uint64_t *xb, int team, int rank, uint64_t seq, Sum4 acc,
+                                     GMEM int *err) {
+  const int lane = int(threadIdx.x) & (kWave - 1);
+  GMEM uint64_t *data = xb;
+  GMEM uint64_t *flags = xb + 2 * kTeamMax * 4;
+  double p[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) p[i] = __shfl(acc.v[i], 0);
+  if (lane == 0) {
+    GMEM uint64_t *mine = data + ((seq & 1) * kTeamMax + rank) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      __hip_atomic_exchange(mine + i, __builtin_bit_cast(uint64_t, p[i]), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_exchange(flags + rank, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  int ok = 1;
+  if (lane < team && lane != rank) {
+    int64_t n = 0;
+    while (__hip_atomic_load(flags + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < seq) {
+      if (++n > kTeamSpinLimit) {
+        ok = 0;
+        break;
+      }
+    }
+    if (ok) {
+      GMEM uint64_t *theirs = data + ((seq & 1) * kTeamMax + lane) * 4;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        p[i] = __builtin_bit_cast(
+            double, __hip_atomic_load(theirs + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
   }
+  Sum4 out{{0.0, 0.0, 0.0, 0.0}, __all(ok) != 0};
+  if (!out.ok) {
+    if (lane == 0) __hip_atomic_exchange(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return out;
+  }
+  for (int q = 0; q < team; ++q) {
+    out.v[0] += __shfl(p[0], q);
+    out.v[1] += __shfl(p[1], q);
+    out.v[2] += __shfl(p[2], q);
+    out.v[3] = fmax(out.v[3], __shfl(p[3], q));
+  }
+  return out;
 }
 
 // Metropolis test and bookkeeping (Translate/Rotate::Accept,
 // src/moves/Translate.h:97-138; MoveSettings::Update,
 // src/MoveSettings.cpp:94-146), run by thread 0.
-__device__ __noinline__ void Metropolis(const BatchBox &sb, BatchState &st, MoveShared &ms,
-                                        const double *acc4, double pr, int mtype, int64_t traceIdx,
-                                        const BatchTrace &trace) {
-  const bool overlap = acc4[3] > 0.0;
-  const double dE = acc4[0] + acc4[1] + acc4[2];
+__device__ __noinline__ void Metropolis(const BatchBox &sb, BatchState &st, MoveShared &ms, Sum4 acc,
+                                        double pr, int mtype) {
+  const bool overlap = acc.v[3] > 0.0;
+  const double dE = acc.v[0] + acc.v[1] + acc.v[2];
   const bool res = pr < exp(-sb.beta * dE);
   const bool accept = res && !overlap;
   st.tries[mtype]++;
@@ -535,17 +612,11 @@ __device__ __noinline__ void Metropolis(const BatchBox &sb, BatchState &st, Move
   if (accept) {
     st.accepted[mtype]++;
     st.tempAccepted[mtype]++;
-    st.energy[0] += acc4[0];
-    st.energy[2] += acc4[1];
-    st.energy[3] += acc4[2];
+    st.energy[0] += acc.v[0];
+    st.energy[2] += acc.v[1];
+    st.energy[3] += acc.v[2];
     st.energy[6] = st.energy[0] + st.energy[1] + st.energy[2] + st.energy[3] + st.energy[4] +
                    st.energy[5];
-  }
-  if (traceIdx >= 0 && traceIdx < trace.cap) {
-    trace.acc[traceIdx] = accept ? 1 : 0;
-    trace.dlj[traceIdx] = acc4[0];
-    trace.dreal[traceIdx] = acc4[1];
-    trace.drecip[traceIdx] = overlap ? 0.0 : acc4[2];
   }
   ms.accept = accept ? 1 : 0;
 }
@@ -563,133 +634,171 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   __shared__ int nearList[kBatchNearCap];
   __shared__ int waveCnt[kBatchThreads / kWave];
 
-  const int b = blockIdx.x, tid = threadIdx.x;
-  {
-    const uint64_t *src = reinterpret_cast<const uint64_t *>(&a.boxes[b]);
-    uint64_t *dst = reinterpret_cast<uint64_t *>(&sb);
-    for (int i = tid; i < int(sizeof(BatchBox) / 8); i += kBatchThreads) dst[i] = src[i];
-  }
-  __syncthreads();
-  const int len = sb.molLen;
-  const int64_t aoff = sb.atomOff, moff = sb.molOff, koff = sb.kOff;
-  double *X = a.x + aoff, *Y = a.y + aoff, *Z = a.z + aoff;
-  // The box's molecule centres live in LDS for the launch when they fit
-  // (a.ldsMols per workgroup); CXr/CYr/CZr are where the pair phase reads them.
+  const int tid = threadIdx.x;
   extern __shared__ double ldsCom[];
-  const bool inLds = sb.nmols <= a.ldsMols;
-  double *CXr = a.cx + moff, *CYr = a.cy + moff, *CZr = a.cz + moff;
-  if (inLds) {
-    CXr = ldsCom;
-    CYr = ldsCom + a.ldsMols;
-    CZr = ldsCom + 2 * a.ldsMols;
-    for (int i = tid; i < sb.nmols; i += kBatchThreads) {
-      CXr[i] = a.cx[moff + i];
-      CYr[i] = a.cy[moff + i];
-      CZr[i] = a.cz[moff + i];
+  for (int ti = a.wgTask[blockIdx.x]; ti < a.wgTask[blockIdx.x + 1]; ++ti) {
+    const BatchTask task = a.tasks[ti];
+    const int b = __builtin_amdgcn_readfirstlane(task.box);
+    const int rank = __builtin_amdgcn_readfirstlane(task.rank);
+    {
+      const uint64_t *src = reinterpret_cast<const uint64_t *>(&a.boxes[b]);
+      uint64_t *dst = reinterpret_cast<uint64_t *>(&sb);
+      for (int i = tid; i < int(sizeof(BatchBox) / 8); i += kBatchThreads) dst[i] = src[i];
     }
-  }
-  const int32_t *kn = a.kn + koff;
-  const double *pref = a.pref + koff;
+    if (tid == 0) ms.abort = 0;
+    __syncthreads();
+    // box constants read from LDS are the same in every lane: kept in
+    // scalar registers, with the pointers derived from them
+    const int len = __builtin_amdgcn_readfirstlane(sb.molLen);
+    const int team = __builtin_amdgcn_readfirstlane(sb.team > 1 ? sb.team : 1);
+    // member `rank` works on its own copy of the coordinates and centres
+    const int64_t aoff = UniformI64(sb.atomOff + int64_t(rank) * sb.natoms);
+    const int64_t moff = UniformI64(sb.molOff + int64_t(rank) * sb.nmols);
+    const int64_t koff = UniformI64(sb.kOff);
+    double *X = a.x + aoff, *Y = a.y + aoff, *Z = a.z + aoff;
+    double *CXg = a.cx + moff, *CYg = a.cy + moff, *CZg = a.cz + moff;
+    // The box's molecule centres live in LDS for the launch when they fit
+    // (a.ldsMols per workgroup); CXr/CYr/CZr are where the pair phase reads them.
+    const bool inLds = __builtin_amdgcn_readfirstlane(sb.nmols) <= a.ldsMols;
+    const bool ewald = __builtin_amdgcn_readfirstlane(sb.ewald) != 0;
+    double *CXr = CXg, *CYr = CYg, *CZr = CZg;
+    if (inLds) {
+      CXr = ldsCom;
+      CYr = ldsCom + a.ldsMols;
+      CZr = ldsCom + 2 * a.ldsMols;
+      for (int i = tid; i < sb.nmols; i += kBatchThreads) {
+        CXr[i] = CXg[i];
+        CYr[i] = CYg[i];
+        CZr[i] = CZg[i];
+      }
+    }
+    const int32_t *kn = a.kn + koff;
+    const double *pref = a.pref + koff;
+    GMEM uint64_t *xb = Gp(a.xchg + int64_t(b) * kXchgWords);
 
-  if (tid == 0) st = a.state[b];
-  int64_t step = a.state[b].step;
-  const int64_t nMoves = a.moves[b];
-  const int nTab = 2 * len * ((sb.nmax[0] + 1) + (sb.nmax[1] + 1) + (sb.nmax[2] + 1));
+    if (tid == 0) st = a.state[b];
+    int64_t step = UniformI64(a.state[b].step);
+    const int64_t nMoves = UniformI64(a.moves[b]);
+    const int nTab = 2 * len * ((sb.nmax[0] + 1) + (sb.nmax[1] + 1) + (sb.nmax[2] + 1));
+    const int64_t traceBox = rank == 0 ? a.trace.box : -1;
 
-  bool pend = false;  // the last move was accepted and S(k) awaits its change
-  for (int64_t mv = 0; mv < nMoves; ++mv, ++step) {
-    // ---- Philox draws for the next kDrawChunk moves ----
-    if ((mv % kDrawChunk) == 0) {
-      if (tid < kDrawChunk) MoveDraws(sb.seed, uint64_t(step + tid), draws[tid]);
+    bool pend = false;  // the last move was accepted and S(k) awaits its change
+    for (int64_t mv = 0; mv < nMoves; ++mv, ++step) {
+      // ---- Philox draws for the next kDrawChunk moves ----
+      if ((mv % kDrawChunk) == 0) {
+        if (tid < kDrawChunk) MoveDraws(sb.seed, uint64_t(step + tid), draws[tid]);
+        __syncthreads();
+      }
+      const double *u = draws[mv % kDrawChunk];
+
+      // ---- proposal (thread 0): move settings, move kind, molecule ----
+      if (tid == 0) Propose(sb, st, ms, u, step, CXr, CYr, CZr);
       __syncthreads();
-    }
-    const double *u = draws[mv % kDrawChunk];
-
-    // ---- proposal (thread 0): move settings, move kind, molecule ----
-    if (tid == 0) Propose(sb, st, ms, u, step, CXr, CYr, CZr);
-    __syncthreads();
-    const int m = ms.mol, mtype = ms.type;
-    if (tid < len) {
-      const int64_t ai = int64_t(m) * len + tid;
-      const double o[3] = {X[ai], Y[ai], Z[ai]};
-      double nw[3];
-      if (mtype == 0) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) nw[d] = Wrap(o[d] + ms.shift[d], sb.L[d]);
-      } else {
-        double v[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) v[d] = Unwrap(o[d], ms.com[d], sb.L[d], sb.half[d]) + (-ms.com[d]);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          const double r = ms.rot[d * 3 + 0] * v[0] + ms.rot[d * 3 + 1] * v[1] + ms.rot[d * 3 + 2] * v[2];
-          nw[d] = Wrap(r + ms.com[d], sb.L[d]);
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        ms.oldp[d][tid] = o[d];
-        ms.newp[d][tid] = nw[d];
-      }
-    }
-    __syncthreads();
-
-    // ---- phase tables for the 2*len positions ----
-    if (sb.ewald) BuildTables<T>(sb, ms, len, nTab, tab);
-
-    // ---- pair phase ----
-    double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
-    LjReal pr;
-    if (inLds)
-      pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Lp(CXr), Lp(CYr), Lp(CZr), m, len, sb.nmols,
-                        queues, nearList, waveCnt);
-    else
-      pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Gp(CXr), Gp(CYr), Gp(CZr), m, len, sb.nmols,
-                        queues, nearList, waveCnt);
-    acc4[0] = pr.lj;
-    acc4[1] = pr.real;
-    acc4[3] = pr.ov;
-    __syncthreads();  // tables complete
-
-    // ---- reciprocal phase: energy change per k-vector ----
-    if (sb.ewald)
-      acc4[2] = RecipDispatch<T>(sb, tab, len, Gp(kn), Gp(pref), Gp(a.sumR + koff),
-                                 Gp(a.sumI + koff), Gp(a.dR + koff), Gp(a.dI + koff), pend);
-    BlockSum4(acc4, red);
-
-    // ---- Metropolis (thread 0) ----
-    if (tid == 0) Metropolis(sb, st, ms, acc4, u[5], mtype, (b == a.trace.box) ? mv : -1, a.trace);
-    __syncthreads();
-
-    // ---- commit (S(k) in the next move's reciprocal phase) ----
-    pend = ms.accept != 0;
-    if (ms.accept) {
+      const int m = __builtin_amdgcn_readfirstlane(ms.mol);
+      const int mtype = __builtin_amdgcn_readfirstlane(ms.type);
       if (tid < len) {
         const int64_t ai = int64_t(m) * len + tid;
-        X[ai] = ms.newp[0][tid];
-        Y[ai] = ms.newp[1][tid];
-        Z[ai] = ms.newp[2][tid];
-      }
-      if (tid == 0 && mtype == 0) {
-        a.cx[moff + m] = ms.newCom[0];
-        a.cy[moff + m] = ms.newCom[1];
-        a.cz[moff + m] = ms.newCom[2];
-        if (inLds) {
-          CXr[m] = ms.newCom[0];
-          CYr[m] = ms.newCom[1];
-          CZr[m] = ms.newCom[2];
+        const double o[3] = {X[ai], Y[ai], Z[ai]};
+        double nw[3];
+        if (mtype == 0) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) nw[d] = Wrap(o[d] + ms.shift[d], sb.L[d]);
+        } else {
+          double v[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) v[d] = Unwrap(o[d], ms.com[d], sb.L[d], sb.half[d]) + (-ms.com[d]);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double r = ms.rot[d * 3 + 0] * v[0] + ms.rot[d * 3 + 1] * v[1] + ms.rot[d * 3 + 2] * v[2];
+            nw[d] = Wrap(r + ms.com[d], sb.L[d]);
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          ms.oldp[d][tid] = o[d];
+          ms.newp[d][tid] = nw[d];
         }
       }
-    }
-    __syncthreads();
-  }
+      __syncthreads();
 
-  if (pend && sb.ewald)
-    RecipFold(sb.K, Gp(kn), Gp(a.sumR + koff), Gp(a.sumI + koff), Gp(a.dR + koff),
-              Gp(a.dI + koff));
-  if (tid == 0) {
-    st.step = step;
-    a.state[b] = st;
+      // ---- phase tables for the 2*len positions ----
+      if (ewald) BuildTables<T>(sb, ms, len, nTab, tab);
+
+      // ---- pair phase ----
+      double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
+      LjReal pr;
+      if (inLds)
+        pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Lp(CXr), Lp(CYr), Lp(CZr), m, len, sb.nmols,
+                          rank, team, queues, nearList, waveCnt);
+      else
+        pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Gp(CXr), Gp(CYr), Gp(CZr), m, len, sb.nmols,
+                          rank, team, queues, nearList, waveCnt);
+      acc4[0] = pr.lj;
+      acc4[1] = pr.real;
+      acc4[3] = pr.ov;
+      __syncthreads();  // tables complete
+
+      // ---- reciprocal phase: energy change per k-vector ----
+      if (ewald)
+        acc4[2] = RecipDispatch<T>(sb, tab, len, Gp(kn), Gp(pref), Gp(a.sumR + koff),
+                                   Gp(a.sumI + koff), Gp(a.dR + koff), Gp(a.dI + koff), rank, team,
+                                   pend);
+      BlockSum4(acc4, red);
+
+      // ---- team total (wave 0), then Metropolis (thread 0) ----
+      if (team > 1 && tid < kWave) {
+        const Sum4 t = TeamSum(xb, team, rank, uint64_t(step) + 1,
+                               Sum4{{acc4[0], acc4[1], acc4[2], acc4[3]}, true}, Gp(a.err));
+        if (tid == 0) {
+          if (!t.ok) ms.abort = 1;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc4[i] = t.v[i];
+        }
+      }
+      if (tid == 0 && !ms.abort) {
+        const Sum4 tot{{acc4[0], acc4[1], acc4[2], acc4[3]}, true};
+        Metropolis(sb, st, ms, tot, u[5], mtype);
+        if (b == traceBox && mv < a.trace.cap) {
+          a.trace.acc[mv] = ms.accept;
+          a.trace.dlj[mv] = acc4[0];
+          a.trace.dreal[mv] = acc4[1];
+          a.trace.drecip[mv] = acc4[3] > 0.0 ? 0.0 : acc4[2];
+        }
+      }
+      __syncthreads();
+      if (__builtin_amdgcn_readfirstlane(ms.abort)) break;
+
+      // ---- commit (S(k) in the next move's reciprocal phase) ----
+      pend = __builtin_amdgcn_readfirstlane(ms.accept) != 0;
+      if (pend) {
+        if (tid < len) {
+          const int64_t ai = int64_t(m) * len + tid;
+          X[ai] = ms.newp[0][tid];
+          Y[ai] = ms.newp[1][tid];
+          Z[ai] = ms.newp[2][tid];
+        }
+        if (tid == 0 && mtype == 0) {
+          CXg[m] = ms.newCom[0];
+          CYg[m] = ms.newCom[1];
+          CZg[m] = ms.newCom[2];
+          if (inLds) {
+            CXr[m] = ms.newCom[0];
+            CYr[m] = ms.newCom[1];
+            CZr[m] = ms.newCom[2];
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    if (pend && ewald)
+      RecipFold(sb, rank, team, Gp(kn), Gp(a.sumR + koff), Gp(a.sumI + koff), Gp(a.dR + koff),
+                Gp(a.dI + koff));
+    if (tid == 0 && rank == 0) {
+      st.step = step;
+      a.state[b] = st;
+    }
+    __syncthreads();  // LDS is reused by the next task
   }
 }
 
@@ -703,13 +812,13 @@ static hipError_t SetLds(int ldsMols) {
 }
 
 template <int PREC>
-static hipError_t LaunchOne(int nboxes, const BatchArgs &a, hipStream_t s, bool setAttr) {
+static hipError_t LaunchOne(int grid, const BatchArgs &a, hipStream_t s, bool setAttr) {
   const size_t dyn = sizeof(double) * 3 * size_t(a.ldsMols);
   if (setAttr) {
     hipError_t e = SetLds<PREC>(a.ldsMols);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(McBatchKernel<PREC>, dim3(nboxes), dim3(kBatchThreads), dyn, s, a);
+  hipLaunchKernelGGL(McBatchKernel<PREC>, dim3(grid), dim3(kBatchThreads), dyn, s, a);
   return hipGetLastError();
 }
 
@@ -719,8 +828,8 @@ hipError_t PrepareBatch(int precision, int ldsMols) {
 
 int BatchLdsMols(int maxMols) { return maxMols < kBatchLdsMolCap ? maxMols : kBatchLdsMolCap; }
 
-hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s, bool setAttr) {
-  return precision == 1 ? LaunchOne<1>(nboxes, a, s, setAttr) : LaunchOne<0>(nboxes, a, s, setAttr);
+hipError_t LaunchBatch(int precision, int grid, const BatchArgs &a, hipStream_t s, bool setAttr) {
+  return precision == 1 ? LaunchOne<1>(grid, a, s, setAttr) : LaunchOne<0>(grid, a, s, setAttr);
 }
 
 }  // namespace gomc

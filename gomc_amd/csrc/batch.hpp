@@ -19,6 +19,15 @@ constexpr int kBatchMaxTypes = 4;    // LJ atom types per box
 constexpr int kBatchMaxN = 32;       // largest |n| of a k-vector component
 constexpr int kDrawChunk = 64;       // moves whose Philox draws are made at once
 constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of every box stream
+// Teams: a large box may be run by T = 2, 4 or 8 workgroups at once, each
+// holding its own copy of the coordinates and a share of the pair items and
+// k-vector rounds; per move the members exchange their partial energy sums.
+constexpr int kTeamMax = 8;
+// Exchange block per box (64-bit words): partial sums [2 parities][8][4],
+// then one move counter per member.
+constexpr int kXchgWords = 2 * kTeamMax * 4 + kTeamMax;
+// Polls of a member's counter before a team gives up and flags an error.
+constexpr int64_t kTeamSpinLimit = int64_t(1) << 22;
 // Molecules whose FP64 centres fit in the LDS left beside the static
 // tables, pair queues and near list (160 KiB per CU on gfx950, ~60 KiB
 // static): 4096 x 24 B.
@@ -50,6 +59,9 @@ struct BatchBox {
   int kSeg;    // S: k-vectors per segment (1, 4 or 8)
   int nmax[3], ewald, elect;
   int cull;  // 1: cull far molecules by their centres before atom pairs
+  int team;  // workgroups running the box (1, 2, 4, 8); member r's coordinates
+             // sit at atomOff + r * natoms and molOff + r * nmols
+  int pad;
 };
 
 static_assert(sizeof(BatchBox) % 8 == 0, "BatchBox is copied as 64-bit words");
@@ -70,7 +82,16 @@ struct BatchTrace {
   double *dlj, *dreal, *drecip;
 };
 
+// One entry of a workgroup's task list: run `box` as member `rank`.
+struct BatchTask {
+  int box, rank;
+};
+
 struct BatchArgs {
+  const BatchTask *tasks;     // task lists of all workgroups, back to back
+  const int *wgTask;          // [grid + 1] start of each workgroup's list
+  uint64_t *xchg;             // [nboxes * kXchgWords] team exchange blocks
+  int *err;                   // set to 1 when a team member's wait runs out
   const BatchBox *boxes;
   BatchState *state;
   double *x, *y, *z;          // [sum atoms]
@@ -103,6 +124,14 @@ struct gomc_batch {
   int32_t *kn = nullptr;
   double *pref = nullptr, *sumR = nullptr, *sumI = nullptr, *dR = nullptr, *dI = nullptr;
   int64_t *moves = nullptr;
+  // workgroup task lists (device) and their capacity; team exchange blocks
+  gomc::BatchTask *tasks = nullptr;
+  int *wgTask = nullptr;
+  uint64_t *xchg = nullptr;
+  int *errFlag = nullptr;
+  int grid = 0;        // workgroups per launch
+  int gridCap = 0;     // workgroups that fit on the device at once
+  bool teams = true;   // false: every box runs on one workgroup (serving)
   int64_t totalAtoms = 0, totalMols = 0, totalK = 0;
   int ldsMols = 0;
   gomc::BatchTrace trace{-1, 0, nullptr, nullptr, nullptr, nullptr};
@@ -116,7 +145,7 @@ namespace gomc {
 // Launch the MC kernel at the given precision (0 FP64, 1 FP32 pair terms);
 // setAttr = false when PrepareBatch already sized its dynamic LDS (inside a
 // stream capture).
-hipError_t LaunchBatch(int precision, int nboxes, const BatchArgs &a, hipStream_t s,
+hipError_t LaunchBatch(int precision, int grid, const BatchArgs &a, hipStream_t s,
                        bool setAttr = true);
 hipError_t PrepareBatch(int precision, int ldsMols);
 // LDS centre capacity for a batch whose largest box has maxMols molecules
@@ -140,6 +169,11 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
             int64_t kOff, PackedBox &out);
 // Device buffers for `nboxes` boxes and the given flat-array lengths.
 int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t kvec);
+// Team size of a box of n molecules (a function of n alone, so a box's
+// results do not depend on the other boxes of its batch).
+int TeamSize(int nmols);
+// Builds and uploads the workgroup task lists for the given move counts.
+int BuildTasks(gomc_batch *bt, const std::vector<int64_t> &moves);
 // Uploads a packed box: parameters and state at index `slot`, arrays at
 // the box's offsets.
 int UploadBox(gomc_batch *bt, int slot, const PackedBox &pb);

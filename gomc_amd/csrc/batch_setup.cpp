@@ -27,7 +27,8 @@ int BFail(gomc_batch *bt, int code, const std::string &msg) {
 void FreeBatch(gomc_batch *bt) {
   void *ptrs[] = {bt->boxes, bt->state, bt->x,    bt->y,    bt->z,    bt->cx,         bt->cy,
                   bt->cz,    bt->kn,    bt->pref, bt->sumR, bt->sumI, bt->dR, bt->dI, bt->moves,      bt->trace.acc,
-                  bt->trace.dlj, bt->trace.dreal, bt->trace.drecip};
+                  bt->trace.dlj, bt->trace.dreal, bt->trace.drecip, bt->tasks, bt->wgTask,
+                  bt->xchg, bt->errFlag};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (bt->graphExec) (void)hipGraphExecDestroy(bt->graphExec);
@@ -71,6 +72,7 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
   } guard{b};
   BatchBox &P = out.P;
   std::memset(&P, 0, sizeof(P));
+  P.team = 1;
   for (int a = 0; a < 3; ++a) {
     P.L[a] = b->geom.L[a];
     P.half[a] = b->geom.half[a];
@@ -164,14 +166,16 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
       }
     }
     // per round a thread pays about three table products to start a
-    // segment and one complex product per step
+    // segment and one complex product per step; a team of T splits the
+    // rounds
+    const int64_t T = bt->teams ? TeamSize(b->nmols) : 1;
     int bestS = 1;
     int64_t bestCost = -1;
     for (int S : {1, 4, 8}) {
       int64_t segs = 0;
       for (int L : runLen) segs += (L + S - 1) / S;
       const int64_t rounds = (segs + kBatchThreads - 1) / kBatchThreads;
-      const int64_t cost = rounds * (S + 3);
+      const int64_t cost = ((rounds + T - 1) / T) * (S + 3);
       if (bestCost < 0 || cost < bestCost) {
         bestCost = cost;
         bestS = S;
@@ -236,6 +240,10 @@ BatchArgs MakeArgs(const gomc_batch *bt) {
   a.dR = bt->dR;
   a.dI = bt->dI;
   a.moves = bt->moves;
+  a.tasks = bt->tasks;
+  a.wgTask = bt->wgTask;
+  a.xchg = bt->xchg;
+  a.err = bt->errFlag;
   a.ldsMols = bt->ldsMols;
   a.trace = bt->trace;
   return a;
@@ -262,7 +270,19 @@ int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t 
   ok &= alloc((void **)&bt->sumI, sizeof(double) * kvec);
   ok &= alloc((void **)&bt->dR, sizeof(double) * kvec);
   ok &= alloc((void **)&bt->dI, sizeof(double) * kvec);
+  // one workgroup per compute unit: the kernel's 1024 threads at 128
+  // registers fill a CU, and team members must all be resident at once
+  int cus = 0;
+  ok &= hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, bt->device) == hipSuccess;
+  bt->gridCap = std::max(8, cus - cus % 8);
+  bt->grid = bt->gridCap;
+  ok &= alloc((void **)&bt->tasks, sizeof(BatchTask) * size_t(nboxes) * kTeamMax);
+  ok &= alloc((void **)&bt->wgTask, sizeof(int) * size_t(bt->gridCap + 1));
+  ok &= alloc((void **)&bt->xchg, sizeof(uint64_t) * size_t(nboxes) * kXchgWords);
+  ok &= alloc((void **)&bt->errFlag, sizeof(int));
   if (!ok) return BFail(bt, GOMC_ERR_HIP, "batch allocation failed");
+  BHIP(bt, hipMemsetAsync(bt->xchg, 0, sizeof(uint64_t) * size_t(nboxes) * kXchgWords, bt->stream));
+  BHIP(bt, hipMemsetAsync(bt->errFlag, 0, sizeof(int), bt->stream));
   // the structure-factor changes start defined (unused k slots are read)
   if (kvec > 0) {
     BHIP(bt, hipMemsetAsync(bt->dR, 0, sizeof(double) * kvec, bt->stream));
@@ -282,18 +302,96 @@ int UploadBox(gomc_batch *bt, int slot, const PackedBox &pb) {
   };
   BHIP(bt, up(bt->boxes + slot, &pb.P, sizeof(BatchBox)));
   BHIP(bt, up(bt->state + slot, &pb.S, sizeof(BatchState)));
-  BHIP(bt, up(bt->x + P.atomOff, pb.x.data(), sizeof(double) * pb.x.size()));
-  BHIP(bt, up(bt->y + P.atomOff, pb.y.data(), sizeof(double) * pb.y.size()));
-  BHIP(bt, up(bt->z + P.atomOff, pb.z.data(), sizeof(double) * pb.z.size()));
-  BHIP(bt, up(bt->cx + P.molOff, pb.cx.data(), sizeof(double) * pb.cx.size()));
-  BHIP(bt, up(bt->cy + P.molOff, pb.cy.data(), sizeof(double) * pb.cy.size()));
-  BHIP(bt, up(bt->cz + P.molOff, pb.cz.data(), sizeof(double) * pb.cz.size()));
+  for (int r = 0; r < std::max(1, P.team); ++r) {  // one copy per team member
+    const int64_t ao = P.atomOff + int64_t(r) * P.natoms, mo = P.molOff + int64_t(r) * P.nmols;
+    BHIP(bt, up(bt->x + ao, pb.x.data(), sizeof(double) * pb.x.size()));
+    BHIP(bt, up(bt->y + ao, pb.y.data(), sizeof(double) * pb.y.size()));
+    BHIP(bt, up(bt->z + ao, pb.z.data(), sizeof(double) * pb.z.size()));
+    BHIP(bt, up(bt->cx + mo, pb.cx.data(), sizeof(double) * pb.cx.size()));
+    BHIP(bt, up(bt->cy + mo, pb.cy.data(), sizeof(double) * pb.cy.size()));
+    BHIP(bt, up(bt->cz + mo, pb.cz.data(), sizeof(double) * pb.cz.size()));
+  }
   BHIP(bt, up(bt->kn + P.kOff, pb.kn.data(), sizeof(int32_t) * pb.kn.size()));
   BHIP(bt, up(bt->pref + P.kOff, pb.pref.data(), sizeof(double) * pb.pref.size()));
   BHIP(bt, up(bt->sumR + P.kOff, pb.sr.data(), sizeof(double) * pb.sr.size()));
   BHIP(bt, up(bt->sumI + P.kOff, pb.si.data(), sizeof(double) * pb.si.size()));
   BHIP(bt, hipStreamSynchronize(s));
   bt->hboxes[slot] = pb.P;
+  return GOMC_OK;
+}
+
+int TeamSize(int nmols) {
+  // Per-move cost of one workgroup grows about 18 us + 4.5 ns per molecule
+  // (measured on MI355X, water); a team of T repeats ~7 us of it per member
+  // (draws, trial coordinates, tables, the centre cull, the exchange) and
+  // splits the rest.  The thresholds keep a box's cycle near 50 ms.
+  if (nmols >= 3600) return 8;
+  if (nmols >= 2800) return 4;
+  if (nmols >= 2000) return 2;
+  return 1;
+}
+
+int BuildTasks(gomc_batch *bt, const std::vector<int64_t> &moves) {
+  const int nb = bt->nboxes, grid = bt->grid, cols = 8, perCol = grid / cols;
+  std::vector<std::vector<BatchTask>> lists(grid);
+  // teams first, largest first; members of a team share one column of
+  // workgroups (index = 8 k + column), which the hardware places on one XCD
+  std::vector<int> order;
+  for (int b = 0; b < nb; ++b)
+    if (bt->teams && bt->hboxes[b].team > 1) order.push_back(b);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int p, int q) { return bt->hboxes[p].team > bt->hboxes[q].team; });
+  std::vector<int> used(cols, 0);
+  for (int b : order) {
+    const int T = bt->hboxes[b].team;
+    int best = 0;
+    for (int c = 1; c < cols; ++c)
+      if (used[c] < used[best]) best = c;
+    if (perCol - used[best] < T) return BFail(bt, GOMC_ERR_STATE, "team does not fit the device");
+    for (int r = 0; r < T; ++r) lists[size_t(8) * (used[best] + r) + best].push_back(BatchTask{b, r});
+    used[best] += T;
+  }
+  // single-workgroup boxes: longest first onto the least-loaded free
+  // workgroup (cost ~ moves * (18 us + 4.5 ns * N))
+  std::vector<int> solo;
+  for (int b = 0; b < nb; ++b)
+    if (!bt->teams || bt->hboxes[b].team <= 1) solo.push_back(b);
+  auto cost = [&](int b) { return double(moves[b]) * (18.0 + 0.0045 * bt->hboxes[b].nmols); };
+  std::stable_sort(solo.begin(), solo.end(), [&](int p, int q) { return cost(p) > cost(q); });
+  std::vector<int> freeWg;
+  for (int g = 0; g < grid; ++g)
+    if (lists[g].empty()) freeWg.push_back(g);
+  if (!solo.empty() && freeWg.empty()) return BFail(bt, GOMC_ERR_STATE, "no workgroup left for single boxes");
+  std::vector<double> load(freeWg.size(), 0.0);
+  for (int b : solo) {
+    size_t best = 0;
+    for (size_t i = 1; i < load.size(); ++i)
+      if (load[i] < load[best]) best = i;
+    lists[freeWg[best]].push_back(BatchTask{b, 0});
+    load[best] += cost(b);
+  }
+  std::vector<BatchTask> flat;
+  std::vector<int> start(grid + 1, 0);
+  for (int g = 0; g < grid; ++g) {
+    start[g] = int(flat.size());
+    flat.insert(flat.end(), lists[g].begin(), lists[g].end());
+  }
+  start[grid] = int(flat.size());
+  if (flat.size() > size_t(nb) * kTeamMax) return BFail(bt, GOMC_ERR_STATE, "task list overflow");
+  if (!flat.empty())
+    BHIP(bt, hipMemcpyAsync(bt->tasks, flat.data(), sizeof(BatchTask) * flat.size(),
+                            hipMemcpyHostToDevice, bt->stream));
+  BHIP(bt, hipMemcpyAsync(bt->wgTask, start.data(), sizeof(int) * start.size(), hipMemcpyHostToDevice,
+                          bt->stream));
+  BHIP(bt, hipStreamSynchronize(bt->stream));
+  return GOMC_OK;
+}
+
+// After a launch: a team member that waited out its limit set the flag.
+int CheckTeams(gomc_batch *bt) {
+  int flag = 0;
+  BHIP(bt, hipMemcpy(&flag, bt->errFlag, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) return BFail(bt, GOMC_ERR_STATE, "a team member timed out waiting for its partners");
   return GOMC_OK;
 }
 
@@ -314,16 +412,64 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
     return code;
   };
   if (hipSetDevice(device) != hipSuccess) return fail(GOMC_ERR_NODEV);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return fail(GOMC_ERR_NODEV);
+  const int gridCap = std::max(8, cus - cus % 8);
   std::vector<PackedBox> packed(nboxes);
+  for (int i = 0; i < nboxes; ++i) {
+    int rc = PackBox(bt, descs[i], 0, 0, 0, packed[i]);
+    if (rc) return fail(rc);
+    packed[i].P.team = TeamSize(packed[i].P.nmols);
+  }
+  // Teams need their members resident together: when the batch holds more
+  // large boxes than the device fits (a column of gridCap / 8 workgroups per
+  // team, and up to a quarter of the workgroups left for the other boxes),
+  // the smallest teams are halved.
+  {
+    int nsolo = 0;
+    for (auto &pb : packed) nsolo += pb.P.team <= 1;
+    const int perCol = gridCap / 8;
+    auto fits = [&]() {
+      std::vector<int> t;
+      int sum = 0;
+      for (auto &pb : packed)
+        if (pb.P.team > 1) {
+          t.push_back(pb.P.team);
+          sum += pb.P.team;
+        }
+      if (sum + std::min(nsolo, gridCap / 4) > gridCap) return false;
+      std::sort(t.rbegin(), t.rend());
+      std::vector<int> used(8, 0);
+      for (int T : t) {
+        int best = int(std::min_element(used.begin(), used.end()) - used.begin());
+        if (perCol - used[best] < T) return false;
+        used[best] += T;
+      }
+      return true;
+    };
+    while (!fits()) {
+      int pick = -1;
+      for (int i = 0; i < nboxes; ++i)
+        if (packed[i].P.team > 1 &&
+            (pick < 0 || packed[i].P.nmols < packed[pick].P.nmols ||
+             (packed[i].P.nmols == packed[pick].P.nmols && packed[i].P.team > packed[pick].P.team)))
+          pick = i;
+      packed[pick].P.team /= 2;
+      if (packed[pick].P.team <= 1) ++nsolo;
+    }
+  }
   int64_t atoms = 0, mols = 0, kvec = 0;
   int maxMols = 0;
   for (int i = 0; i < nboxes; ++i) {
-    int rc = PackBox(bt, descs[i], atoms, mols, kvec, packed[i]);
-    if (rc) return fail(rc);
-    atoms += packed[i].P.natoms;
-    mols += packed[i].P.nmols;
-    kvec += packed[i].P.K;
-    maxMols = std::max(maxMols, packed[i].P.nmols);
+    BatchBox &P = packed[i].P;
+    P.atomOff = atoms;
+    P.molOff = mols;
+    P.kOff = kvec;
+    atoms += int64_t(P.natoms) * P.team;
+    mols += int64_t(P.nmols) * P.team;
+    kvec += P.K;
+    maxMols = std::max(maxMols, P.nmols);
   }
   // Boxes up to the cap keep their molecule centres in LDS.
   bt->ldsMols = BatchLdsMols(maxMols);
@@ -348,16 +494,18 @@ int gomc_batch_run(gomc_batch *bt, const int64_t *moves) {
   }
   BHIP(bt, hipMemcpyAsync(bt->moves, mv.data(), sizeof(int64_t) * bt->nboxes, hipMemcpyHostToDevice,
                           bt->stream));
+  int rc = BuildTasks(bt, mv);
+  if (rc) return rc;
   const BatchArgs a = MakeArgs(bt);
   BHIP(bt, hipEventRecord(bt->ev0, bt->stream));
-  BHIP(bt, LaunchBatch(bt->precision, bt->nboxes, a, bt->stream));
+  BHIP(bt, LaunchBatch(bt->precision, bt->grid, a, bt->stream));
   BHIP(bt, hipEventRecord(bt->ev1, bt->stream));
   BHIP(bt, hipStreamSynchronize(bt->stream));
   float ms = 0.0f;
   BHIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
   bt->lastKernelMs = ms;
   if (bt->trace.box >= 0) bt->traceLen = std::min<int64_t>(bt->trace.cap, mv[bt->trace.box]);
-  return GOMC_OK;
+  return CheckTeams(bt);
 }
 
 int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph) {
@@ -368,13 +516,15 @@ int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph) {
   BHIP(bt, hipMemcpyAsync(bt->moves, mv.data(), sizeof(int64_t) * bt->nboxes, hipMemcpyHostToDevice,
                           bt->stream));
   BHIP(bt, hipStreamSynchronize(bt->stream));
+  int rc = BuildTasks(bt, mv);
+  if (rc) return rc;
   const BatchArgs a = MakeArgs(bt);
   if (use_graph && !bt->graphExec) {
     // one cycle = one kernel node; the move counts are read from device
     // memory when a replay runs, so the graph stays valid while they hold
     BHIP(bt, PrepareBatch(bt->precision, bt->ldsMols));
     BHIP(bt, hipStreamBeginCapture(bt->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = LaunchBatch(bt->precision, bt->nboxes, a, bt->stream, false);
+    hipError_t e = LaunchBatch(bt->precision, bt->grid, a, bt->stream, false);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(bt->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -389,14 +539,14 @@ int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph) {
     if (use_graph)
       BHIP(bt, hipGraphLaunch(bt->graphExec, bt->stream));
     else
-      BHIP(bt, LaunchBatch(bt->precision, bt->nboxes, a, bt->stream));
+      BHIP(bt, LaunchBatch(bt->precision, bt->grid, a, bt->stream));
   }
   BHIP(bt, hipEventRecord(bt->ev1, bt->stream));
   BHIP(bt, hipStreamSynchronize(bt->stream));
   float ms = 0.0f;
   BHIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
   bt->lastKernelMs = ms;
-  return GOMC_OK;
+  return CheckTeams(bt);
 }
 
 int gomc_batch_stats(const gomc_batch *bt_, double *energies, int64_t *accepted, int64_t *moves_done) {
@@ -423,6 +573,16 @@ int gomc_batch_get_coords(const gomc_batch *bt_, int box, double *x, double *y, 
   BHIP(bt, hipMemcpy(x, bt->x + P.atomOff, n, hipMemcpyDeviceToHost));
   BHIP(bt, hipMemcpy(y, bt->y + P.atomOff, n, hipMemcpyDeviceToHost));
   BHIP(bt, hipMemcpy(z, bt->z + P.atomOff, n, hipMemcpyDeviceToHost));
+  return GOMC_OK;
+}
+
+int gomc_batch_box_layout(const gomc_batch *bt, int box, int32_t out[4]) {
+  if (!bt || !out || box < 0 || box >= bt->nboxes) return GOMC_ERR_ARG;
+  const BatchBox &P = bt->hboxes[box];
+  out[0] = std::max(1, P.team);
+  out[1] = P.kSeg;
+  out[2] = P.kChunk;
+  out[3] = P.cull;
   return GOMC_OK;
 }
 

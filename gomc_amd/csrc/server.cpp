@@ -281,6 +281,7 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
   bt->device = device;
   bt->precision = precision;
   bt->nboxes = slots;
+  bt->teams = false;  // a job's result must not depend on the jobs beside it
   bt->hboxes.assign(slots, BatchBox());
   std::memset(bt->hboxes.data(), 0, sizeof(BatchBox) * slots);
   int rc = AllocBatch(bt, slots, sv->atomCap, sv->molCap, sv->kCap);
@@ -312,6 +313,7 @@ int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d, int64_t moves
   // the caller's thread, while the loop keeps the device busy
   gomc_batch scratch;  // holds the packing error, not shared with the loop
   scratch.device = sv->device;
+  scratch.teams = false;
   int rc = PackBox(&scratch, *d, 0, 0, 0, j->pb);
   if (rc) {
     std::lock_guard<std::mutex> g(sv->mu);

@@ -232,10 +232,14 @@ int gomc_get_scales(const gomc_box *b, double *disp, double *rot);
 
 /* ---- batched persistent MC (many independent boxes, one launch) ------ */
 typedef struct gomc_batch gomc_batch;
-/* Boxes hold one molecule kind each (rigid, up to 8 atoms).  Each box runs
+/* Boxes hold one molecule kind each (rigid, up to 4 atoms).  Each box runs
  * its own Philox4x64-10 stream (key {seed, tag}, counter {move, block}).
  * precision: 0 = FP64 everywhere; 1 = FP32 pair terms and phases with FP64
- * accumulation, running totals and Metropolis exponent. */
+ * accumulation, running totals and Metropolis exponent.
+ * A box of 2000 molecules or more runs on a team of 2, 4 or 8 workgroups
+ * (a function of its molecule count), unless the batch holds more large
+ * boxes than the device fits, when the smallest teams are halved.  A box's
+ * results depend only on the box and its team size. */
 int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device,
                       int precision, gomc_batch **out);
 int gomc_batch_destroy(gomc_batch *bt);
@@ -253,6 +257,10 @@ int gomc_batch_stats(const gomc_batch *bt, double *energies,
                      int64_t *accepted, int64_t *moves_done);
 int gomc_batch_get_coords(const gomc_batch *bt, int box, double *x, double *y,
                           double *z);
+/* How box `box` is laid out: out = {team size, k-vectors per segment,
+ * segment rounds, centre cull on}.  Device-layout query with no
+ * counterpart in the reference. */
+int gomc_batch_box_layout(const gomc_batch *bt, int box, int32_t out[4]);
 /* Per-move trace of box `box` over the last gomc_batch_run (first `cap`
  * moves): accepted flag and the three energy deltas. */
 int gomc_batch_set_trace(gomc_batch *bt, int box, int64_t cap);

@@ -46,6 +46,57 @@ def test_batch_box_matches_oracle_philox(nmol, n):
     assert np.allclose(bt.coords(0), o.coords(), rtol=0, atol=1e-9)
 
 
+@pytest.mark.parametrize("nmol,team,n", [(2200, 2, 2500), (3000, 4, 2000), (3700, 8, 1500)])
+def test_team_box_matches_oracle_philox(nmol, team, n):
+    """A box run by a team of workgroups (split pair items and k-vector
+    rounds, partial sums exchanged each move) makes the oracle's decisions."""
+    s = systems.water_box(nmol, seed=61, adj_steps=500)
+    bt = Batch([s])
+    assert bt.layout(0)["team"] == team
+    o = Oracle(s, threads=8)
+    e0 = o.energies()
+    bt.set_trace(0, n)
+    bt.run([n])
+    tr = bt.trace()
+    no, ro = o.run(n, mode=1)
+    assert np.array_equal(tr["acc"], ro["acc"]), np.nonzero(tr["acc"] != ro["acc"])[0][:5]
+    assert dE_close(tr["dlj"], ro["dlj"], e0[0])
+    assert dE_close(tr["dreal"], ro["dreal"], e0[2])
+    assert dE_close(tr["drecip"], ro["drecip"], e0[3])
+    e, acc, done = bt.stats()
+    assert done[0] == n and acc[0] == no
+    assert np.allclose(e[0], o.energies(), rtol=1e-10, atol=0)
+    assert np.allclose(bt.coords(0), o.coords(), rtol=0, atol=1e-9)
+
+
+def test_team_box_same_alone_and_in_mix():
+    """Team sizes depend on the box alone, so a team box gives the same bits
+    alone and beside other boxes (teams and single-workgroup boxes)."""
+    big = systems.water_box(2600, seed=71)
+    mix = [systems.water_box(300, seed=72), big, systems.water_box(3100, seed=73),
+           systems.water_box(500, seed=74)]
+    n = [600, 1200, 900, 800]
+    full = Batch(mix)
+    assert [full.layout(i)["team"] for i in range(4)] == [1, 2, 4, 1]
+    full.set_trace(1, n[1])
+    full.run(n)
+    t_mix = full.trace()
+    e_mix, _, _ = full.stats()
+    alone = Batch([big])
+    alone.set_trace(0, n[1])
+    alone.run([n[1]])
+    t_alone = alone.trace()
+    e_alone, _, _ = alone.stats()
+    for k in ("acc", "dlj", "dreal", "drecip"):
+        assert np.array_equal(t_mix[k], t_alone[k]), k
+    assert np.array_equal(e_mix[1], e_alone[0])
+    assert np.array_equal(full.coords(1), alone.coords(0))
+    # the running energies of the team boxes match a recomputation
+    for i in (1, 2):
+        s2 = systems.System(**{**mix[i].__dict__, "xyz": full.coords(i), "_keep": []})
+        assert np.allclose(e_mix[i], Box(s2).system_total(), rtol=1e-10, atol=1e-6)
+
+
 def test_batch_alone_equals_in_mix():
     mix = [systems.water_box(n, seed=100 + i, temperature=t)
            for i, (n, t) in enumerate([(250, 290.0), (600, 330.0), (400, 300.0), (900, 350.0)])]
