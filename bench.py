@@ -55,6 +55,23 @@ def move_flops(s, nkvec: int) -> float:
     return D * W_DIST + C * W_COUL + Lj * W_LJ + S * W_SINCOS + nkvec * W_KENERGY
 
 
+def copy_reference(nbytes: int) -> None:
+    """Three device-to-device copies of `nbytes` through the HIP runtime (its
+    blit kernel): a known-size reference for the HBM byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +86,9 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=60, help="0 skips the serving run")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (a known-size reference "
+                         "kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -186,6 +206,11 @@ def main() -> None:
                           f"boxes ({frac:.3f} of one MC cycle per box), one box per thread")
         print(f"# cpu oracle {cpu['value']:.0f} moves/s on {threads} threads", file=sys.stderr,
               flush=True)
+
+    if args.calibrate_copy > 0:
+        copy_reference(args.calibrate_copy << 20)
+        print(f"# copy reference: {args.calibrate_copy} MiB read + written per copy, 3 copies",
+              file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,

@@ -15,6 +15,7 @@
 // positions (one sincos per table entry), and advanced along nz.
 #include "batch.hpp"
 #include "device_common.hpp"
+#include "erfcx_poly.hpp"
 
 namespace gomc {
 namespace {
@@ -227,6 +228,47 @@ struct LjReal {
   double lj, real, ov;
 };
 
+// erfc for 0 <= x <= 4 as exp(-x^2) erfcx(x), erfcx from the fitted
+// polynomial of erfcx_poly.hpp (relative error about 3e-15); libm beyond.
+__device__ __forceinline__ double ErfcPoly(double x) {
+  if (!(x <= kErfcxUse)) return erfc(x);
+  const double t = (x - kErfcxMid) * kErfcxInvHalf;
+  double p = kErfcxPoly[kErfcxDeg];
+#pragma unroll
+  for (int k = kErfcxDeg - 1; k >= 0; --k) p = fma(p, t, kErfcxPoly[k]);
+  return exp(-(x * x)) * p;
+}
+
+// 1/sqrt(d2) to double precision: the hardware estimate and two Newton
+// steps y <- y (3/2 - d2 y^2 / 2).
+__device__ __forceinline__ double Rsqrt64(double d2) {
+  const double h = 0.5 * d2;
+  double y = __builtin_amdgcn_rsq(d2);
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
+}
+
+// FP64 pair terms: one reciprocal square root serves the Coulomb distance
+// and 1/r^2 of the Lennard-Jones ratio.
+template <>
+__device__ __forceinline__ void PairTerm<double>(const BatchBox &sb, double d2, int code,
+                                                 double &lj, double &real) {
+  const int pq = code & 0xff;
+  const double sign = (code >> 8) ? 1.0 : -1.0;
+  const double qiqj = sb.qq[pq], eps = sb.epsCn[pq];
+  const double ir = Rsqrt64(d2);
+  if (sb.elect && qiqj != 0.0 && !(sb.rCutCoulSq < d2)) {
+    const double ec = sb.ewald ? qiqj * ErfcPoly(sb.alpha * (d2 * ir)) * ir : qiqj * ir;
+    real += sign * ec;
+  }
+  if (eps != 0.0 && !(sb.rCutLJSq < d2)) {
+    const double rRat2 = sb.sigSq[pq] * (ir * ir);
+    const double att = rRat2 * rRat2 * rRat2;
+    lj += sign * (eps * (att * att - att));
+  }
+}
+
 // Evaluates `n` queued pairs (one per lane) and adds them to acc.
 template <typename T>
 __device__ __noinline__ LjReal DrainQueue(const BatchBox &sb, const LMEM PairQueue *q, int n,
@@ -236,80 +278,61 @@ __device__ __noinline__ LjReal DrainQueue(const BatchBox &sb, const LMEM PairQue
   return acc;
 }
 
+// The pair phase and the reciprocal phase of a move are independent once
+// the trial coordinates and phase tables exist, and both are bound by memory
+// and pipeline latency: waves 0-7 run the pair phase while waves 8-15 run
+// the reciprocal phase, with no barrier inside either.
+constexpr int kPairThreads = kBatchThreads / 2;
+constexpr int kPairWaves = kPairThreads / kWave;
+constexpr int kNearPerWave = kBatchNearCap / kPairWaves;
+
 // Pair-energy change of the move (MoleculeInter, src/CalculateEnergy.cpp:
-// 581-686) in two levels.  Level 1 (boxes with sb.cull): a molecule whose
-// centre is at least rc + 2 r_max from both the old and the new centre of
-// the moving molecule (minimum image) holds no atom pair inside the cutoff
-// (triangle inequality), so only the others ("near") are listed: a count
-// pass, a workgroup prefix over the waves, a store pass -- a fixed order and
-// two barriers per move.  Level 2: one atom of a near molecule (of every
-// other molecule without the cull) per work item, tested against the old and
-// new positions of all len moving atoms; distances inside the cutoff are
-// compacted into a per-wave queue (ballot + prefix count) and their energies
+// 581-686), in two levels, by the 8 waves of the pair group.  Level 1
+// (boxes with sb.cull): a molecule whose centre is at least rc + 2 r_max
+// from both the old and the new centre of the moving molecule (minimum
+// image) holds no atom pair inside the cutoff (triangle inequality); each
+// wave tests its own chunks of 64 molecules and lists the near ones in its
+// own part of nearList (at most kBatchNearCap / 8 = 512 molecules reach a
+// wave, so the list cannot overflow).  Level 2: one atom of a near molecule
+// (of every other molecule without the cull) per lane, tested against the
+// old and new positions of the moving atoms; distances inside the cutoff are
+// compacted into the wave's queue (ballot + prefix count) and their energies
 // evaluated 64 at a time, so the erfc path runs with full waves.  A team
-// member evaluates its share of the level-2 items.
-// CX/CY/CZ point to LDS or HBM (ComPtr).  Returns this thread's {lj, real,
-// overlap}: overlap = 1 when a new position lies inside rCutLow of an atom.
-template <typename T, typename ComPtr>
+// member culls its share of the molecules (or, without the cull, takes its
+// share of the items).  CX/CY/CZ point to LDS or HBM (ComPtr).  Returns this
+// thread's {lj, real, overlap}: overlap = 1 when a new position lies inside
+// rCutLow of an atom.
+template <typename T, int LEN, typename ComPtr>
 __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &ms,
                                          const GMEM double *X, const GMEM double *Y,
                                          const GMEM double *Z, ComPtr CX, ComPtr CY, ComPtr CZ,
-                                         int m, int len, int nmols, int rank, int team,
-                                         PairQueue *queues, int *nearList, int *waveCnt) {
+                                         int m, int nmols, int rank, int team, PairQueue *queues,
+                                         int *nearList) {
+  constexpr int len = LEN;
   BoxGeom g;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     g.L[d] = sb.L[d];
     g.half[d] = sb.half[d];
   }
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;  // wave < kPairWaves
   PairQueue &q = queues[wave];
+  int *myNear = nearList + wave * kNearPerWave;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int count = 0;  // wave-uniform
   LjReal acc{0.0, 0.0, 0.0};
   auto isNear = [&](int j) {
-    if (j >= nmols || j == m) return false;
+    if (j == m) return false;
     const double cx = CX[j], cy = CY[j], cz = CZ[j];
     const double d2o = DistSq(g, ms.com[0], ms.com[1], ms.com[2], cx, cy, cz);
     const double d2n = DistSq(g, ms.newCom[0], ms.newCom[1], ms.newCom[2], cx, cy, cz);
     return d2o < sb.nearCutSq || d2n < sb.nearCutSq;
   };
-  int total = nmols - 1;
-  if (sb.cull) {
-    int wc = 0;
-    for (int base = 0; base < nmols; base += kBatchThreads)
-      wc += __popcll(__ballot(isNear(base + int(threadIdx.x))));
-    if (lane == 0) waveCnt[wave] = wc;
-    __syncthreads();
-    int pos = 0;
-    total = 0;
-    for (int w = 0; w < kBatchThreads / kWave; ++w) {
-      const int c = waveCnt[w];
-      pos += (w < wave) ? c : 0;
-      total += c;
-    }
-    for (int base = 0; base < nmols; base += kBatchThreads) {
-      const int j = base + int(threadIdx.x);
-      const bool near = isNear(j);
-      const uint64_t nmask = __ballot(near);
-      if (near) nearList[pos + __popcll(nmask & below)] = j;
-      pos += __popcll(nmask);
-    }
-    __syncthreads();
-  }
-  // a team member takes its share [lo, hi) of the items
-  const int items = total * len;
-  const int per = (items + team - 1) / team;
-  const int lo = rank * per, hi = min(items, lo + per);
-  for (int ib = lo; ib < hi; ib += kBatchThreads) {
-    const int it = ib + int(threadIdx.x);
-    const bool valid = it < hi;
-    const int itc = valid ? it : 0;
-    const uint32_t k = uint32_t((uint64_t(uint32_t(itc)) * sb.magicLen) >> 32);  // itc / len
-    const int qa = itc - int(k) * len;
-    const int jm = sb.cull ? nearList[k] : (int(k) < m ? int(k) : int(k) + 1);
+  // one level-2 item: atom qa of molecule jm against the moving atoms
+  auto item = [&](bool valid, int jm, int qa) {
     const int jj = jm * len + qa;
     const double xj = X[jj], yj = Y[jj], zj = Z[jj];
+#pragma unroll
     for (int p = 0; p < len; ++p) {
       const int pq = p * kBatchMaxAtoms + qa;
       const bool live = valid && (sb.qq[pq] != 0.0 || sb.epsCn[pq] != 0.0);
@@ -342,9 +365,59 @@ __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &m
         }
       }
     }
+  };
+  if (sb.cull) {
+    const int mper = (nmols + team - 1) / team;
+    const int m0 = rank * mper, m1 = min(nmols, m0 + mper);
+    int nnear = 0;  // wave-uniform
+    for (int base = m0 + wave * kWave; base < m1; base += kPairThreads) {
+      const int j = base + lane;
+      const bool near = j < m1 && isNear(j);
+      const uint64_t nmask = __ballot(near);
+      if (near) myNear[nnear + __popcll(nmask & below)] = j;
+      nnear += __popcll(nmask);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int items = nnear * len;
+    for (int ib = 0; ib < items; ib += kWave) {
+      const int it = ib + lane;
+      const bool valid = it < items;
+      const int itc = valid ? it : 0;
+      const uint32_t k = uint32_t((uint64_t(uint32_t(itc)) * sb.magicLen) >> 32);  // itc / len
+      item(valid, myNear[k], itc - int(k) * len);
+    }
+  } else {
+    const int items = (nmols - 1) * len;
+    const int per = (items + team - 1) / team;
+    const int lo = rank * per, hi = min(items, lo + per);
+    for (int ib = lo + wave * kWave; ib < hi; ib += kPairThreads) {
+      const int it = ib + lane;
+      const bool valid = it < hi;
+      const int itc = valid ? it : 0;
+      const uint32_t k = uint32_t((uint64_t(uint32_t(itc)) * sb.magicLen) >> 32);  // itc / len
+      item(valid, int(k) < m ? int(k) : int(k) + 1, itc - int(k) * len);
+    }
   }
   __builtin_amdgcn_wave_barrier();
   return DrainQueue<T>(sb, Lp(&q), count, acc);
+}
+
+template <typename T, typename ComPtr>
+__device__ __forceinline__ LjReal PairDispatch(const BatchBox &sb, const MoveShared &ms,
+                                               const GMEM double *X, const GMEM double *Y,
+                                               const GMEM double *Z, ComPtr CX, ComPtr CY,
+                                               ComPtr CZ, int m, int len, int nmols, int rank,
+                                               int team, PairQueue *queues, int *nearList) {
+  switch (len) {
+    case 1:
+      return PairPhase<T, 1>(sb, ms, X, Y, Z, CX, CY, CZ, m, nmols, rank, team, queues, nearList);
+    case 2:
+      return PairPhase<T, 2>(sb, ms, X, Y, Z, CX, CY, CZ, m, nmols, rank, team, queues, nearList);
+    case 3:
+      return PairPhase<T, 3>(sb, ms, X, Y, Z, CX, CY, CZ, m, nmols, rank, team, queues, nearList);
+    default:
+      return PairPhase<T, 4>(sb, ms, X, Y, Z, CX, CY, CZ, m, nmols, rank, team, queues, nearList);
+  }
 }
 
 // Reciprocal-space energy change of the move (MolReciprocal,
@@ -357,8 +430,8 @@ __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &m
 // fixed (nx, ny).  The phase of each of the 2 LEN positions is formed from
 // the tables at the segment start, (ex * ey) * ez, and advanced along nz by
 // one complex product with e^{i c_z z} per step, so a k-vector costs no
-// table reads after the first of its segment.  The loads of two (FP64) or
-// four (mixed mode) steps are issued before they are used.
+// table reads after the first of its segment.  The loads of two steps are
+// issued before they are used.
 // pend: the previous move was accepted and its (dR, dI), still in the
 // thread's slots, are added to S(k) here (Ewald::UpdateRecip,
 // src/Ewald.cpp:1420) before the slots are reused.  Returns this thread's
@@ -402,9 +475,8 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
                                           GMEM double *__restrict__ sumR, GMEM double *__restrict__ sumI,
                                           GMEM double *__restrict__ dRs, GMEM double *__restrict__ dIs,
                                           int rank, int team, bool pend) {
-  // steps whose loads are issued together (FP64 phases leave room for two)
-  constexpr int GMax = sizeof(T) == 8 ? 2 : 4;
-  constexpr int G = S < GMax ? S : GMax;
+  // steps whose loads are issued together (registers leave room for two)
+  constexpr int G = S < 2 ? S : 2;
   T q[LEN];
   CplxT<T> stepO[LEN], stepN[LEN];
 #pragma unroll
@@ -415,7 +487,12 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
     stepN[p] = {Uniform(sn.re), Uniform(sn.im)};
   }
   double e = 0.0;
+  // the 512 threads of the reciprocal group each own two segment columns
+  // of the 1024-wide layout, t and t + 512
+  const int gt = int(threadIdx.x) - kPairThreads;
   for (int r = rank; r < sb.kChunk; r += team) {  // a team member's rounds
+   for (int h = 0; h < 2; ++h) {
+    const int t = gt + h * kPairThreads;
     CplxT<T> eO[LEN], eN[LEN];
 #pragma unroll 1
     for (int jb = 0; jb < S; jb += G) {
@@ -423,7 +500,7 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
       double r0[G], i0[G], pf[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int s = ((r * S) + jb + g) * kBatchThreads + int(threadIdx.x);
+        const int s = ((r * S) + jb + g) * kBatchThreads + t;
         code[g] = kn[s];
         r0[g] = sumR[s];
         i0[g] = sumI[s];
@@ -433,7 +510,7 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
         double ar[G], ai[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          const int s = ((r * S) + jb + g) * kBatchThreads + int(threadIdx.x);
+          const int s = ((r * S) + jb + g) * kBatchThreads + t;
           ar[g] = dRs[s];
           ai[g] = dIs[s];
         }
@@ -441,7 +518,7 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
         for (int g = 0; g < G; ++g) {
           // unused slots hold no change (their dRs/dIs are never written)
           if (code[g] >= 0) {
-            const int s = ((r * S) + jb + g) * kBatchThreads + int(threadIdx.x);
+            const int s = ((r * S) + jb + g) * kBatchThreads + t;
             r0[g] += ar[g];
             i0[g] += ai[g];
             sumR[s] = r0[g];
@@ -482,13 +559,14 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
         const double dR = used ? double(sRN - sRO) : 0.0;
         const double dI = used ? double(sIN - sIO) : 0.0;
         if (used) {
-          const int s = ((r * S) + j) * kBatchThreads + int(threadIdx.x);
+          const int s = ((r * S) + j) * kBatchThreads + t;
           dRs[s] = dR;
           dIs[s] = dI;
         }
         e += pf[g] * (dR * (2.0 * r0[g] + dR) + dI * (2.0 * i0[g] + dI));
       }
     }
+   }
   }
   return e;
 }
@@ -519,6 +597,7 @@ __device__ __forceinline__ double RecipDispatch(const BatchBox &sb,
                                                 GMEM double *dIs, int rank, int team, bool pend) {
   if (sb.kSeg == 8) return RecipPhaseLen<T, 8>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
   if (sb.kSeg == 4) return RecipPhaseLen<T, 4>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
+  if (sb.kSeg == 2) return RecipPhaseLen<T, 2>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
   return RecipPhaseLen<T, 1>(sb, tab, len, kn, pref, sumR, sumI, dRs, dIs, rank, team, pend);
 }
 
@@ -630,9 +709,8 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   __shared__ MoveShared ms;
   __shared__ double red[kBatchThreads / kWave][4];
   __shared__ BatchState st;  // running energies and move settings
-  __shared__ PairQueue queues[kBatchThreads / kWave];
+  __shared__ PairQueue queues[kPairWaves];
   __shared__ int nearList[kBatchNearCap];
-  __shared__ int waveCnt[kBatchThreads / kWave];
 
   const int tid = threadIdx.x;
   extern __shared__ double ldsCom[];
@@ -723,26 +801,26 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
 
       // ---- phase tables for the 2*len positions ----
       if (ewald) BuildTables<T>(sb, ms, len, nTab, tab);
-
-      // ---- pair phase ----
-      double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
-      LjReal pr;
-      if (inLds)
-        pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Lp(CXr), Lp(CYr), Lp(CZr), m, len, sb.nmols,
-                          rank, team, queues, nearList, waveCnt);
-      else
-        pr = PairPhase<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Gp(CXr), Gp(CYr), Gp(CZr), m, len, sb.nmols,
-                          rank, team, queues, nearList, waveCnt);
-      acc4[0] = pr.lj;
-      acc4[1] = pr.real;
-      acc4[3] = pr.ov;
       __syncthreads();  // tables complete
 
-      // ---- reciprocal phase: energy change per k-vector ----
-      if (ewald)
+      // ---- pair phase (waves 0-7) beside the reciprocal phase (waves 8-15) ----
+      double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
+      if (tid < kPairThreads) {
+        LjReal pr;
+        if (inLds)
+          pr = PairDispatch<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Lp(CXr), Lp(CYr), Lp(CZr), m, len,
+                               sb.nmols, rank, team, queues, nearList);
+        else
+          pr = PairDispatch<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Gp(CXr), Gp(CYr), Gp(CZr), m, len,
+                               sb.nmols, rank, team, queues, nearList);
+        acc4[0] = pr.lj;
+        acc4[1] = pr.real;
+        acc4[3] = pr.ov;
+      } else if (ewald) {
         acc4[2] = RecipDispatch<T>(sb, tab, len, Gp(kn), Gp(pref), Gp(a.sumR + koff),
                                    Gp(a.sumI + koff), Gp(a.dR + koff), Gp(a.dI + koff), rank, team,
                                    pend);
+      }
       BlockSum4(acc4, red);
 
       // ---- team total (wave 0), then Metropolis (thread 0) ----

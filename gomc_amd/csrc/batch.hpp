@@ -56,7 +56,7 @@ struct BatchBox {
   int natoms, nmols, molLen;
   int K;       // k-vector slots of the box: kChunk * kSeg * kBatchThreads
   int kChunk;  // rounds: segments per thread
-  int kSeg;    // S: k-vectors per segment (1, 4 or 8)
+  int kSeg;    // S: k-vectors per segment (1, 2, 4 or 8)
   int nmax[3], ewald, elect;
   int cull;  // 1: cull far molecules by their centres before atom pairs
   int team;  // workgroups running the box (1, 2, 4, 8); member r's coordinates

@@ -171,7 +171,7 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     const int64_t T = bt->teams ? TeamSize(b->nmols) : 1;
     int bestS = 1;
     int64_t bestCost = -1;
-    for (int S : {1, 4, 8}) {
+    for (int S : {1, 2, 4, 8}) {
       int64_t segs = 0;
       for (int L : runLen) segs += (L + S - 1) / S;
       const int64_t rounds = (segs + kBatchThreads - 1) / kBatchThreads;
