@@ -280,20 +280,18 @@ __device__ __noinline__ LjReal DrainQueue(const BatchBox &sb, const LMEM PairQue
 
 // The pair phase and the reciprocal phase of a move are independent once
 // the trial coordinates and phase tables exist, and both are bound by memory
-// and pipeline latency: waves 0-7 run the pair phase while waves 8-15 run
-// the reciprocal phase, with no barrier inside either.
-constexpr int kPairThreads = kBatchThreads / 2;
-constexpr int kPairWaves = kPairThreads / kWave;
-constexpr int kNearPerWave = kBatchNearCap / kPairWaves;
+// and pipeline latency: waves 0 .. sb.pairWaves-1 run the pair phase while
+// the other sb.kWidth threads run the reciprocal phase, with no barrier
+// inside either.
 
 // Pair-energy change of the move (MoleculeInter, src/CalculateEnergy.cpp:
-// 581-686), in two levels, by the 8 waves of the pair group.  Level 1
+// 581-686), in two levels, by the waves of the pair group.  Level 1
 // (boxes with sb.cull): a molecule whose centre is at least rc + 2 r_max
 // from both the old and the new centre of the moving molecule (minimum
 // image) holds no atom pair inside the cutoff (triangle inequality); each
 // wave tests its own chunks of 64 molecules and lists the near ones in its
-// own part of nearList (at most kBatchNearCap / 8 = 512 molecules reach a
-// wave, so the list cannot overflow).  Level 2: one atom of a near molecule
+// own part of nearList (kNearListLen / pairWaves entries: no more molecules
+// than that reach a wave, so the list cannot overflow).  Level 2: one atom of a near molecule
 // (of every other molecule without the cull) per lane, tested against the
 // old and new positions of the moving atoms; distances inside the cutoff are
 // compacted into the wave's queue (ballot + prefix count) and their energies
@@ -315,9 +313,10 @@ __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &m
     g.L[d] = sb.L[d];
     g.half[d] = sb.half[d];
   }
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;  // wave < kPairWaves
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;  // wave < pairWaves
+  const int pt = sb.pairWaves * kWave;
   PairQueue &q = queues[wave];
-  int *myNear = nearList + wave * kNearPerWave;
+  int *myNear = nearList + wave * (kNearListLen / sb.pairWaves);
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int count = 0;  // wave-uniform
   LjReal acc{0.0, 0.0, 0.0};
@@ -335,7 +334,7 @@ __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &m
 #pragma unroll
     for (int p = 0; p < len; ++p) {
       const int pq = p * kBatchMaxAtoms + qa;
-      const bool live = valid && (sb.qq[pq] != 0.0 || sb.epsCn[pq] != 0.0);
+      const bool live = valid && ((sb.live >> pq) & 1);
 #pragma unroll
       for (int isNew = 0; isNew < 2; ++isNew) {
         const double px = isNew ? ms.newp[0][p] : ms.oldp[0][p];
@@ -370,7 +369,7 @@ __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &m
     const int mper = (nmols + team - 1) / team;
     const int m0 = rank * mper, m1 = min(nmols, m0 + mper);
     int nnear = 0;  // wave-uniform
-    for (int base = m0 + wave * kWave; base < m1; base += kPairThreads) {
+    for (int base = m0 + wave * kWave; base < m1; base += pt) {
       const int j = base + lane;
       const bool near = j < m1 && isNear(j);
       const uint64_t nmask = __ballot(near);
@@ -390,7 +389,7 @@ __device__ __noinline__ LjReal PairPhase(const BatchBox &sb, const MoveShared &m
     const int items = (nmols - 1) * len;
     const int per = (items + team - 1) / team;
     const int lo = rank * per, hi = min(items, lo + per);
-    for (int ib = lo + wave * kWave; ib < hi; ib += kPairThreads) {
+    for (int ib = lo + wave * kWave; ib < hi; ib += pt) {
       const int it = ib + lane;
       const bool valid = it < hi;
       const int itc = valid ? it : 0;
@@ -487,12 +486,9 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
     stepN[p] = {Uniform(sn.re), Uniform(sn.im)};
   }
   double e = 0.0;
-  // the 512 threads of the reciprocal group each own two segment columns
-  // of the 1024-wide layout, t and t + 512
-  const int gt = int(threadIdx.x) - kPairThreads;
+  // thread t of the reciprocal group owns column t of the W-wide layout
+  const int W = sb.kWidth, t = int(threadIdx.x) - sb.pairWaves * kWave;
   for (int r = rank; r < sb.kChunk; r += team) {  // a team member's rounds
-   for (int h = 0; h < 2; ++h) {
-    const int t = gt + h * kPairThreads;
     CplxT<T> eO[LEN], eN[LEN];
 #pragma unroll 1
     for (int jb = 0; jb < S; jb += G) {
@@ -500,7 +496,7 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
       double r0[G], i0[G], pf[G];
 #pragma unroll
       for (int g = 0; g < G; ++g) {
-        const int s = ((r * S) + jb + g) * kBatchThreads + t;
+        const int s = ((r * S) + jb + g) * W + t;
         code[g] = kn[s];
         r0[g] = sumR[s];
         i0[g] = sumI[s];
@@ -510,7 +506,7 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
         double ar[G], ai[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-          const int s = ((r * S) + jb + g) * kBatchThreads + t;
+          const int s = ((r * S) + jb + g) * W + t;
           ar[g] = dRs[s];
           ai[g] = dIs[s];
         }
@@ -518,7 +514,7 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
         for (int g = 0; g < G; ++g) {
           // unused slots hold no change (their dRs/dIs are never written)
           if (code[g] >= 0) {
-            const int s = ((r * S) + jb + g) * kBatchThreads + t;
+            const int s = ((r * S) + jb + g) * W + t;
             r0[g] += ar[g];
             i0[g] += ai[g];
             sumR[s] = r0[g];
@@ -559,14 +555,13 @@ __device__ __noinline__ double RecipPhase(const BatchBox &sb, const CplxT<T> (*t
         const double dR = used ? double(sRN - sRO) : 0.0;
         const double dI = used ? double(sIN - sIO) : 0.0;
         if (used) {
-          const int s = ((r * S) + j) * kBatchThreads + t;
+          const int s = ((r * S) + j) * W + t;
           dRs[s] = dR;
           dIs[s] = dI;
         }
         e += pf[g] * (dR * (2.0 * r0[g] + dR) + dI * (2.0 * i0[g] + dI));
       }
     }
-   }
   }
   return e;
 }
@@ -606,9 +601,9 @@ __device__ __forceinline__ double RecipDispatch(const BatchBox &sb,
 __device__ __noinline__ void RecipFold(const BatchBox &sb, int rank, int team, const GMEM int32_t *kn,
                                        GMEM double *sumR, GMEM double *sumI, const GMEM double *dRs,
                                        const GMEM double *dIs) {
+  const int per = sb.kSeg * sb.kWidth;  // slots of a round
   for (int r = rank; r < sb.kChunk; r += team)
-    for (int j = 0; j < sb.kSeg; ++j) {
-      const int s = (r * sb.kSeg + j) * kBatchThreads + int(threadIdx.x);
+    for (int s = r * per + int(threadIdx.x); s < (r + 1) * per; s += kBatchThreads) {
       if (kn[s] >= 0) {
         sumR[s] += dRs[s];
         sumI[s] += dIs[s];
@@ -616,51 +611,53 @@ __device__ __noinline__ void RecipFold(const BatchBox &sb, int rank, int team, c
     }
 }
 
-// Team exchange, run by wave 0 of each member (acc valid in lane 0): lane 0
-// publishes this member's partial sums {lj, real, recip, overlap} for move
-// `seq`; lane q (q < team, q != rank) waits for member q's counter and
-// loads its partials, so all waits and loads are in flight together; every
-// lane then forms the team total in member order, the same bits in every
-// member.  Words go through device-coherent atomics and the counter is
-// written after the partials have landed.  A wait that outlasts
-// kTeamSpinLimit polls sets *err and ok = false.
+// Team exchange, run by wave 0 of each member (acc valid in lane 0).  Each
+// partial sum {lj, real, recip, overlap} is published with its 8 lowest
+// mantissa bits replaced by the move tag (seq & 255), so a reader knows a
+// word is this move's without a separate counter: lane 0 writes the
+// member's four tagged words for move `seq`; lane q (q < team, q != rank)
+// polls member q's words until all four carry the tag.  Slots alternate
+// with the move parity, so a stale word is two moves old and carries
+// another tag.  Every member clears the tag bits and sums the same values
+// in member order (a partial loses at most 2^-44 of itself), so all
+// members hold the same bits.  Words go through device-coherent atomics.  A wait
+// that outlasts kTeamSpinLimit polls sets *err and ok = false.
 struct Sum4 {
   double v[4];
   bool ok;
 };
 
+__device__ __forceinline__ uint64_t Tagged(double v, uint64_t tag) {
+  return (__builtin_bit_cast(uint64_t, v) & ~uint64_t(0xff)) | tag;
+}
+
 __device__ __noinline__ Sum4 TeamSum(GMEM uint64_t *xb, int team, int rank, uint64_t seq, Sum4 acc,
                                      GMEM int *err) {
   const int lane = int(threadIdx.x) & (kWave - 1);
-  GMEM uint64_t *data = xb;
-  GMEM uint64_t *flags = xb + 2 * kTeamMax * 4;
-  double p[4];
+  const uint64_t tag = seq & 0xff;
+  GMEM uint64_t *data = xb + (seq & 1) * kTeamMax * 4;
+  uint64_t w[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) p[i] = __shfl(acc.v[i], 0);
+  for (int i = 0; i < 4; ++i) w[i] = Tagged(__shfl(acc.v[i], 0), tag);
   if (lane == 0) {
-    GMEM uint64_t *mine = data + ((seq & 1) * kTeamMax + rank) * 4;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-      __hip_atomic_exchange(mine + i, __builtin_bit_cast(uint64_t, p[i]), __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);
-    __hip_atomic_exchange(flags + rank, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_exchange(data + rank * 4 + i, w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   int ok = 1;
   if (lane < team && lane != rank) {
-    int64_t n = 0;
-    while (__hip_atomic_load(flags + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < seq) {
-      if (++n > kTeamSpinLimit) {
+    GMEM uint64_t *theirs = data + lane * 4;
+    for (int64_t n = 0;; ++n) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        w[i] = __hip_atomic_load(theirs + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((w[0] & 0xff) == tag && (w[1] & 0xff) == tag && (w[2] & 0xff) == tag &&
+          (w[3] & 0xff) == tag)
+        break;
+      if (n > kTeamSpinLimit) {
         ok = 0;
         break;
       }
-    }
-    if (ok) {
-      GMEM uint64_t *theirs = data + ((seq & 1) * kTeamMax + lane) * 4;
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-        p[i] = __builtin_bit_cast(
-            double, __hip_atomic_load(theirs + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     }
   }
   Sum4 out{{0.0, 0.0, 0.0, 0.0}, __all(ok) != 0};
@@ -668,6 +665,9 @@ __device__ __noinline__ Sum4 TeamSum(GMEM uint64_t *xb, int team, int rank, uint
     if (lane == 0) __hip_atomic_exchange(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return out;
   }
+  double p[4];  // the tag bits cleared: the partials truncated alike in every member
+#pragma unroll
+  for (int i = 0; i < 4; ++i) p[i] = __builtin_bit_cast(double, w[i] & ~uint64_t(0xff));
   for (int q = 0; q < team; ++q) {
     out.v[0] += __shfl(p[0], q);
     out.v[1] += __shfl(p[1], q);
@@ -709,8 +709,8 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
   __shared__ MoveShared ms;
   __shared__ double red[kBatchThreads / kWave][4];
   __shared__ BatchState st;  // running energies and move settings
-  __shared__ PairQueue queues[kPairWaves];
-  __shared__ int nearList[kBatchNearCap];
+  __shared__ PairQueue queues[kPairWavesMax];
+  __shared__ int nearList[kNearListLen];
 
   const int tid = threadIdx.x;
   extern __shared__ double ldsCom[];
@@ -739,6 +739,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     // (a.ldsMols per workgroup); CXr/CYr/CZr are where the pair phase reads them.
     const bool inLds = __builtin_amdgcn_readfirstlane(sb.nmols) <= a.ldsMols;
     const bool ewald = __builtin_amdgcn_readfirstlane(sb.ewald) != 0;
+    const int pairThreads = __builtin_amdgcn_readfirstlane(sb.pairWaves) * kWave;
     double *CXr = CXg, *CYr = CYg, *CZr = CZg;
     if (inLds) {
       CXr = ldsCom;
@@ -803,9 +804,9 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
       if (ewald) BuildTables<T>(sb, ms, len, nTab, tab);
       __syncthreads();  // tables complete
 
-      // ---- pair phase (waves 0-7) beside the reciprocal phase (waves 8-15) ----
+      // ---- pair phase beside the reciprocal phase ----
       double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // lj, real, recip, overlap
-      if (tid < kPairThreads) {
+      if (tid < pairThreads) {
         LjReal pr;
         if (inLds)
           pr = PairDispatch<T>(sb, ms, Gp(X), Gp(Y), Gp(Z), Lp(CXr), Lp(CYr), Lp(CZr), m, len,

@@ -23,9 +23,9 @@ constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of every box strea
 // holding its own copy of the coordinates and a share of the pair items and
 // k-vector rounds; per move the members exchange their partial energy sums.
 constexpr int kTeamMax = 8;
-// Exchange block per box (64-bit words): partial sums [2 parities][8][4],
-// then one move counter per member.
-constexpr int kXchgWords = 2 * kTeamMax * 4 + kTeamMax;
+// Exchange block per box (64-bit words): tagged partial sums
+// [2 parities][8 members][4].
+constexpr int kXchgWords = 2 * kTeamMax * 4;
 // Polls of a member's counter before a team gives up and flags an error.
 constexpr int64_t kTeamSpinLimit = int64_t(1) << 22;
 // Molecules whose FP64 centres fit in the LDS left beside the static
@@ -35,6 +35,10 @@ constexpr int kBatchLdsMolCap = 4096;
 // Capacity of the per-move list of molecules near the moving one; boxes of
 // more molecules skip the centre cull.
 constexpr int kBatchNearCap = 4096;
+// Pair-phase waves per box and the near-list entries each wave may hold:
+// a wave tests at most ceil(4096 / (64 w)) chunks of 64 molecules.
+constexpr int kPairWavesMax = 12;
+constexpr int kNearListLen = 4608;  // 12 x 384, 8 x 576, 4 x 1152
 
 // Read-only description of one box (host-filled, 8-byte aligned fields).
 struct BatchBox {
@@ -61,7 +65,10 @@ struct BatchBox {
   int cull;  // 1: cull far molecules by their centres before atom pairs
   int team;  // workgroups running the box (1, 2, 4, 8); member r's coordinates
              // sit at atomOff + r * natoms and molOff + r * nmols
-  int pad;
+  int live;  // bit p * kBatchMaxAtoms + q: atoms p, q interact (charge or LJ)
+  int pairWaves;  // waves 0 .. pairWaves-1 run the pair phase (4, 8 or 12)
+  int kWidth;     // W = 1024 - 64 pairWaves: threads of the reciprocal phase and
+                  // columns of the k-vector layout
 };
 
 static_assert(sizeof(BatchBox) % 8 == 0, "BatchBox is copied as 64-bit words");
@@ -97,7 +104,7 @@ struct BatchArgs {
   double *x, *y, *z;          // [sum atoms]
   double *cx, *cy, *cz;       // [sum mols] centres of mass
   // k-vector arrays, [sum K] slots; per box, segments of S consecutive nz
-  // at fixed (nx, ny): thread t, round r, step j at slot (r*S + j)*1024 + t
+  // at fixed (nx, ny): column t, round r, step j at slot (r*S + j)*W + t
   const int32_t *kn;          // packed (nx, ny+128, nz+128) bytes, -1 unused slot
   const double *pref;
   double *sumR, *sumI;        // reference structure factors

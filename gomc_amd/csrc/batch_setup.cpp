@@ -100,6 +100,20 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
   // the cull pays when the near sphere is well inside the box
   P.cull = (b->nmols <= kBatchNearCap &&
             (4.0 / 3.0) * M_PI * nearCut * nearCut * nearCut < 0.5 * b->volume) ? 1 : 0;
+  {
+    // Waves for the pair phase; the rest run the reciprocal phase beside
+    // it.  Measured on MI355X (water, 200-4000 molecules): 12 pair waves
+    // are fastest while a single workgroup holds at most 2048 k-vectors
+    // (8 per reciprocal thread), 8 beyond that and in teams.
+    const int T = bt->teams ? TeamSize(b->nmols) : 1;
+    const int kper = b->ewald ? b->K : 0;
+    P.pairWaves = (T == 1 && kper <= 2048) ? 12 : 8;
+    if (const char *ov = std::getenv("GOMC_PAIR_WAVES")) {
+      const int w = std::atoi(ov);
+      if (w == 4 || w == 8 || w == 12) P.pairWaves = w;
+    }
+    P.kWidth = kBatchThreads - kWave * P.pairWaves;
+  }
   P.rCutSq = b->geom.rCutSq;
   P.rCutLJSq = b->geom.rCutLJSq;
   P.rCutCoulSq = b->geom.rCutCoulSq;
@@ -118,6 +132,7 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
       if (b->epsCn[idx] != 0.0 && b->nHalf[idx] != 6.0)
         return BFail(bt, GOMC_ERR_ARG, "batched boxes use the 12-6 Lennard-Jones form");
       P.qq[pq] = b->kindCharge[0][p] * b->kindCharge[0][q] * kQQFact;
+      if (P.qq[pq] != 0.0 || P.epsCn[pq] != 0.0) P.live |= 1 << pq;
     }
   }
   P.atomOff = atomOff;
@@ -154,8 +169,8 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     if (rc || int(pref.size()) != b->K) return BFail(bt, GOMC_ERR_STATE, "k-vector setup mismatch");
     // Segments: each run of consecutive nz at fixed (nx, ny) (contiguous
     // in the reference order) is cut into pieces of at most S k-vectors;
-    // thread t takes segment r * kBatchThreads + t in round r, its step j
-    // at slot (r * S + j) * kBatchThreads + t.  Unused slots carry kn -1.
+    // column t of the W-wide layout takes segment r * W + t in round r, its
+    // step j at slot (r * S + j) * W + t.  Unused slots carry kn -1.
     std::vector<int> runStart, runLen;
     for (int k = 0; k < b->K; ++k) {
       if (k > 0 && packed[k] == packed[k - 1] + (1 << 16)) {
@@ -165,6 +180,7 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
         runLen.push_back(1);
       }
     }
+    const int W = P.kWidth;
     // per round a thread pays about three table products to start a
     // segment and one complex product per step; a team of T splits the
     // rounds
@@ -174,7 +190,7 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     for (int S : {1, 2, 4, 8}) {
       int64_t segs = 0;
       for (int L : runLen) segs += (L + S - 1) / S;
-      const int64_t rounds = (segs + kBatchThreads - 1) / kBatchThreads;
+      const int64_t rounds = (segs + W - 1) / W;
       const int64_t cost = ((rounds + T - 1) / T) * (S + 3);
       if (bestCost < 0 || cost < bestCost) {
         bestCost = cost;
@@ -184,12 +200,12 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     const int S = bestS;
     int64_t segs = 0;
     for (int L : runLen) segs += (L + S - 1) / S;
-    const int64_t rounds = (segs + kBatchThreads - 1) / kBatchThreads;
-    if (rounds * S * kBatchThreads > (int64_t(1) << 30))
+    const int64_t rounds = (segs + W - 1) / W;
+    if (rounds * S * W > (int64_t(1) << 30))
       return BFail(bt, GOMC_ERR_ARG, "too many k-vectors for the batched layout");
     P.kSeg = S;
     P.kChunk = int(rounds);
-    P.K = int(rounds * S * kBatchThreads);
+    P.K = int(rounds * S * W);
     out.pref.assign(P.K, 0.0);
     out.kn.assign(P.K, -1);
     out.sr.assign(P.K, 0.0);
@@ -197,10 +213,10 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     int64_t seg = 0;
     for (size_t r = 0; r < runLen.size(); ++r) {
       for (int off = 0; off < runLen[r]; off += S, ++seg) {
-        const int64_t round = seg / kBatchThreads, t = seg % kBatchThreads;
+        const int64_t round = seg / W, t = seg % W;
         for (int j = 0; j < S && off + j < runLen[r]; ++j) {
           const int k = runStart[r] + off + j;
-          const int64_t slot = (round * S + j) * kBatchThreads + t;
+          const int64_t slot = (round * S + j) * W + t;
           out.pref[slot] = pref[k];
           out.kn[slot] = packed[k];
           out.sr[slot] = sr[k];
@@ -321,13 +337,13 @@ int UploadBox(gomc_batch *bt, int slot, const PackedBox &pb) {
 }
 
 int TeamSize(int nmols) {
-  // Per-move cost of one workgroup grows about 18 us + 4.5 ns per molecule
+  // Per-move cost of one workgroup grows about 10 us + 6 ns per molecule
   // (measured on MI355X, water); a team of T repeats ~7 us of it per member
-  // (draws, trial coordinates, tables, the centre cull, the exchange) and
-  // splits the rest.  The thresholds keep a box's cycle near 50 ms.
-  if (nmols >= 3600) return 8;
-  if (nmols >= 2800) return 4;
-  if (nmols >= 2000) return 2;
+  // (draws, trial coordinates, tables, the exchange) and splits the rest.
+  // The thresholds keep a box's cycle near 50 ms.
+  if (nmols >= 3500) return 8;
+  if (nmols >= 2900) return 4;
+  if (nmols >= 1800) return 2;
   return 1;
 }
 

@@ -11,13 +11,11 @@ namespace gomc {
 constexpr double kQQFact = 167103.208067979;  // lib/NumLib.h:22
 constexpr int kWave = 64;
 
-// BoxDimensions::MinImageSigned (src/BoxDimensions.h:169-175).
+// BoxDimensions::MinImageSigned (src/BoxDimensions.h:169-175): the same
+// single add or subtract, chosen by selects rather than branches.
 __device__ __forceinline__ double MinImage(double d, double ax, double half) {
-  if (d > half)
-    d -= ax;
-  else if (d < -half)
-    d += ax;
-  return d;
+  const double down = d - ax, up = d + ax;
+  return d > half ? down : (d < -half ? up : d);
 }
 
 // BoxDimensions::WrapPBC (src/BoxDimensions.cpp:261-296).

@@ -236,7 +236,7 @@ typedef struct gomc_batch gomc_batch;
  * its own Philox4x64-10 stream (key {seed, tag}, counter {move, block}).
  * precision: 0 = FP64 everywhere; 1 = FP32 pair terms and phases with FP64
  * accumulation, running totals and Metropolis exponent.
- * A box of 2000 molecules or more runs on a team of 2, 4 or 8 workgroups
+ * A box of 1800 molecules or more runs on a team of 2, 4 or 8 workgroups
  * (a function of its molecule count), unless the batch holds more large
  * boxes than the device fits, when the smallest teams are halved.  A box's
  * results depend only on the box and its team size. */
