@@ -729,9 +729,13 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     // scalar registers, with the pointers derived from them
     const int len = __builtin_amdgcn_readfirstlane(sb.molLen);
     const int team = __builtin_amdgcn_readfirstlane(sb.team > 1 ? sb.team : 1);
-    // member `rank` works on its own copy of the coordinates and centres
-    const int64_t aoff = UniformI64(sb.atomOff + int64_t(rank) * sb.natoms);
-    const int64_t moff = UniformI64(sb.molOff + int64_t(rank) * sb.nmols);
+    // Team members share the box's coordinates and centres.  Every member
+    // writes the same accepted values at the same point of a move, after the
+    // exchange that needs all members' pair phases of that move, and reads
+    // them only after its own write and a barrier: a read sees its own
+    // member's write or another's identical one.
+    const int64_t aoff = UniformI64(sb.atomOff);
+    const int64_t moff = UniformI64(sb.molOff);
     const int64_t koff = UniformI64(sb.kOff);
     double *X = a.x + aoff, *Y = a.y + aoff, *Z = a.z + aoff;
     double *CXg = a.cx + moff, *CYg = a.cy + moff, *CZg = a.cz + moff;

@@ -20,8 +20,8 @@ constexpr int kBatchMaxN = 32;       // largest |n| of a k-vector component
 constexpr int kDrawChunk = 64;       // moves whose Philox draws are made at once
 constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of every box stream
 // Teams: a large box may be run by T = 2, 4 or 8 workgroups at once, each
-// holding its own copy of the coordinates and a share of the pair items and
-// k-vector rounds; per move the members exchange their partial energy sums.
+// taking a share of the pair items and k-vector rounds; per move the
+// members exchange their partial energy sums.
 constexpr int kTeamMax = 8;
 // Exchange block per box (64-bit words): tagged partial sums
 // [2 parities][8 members][4].
@@ -63,8 +63,7 @@ struct BatchBox {
   int kSeg;    // S: k-vectors per segment (1, 2, 4 or 8)
   int nmax[3], ewald, elect;
   int cull;  // 1: cull far molecules by their centres before atom pairs
-  int team;  // workgroups running the box (1, 2, 4, 8); member r's coordinates
-             // sit at atomOff + r * natoms and molOff + r * nmols
+  int team;  // workgroups running the box (1, 2, 4, 8)
   int live;  // bit p * kBatchMaxAtoms + q: atoms p, q interact (charge or LJ)
   int pairWaves;  // waves 0 .. pairWaves-1 run the pair phase (4, 8 or 12)
   int kWidth;     // W = 1024 - 64 pairWaves: threads of the reciprocal phase and

@@ -318,8 +318,8 @@ int UploadBox(gomc_batch *bt, int slot, const PackedBox &pb) {
   };
   BHIP(bt, up(bt->boxes + slot, &pb.P, sizeof(BatchBox)));
   BHIP(bt, up(bt->state + slot, &pb.S, sizeof(BatchState)));
-  for (int r = 0; r < std::max(1, P.team); ++r) {  // one copy per team member
-    const int64_t ao = P.atomOff + int64_t(r) * P.natoms, mo = P.molOff + int64_t(r) * P.nmols;
+  {
+    const int64_t ao = P.atomOff, mo = P.molOff;
     BHIP(bt, up(bt->x + ao, pb.x.data(), sizeof(double) * pb.x.size()));
     BHIP(bt, up(bt->y + ao, pb.y.data(), sizeof(double) * pb.y.size()));
     BHIP(bt, up(bt->z + ao, pb.z.data(), sizeof(double) * pb.z.size()));
@@ -482,8 +482,8 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
     P.atomOff = atoms;
     P.molOff = mols;
     P.kOff = kvec;
-    atoms += int64_t(P.natoms) * P.team;
-    mols += int64_t(P.nmols) * P.team;
+    atoms += P.natoms;
+    mols += P.nmols;
     kvec += P.K;
     maxMols = std::max(maxMols, P.nmols);
   }
