@@ -33,6 +33,10 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("MC trial moves/s (1 GPU), batched+served vs plain at ≤1e-6 rel ΔE; "
           "p99 latency")
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
+# HBM bytes per launch of McBatchKernel<0> on the default 256-box workload,
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
+# (profiles/McBatchKernel_water256_fp64_hbm.md); reported for that workload.
+PROFILED_TRAFFIC = {"bytes": 158.3e9, "source": "profiles/McBatchKernel_water256_fp64_hbm.md"}
 # Fixed per-term FLOP weights of the algorithmic work model: distance +
 # minimum image 12, erfc-Coulomb 30, LJ 15, sincos + 2 FMA 24, per-k energy 6.
 W_DIST, W_COUL, W_LJ, W_SINCOS, W_KENERGY = 12, 30, 15, 24, 6
@@ -212,6 +216,7 @@ def main() -> None:
         print(f"# copy reference: {args.calibrate_copy} MiB read + written per copy, 3 copies",
               file=sys.stderr, flush=True)
 
+    default_workload = args.boxes == 256 and args.precision == 0
     out = {
         "metric": METRIC,
         "value": value,
@@ -249,7 +254,8 @@ def main() -> None:
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": PROFILED_TRAFFIC["bytes"] if default_workload else None,
+            "traffic_source": PROFILED_TRAFFIC["source"] if default_workload else None,
             "kernel": "McBatchKernel",
             "flops_per_launch": flops_step,
         },

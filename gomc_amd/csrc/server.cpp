@@ -326,7 +326,7 @@ int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d, int64_t moves
     return GOMC_ERR_ARG;
   }
   // per-move cost of the batched kernel at this size, FP64, microseconds
-  j->costUs = 16.0 + 0.0125 * double(j->pb.P.nmols);
+  j->costUs = 10.0 + 0.006 * double(j->pb.P.nmols);
   std::lock_guard<std::mutex> g(sv->mu);
   j->id = sv->nextId++;
   j->tSubmit = Now(sv->t0);
