@@ -90,6 +90,8 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=60, help="0 skips the serving run")
+    ap.add_argument("--serve-gcmc-jobs", type=int, default=12,
+                    help="GCMC state-point jobs in the served run (0 skips)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (a known-size reference "
                          "kernel for HBM byte counters)")
@@ -175,6 +177,40 @@ def main() -> None:
         print(f"# served {served['moves_per_s']:.0f} moves/s p50 {served['p50_s']:.2f} s "
               f"p99 {served['p99_s']:.2f} s", file=sys.stderr, flush=True)
 
+    served_gcmc = None
+    if args.serve_gcmc_jobs > 0:
+        from gomc_amd.serve import Server, poisson_stream_gcmc, run_stream
+        from oracle.pyoracle import Oracle
+        jobs = poisson_stream_gcmc(args.serve_gcmc_jobs, rate=20.0, seed=17, lmin=20.0,
+                                   lmax=30.0, cycles=5)
+        sv = Server(slots=64, budget_bytes=8 << 30, tick_ms=20.0)
+        rep = run_stream(sv, jobs, timeout=600.0)
+        sv.close()
+        # <N> of the first job against the oracle under the same draws
+        j0, r0 = jobs[0], rep["results"][0]
+        o = Oracle(j0.system, threads=min(16, os.cpu_count() or 1))
+        done, sums, samples = 0, np.zeros(2), 0
+        while done < j0.moves:
+            n_in, _ = o.mol_boxes()
+            chunk = min(max(1, n_in), j0.moves - done)
+            o.run(chunk, start_step=done, mode=0, record=False)
+            done += chunk
+            _, where = o.mol_boxes()
+            for k in (0, 1):
+                sums[k] += np.count_nonzero((where == 0) & (j0.system.mol_kind == k))
+            samples += 1
+        served_gcmc = dict(
+            workload=f"Poisson(20/s) stream of {rep['jobs']} GCMC methane/CO2 adsorption jobs, "
+                     "L 20-30 A, T 250-350 K, fugacities log-uniform 1e3-1e7 Pa, 5 cycles each; "
+                     "GCMC lane (plain FP64 path, 8 concurrent jobs on their own streams)",
+            moves_per_s=rep["moves_per_s"], p50_s=rep["p50_s"], p99_s=rep["p99_s"],
+            moves=rep["moves"], wall_s=rep["wall_s"],
+            mean_n_job0=r0["mean_n"].tolist(), mean_n_job0_oracle=(sums / samples).tolist())
+        print(f"# served GCMC {served_gcmc['moves_per_s']:.0f} moves/s p50 "
+              f"{served_gcmc['p50_s']:.2f} s p99 {served_gcmc['p99_s']:.2f} s; <N> job 0 "
+              f"{served_gcmc['mean_n_job0']} oracle {served_gcmc['mean_n_job0_oracle']}",
+              file=sys.stderr, flush=True)
+
     plain = None
     if not args.no_plain:
         idx = np.linspace(0, len(boxes) - 1, min(args.plain_boxes, len(boxes))).astype(int)
@@ -244,6 +280,7 @@ def main() -> None:
         "graph": "one HIP graph per MC cycle, replayed",
         "mixed_precision": mixed,
         "served": served,
+        "served_gcmc": served_gcmc,
         "plain_fp64_gpu": plain,
         "speedup_vs_plain": (value / plain["moves_per_s"]) if plain else None,
         "mixed_speedup_vs_plain": (mixed["moves_per_s"] / plain["moves_per_s"])

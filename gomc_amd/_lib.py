@@ -69,6 +69,7 @@ SIGNATURES = {
     "gomc_server_last_error": (C.c_char_p, [_vp]),
     "gomc_server_submit": (C.c_int, [_vp, C.POINTER(SystemDesc), C.c_int64, _lp]),
     "gomc_server_poll": (C.c_int, [_vp, C.c_int64, _ip, _dp, _dp, _lp, _lp]),
+    "gomc_server_poll_counts": (C.c_int, [_vp, C.c_int64, _dp]),
     "gomc_server_drain": (C.c_int, [_vp, C.c_double]),
     "gomc_server_stats": (C.c_int, [_vp, _dp]),
     "gomc_batch_run_cycles": (C.c_int, [_vp, C.c_int, C.c_int]),

@@ -15,7 +15,17 @@
 // A job's moves draw from its own Philox stream indexed by its move
 // number, so its result does not depend on which other jobs share the
 // launches or how its moves are split between launches.
+//
+// GCMC jobs (ensemble 1: insertion/deletion with CBMC growth, two molecule
+// kinds) run in a second lane: kGcmcWorkers host threads each take the next
+// queued GCMC job while the budget holds its bytes, build it as a device box
+// on its own stream and run the reference move loop (MT19937 draws) to the
+// end, sampling the molecules of each kind in the box after every MC cycle.
+// The lanes share the byte budget (the batched lane's arenas are reserved
+// from it at creation; a GCMC job's device box is allocated on admission);
+// jobs of both lanes overlap on the device.
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -69,9 +79,45 @@ class Arena {
   std::map<int64_t, int64_t> free_;
 };
 
+constexpr int kGcmcWorkers = 8;  // GCMC jobs running at once, each on its own stream
 constexpr int64_t kAtomBytes = 3 * sizeof(double);                          // x, y, z
 constexpr int64_t kMolBytes = 3 * sizeof(double);                           // centres
 constexpr int64_t kKBytes = sizeof(int32_t) + 5 * sizeof(double);           // kn, pref, S, dS
+
+// A descriptor deep-copied at submission, for jobs that are built later
+// (the caller's arrays may be gone by then).
+struct DescCopy {
+  gomc_system_desc d{};
+  std::vector<double> sigma, eps, nrep, kcharge, x, y, z, fug;
+  std::vector<int32_t> knat, ktype, molKind, molBox;
+};
+
+std::unique_ptr<DescCopy> CopyDesc(const gomc_system_desc &d) {
+  auto c = std::make_unique<DescCopy>();
+  c->d = d;
+  const int nt = d.n_atom_types, nk = d.n_kinds, nm = d.n_mols;
+  auto cp = [](auto *src, size_t n, auto &dst) {
+    dst.assign(src, src + n);
+    return dst.data();
+  };
+  c->d.sigma = cp(d.sigma, nt, c->sigma);
+  c->d.epsilon = cp(d.epsilon, nt, c->eps);
+  c->d.n_rep = cp(d.n_rep, nt, c->nrep);
+  c->d.kind_natoms = cp(d.kind_natoms, nk, c->knat);
+  int64_t kindAtoms = 0;
+  for (int k = 0; k < nk; ++k) kindAtoms += d.kind_natoms[k];
+  c->d.kind_atom_type = cp(d.kind_atom_type, kindAtoms, c->ktype);
+  c->d.kind_charge = cp(d.kind_charge, kindAtoms, c->kcharge);
+  c->d.mol_kind = cp(d.mol_kind, nm, c->molKind);
+  int64_t natoms = 0;
+  for (int m = 0; m < nm; ++m) natoms += d.kind_natoms[d.mol_kind[m]];
+  c->d.x = cp(d.x, natoms, c->x);
+  c->d.y = cp(d.y, natoms, c->y);
+  c->d.z = cp(d.z, natoms, c->z);
+  c->d.mol_box = d.mol_box ? cp(d.mol_box, nm, c->molBox) : nullptr;
+  c->d.kind_fugacity = d.kind_fugacity ? cp(d.kind_fugacity, nk, c->fug) : nullptr;
+  return c;
+}
 
 struct Job {
   int64_t id = 0, totalMoves = 0, done = 0;
@@ -83,6 +129,13 @@ struct Job {
   int64_t accepted = 0;
   double costUs = 1.0;  // per-move cost estimate
   PackedBox pb;
+  // GCMC lane: the job runs on its own device box and stream (the plain
+  // path), from a deep copy of its descriptor
+  bool gcmc = false;
+  std::unique_ptr<DescCopy> desc;
+  int64_t gcmcBytes = 0;
+  double sumN[2] = {0, 0};  // molecules of kinds 0, 1 in the box, summed over samples
+  int64_t samples = 0;
 };
 
 double Now(const std::chrono::steady_clock::time_point &t0) {
@@ -108,6 +161,10 @@ struct gomc_server {
   std::string err;
   std::thread loop;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  // GCMC lane
+  std::deque<Job *> gcmcQueue;
+  std::vector<std::thread> workers;
+  int gcmcRunning = 0;
 };
 
 namespace {
@@ -248,6 +305,105 @@ void Loop(gomc_server *sv) {
   }
 }
 
+// Device bytes of a GCMC job on the plain path: coordinates, charges,
+// kinds, cell list, forces are not built; k-vector arrays (index triple,
+// prefactor, two structure-factor pairs) and per-molecule data.
+int64_t GcmcBytes(const gomc_system_desc &d) {
+  int64_t natoms = 0;
+  for (int m = 0; m < d.n_mols; ++m) natoms += d.kind_natoms[d.mol_kind[m]];
+  const double a = std::sqrt(-std::log(d.tolerance)) / d.rcut_coulomb;
+  const double kc = -2.0 * std::log(d.tolerance) / d.rcut_coulomb;
+  (void)a;
+  // k-vectors in the half sphere |k| < kc: (2/3) pi (kc L / 2 pi)^3
+  double kv = 2.0 / 3.0 * M_PI;
+  for (int i = 0; i < 3; ++i) kv *= kc * d.box[i] / (2.0 * M_PI);
+  return natoms * 96 + int64_t(d.n_mols) * 64 + int64_t(kv) * 64 + (int64_t(1) << 20);
+}
+
+// Worker of the GCMC lane: takes queued GCMC jobs in arrival order while
+// the byte budget allows, and runs each to completion on its own box
+// (gomc_run_moves in chunks of one MC cycle, N_box moves), sampling the
+// number of molecules of each kind in the box after every chunk.
+void GcmcWorker(gomc_server *sv) {
+  (void)hipSetDevice(sv->device);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (true) {
+    Job *j = nullptr;
+    while (!sv->stop) {
+      if (!sv->gcmcQueue.empty() &&
+          sv->bytesUsed + sv->gcmcQueue.front()->gcmcBytes <= sv->budget) {
+        j = sv->gcmcQueue.front();
+        sv->gcmcQueue.pop_front();
+        break;
+      }
+      sv->cv.wait_for(lk, std::chrono::milliseconds(2));
+    }
+    if (!j) return;
+    sv->bytesUsed += j->gcmcBytes;
+    sv->bytesPeak = std::max(sv->bytesPeak, sv->bytesUsed);
+    ++sv->gcmcRunning;
+    j->status = 1;
+    j->tAdmit = Now(sv->t0);
+    lk.unlock();
+    gomc_box *b = nullptr;
+    std::string failure;
+    int rc = gomc_box_create(&j->desc->d, sv->device, &b);
+    if (rc) failure = "GCMC job setup failed";
+    const int nm = j->desc->d.n_mols;
+    std::vector<int32_t> where(nm);
+    int64_t moves = 0, accepted = 0;
+    while (!rc && moves < j->totalMoves) {
+      const int nIn = gomc_mol_boxes(b, where.data());
+      if (nIn < 0) {
+        rc = nIn;
+        break;
+      }
+      const int64_t chunk = std::min<int64_t>(std::max(1, nIn), j->totalMoves - moves);
+      const int64_t acc = gomc_run_moves(b, moves, chunk, nullptr, nullptr, nullptr, nullptr,
+                                         nullptr, nullptr, nullptr);
+      if (acc < 0) {
+        rc = int(acc);
+        break;
+      }
+      accepted += acc;
+      moves += chunk;
+      if (gomc_mol_boxes(b, where.data()) < 0) {
+        rc = GOMC_ERR_STATE;
+        break;
+      }
+      double n[2] = {0, 0};
+      for (int m = 0; m < nm; ++m)
+        if (where[m] == 0 && j->desc->molKind[m] < 2) n[j->desc->molKind[m]] += 1.0;
+      std::lock_guard<std::mutex> g(sv->mu);
+      j->sumN[0] += n[0];
+      j->sumN[1] += n[1];
+      ++j->samples;
+      j->done = moves;
+      sv->movesDone += chunk;
+    }
+    double e[7] = {0};
+    if (!rc) rc = gomc_energies(b, e);
+    if (rc && failure.empty()) failure = b ? gomc_last_error(b) : "GCMC job failed";
+    if (b) gomc_box_destroy(b);
+    lk.lock();
+    sv->bytesUsed -= j->gcmcBytes;
+    --sv->gcmcRunning;
+    if (rc) {
+      j->status = 3;
+      sv->err = failure;
+    } else {
+      for (int i = 0; i < 7; ++i) j->energy[i] = e[i];
+      j->accepted = accepted;
+      j->status = 2;
+    }
+    j->tDone = Now(sv->t0);
+    j->desc.reset();
+    --sv->outstanding;
+    sv->doneCv.notify_all();
+    sv->cv.notify_all();
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -301,6 +457,7 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
   sv->slotJob.assign(slots, nullptr);
   gomc_server *raw = sv.release();
   raw->loop = std::thread(Loop, raw);
+  for (int w = 0; w < kGcmcWorkers; ++w) raw->workers.emplace_back(GcmcWorker, raw);
   *out = raw;
   return GOMC_OK;
 }
@@ -309,6 +466,25 @@ int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d, int64_t moves
   if (!sv || !d || moves < 0 || !job_id) return GOMC_ERR_ARG;
   auto j = std::make_unique<Job>();
   j->totalMoves = moves;
+  if (d->ensemble == 1) {  // GCMC lane
+    j->gcmc = true;
+    j->desc = CopyDesc(*d);
+    j->gcmcBytes = GcmcBytes(*d);
+    std::lock_guard<std::mutex> g(sv->mu);
+    if (j->gcmcBytes > sv->budget) {
+      sv->err = "job larger than the memory budget";
+      return GOMC_ERR_ARG;
+    }
+    j->id = sv->nextId++;
+    j->tSubmit = Now(sv->t0);
+    *job_id = j->id;
+    Job *raw = j.get();
+    sv->jobs[j->id] = std::move(j);
+    ++sv->outstanding;
+    sv->gcmcQueue.push_back(raw);
+    sv->cv.notify_all();
+    return GOMC_OK;
+  }
   // build and pack the job (initial energies, structure factors) now, on
   // the caller's thread, while the loop keeps the device busy
   gomc_batch scratch;  // holds the packing error, not shared with the loop
@@ -365,6 +541,23 @@ int gomc_server_poll(gomc_server *sv, int64_t job_id, int32_t *status, double ti
   return GOMC_OK;
 }
 
+int gomc_server_poll_counts(gomc_server *sv, int64_t job_id, double mean_n[2]) {
+  if (!sv || !mean_n) return GOMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(sv->mu);
+  auto it = sv->jobs.find(job_id);
+  if (it == sv->jobs.end()) return GOMC_ERR_ARG;
+  const Job &j = *it->second;
+  if (j.gcmc) {
+    const double n = j.samples > 0 ? double(j.samples) : 1.0;
+    mean_n[0] = j.sumN[0] / n;
+    mean_n[1] = j.sumN[1] / n;
+  } else {
+    mean_n[0] = j.pb.P.nmols;
+    mean_n[1] = 0.0;
+  }
+  return GOMC_OK;
+}
+
 int gomc_server_drain(gomc_server *sv, double timeout_s) {
   if (!sv) return GOMC_ERR_ARG;
   std::unique_lock<std::mutex> lk(sv->mu);
@@ -382,7 +575,7 @@ int gomc_server_stats(gomc_server *sv, double out[8]) {
   out[3] = double(sv->bytesPeak);
   out[4] = double(sv->budget);
   out[5] = Now(sv->t0);
-  out[6] = double(sv->queue.size());
+  out[6] = double(sv->queue.size() + sv->gcmcQueue.size());
   out[7] = double(sv->outstanding);
   return GOMC_OK;
 }
@@ -394,10 +587,13 @@ int gomc_server_destroy(gomc_server *sv) {
     sv->stop = true;
     // queued jobs are dropped; running ones finish their current launch
     sv->queue.clear();
+    sv->gcmcQueue.clear();
     for (int s = 0; s < sv->nslots; ++s) sv->slotJob[s] = nullptr;
     sv->cv.notify_all();
   }
   if (sv->loop.joinable()) sv->loop.join();
+  for (auto &w : sv->workers)  // a running GCMC job finishes first
+    if (w.joinable()) w.join();
   if (sv->bt) gomc_batch_destroy(sv->bt);
   delete sv;
   return GOMC_OK;

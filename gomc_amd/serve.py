@@ -51,8 +51,13 @@ class Server:
         acc, done = C.c_int64(), C.c_int64()
         self._check(self.lib.gomc_server_poll(self.h, job, C.byref(st), ptr(times), ptr(en),
                                               C.byref(acc), C.byref(done)))
+        mean_n = np.zeros(2)
+        self._check(self.lib.gomc_server_poll_counts(self.h, job, ptr(mean_n)))
         return dict(status=st.value, submitted=times[0], admitted=times[1], done=times[2],
-                    energies=en, accepted=acc.value, moves_done=done.value)
+                    energies=en, accepted=acc.value, moves_done=done.value, mean_n=mean_n)
+
+    def last_error(self) -> str:
+        return self.lib.gomc_server_last_error(self.h).decode()
 
     def drain(self, timeout: float = 600.0):
         self._check(self.lib.gomc_server_drain(self.h, timeout))
@@ -99,6 +104,33 @@ def poisson_stream(njobs: int, rate: float = 20.0, seed: int = 7, nmin: int = 20
         s = systems.water_box(n, temperature=temp, seed=seed * 100003 + i,
                               place_seed=seed * 7919 + i)
         jobs.append(Job(t, s, cycles * n))
+    return jobs
+
+
+def poisson_stream_gcmc(njobs: int, rate: float = 20.0, seed: int = 17, lmin: float = 20.0,
+                        lmax: float = 60.0, tmin: float = 250.0, tmax: float = 350.0,
+                        pmin: float = 1e3, pmax: float = 1e7, cycles: int = 2000,
+                        density: float = 0.004) -> list[Job]:
+    """Seeded stream of GCMC adsorption state points: methane / CO2 boxes of
+    edge uniform in [lmin, lmax] A, temperature uniform in [tmin, tmax] K,
+    each kind's fugacity log-uniform in [pmin, pmax] Pa; the box starts with
+    `density` molecules per A^3 split evenly between the kinds, beside a
+    reservoir of 0.015 molecules per A^3 of each kind to insert from (more
+    than the box holds at these state points: the reference stops when the
+    reservoir runs dry); `cycles` MC cycles of the initial molecule count
+    each.  40 % transfers, 30 % translations, 30 % rotations."""
+    rng = np.random.default_rng(seed)
+    t, jobs = 0.0, []
+    for i in range(njobs):
+        t += rng.exponential(1.0 / rate)
+        L = float(rng.uniform(lmin, lmax))
+        temp = float(rng.uniform(tmin, tmax))
+        fug = tuple(float(math.exp(rng.uniform(math.log(pmin), math.log(pmax)))) * 1e-5
+                    for _ in range(2))  # Pa -> bar
+        n = max(2, int(density * L ** 3) // 2)
+        s = systems.gcmc_mixture(n, n, L, temperature=temp, fugacity_bar=fug,
+                                 reservoir=int(0.015 * L ** 3) + 32, seed=seed * 1009 + i)
+        jobs.append(Job(t, s, cycles * 2 * n))
     return jobs
 
 

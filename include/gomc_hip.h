@@ -285,7 +285,10 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
 int gomc_server_destroy(gomc_server *sv);
 const char *gomc_server_last_error(const gomc_server *sv);
 /* Builds the job's initial state (on the caller's thread) and queues it for
- * `moves` MC moves; *job_id identifies it. */
+ * `moves` MC moves; *job_id identifies it.  A GCMC descriptor (ensemble 1)
+ * goes to the GCMC lane: up to 8 such jobs run at once, each on its own
+ * device box and stream (the plain path, FP64, MT19937 draws), admitted in
+ * arrival order while the budget holds their bytes. */
 int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d,
                        int64_t moves, int64_t *job_id);
 /* status 0 queued, 1 running, 2 done, 3 failed; times = {submitted,
@@ -294,6 +297,11 @@ int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d,
 int gomc_server_poll(gomc_server *sv, int64_t job_id, int32_t *status,
                      double times[3], double energies[7], int64_t *accepted,
                      int64_t *moves_done);
+/* Mean number of molecules of kinds 0 and 1 in the simulation box over a
+ * GCMC job's MC cycles (sampled after each cycle); for a batched NVT job,
+ * {N, 0}.  Serving query with no counterpart in the reference (its
+ * observable is the block average of BoxMolecules, src/BlockOutput.cpp). */
+int gomc_server_poll_counts(gomc_server *sv, int64_t job_id, double mean_n[2]);
 /* Waits until every submitted job has finished (GOMC_ERR_STATE on timeout). */
 int gomc_server_drain(gomc_server *sv, double timeout_s);
 /* {moves done, kernel seconds, launches, peak bytes in use, budget bytes,

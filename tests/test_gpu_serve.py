@@ -74,3 +74,56 @@ def test_admission_respects_memory_budget():
     # later jobs waited for memory: admission times are not all equal
     adm = sorted(r["admitted"] for r in res)
     assert adm[-1] > adm[0]
+
+
+def _gcmc_job(seed, nch4=12, nco2=12, L=24.0, fug=(80.0, 30.0)):
+    # the reservoir must not run dry (the reference stops when it does)
+    return systems.gcmc_mixture(nch4, nco2, L, fugacity_bar=fug, reservoir=250, seed=seed,
+                                adj_steps=200)
+
+
+def test_gcmc_job_same_alone_and_in_mix_and_matches_oracle():
+    """GCMC lane: a methane/CO2 job gives the same energies, accepted count
+    and mean molecule counts alone and while NVT and other GCMC jobs run
+    beside it (each GCMC job has its own box, stream and MT19937 stream);
+    its accepted count and final energies equal the FP64 oracle's under the
+    same draws (1e-10 relative)."""
+    g = _gcmc_job(5)
+    moves = 1500
+    sv = Server(slots=8, budget_bytes=256 << 20, tick_ms=2.0)
+    ja = sv.submit(g, moves)
+    sv.drain(300)
+    alone = sv.poll(ja)
+    sv.close()
+    assert alone["status"] == 2 and alone["moves_done"] == moves
+
+    sv = Server(slots=8, budget_bytes=256 << 20, tick_ms=2.0)
+    others = [sv.submit(_job(400, 7, 300.0), 4000), sv.submit(_gcmc_job(9), 1200)]
+    jb = sv.submit(g, moves)
+    others.append(sv.submit(_gcmc_job(11, fug=(300.0, 10.0)), 900))
+    sv.drain(300)
+    mixed = sv.poll(jb)
+    assert all(sv.poll(o)["status"] == 2 for o in others), (
+        [sv.poll(o)["status"] for o in others], sv.last_error())
+    sv.close()
+    assert np.array_equal(alone["energies"], mixed["energies"])
+    assert alone["accepted"] == mixed["accepted"]
+    assert np.array_equal(alone["mean_n"], mixed["mean_n"])
+
+    # the oracle under the same draws, sampled the same way (after every
+    # cycle of N_box moves): identical sequences give identical counts
+    o = Oracle(g, threads=8)
+    done, n_acc, sums, samples = 0, 0, np.zeros(2), 0
+    while done < moves:
+        n_in, _ = o.mol_boxes()
+        chunk = min(max(1, n_in), moves - done)
+        acc, _ = o.run(chunk, start_step=done, mode=0, record=False)
+        n_acc += acc
+        done += chunk
+        _, where = o.mol_boxes()
+        for k in (0, 1):
+            sums[k] += np.count_nonzero((where == 0) & (g.mol_kind == k))
+        samples += 1
+    assert n_acc == mixed["accepted"]
+    assert np.allclose(mixed["energies"], o.energies(), rtol=1e-10, atol=1e-6)
+    assert np.array_equal(mixed["mean_n"], sums / samples)
