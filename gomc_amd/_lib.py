@@ -11,6 +11,11 @@ from pathlib import Path
 
 from .systems import SystemDesc
 
+
+class BatchTuning(C.Structure):
+    """gomc_batch_tuning (include/gomc_hip.h)."""
+    _fields_ = [("team_min", C.c_int32 * 3), ("pair12_max_k", C.c_int32)]
+
 LIB_PATH = Path(__file__).resolve().parent / "libgomc_hip.so"
 HEADER = Path(__file__).resolve().parent.parent / "include" / "gomc_hip.h"
 
@@ -61,6 +66,10 @@ SIGNATURES = {
     "gomc_batch_stats": (C.c_int, [_vp, _dp, _lp, _lp]),
     "gomc_batch_get_coords": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "gomc_batch_box_layout": (C.c_int, [_vp, C.c_int, _ip]),
+    "gomc_batch_box_times": (C.c_int, [_vp, _dp]),
+    "gomc_batch_default_tuning": (C.c_int, [C.POINTER(BatchTuning)]),
+    "gomc_batch_create_tuned": (C.c_int, [C.POINTER(SystemDesc), C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(BatchTuning), C.POINTER(_vp)]),
     "gomc_batch_set_trace": (C.c_int, [_vp, C.c_int, C.c_int64]),
     "gomc_batch_get_trace": (C.c_int, [_vp, _ip, _dp, _dp, _dp, _lp]),
     "gomc_server_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_double,

@@ -4,6 +4,8 @@ every energy; nothing here computes an energy on the CPU."""
 from __future__ import annotations
 
 import ctypes as C
+import json
+from pathlib import Path
 
 import numpy as np
 
@@ -214,22 +216,54 @@ class Box:
         return d, r
 
 
-class Batch:
-    """Many independent boxes advanced by one persistent launch."""
+TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"
 
-    def __init__(self, systems: list[System], device: int = 0, precision: int = 0):
+
+def load_tuning(path=TUNING_FILE) -> dict | None:
+    """The launch shape written by gomc_amd.autotune, if there is one."""
+    try:
+        return json.loads(Path(path).read_text())["tuning"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+class Batch:
+    """Many independent boxes advanced by one persistent launch.
+
+    tuning: {"team_min": [n2, n4, n8], "pair12_max_k": k} (see
+    gomc_batch_tuning); None takes gomc_amd/tuning.json when present, else
+    the library defaults; "default" forces the library defaults."""
+
+    def __init__(self, systems: list[System], device: int = 0, precision: int = 0,
+                 tuning: dict | str | None = None):
         self.lib = _lib.load()
         self.systems = systems
         arr = (SystemDesc * len(systems))()
         for i, s in enumerate(systems):
             arr[i] = s.desc()
         self._descs = arr
+        if tuning is None:
+            tuning = load_tuning()
+        tp = None
+        if isinstance(tuning, dict):
+            t = _lib.BatchTuning()
+            self.lib.gomc_batch_default_tuning(C.byref(t))
+            for i, v in enumerate(tuning.get("team_min", list(t.team_min))):
+                t.team_min[i] = int(v)
+            t.pair12_max_k = int(tuning.get("pair12_max_k", t.pair12_max_k))
+            tp = C.byref(t)
         h = C.c_void_p()
-        rc = self.lib.gomc_batch_create(arr, len(systems), device, precision, C.byref(h))
+        rc = self.lib.gomc_batch_create_tuned(arr, len(systems), device, precision, tp, C.byref(h))
         if rc != 0:
             raise GomcError(f"gomc_batch_create failed ({rc})")
         self.h = h
         self.nboxes = len(systems)
+
+    def box_times(self) -> np.ndarray:
+        """Per-box wall time of the last launch, microseconds."""
+        out = np.zeros(self.nboxes)
+        self._check(self.lib.gomc_batch_box_times(self.h, ptr(out)))
+        return out
 
     def _check(self, rc):
         if rc < 0:

@@ -766,6 +766,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     const int64_t traceBox = rank == 0 ? a.trace.box : -1;
 
     bool pend = false;  // the last move was accepted and S(k) awaits its change
+    const uint64_t t0 = wall_clock64();
     for (int64_t mv = 0; mv < nMoves; ++mv, ++step) {
       // ---- Philox draws for the next kDrawChunk moves ----
       if ((mv % kDrawChunk) == 0) {
@@ -880,6 +881,7 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
     if (tid == 0 && rank == 0) {
       st.step = step;
       a.state[b] = st;
+      a.boxTicks[b] = wall_clock64() - t0;
     }
     __syncthreads();  // LDS is reused by the next task
   }

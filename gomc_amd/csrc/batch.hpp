@@ -98,6 +98,7 @@ struct BatchArgs {
   const int *wgTask;          // [grid + 1] start of each workgroup's list
   uint64_t *xchg;             // [nboxes * kXchgWords] team exchange blocks
   int *err;                   // set to 1 when a team member's wait runs out
+  uint64_t *boxTicks;         // [nboxes] device wall-clock ticks of each box's moves
   const BatchBox *boxes;
   BatchState *state;
   double *x, *y, *z;          // [sum atoms]
@@ -138,6 +139,10 @@ struct gomc_batch {
   int grid = 0;        // workgroups per launch
   int gridCap = 0;     // workgroups that fit on the device at once
   bool teams = true;   // false: every box runs on one workgroup (serving)
+  gomc_batch_tuning tune{{1800, 2900, 3500}, 2048};
+  uint64_t *boxTicks = nullptr;
+  std::vector<double> usPerMove;  // measured per-box cost of the last launch (0: none)
+  std::vector<double> lastUs;     // per-box time of the last launch
   int64_t totalAtoms = 0, totalMols = 0, totalK = 0;
   int ldsMols = 0;
   gomc::BatchTrace trace{-1, 0, nullptr, nullptr, nullptr, nullptr};
@@ -177,7 +182,7 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
 int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t kvec);
 // Team size of a box of n molecules (a function of n alone, so a box's
 // results do not depend on the other boxes of its batch).
-int TeamSize(int nmols);
+int TeamSize(const gomc_batch *bt, int nmols);
 // Builds and uploads the workgroup task lists for the given move counts.
 int BuildTasks(gomc_batch *bt, const std::vector<int64_t> &moves);
 // Uploads a packed box: parameters and state at index `slot`, arrays at

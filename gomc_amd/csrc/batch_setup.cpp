@@ -28,7 +28,7 @@ void FreeBatch(gomc_batch *bt) {
   void *ptrs[] = {bt->boxes, bt->state, bt->x,    bt->y,    bt->z,    bt->cx,         bt->cy,
                   bt->cz,    bt->kn,    bt->pref, bt->sumR, bt->sumI, bt->dR, bt->dI, bt->moves,      bt->trace.acc,
                   bt->trace.dlj, bt->trace.dreal, bt->trace.drecip, bt->tasks, bt->wgTask,
-                  bt->xchg, bt->errFlag};
+                  bt->xchg, bt->errFlag, bt->boxTicks};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (bt->graphExec) (void)hipGraphExecDestroy(bt->graphExec);
@@ -105,13 +105,9 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     // it.  Measured on MI355X (water, 200-4000 molecules): 12 pair waves
     // are fastest while a single workgroup holds at most 2048 k-vectors
     // (8 per reciprocal thread), 8 beyond that and in teams.
-    const int T = bt->teams ? TeamSize(b->nmols) : 1;
+    const int T = bt->teams ? TeamSize(bt, b->nmols) : 1;
     const int kper = b->ewald ? b->K : 0;
-    P.pairWaves = (T == 1 && kper <= 2048) ? 12 : 8;
-    if (const char *ov = std::getenv("GOMC_PAIR_WAVES")) {
-      const int w = std::atoi(ov);
-      if (w == 4 || w == 8 || w == 12) P.pairWaves = w;
-    }
+    P.pairWaves = (T == 1 && kper <= bt->tune.pair12_max_k) ? 12 : 8;
     P.kWidth = kBatchThreads - kWave * P.pairWaves;
   }
   P.rCutSq = b->geom.rCutSq;
@@ -184,7 +180,7 @@ int PackBox(gomc_batch *bt, const gomc_system_desc &d, int64_t atomOff, int64_t 
     // per round a thread pays about three table products to start a
     // segment and one complex product per step; a team of T splits the
     // rounds
-    const int64_t T = bt->teams ? TeamSize(b->nmols) : 1;
+    const int64_t T = bt->teams ? TeamSize(bt, b->nmols) : 1;
     int bestS = 1;
     int64_t bestCost = -1;
     for (int S : {1, 2, 4, 8}) {
@@ -260,6 +256,7 @@ BatchArgs MakeArgs(const gomc_batch *bt) {
   a.wgTask = bt->wgTask;
   a.xchg = bt->xchg;
   a.err = bt->errFlag;
+  a.boxTicks = bt->boxTicks;
   a.ldsMols = bt->ldsMols;
   a.trace = bt->trace;
   return a;
@@ -296,6 +293,7 @@ int AllocBatch(gomc_batch *bt, int nboxes, int64_t atoms, int64_t mols, int64_t 
   ok &= alloc((void **)&bt->wgTask, sizeof(int) * size_t(bt->gridCap + 1));
   ok &= alloc((void **)&bt->xchg, sizeof(uint64_t) * size_t(nboxes) * kXchgWords);
   ok &= alloc((void **)&bt->errFlag, sizeof(int));
+  ok &= alloc((void **)&bt->boxTicks, sizeof(uint64_t) * size_t(nboxes));
   if (!ok) return BFail(bt, GOMC_ERR_HIP, "batch allocation failed");
   BHIP(bt, hipMemsetAsync(bt->xchg, 0, sizeof(uint64_t) * size_t(nboxes) * kXchgWords, bt->stream));
   BHIP(bt, hipMemsetAsync(bt->errFlag, 0, sizeof(int), bt->stream));
@@ -336,14 +334,15 @@ int UploadBox(gomc_batch *bt, int slot, const PackedBox &pb) {
   return GOMC_OK;
 }
 
-int TeamSize(int nmols) {
+int TeamSize(const gomc_batch *bt, int nmols) {
   // Per-move cost of one workgroup grows about 10 us + 6 ns per molecule
   // (measured on MI355X, water); a team of T repeats ~7 us of it per member
   // (draws, trial coordinates, tables, the exchange) and splits the rest.
-  // The thresholds keep a box's cycle near 50 ms.
-  if (nmols >= 3500) return 8;
-  if (nmols >= 2900) return 4;
-  if (nmols >= 1800) return 2;
+  // The default thresholds {1800, 2900, 3500} keep a 256-box cycle near 50 ms.
+  const int32_t *t = bt->tune.team_min;
+  if (t[2] > 0 && nmols >= t[2]) return 8;
+  if (t[1] > 0 && nmols >= t[1]) return 4;
+  if (t[0] > 0 && nmols >= t[0]) return 2;
   return 1;
 }
 
@@ -368,11 +367,18 @@ int BuildTasks(gomc_batch *bt, const std::vector<int64_t> &moves) {
     used[best] += T;
   }
   // single-workgroup boxes: longest first onto the least-loaded free
-  // workgroup (cost ~ moves * (18 us + 4.5 ns * N))
+  // workgroup
   std::vector<int> solo;
   for (int b = 0; b < nb; ++b)
     if (!bt->teams || bt->hboxes[b].team <= 1) solo.push_back(b);
-  auto cost = [&](int b) { return double(moves[b]) * (18.0 + 0.0045 * bt->hboxes[b].nmols); };
+  // per-move cost: measured in the last launch when there was one, else the
+  // model 10 us + 6 ns per molecule
+  auto cost = [&](int b) {
+    const double us = (b < int(bt->usPerMove.size()) && bt->usPerMove[b] > 0.0)
+                          ? bt->usPerMove[b]
+                          : 10.0 + 0.006 * bt->hboxes[b].nmols;
+    return double(moves[b]) * us;
+  };
   std::stable_sort(solo.begin(), solo.end(), [&](int p, int q) { return cost(p) > cost(q); });
   std::vector<int> freeWg;
   for (int g = 0; g < grid; ++g)
@@ -403,11 +409,24 @@ int BuildTasks(gomc_batch *bt, const std::vector<int64_t> &moves) {
   return GOMC_OK;
 }
 
-// After a launch: a team member that waited out its limit set the flag.
-int CheckTeams(gomc_batch *bt) {
+// After a launch: a team member that waited out its limit set the flag;
+// the boxes' times become the cost model of the next task lists.
+int CheckTeams(gomc_batch *bt, const std::vector<int64_t> &moves) {
   int flag = 0;
   BHIP(bt, hipMemcpy(&flag, bt->errFlag, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) return BFail(bt, GOMC_ERR_STATE, "a team member timed out waiting for its partners");
+  std::vector<uint64_t> ticks(bt->nboxes);
+  BHIP(bt, hipMemcpy(ticks.data(), bt->boxTicks, sizeof(uint64_t) * bt->nboxes,
+                     hipMemcpyDeviceToHost));
+  int rate = 0;  // device wall clock, kHz
+  BHIP(bt, hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, bt->device));
+  bt->usPerMove.assign(bt->nboxes, 0.0);
+  bt->lastUs.assign(bt->nboxes, 0.0);
+  for (int b = 0; b < bt->nboxes; ++b) {
+    const double us = rate > 0 ? double(ticks[b]) * 1e3 / rate : 0.0;
+    bt->lastUs[b] = us;
+    if (moves[b] > 0) bt->usPerMove[b] = us / double(moves[b]);
+  }
   return GOMC_OK;
 }
 
@@ -415,11 +434,29 @@ int CheckTeams(gomc_batch *bt) {
 
 extern "C" {
 
+int gomc_batch_default_tuning(gomc_batch_tuning *out) {
+  if (!out) return GOMC_ERR_ARG;
+  *out = gomc_batch().tune;
+  return GOMC_OK;
+}
+
 int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int precision,
                       gomc_batch **out) {
+  return gomc_batch_create_tuned(descs, nboxes, device, precision, nullptr, out);
+}
+
+int gomc_batch_box_times(const gomc_batch *bt, double *us) {
+  if (!bt || !us) return GOMC_ERR_ARG;
+  for (int b = 0; b < bt->nboxes; ++b) us[b] = b < int(bt->lastUs.size()) ? bt->lastUs[b] : 0.0;
+  return GOMC_OK;
+}
+
+int gomc_batch_create_tuned(const gomc_system_desc *descs, int nboxes, int device, int precision,
+                            const gomc_batch_tuning *tuning, gomc_batch **out) {
   if (!descs || nboxes <= 0 || !out || precision < 0 || precision > 1) return GOMC_ERR_ARG;
   *out = nullptr;
   gomc_batch *bt = new gomc_batch();
+  if (tuning) bt->tune = *tuning;
   bt->device = device;
   bt->precision = precision;
   bt->nboxes = nboxes;
@@ -436,7 +473,7 @@ int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device, int
   for (int i = 0; i < nboxes; ++i) {
     int rc = PackBox(bt, descs[i], 0, 0, 0, packed[i]);
     if (rc) return fail(rc);
-    packed[i].P.team = TeamSize(packed[i].P.nmols);
+    packed[i].P.team = TeamSize(bt, packed[i].P.nmols);
   }
   // Teams need their members resident together: when the batch holds more
   // large boxes than the device fits (a column of gridCap / 8 workgroups per
@@ -521,7 +558,7 @@ int gomc_batch_run(gomc_batch *bt, const int64_t *moves) {
   BHIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
   bt->lastKernelMs = ms;
   if (bt->trace.box >= 0) bt->traceLen = std::min<int64_t>(bt->trace.cap, mv[bt->trace.box]);
-  return CheckTeams(bt);
+  return CheckTeams(bt, mv);
 }
 
 int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph) {
@@ -562,7 +599,7 @@ int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph) {
   float ms = 0.0f;
   BHIP(bt, hipEventElapsedTime(&ms, bt->ev0, bt->ev1));
   bt->lastKernelMs = ms;
-  return CheckTeams(bt);
+  return CheckTeams(bt, mv);
 }
 
 int gomc_batch_stats(const gomc_batch *bt_, double *energies, int64_t *accepted, int64_t *moves_done) {

@@ -242,6 +242,23 @@ typedef struct gomc_batch gomc_batch;
  * results depend only on the box and its team size. */
 int gomc_batch_create(const gomc_system_desc *descs, int nboxes, int device,
                       int precision, gomc_batch **out);
+/* Launch-shape choices of a batch (no counterpart in the reference; the
+ * autotuner gomc_amd/autotune.py picks them by timing MC cycles):
+ * team_min[i]: molecule count from which a box runs on 2, 4, 8 workgroups
+ * (<= 0: never); pair12_max_k: a single-workgroup box with at most this many
+ * k-vectors runs its pair phase on 12 waves beside a 4-wave reciprocal
+ * phase, other boxes on 8 + 8.  Results depend on a box's team size. */
+typedef struct gomc_batch_tuning {
+  int32_t team_min[3];
+  int32_t pair12_max_k;
+} gomc_batch_tuning;
+int gomc_batch_default_tuning(gomc_batch_tuning *out);
+int gomc_batch_create_tuned(const gomc_system_desc *descs, int nboxes, int device,
+                            int precision, const gomc_batch_tuning *tuning,
+                            gomc_batch **out);
+/* Wall time of each box's moves in the last launch, microseconds
+ * (device clock; a team box counts its first member). */
+int gomc_batch_box_times(const gomc_batch *bt, double *us);
 int gomc_batch_destroy(gomc_batch *bt);
 const char *gomc_batch_last_error(const gomc_batch *bt);
 /* Advance box i by moves[i] moves (NULL: by its molecule count, one MC

@@ -51,7 +51,7 @@ def test_team_box_matches_oracle_philox(nmol, team, n):
     """A box run by a team of workgroups (split pair items and k-vector
     rounds, partial sums exchanged each move) makes the oracle's decisions."""
     s = systems.water_box(nmol, seed=61, adj_steps=500)
-    bt = Batch([s])
+    bt = Batch([s], tuning="default")
     assert bt.layout(0)["team"] == team
     o = Oracle(s, threads=8)
     e0 = o.energies()
@@ -76,13 +76,13 @@ def test_team_box_same_alone_and_in_mix():
     mix = [systems.water_box(300, seed=72), big, systems.water_box(3100, seed=73),
            systems.water_box(500, seed=74)]
     n = [600, 1200, 900, 800]
-    full = Batch(mix)
+    full = Batch(mix, tuning="default")
     assert [full.layout(i)["team"] for i in range(4)] == [1, 2, 4, 1]
     full.set_trace(1, n[1])
     full.run(n)
     t_mix = full.trace()
     e_mix, _, _ = full.stats()
-    alone = Batch([big])
+    alone = Batch([big], tuning="default")
     alone.set_trace(0, n[1])
     alone.run([n[1]])
     t_alone = alone.trace()
@@ -218,3 +218,17 @@ def test_mixed_precision_observables_within_two_sigma():
         z.append(abs(blocks[0].mean() - blocks[1].mean()) / se)
     z = np.array(z)
     assert np.sum(z < 2.0) >= 3 and np.all(z < 4.0), z
+
+
+def test_autotuner_picks_a_timed_candidate():
+    """The autotuner times every candidate launch shape on a small batch and
+    returns the fastest; the per-box times it relies on are positive."""
+    from gomc_amd import autotune
+    boxes = [systems.water_box(n, seed=300 + i) for i, n in enumerate((250, 600, 1900, 2100))]
+    rep = autotune.tune(boxes, cycles=1, log=lambda m: None)
+    ms = [c["ms_per_cycle"] for c in rep["candidates"]]
+    assert len(ms) == len(autotune.PAIR12) + len(autotune.TEAMS) - 1
+    assert rep["ms_per_cycle"] == min(ms) and all(m > 0 for m in ms)
+    bt = Batch(boxes, tuning=rep["tuning"])
+    bt.run_cycles(1, graph=False)
+    assert np.all(bt.box_times() > 0)
