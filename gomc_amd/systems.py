@@ -309,9 +309,11 @@ def gcmc_mixture(n_ch4: int, n_co2: int, box: float, temperature: float = 300.0,
 
 
 def config3(seed: int = 12345) -> System:
-    """GCMC methane + CO2, 40 A box, 300 K, about 2000 molecules in total at
-    the chosen fugacities; 40 % transfers with 16 first-atom trials, 30 %
+    """GCMC methane + CO2, 40 A box, 300 K, fugacities chosen for about 2000
+    molecules in total; the box is seeded uniformly with 600 + 600 molecules
+    (random sequential insertion cannot reach 2000 at liquid-like density)
+    and the transfers fill it; 40 % transfers with 16 first-atom trials, 30 %
     translate, 30 % rotate, Ewald tolerance 1e-5."""
-    return gcmc_mixture(1000, 1000, 40.0, temperature=300.0, fugacity_bar=(60.0, 25.0),
+    return gcmc_mixture(600, 600, 40.0, temperature=300.0, fugacity_bar=(60.0, 25.0),
                         reservoir=1500, seed=seed, cbmc_first=16, cbmc_nth=8, cbmc_ang=100)
 
