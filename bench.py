@@ -165,11 +165,12 @@ def main() -> None:
         from gomc_amd.serve import Server, poisson_stream, run_stream
         jobs = poisson_stream(args.serve_jobs, rate=20.0, seed=11, nmin=200, nmax=2000,
                               cycles=20)
-        sv = Server(slots=256, budget_bytes=8 << 30, tick_ms=20.0)
+        sv = Server(precision=1, slots=256, budget_bytes=8 << 30, tick_ms=20.0)
         rep = run_stream(sv, jobs, timeout=300.0)
         sv.close()
         served = dict(workload=f"Poisson(20/s) stream of {rep['jobs']} NVT water state-point "
-                               "jobs, N log-uniform 200-2000, T 250-350 K, 20 cycles each",
+                               "jobs, N log-uniform 200-2000, T 250-350 K, 20 cycles each; "
+                               "mixed precision (FP32 pair terms and phases, FP64 sums)",
                       moves_per_s=rep["moves_per_s"], p50_s=rep["p50_s"], p99_s=rep["p99_s"],
                       mean_queue_s=rep["mean_queue_s"], wall_s=rep["wall_s"],
                       moves=rep["moves"], budget_bytes=rep["stats"]["budget_bytes"],
@@ -183,10 +184,10 @@ def main() -> None:
         from oracle.pyoracle import Oracle
         jobs = poisson_stream_gcmc(args.serve_gcmc_jobs, rate=20.0, seed=17, lmin=20.0,
                                    lmax=30.0, cycles=5)
-        sv = Server(slots=64, budget_bytes=8 << 30, tick_ms=20.0)
+        sv = Server(precision=1, slots=64, budget_bytes=8 << 30, tick_ms=20.0)
         rep = run_stream(sv, jobs, timeout=600.0)
         sv.close()
-        # <N> of the first job against the oracle under the same draws
+        # <N> of the first job against the FP64 oracle under the same draws
         j0, r0 = jobs[0], rep["results"][0]
         o = Oracle(j0.system, threads=min(16, os.cpu_count() or 1))
         done, sums, samples = 0, np.zeros(2), 0
@@ -202,7 +203,8 @@ def main() -> None:
         served_gcmc = dict(
             workload=f"Poisson(20/s) stream of {rep['jobs']} GCMC methane/CO2 adsorption jobs, "
                      "L 20-30 A, T 250-350 K, fugacities log-uniform 1e3-1e7 Pa, 5 cycles each; "
-                     "GCMC lane (plain FP64 path, 8 concurrent jobs on their own streams)",
+                     "GCMC lane (plain path in mixed precision, 8 concurrent jobs on their own "
+                     "streams)",
             moves_per_s=rep["moves_per_s"], p50_s=rep["p50_s"], p99_s=rep["p99_s"],
             moves=rep["moves"], wall_s=rep["wall_s"],
             mean_n_job0=r0["mean_n"].tolist(), mean_n_job0_oracle=(sums / samples).tolist())

@@ -58,6 +58,7 @@ SIGNATURES = {
     "gomc_energies": (C.c_int, [_vp, _dp]),
     "gomc_mol_boxes": (C.c_int, [_vp, _ip]),
     "gomc_get_scales": (C.c_int, [_vp, _dp, _dp]),
+    "gomc_set_precision": (C.c_int, [_vp, C.c_int]),
     "gomc_batch_create": (C.c_int, [C.POINTER(SystemDesc), C.c_int, C.c_int, C.c_int,
                                     C.POINTER(_vp)]),
     "gomc_batch_destroy": (C.c_int, [_vp]),

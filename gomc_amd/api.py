@@ -113,6 +113,10 @@ class Box:
                                              ptr(z), C.byref(e)))
         return e.value
 
+    def set_precision(self, precision: int):
+        """0 FP64; 1 FP32 pair terms and phases in trials, FP64 sums."""
+        self._check(self.lib.gomc_set_precision(self.h, precision))
+
     def recip_cache(self, enable: bool = True):
         """EwaldCached mode: per-molecule structure factors kept in HBM."""
         self._check(self.lib.gomc_recip_cache(self.h, int(enable)))

@@ -57,6 +57,24 @@ struct TrialArgs {
 // positions of the molecule (CalculateEnergy.cpp:593-679).  Blocks after
 // that: one k-vector per thread, the structure-factor delta and the new
 // per-k energy (Ewald.cpp:436-467).
+// P = 1: pair terms and phases in FP32, sums in FP64 (the mixed mode).
+template <int P>
+__device__ __forceinline__ double Coul(const BoxGeom &g, double d2, double qiqj) {
+  return P ? CoulombEnergyF32(g, d2, qiqj) : CoulombEnergy(g, d2, qiqj);
+}
+template <int P>
+__device__ __forceinline__ double LJ(double d2, double sig, double eps, double nh) {
+  return P ? LJEnergyF32(d2, sig, eps, nh) : LJEnergy(d2, sig, eps, nh);
+}
+template <int P>
+__device__ __forceinline__ void Sincos(double ang, double *s, double *c) {
+  if (P)
+    SincosF32(ang, s, c);
+  else
+    sincos(ang, s, c);
+}
+
+template <int P>
 __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
   __shared__ double red[kBlock / kWave];
   Partial4 part{0, 0, 0, 0};
@@ -89,15 +107,15 @@ __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
         double d2 = DistSq(a.g, a.ox[p], a.oy[p], a.oz[p], xj, yj, zj);
         if (a.g.rCutSq > d2) {
           if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2))
-            real += -CoulombEnergy(a.g, d2, qiqj);
-          if (!(a.g.rCutLJSq < d2)) lj += -LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+            real += -Coul<P>(a.g, d2, qiqj);
+          if (!(a.g.rCutLJSq < d2)) lj += -LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
         }
         d2 = DistSq(a.g, a.nx[p], a.ny[p], a.nz[p], xj, yj, zj);
         if (a.g.rCutSq > d2) {
           if (d2 < a.g.rCutLowSq) ov = 1.0;
           if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2))
-            real += CoulombEnergy(a.g, d2, qiqj);
-          if (!(a.g.rCutLJSq < d2)) lj += LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+            real += Coul<P>(a.g, d2, qiqj);
+          if (!(a.g.rCutLJSq < d2)) lj += LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
         }
       }
     }
@@ -115,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
           if (fabs(a.mq[p]) < 0.000000001) continue;
           double dNew = a.nx[p] * kxk + a.ny[p] * kyk + a.nz[p] * kzk;
           double sn, cn;
-          sincos(dNew, &sn, &cn);
+          Sincos<P>(dNew, &sn, &cn);
           sRN += a.mq[p] * cn;
           sIN += a.mq[p] * sn;
         }
@@ -129,8 +147,8 @@ __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
           double dNew = a.nx[p] * kxk + a.ny[p] * kyk + a.nz[p] * kzk;
           double dOld = a.ox[p] * kxk + a.oy[p] * kyk + a.oz[p] * kzk;
           double sn, cn, so, co;
-          sincos(dNew, &sn, &cn);
-          sincos(dOld, &so, &co);
+          Sincos<P>(dNew, &sn, &cn);
+          Sincos<P>(dOld, &so, &co);
           sRN += a.mq[p] * cn;
           sIN += a.mq[p] * sn;
           sRO += a.mq[p] * co;
@@ -365,6 +383,7 @@ struct SwapArgs {
   double *result;
 };
 
+template <int P>
 __global__ void __launch_bounds__(kBlock) SwapRecipKernel(const SwapArgs a) {
   __shared__ double red[kBlock / kWave];
   const int k = blockIdx.x * kBlock + threadIdx.x;
@@ -375,7 +394,7 @@ __global__ void __launch_bounds__(kBlock) SwapRecipKernel(const SwapArgs a) {
       if (fabs(a.pq[p]) < 0.000000001) continue;
       double dot = a.px[p] * a.kx[k] + a.py[p] * a.ky[k] + a.pz[p] * a.kz[k];
       double s, c;
-      sincos(dot, &s, &c);
+      Sincos<P>(dot, &s, &c);
       sR += a.pq[p] * c;
       sI += a.pq[p] * s;
     }
@@ -440,6 +459,7 @@ struct ParticleArgs {
   double *en, *real, *overlap;
 };
 
+template <int P>
 __global__ void __launch_bounds__(kBlock) ParticleInterKernel(const ParticleArgs a) {
   __shared__ double red[kBlock / kWave];
   const int t = blockIdx.x;
@@ -452,11 +472,11 @@ __global__ void __launch_bounds__(kBlock) ParticleInterKernel(const ParticleArgs
       if (d2 < a.g.rCutLowSq) ov = 1.0;
       if (!(a.g.rCutLJSq < d2)) {
         int idx = a.atype + a.type[j] * a.ntypes;
-        lj += LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+        lj += LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
       }
       if (a.g.elect) {
         double qiqj = a.q[j] * a.charge * kQQFact;
-        if (qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += CoulombEnergy(a.g, d2, qiqj);
+        if (qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += Coul<P>(a.g, d2, qiqj);
       }
     }
   }
@@ -566,7 +586,10 @@ int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const do
   a.partials = b->partials;
   a.counter = b->counter;
   a.result = b->devRes;
-  hipLaunchKernelGGL(TrialKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  if (b->precision == 1)
+    hipLaunchKernelGGL(TrialKernel<1>, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  else
+    hipLaunchKernelGGL(TrialKernel<0>, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   rc = Sync(b);
   if (rc) return rc;
@@ -867,7 +890,10 @@ int ParticleInterImpl(gomc_box *b, int atomType, double charge, int skipMol, int
   a.en = buf + 3 * n;
   a.real = buf + 4 * n;
   a.overlap = buf + 5 * n;
-  hipLaunchKernelGGL(ParticleInterKernel, dim3(n), dim3(kBlock), 0, b->stream, a);
+  if (b->precision == 1)
+    hipLaunchKernelGGL(ParticleInterKernel<1>, dim3(n), dim3(kBlock), 0, b->stream, a);
+  else
+    hipLaunchKernelGGL(ParticleInterKernel<0>, dim3(n), dim3(kBlock), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   std::vector<double> host(3 * size_t(n));
   HIP_TRY(b, hipMemcpyAsync(host.data(), buf + 3 * n, 3 * nb, hipMemcpyDeviceToHost, b->stream));
@@ -920,7 +946,10 @@ int SwapRecipImpl(gomc_box *b, int kind, int insert, const double *x, const doub
   a.partials = b->partials;
   a.counter = b->counter;
   a.result = b->devRes;
-  hipLaunchKernelGGL(SwapRecipKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  if (b->precision == 1)
+    hipLaunchKernelGGL(SwapRecipKernel<1>, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
+  else
+    hipLaunchKernelGGL(SwapRecipKernel<0>, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   rc = Sync(b);
   if (rc) return rc;
@@ -1029,6 +1058,12 @@ int gomc_swap_recip(gomc_box *b, int kind, int mol, int insert, const double *x,
     return GOMC_ERR_ARG;
   (void)hipSetDevice(b->device);
   return SwapRecipImpl(b, kind, insert, x, y, z, mol, enew);
+}
+
+int gomc_set_precision(gomc_box *b, int precision) {
+  if (!b || precision < 0 || precision > 1) return GOMC_ERR_ARG;
+  b->precision = precision;
+  return GOMC_OK;
 }
 
 int gomc_recip_cache(gomc_box *b, int enable) {

@@ -100,6 +100,7 @@ struct Energies {
 struct gomc_box {
   int device = 0;
   hipStream_t stream = nullptr;
+  int precision = 0;  // 1: trial kernels with FP32 pair terms and phases, FP64 sums
   std::string err;
 
   // ---- topology and force field (host) ----

@@ -92,6 +92,32 @@ __device__ __forceinline__ double CoulombEnergy(const BoxGeom &g, double distSq,
   return qiqj / dist;
 }
 
+// Mixed precision (FP32 pair terms and phases, FP64 accumulation): the
+// same terms in FP32 from the FP64 squared distance.
+__device__ __forceinline__ double CoulombEnergyF32(const BoxGeom &g, double distSq, double qiqj) {
+  const float d = sqrtf(float(distSq));
+  if (g.ewald) return double(float(qiqj) * erfcf(float(g.alpha) * d) / d);
+  return double(float(qiqj) / d);
+}
+
+__device__ __forceinline__ double LJEnergyF32(double distSq, double sigmaSq, double epsCn,
+                                              double nHalf) {
+  const float rRat2 = float(sigmaSq) / float(distSq);
+  const float att = rRat2 * rRat2 * rRat2;
+  const float rep = (nHalf == 6.0) ? att * att : powf(rRat2, float(nHalf));
+  return double(float(epsCn) * (rep - att));
+}
+
+// sincos(ang) in FP32 after reducing the FP64 argument to [-pi, pi).
+__device__ __forceinline__ void SincosF32(double ang, double *s, double *c) {
+  const double turns = ang * (0.5 * M_1_PI);
+  const float red = float((turns - rint(turns)) * (2.0 * M_PI));
+  float sf, cf;
+  sincosf(red, &sf, &cf);
+  *s = sf;
+  *c = cf;
+}
+
 // FFParticle::CalcCoulombVir (src/FFParticle.cpp:432-446).
 __device__ __forceinline__ double CoulombVirial(const BoxGeom &g, double distSq,
                                                 double qiqj) {

@@ -349,6 +349,7 @@ void GcmcWorker(gomc_server *sv) {
     std::string failure;
     int rc = gomc_box_create(&j->desc->d, sv->device, &b);
     if (rc) failure = "GCMC job setup failed";
+    if (!rc && sv->precision == 1) rc = gomc_set_precision(b, 1);
     const int nm = j->desc->d.n_mols;
     std::vector<int32_t> where(nm);
     int64_t moves = 0, accepted = 0;

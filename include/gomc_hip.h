@@ -149,6 +149,12 @@ int gomc_swap_recip(gomc_box *b, int kind, int mol, int insert, const double *x,
  * the new positions' terms; the cache follows accepted moves.  enable=0
  * frees it.  Energies are the same as without the cache. */
 int gomc_recip_cache(gomc_box *b, int enable);
+/* Trial precision (no counterpart in the reference): 1 evaluates the pair
+ * terms and e^{ik.r} phases of MoleculeInter / MolReciprocal trials, CBMC
+ * trial energies (ParticleInter) and SwapReciprocal in FP32 from FP64
+ * distances and arguments, with FP64 sums, S(k), running energies and
+ * Metropolis; 0 (default) is FP64 throughout.  Whole-box totals stay FP64. */
+int gomc_set_precision(gomc_box *b, int precision);
 /* UpdateRecip (Ewald.cpp:1420-1434): the new sums become the reference
  * sums (a pointer swap); replaces UpdateRecipCUDA. */
 int gomc_recip_accept(gomc_box *b);

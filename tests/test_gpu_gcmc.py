@@ -59,3 +59,58 @@ def test_gcmc_moves_same_sequence_as_oracle(nch4, nco2, L, fug, nsteps):
     assert np.allclose(b.coords(), o.coords(), rtol=0, atol=1e-9)
     assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=1e-6)
     assert np.allclose(b.system_total(), b.energies(), rtol=1e-10, atol=1e-6)
+
+
+def _counts_run(s, cycles, precision):
+    """GCMC run in chunks of one MC cycle (N_box moves), the molecules of
+    each kind in the box sampled after each chunk."""
+    b = Box(s)
+    b.set_precision(precision)
+    mk = np.asarray(s.mol_kind)
+    done, out = 0, []
+    for _ in range(cycles):
+        n_in, _ = b.mol_boxes()
+        chunk = max(1, n_in)
+        b.run(chunk, start_step=done, record=False)
+        done += chunk
+        _, where = b.mol_boxes()
+        out.append([np.count_nonzero((where == 0) & (mk == k)) for k in (0, 1)])
+    return np.array(out, dtype=float)
+
+
+def test_gcmc_mixed_precision_error_against_fp64():
+    """Mixed mode (FP32 pair terms, CBMC trial energies and phases; FP64
+    sums, Rosenbluth weights, S(k), Metropolis): while the two runs make the
+    same decisions, the per-move energy changes agree to 1e-6 x max(|dE|,
+    |E_total|)."""
+    s = systems.gcmc_mixture(40, 40, 25.0, fugacity_bar=(200.0, 60.0), reservoir=200, seed=8)
+    b64, b32 = Box(s), Box(s)
+    b32.set_precision(1)
+    etot = abs(Oracle(s, threads=8).energies()[6])
+    n = 1500
+    _, r64 = b64.run(n)
+    _, r32 = b32.run(n)
+    same = (r64["acc"] == r32["acc"]) & (r64["kind"] == r32["kind"]) & (r64["mol"] == r32["mol"])
+    first = int(np.argmin(same)) if not same.all() else n
+    assert first >= 50, first
+    for k in ("dlj", "dreal", "drecip"):
+        ref, got = r64[k][:first], r32[k][:first]
+        err = np.abs(got - ref)
+        assert np.all(err <= 1e-6 * np.maximum(np.abs(ref), etot)), (k, err.max())
+
+
+def test_gcmc_mixed_precision_mean_counts_within_error():
+    """<N> of each kind in mixed precision against FP64, as independent
+    samplers (their own MT19937 seeds) from one starting box: the means of
+    cycles 300-1499 (12 blocks of 100) agree within 3 combined standard
+    errors for both kinds (probability of a false failure about 0.5 %)."""
+    base = dict(fugacity_bar=(80.0, 30.0), reservoir=250, adj_steps=200)
+    s64 = systems.gcmc_mixture(12, 12, 24.0, seed=21, **base)
+    s32 = systems.System(**{**s64.__dict__, "seed": 977, "_keep": []})
+    c64 = _counts_run(s64, 1500, 0)[300:]
+    c32 = _counts_run(s32, 1500, 1)[300:]
+    for k in (0, 1):
+        blocks = [c[:, k].reshape(12, -1).mean(1) for c in (c64, c32)]
+        se = np.sqrt(sum(x.var(ddof=1) / len(x) for x in blocks))
+        z = abs(blocks[0].mean() - blocks[1].mean()) / max(se, 1e-12)
+        assert z < 3.0, (k, blocks[0].mean(), blocks[1].mean(), se)
