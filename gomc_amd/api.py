@@ -194,14 +194,18 @@ class Box:
         return out
 
     # ---- MC ----
-    def run(self, nsteps: int, start_step: int = 0, record: bool = True):
+    def run(self, nsteps: int, start_step: int = 0, record: bool = True, device: bool = False):
+        """Run `nsteps` moves from `start_step`: on the host driver (one fused
+        launch per energy call) or, with device=True, in one launch of the
+        device move loop (same draws, same moves, same outcomes)."""
+        fn = self.lib.gomc_run_moves_device if device else self.lib.gomc_run_moves
         if not record:
-            return self._check(self.lib.gomc_run_moves(self.h, start_step, nsteps, None, None, None,
-                                                       None, None, None, None)), None
+            return self._check(fn(self.h, start_step, nsteps, None, None, None,
+                                  None, None, None, None)), None
         rec = dict(kind=np.zeros(nsteps, np.int32), mol=np.zeros(nsteps, np.int32),
                    acc=np.zeros(nsteps, np.int32), ovl=np.zeros(nsteps, np.int32),
                    dlj=np.zeros(nsteps), dreal=np.zeros(nsteps), drecip=np.zeros(nsteps))
-        n = self._check(self.lib.gomc_run_moves(self.h, start_step, nsteps, ptr(rec["kind"]),
+        n = self._check(fn(self.h, start_step, nsteps, ptr(rec["kind"]),
                                                 ptr(rec["mol"]), ptr(rec["acc"]), ptr(rec["ovl"]),
                                                 ptr(rec["dlj"]), ptr(rec["dreal"]),
                                                 ptr(rec["drecip"])))
@@ -218,6 +222,30 @@ class Box:
         d, r = np.zeros(nk), np.zeros(nk)
         self._check(self.lib.gomc_get_scales(self.h, ptr(d), ptr(r)))
         return d, r
+
+
+def run_device(boxes, nsteps, start_steps=None, cycles=None):
+    """One launch of the device move loop over several boxes (one workgroup
+    each).  `nsteps` and `cycles` are per box (scalars broadcast); with
+    cycles, each box runs up to that many cycles of N_box moves and samples
+    its kind-0/1 counts after each.  Returns (moves, accepted, count sums
+    (n, 2), samples)."""
+    n = len(boxes)
+    lib = boxes[0].lib
+    hs = (C.c_void_p * n)(*[b.h for b in boxes])
+    ns = np.broadcast_to(np.asarray(nsteps, np.int64), (n,)).copy()
+    st = (np.zeros(n, np.int64) if start_steps is None
+          else np.broadcast_to(np.asarray(start_steps, np.int64), (n,)).copy())
+    cy = (None if cycles is None
+          else np.broadcast_to(np.asarray(cycles, np.int32), (n,)).copy())
+    moves, acc, samples = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+    sums = np.zeros((n, 2))
+    rc = lib.gomc_run_moves_device_multi(hs, n, ptr(st), ptr(ns), None if cy is None else ptr(cy),
+                                         ptr(moves), ptr(acc), ptr(sums), ptr(samples))
+    if rc != 0:
+        msgs = [lib.gomc_last_error(b.h).decode() for b in boxes]
+        raise GomcError(f"device move loop failed ({rc}): {[m for m in msgs if m][:1]}")
+    return moves, acc, sums, samples
 
 
 TUNING_FILE = Path(__file__).resolve().parent / "tuning.json"

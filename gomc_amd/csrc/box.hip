@@ -59,22 +59,6 @@ struct TrialArgs {
 // per-k energy (Ewald.cpp:436-467).
 // P = 1: pair terms and phases in FP32, sums in FP64 (the mixed mode).
 template <int P>
-__device__ __forceinline__ double Coul(const BoxGeom &g, double d2, double qiqj) {
-  return P ? CoulombEnergyF32(g, d2, qiqj) : CoulombEnergy(g, d2, qiqj);
-}
-template <int P>
-__device__ __forceinline__ double LJ(double d2, double sig, double eps, double nh) {
-  return P ? LJEnergyF32(d2, sig, eps, nh) : LJEnergy(d2, sig, eps, nh);
-}
-template <int P>
-__device__ __forceinline__ void Sincos(double ang, double *s, double *c) {
-  if (P)
-    SincosF32(ang, s, c);
-  else
-    sincos(ang, s, c);
-}
-
-template <int P>
 __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
   __shared__ double red[kBlock / kWave];
   Partial4 part{0, 0, 0, 0};
@@ -759,6 +743,7 @@ static void FreeAll(gomc_box *b) {
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   MPFree(b);
+  DevFree(b);
   if (b->trialBuf) (void)hipFree(b->trialBuf);
   for (void *p : {(void *)b->cacheC, (void *)b->cacheS, (void *)b->stageC, (void *)b->stageS})
     if (p) (void)hipFree(p);

@@ -19,6 +19,7 @@
 
 namespace gomc {
 
+struct DevDriver;  // buffers of the device move loop (gcmc_dev.hip)
 constexpr int kMaxMolAtoms = 16;   // atoms per molecule in a trial launch
 constexpr int kBlock = 256;        // threads per workgroup, plain kernels
 
@@ -182,6 +183,7 @@ struct gomc_box {
   gomc::Energies pot;
   double movePerc[3] = {0.5, 0.5, 0.0};
   uint64_t perAdjust = 1000;
+  gomc::DevDriver *dev = nullptr;  // device move loop, allocated on first use
 };
 
 namespace gomc {
@@ -222,4 +224,6 @@ int SwapRecipImpl(gomc_box *b, int kind, int insert, const double *x, const doub
                   const double *z, int mol, double *enew);
 // Rebuilds the per-molecule cache from the current coordinates.
 int CacheBuild(gomc_box *b);
+// Frees the device move loop's buffers.
+void DevFree(gomc_box *b);
 }  // namespace gomc

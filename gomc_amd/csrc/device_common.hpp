@@ -184,6 +184,24 @@ __host__ __device__ __forceinline__ double U01(uint64_t v) {
   return double(v) * 0x1p-64 + 0x1p-65;
 }
 
+// The pair terms and phases at precision P: 0 is FP64 throughout, 1 the
+// FP32 forms above (sums stay FP64 in every caller).
+template <int P>
+__device__ __forceinline__ double Coul(const BoxGeom &g, double d2, double qiqj) {
+  return P ? CoulombEnergyF32(g, d2, qiqj) : CoulombEnergy(g, d2, qiqj);
+}
+template <int P>
+__device__ __forceinline__ double LJ(double d2, double sig, double eps, double nh) {
+  return P ? LJEnergyF32(d2, sig, eps, nh) : LJEnergy(d2, sig, eps, nh);
+}
+template <int P>
+__device__ __forceinline__ void Sincos(double ang, double *s, double *c) {
+  if (P)
+    SincosF32(ang, s, c);
+  else
+    sincos(ang, s, c);
+}
+
 // ------------------------------------------------------- reductions -------
 template <typename T>
 __device__ __forceinline__ T WaveSum(T v) {

@@ -1,0 +1,1471 @@
+// Device-resident move loop: the single-box driver's moves -- Translate,
+// Rotate and (GCMC) MoleculeTransfer with its configurational-bias growth --
+// run on the device, one workgroup per box and many moves per launch, with
+// each box's MT19937 stream held in LDS.  The draw order, the CBMC builders
+// and the bookkeeping are those of the host driver (driver.cpp, gcmc.cpp),
+// which follow src/System.cpp:262-302, src/moves/Translate.h:52-138,
+// src/moves/Rotation.h:41-130, src/moves/MoleculeTransfer.h:85-246,
+// src/cbmc/DCSingle.cpp:23-100, src/cbmc/DCFreeHedron.cpp:48-240,
+// src/MoleculeLookup.cpp:165-198 and src/MoveSettings.cpp:205-258, so a box
+// run here takes the same moves, in the same order, with the same outcomes,
+// as on the host driver or the CPU restatement of the reference.
+//
+// Work split inside a move: thread 0 of the workgroup runs the serial part
+// (random draws, trial geometry, Metropolis, molecule lookup), the whole
+// workgroup runs the energies:
+//   translate / rotate  every atom against the old and new positions of the
+//                       molecule, and every k-vector's structure-factor delta,
+//                       reduced in one pass;
+//   CBMC stage          (trial position, atom) pairs spread over the waves,
+//                       each wave summing one pair over every atom;
+//   swap reciprocal     every k-vector's new structure factor.
+// Energies are summed in a fixed order, so a run does not depend on timing.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "box.hpp"
+
+namespace gomc {
+
+namespace {
+
+constexpr int kDevThreads = 1024;
+constexpr int kDevWaves = kDevThreads / kWave;
+constexpr int kDevKinds = 4;      // molecule kinds of a device-driven box
+constexpr int kDevKindAtoms = 3;  // atoms per molecule (1- and 3-site rigid kinds)
+constexpr int kDevTrials = 32;    // CBMC trials per growth stage
+constexpr int kDevCombos = 2 * kDevTrials;
+constexpr int kMtN = 624, kMtM = 397;
+constexpr int kDisplace = 0, kRotate = 1, kMolTransfer = 10;
+constexpr double kSmallWeight = 1.0e-38;  // lib/NumLib.h:19
+
+// Read-only description of one box for a launch.
+struct DevBox {
+  BoxGeom g;
+  double beta, volume, volInv;
+  double pairEnCorr[kDevKinds * kDevKinds], fugacity[kDevKinds], selfKind[kDevKinds];
+  double kindQ[kDevKinds][kDevKindAtoms];
+  int kindT[kDevKinds][kDevKindAtoms], kindLen[kDevKinds];
+  double hedB[kDevKinds], hedP[kDevKinds], hedTh[kDevKinds];
+  double moveW[13], movePerc[2];
+  int nMoveTypes, lrc, nkinds, natoms, nmols, ntypes, K;
+  int cbmcFirst, cbmcNth, cbmcAng;
+  int nSwapKinds, nMoveKinds, swapKind[kDevKinds], moveKind[kDevKinds];
+  uint64_t perAdjust;
+  double *x, *y, *z;
+  const double *q;
+  const int *type;
+  int *mol;
+  const double *sigSq, *epsCn, *nHalf;
+  const double *kx, *ky, *kz, *pref;
+  double *sumR, *sumI, *newR, *newI;
+  const int *molStart, *molKind;
+  int *molBox;
+  uint32_t *lookup;
+  double *com;  // [3][nmols]
+  struct DevState *st;
+  int64_t step0, nsteps;
+  int cycles;
+  int32_t *tKind, *tMol, *tAcc, *tOvl;
+  double *tE;  // [nsteps][3]
+};
+
+// Mutable state of a box's move loop, copied to LDS for a launch.
+struct DevState {
+  uint32_t mt[kMtN];
+  int mtIdx, status;
+  double pot[7];  // inter, tc, real, recip, self, corr, total
+  double scale[2][kDevKinds];
+  uint64_t tries[2][kDevKinds], acc[2][kDevKinds], tTries[2][kDevKinds], tAcc[2][kDevKinds];
+  uint64_t swapTries[2 * kDevKinds], swapAcc[2 * kDevKinds];
+  uint32_t bkStart[2 * kDevKinds + 1], swapCount[2 * kDevKinds];
+  int kindCount[kDevKinds];
+  double sumN[2];
+  int64_t samples, moves, accepted;
+};
+
+enum : int { kStErr = 1, kStEmptyReservoir = 2, kStWeights = 3, kStMoveKind = 4 };
+
+// One trial molecule (cbmc::TrialMol).
+struct DTrial {
+  double px[kDevKindAtoms], py[kDevKindAtoms], pz[kDevKindAtoms];
+  double weight, inter, real;
+  int overlap, box;
+};
+
+// Per-move control block shared by the workgroup.
+struct Ctl {
+  int op, mol, len, st, commit, doRecip, sign, found, abort, section;
+  int src, dst, kind, mv, mk, tried, pick;
+  double subDraw, pr, ncom[3], thetaW;
+  double tcLose, tcGain, wTc, corrNew, corrOld, selfNew, selfOld, recipLose, recipGain;
+  double ox[kDevKindAtoms], oy[kDevKindAtoms], oz[kDevKindAtoms];
+  double nx[kDevKindAtoms], ny[kDevKindAtoms], nz[kDevKindAtoms];
+  double mq[kDevKindAtoms];
+  int mt_[kDevKindAtoms];
+  // CBMC stage: nc (position, atom) combinations and their energies
+  int nc, skip;
+  double cx[kDevCombos], cy[kDevCombos], cz[kDevCombos], cq[kDevCombos];
+  int ct[kDevCombos];
+  double cE[kDevCombos], cR[kDevCombos], cO[kDevCombos];
+  double w[kDevTrials], sI[kDevTrials], sR[kDevTrials];  // stage weights and sums
+  DTrial nm, om;
+  int64_t done, cycleLeft;
+  int cyclesDone;
+  double red[kDevWaves][4];
+  double res[4];
+};
+
+enum : int { kOpStop = 0, kOpNone = 1, kOpMol = 2, kOpTransfer = 3 };
+
+#define NOINLINE __attribute__((noinline))
+
+// ---------------------------------------------------------------- MT19937
+// The generator of host_rng.hpp over the state in LDS (thread 0 only).
+NOINLINE __device__ void MtTwist(DevState &s) {
+  for (int i = 0; i < kMtN; ++i) {
+    const int i1 = i + 1 < kMtN ? i + 1 : i + 1 - kMtN;
+    const int im = i + kMtM < kMtN ? i + kMtM : i + kMtM - kMtN;
+    const uint32_t y = (s.mt[i] & 0x80000000u) | (s.mt[i1] & 0x7fffffffu);
+    uint32_t v = s.mt[im] ^ (y >> 1);
+    if (s.mt[i1] & 1u) v ^= 0x9908b0dfu;
+    s.mt[i] = v;
+  }
+  s.mtIdx = 0;
+}
+__device__ uint32_t MtNext(DevState &s) {
+  if (s.mtIdx == kMtN) MtTwist(s);
+  uint32_t v = s.mt[s.mtIdx++];
+  v ^= v >> 11;
+  v ^= (v << 7) & 0x9d2c5680u;
+  v ^= (v << 15) & 0xefc60000u;
+  return v ^ (v >> 18);
+}
+NOINLINE __device__ uint32_t MtBelow(DevState &s, uint32_t n) {
+  uint32_t mask = n;
+  for (int sh = 1; sh < 32; sh <<= 1) mask |= mask >> sh;
+  uint32_t v;
+  do {
+    v = MtNext(s) & mask;
+  } while (v > n);
+  return v;
+}
+NOINLINE __device__ double MtClosed(DevState &s) {
+  return double(MtNext(s)) * (1.0 / 4294967295.0);
+}
+NOINLINE __device__ double MtHalfOpen(DevState &s) {
+  return double(MtNext(s)) * (1.0 / 4294967296.0);
+}
+
+struct DM3 {
+  double m[3][3];
+};
+__device__ void Apply(const DM3 &r, double x, double y, double z, double &ox, double &oy,
+                      double &oz) {
+  ox = r.m[0][0] * x + r.m[0][1] * y + r.m[0][2] * z;
+  oy = r.m[1][0] * x + r.m[1][1] * y + r.m[1][2] * z;
+  oz = r.m[2][0] * x + r.m[2][1] * y + r.m[2][2] * z;
+}
+// TransformMatrix::FromAxisAngle, as AxisAngle in host_rng.hpp.
+NOINLINE __device__ DM3 AxisAngleD(double theta, double ax, double ay, double az) {
+  const double av[3] = {ax, ay, az};
+  const double cross[3][3] = {{0.0, -az, ay}, {az, 0.0, -ax}, {-ay, ax, 0.0}};
+  const double c = cos(theta), s = sin(theta);
+  DM3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const double tensor = av[i] * av[j];
+      r.m[i][j] = (i == j ? c : 0.0) + (s * cross[i][j] + (1 - c) * tensor);
+    }
+  return r;
+}
+// TransformMatrix::UniformRandom, as UniformRotation in host_rng.hpp.
+NOINLINE __device__ DM3 UniformRotationD(double u1, double u2, double u3) {
+  u1 *= 2.0 * M_PI;
+  u1 -= M_PI;
+  u2 *= 2.0 * M_PI;
+  u3 *= 2.0;
+  const double r = sqrt(u3);
+  const double vx = sin(u2) * r, vy = cos(u2) * r, vz = sqrt(2.0 - u3);
+  const double sinTh = sin(u1), cosTh = cos(u1);
+  const double sx = vx * cosTh - vy * sinTh;
+  const double sy = vx * sinTh + vy * cosTh;
+  DM3 m;
+  m.m[0][0] = vx * sx - cosTh;
+  m.m[0][1] = vx * sy - sinTh;
+  m.m[0][2] = vx * vz;
+  m.m[1][0] = vy * sx + sinTh;
+  m.m[1][1] = vy * sy - cosTh;
+  m.m[1][2] = vy * vz;
+  m.m[2][0] = vz * sx;
+  m.m[2][1] = vz * sy;
+  m.m[2][2] = 1.0 - u3;
+  return m;
+}
+
+__device__ uint32_t NumKindInBox(const DevBox &B, const DevState &s, int k, int box) {
+  return s.bkStart[box * B.nkinds + k + 1] - s.bkStart[box * B.nkinds + k];
+}
+
+// PRNG::PickWeighted (src/PRNG.h:167-203).
+NOINLINE __device__ int PickWeightedD(DevState &s, const double *w, int n, double total) {
+  const double draw = MtClosed(s) * total;
+  double sum = 0.0;
+  for (int i = 0; i < n; ++i) {
+    sum += w[i];
+    if (sum >= draw) return i;
+  }
+  int lastZero = n;
+  for (int i = n - 1; i > 0 && w[i] == 0.0; --i) lastZero = i;
+  lastZero--;
+  if (fabs(draw - total) < 0.001) return lastZero;
+  return -1;
+}
+
+// Four sums over the workgroup, valid in c.res after the call.
+__device__ void BlockSum4(Ctl &c, double a, double b, double d, double e) {
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  a = WaveSum(a);
+  b = WaveSum(b);
+  d = WaveSum(d);
+  e = WaveSum(e);
+  if (lane == 0) {
+    c.red[wave][0] = a;
+    c.red[wave][1] = b;
+    c.red[wave][2] = d;
+    c.red[wave][3] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double r = 0.0;
+    for (int w = 0; w < kDevWaves; ++w) r += c.red[w][threadIdx.x];
+    c.res[threadIdx.x] = r;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------- workgroup energy phases --
+// ParticleInter (src/CalculateEnergy.cpp:727-785) for the stage's nc
+// (position, atom) combinations: one wave per combination, the lanes over
+// the atoms, then the wave's sum.
+template <int P>
+NOINLINE __device__ void ComboEnergies(const DevBox &B, Ctl &c) {
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int nc = c.nc, skip = c.skip, natoms = B.natoms, ntypes = B.ntypes;
+  const BoxGeom g = B.g;
+  const double *x = B.x, *y = B.y, *z = B.z, *q = B.q;
+  const int *type = B.type, *mol = B.mol;
+  for (int ci = wave; ci < nc; ci += kDevWaves) {
+    const double px = c.cx[ci], py = c.cy[ci], pz = c.cz[ci], charge = c.cq[ci];
+    const int atype = c.ct[ci];
+    double lj = 0.0, real = 0.0, ov = 0.0;
+    for (int j = lane; j < natoms; j += kWave) {
+      const int mj = mol[j];
+      if (mj == skip || mj < 0) continue;
+      const double d2 = DistSq(g, px, py, pz, x[j], y[j], z[j]);
+      if (g.rCutSq > d2) {
+        if (d2 < g.rCutLowSq) ov = 1.0;
+        if (!(g.rCutLJSq < d2)) {
+          const int idx = atype + type[j] * ntypes;
+          lj += LJ<P>(d2, B.sigSq[idx], B.epsCn[idx], B.nHalf[idx]);
+        }
+        if (g.elect) {
+          const double qiqj = q[j] * charge * kQQFact;
+          if (qiqj != 0.0 && !(g.rCutCoulSq < d2)) real += Coul<P>(g, d2, qiqj);
+        }
+      }
+    }
+    lj = WaveSum(lj);
+    real = WaveSum(real);
+    ov = WaveSum(ov);
+    if (lane == 0) {
+      c.cE[ci] = lj;
+      c.cR[ci] = real;
+      c.cO[ci] = ov;
+    }
+  }
+  __syncthreads();
+}
+
+// One translate / rotate trial's energies: every atom against the old and
+// new positions (CalculateEnergy::MoleculeInter, CalculateEnergy.cpp:581-686)
+// and every k-vector's delta (Ewald::MolReciprocal, Ewald.cpp:409-473).
+template <int P>
+NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c) {
+  double lj = 0.0, real = 0.0, ov = 0.0, e = 0.0;
+  const int st = c.st, len = c.len, natoms = B.natoms, ntypes = B.ntypes;
+  const BoxGeom g = B.g;
+  double ox[kDevKindAtoms], oy[kDevKindAtoms], oz[kDevKindAtoms], nx[kDevKindAtoms],
+      ny[kDevKindAtoms], nz[kDevKindAtoms], mq[kDevKindAtoms];
+  int mt[kDevKindAtoms];
+  for (int p = 0; p < kDevKindAtoms; ++p) {
+    ox[p] = c.ox[p];
+    oy[p] = c.oy[p];
+    oz[p] = c.oz[p];
+    nx[p] = c.nx[p];
+    ny[p] = c.ny[p];
+    nz[p] = c.nz[p];
+    mq[p] = c.mq[p];
+    mt[p] = c.mt_[p];
+  }
+  for (int j = threadIdx.x; j < natoms; j += kDevThreads) {
+    if ((j >= st && j < st + len) || B.mol[j] < 0) continue;
+    const double xj = B.x[j], yj = B.y[j], zj = B.z[j], qj = B.q[j];
+    const int tj = B.type[j];
+#pragma unroll
+    for (int p = 0; p < kDevKindAtoms; ++p) {
+      if (p >= len) break;
+      const int idx = mt[p] + tj * ntypes;
+      const double qiqj = mq[p] * qj * kQQFact;
+      double d2 = DistSq(g, ox[p], oy[p], oz[p], xj, yj, zj);
+      if (g.rCutSq > d2) {
+        if (g.elect && qiqj != 0.0 && !(g.rCutCoulSq < d2)) real += -Coul<P>(g, d2, qiqj);
+        if (!(g.rCutLJSq < d2)) lj += -LJ<P>(d2, B.sigSq[idx], B.epsCn[idx], B.nHalf[idx]);
+      }
+      d2 = DistSq(g, nx[p], ny[p], nz[p], xj, yj, zj);
+      if (g.rCutSq > d2) {
+        if (d2 < g.rCutLowSq) ov = 1.0;
+        if (g.elect && qiqj != 0.0 && !(g.rCutCoulSq < d2)) real += Coul<P>(g, d2, qiqj);
+        if (!(g.rCutLJSq < d2)) lj += LJ<P>(d2, B.sigSq[idx], B.epsCn[idx], B.nHalf[idx]);
+      }
+    }
+  }
+  if (c.doRecip)
+    for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
+      const double kxk = B.kx[k], kyk = B.ky[k], kzk = B.kz[k];
+      double sRN = 0.0, sIN = 0.0, sRO = 0.0, sIO = 0.0;
+      for (int p = 0; p < len; ++p) {
+        if (fabs(mq[p]) < 0.000000001) continue;
+        const double dNew = nx[p] * kxk + ny[p] * kyk + nz[p] * kzk;
+        const double dOld = ox[p] * kxk + oy[p] * kyk + oz[p] * kzk;
+        double sn, cn, so, co;
+        Sincos<P>(dNew, &sn, &cn);
+        Sincos<P>(dOld, &so, &co);
+        sRN += mq[p] * cn;
+        sIN += mq[p] * sn;
+        sRO += mq[p] * co;
+        sIO += mq[p] * so;
+      }
+      const double R = B.sumR[k] + (sRN - sRO);
+      const double I = B.sumI[k] + (sIN - sIO);
+      B.newR[k] = R;
+      B.newI[k] = I;
+      e += (R * R + I * I) * B.pref[k];
+    }
+  BlockSum4(c, lj, real, e, ov);
+}
+
+// Ewald::SwapDestRecip / SwapSourceRecip (Ewald.cpp:478-531,657-710).
+template <int P>
+NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, Ctl &c) {
+  double e = 0.0;
+  const int len = c.len, sign = c.sign;
+  double px[kDevKindAtoms], py[kDevKindAtoms], pz[kDevKindAtoms], mq[kDevKindAtoms];
+  for (int p = 0; p < kDevKindAtoms; ++p) {
+    px[p] = c.nx[p];
+    py[p] = c.ny[p];
+    pz[p] = c.nz[p];
+    mq[p] = c.mq[p];
+  }
+  for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
+    double sR = 0.0, sI = 0.0;
+    for (int p = 0; p < len; ++p) {
+      if (fabs(mq[p]) < 0.000000001) continue;
+      const double dot = px[p] * B.kx[k] + py[p] * B.ky[k] + pz[p] * B.kz[k];
+      double sn, cs;
+      Sincos<P>(dot, &sn, &cs);
+      sR += mq[p] * cs;
+      sI += mq[p] * sn;
+    }
+    const double R = (sign > 0) ? B.sumR[k] + sR : B.sumR[k] - sR;
+    const double I = (sign > 0) ? B.sumI[k] + sI : B.sumI[k] - sI;
+    B.newR[k] = R;
+    B.newI[k] = I;
+    e += (R * R + I * I) * B.pref[k];
+  }
+  BlockSum4(c, e, 0.0, 0.0, 0.0);
+}
+
+__device__ void CommitRecip(const DevBox &B) {
+  for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
+    B.sumR[k] = B.newR[k];
+    B.sumI[k] = B.newI[k];
+  }
+}
+
+// ------------------------------------------------- thread-0 serial pieces --
+__device__ void Total(DevState &s) {
+  s.pot[6] = s.pot[0] + s.pot[1] + s.pot[2] + s.pot[3] + s.pot[4] + s.pot[5];
+}
+
+NOINLINE __device__ void Record(const DevBox &B, int64_t i, int kind, int mol, int acc, int ovl,
+                                double e0, double e1, double e2) {
+  if (!B.tKind) return;
+  B.tKind[i] = kind;
+  B.tMol[i] = mol;
+  B.tAcc[i] = acc;
+  B.tOvl[i] = ovl;
+  B.tE[3 * i] = e0;
+  B.tE[3 * i + 1] = e1;
+  B.tE[3 * i + 2] = e2;
+}
+
+// MoveSettings::AdjustMoves / Adjust (src/MoveSettings.cpp:148-162,205-258).
+NOINLINE __device__ void AdjustMovesD(const DevBox &B, DevState &s, uint64_t step) {
+  if ((step + 1) % B.perAdjust != 0) return;
+  const double lmin = fmin(B.g.L[0], fmin(B.g.L[1], B.g.L[2]));
+  for (int mv = 0; mv < 2; ++mv)
+    for (int k = 0; k < B.nkinds; ++k) {
+      double sc = s.scale[mv][k];
+      if (s.tTries[mv][k] > 0) {
+        const double acc = double(s.tAcc[mv][k]) / double(s.tTries[mv][k]);
+        const double fract = acc / 0.5;
+        sc *= (fract > 0.0) ? fract : 0.5;
+      }
+      const double lo = (mv == kDisplace) ? 1.0e-10 : 1.0e-6;
+      const double hi = (mv == kDisplace) ? (lmin * 0.5) - 1.0e-7 : M_PI - 1.0e-7;
+      if (sc < lo) sc = lo;
+      if (sc > hi) sc = hi;
+      s.scale[mv][k] = sc;
+      s.tAcc[mv][k] = 0;
+      s.tTries[mv][k] = 0;
+    }
+}
+
+// Top of the move loop: cycle bookkeeping, the adjustment, and
+// PRNG::PickArbDist over the move kinds (src/PRNG.h:153-163).
+NOINLINE __device__ void LoopTop(const DevBox &B, DevState &S, Ctl &c) {
+  c.op = kOpStop;
+  c.commit = 0;
+  bool go = !c.abort && c.done < B.nsteps && S.status == 0;
+  if (go && B.cycles > 0) {
+    if (c.cycleLeft == 0) {  // a cycle ended: sample, maybe stop
+      S.sumN[0] += B.nkinds > 0 ? double(NumKindInBox(B, S, 0, 0)) : 0.0;
+      S.sumN[1] += B.nkinds > 1 ? double(NumKindInBox(B, S, 1, 0)) : 0.0;
+      S.samples++;
+      c.cyclesDone++;
+      c.cycleLeft = -1;
+      go = c.cyclesDone < B.cycles;
+    }
+    if (go && c.cycleLeft < 0) {
+      const int64_t nIn = int64_t(S.bkStart[B.nkinds] - S.bkStart[0]);
+      c.cycleLeft = nIn > 1 ? nIn : 1;
+      if (c.cycleLeft > B.nsteps - c.done) c.cycleLeft = B.nsteps - c.done;
+    }
+  }
+  if (!go) return;
+  AdjustMovesD(B, S, uint64_t(B.step0 + c.done));
+  const double draw0 = MtClosed(S) * 1.0;
+  double sum = 0.0, prev = 0.0;
+  uint32_t pick = 0;
+  for (pick = 0; pick < uint32_t(B.nMoveTypes) && sum < draw0; ++pick) {
+    prev = sum;
+    sum += B.moveW[pick];
+  }
+  if (pick != 0) --pick;
+  c.subDraw = draw0 - prev;
+  c.pick = int(pick);
+  if (pick == kMolTransfer && B.nMoveTypes == 13)
+    c.op = kOpTransfer;
+  else if (pick == kDisplace || pick == kRotate)
+    c.op = kOpMol;
+  else
+    S.status = kStMoveKind;
+}
+
+// DCSingle::BuildNew / BuildOld (DCSingle.cpp:23-100): cbmcFirst positions
+// uniform in the box (the old build's first is the actual atom).
+NOINLINE __device__ void SinglePrep(const DevBox &B, DevState &s, Ctl &c, const DTrial &t,
+                                    int atom, bool isNew) {
+  const int n = B.cbmcFirst, kind = c.kind;
+  for (int i = 0; i < n; ++i) {
+    const double xx = MtHalfOpen(s) * B.g.L[0];
+    const double yy = MtHalfOpen(s) * B.g.L[1];
+    const double zz = MtHalfOpen(s) * B.g.L[2];
+    c.cx[i] = xx;
+    c.cy[i] = yy;
+    c.cz[i] = zz;
+    c.ct[i] = B.kindT[kind][atom];
+    c.cq[i] = B.kindQ[kind][atom];
+  }
+  if (!isNew) {
+    c.cx[0] = Wrap(t.px[atom], B.g.L[0]);
+    c.cy[0] = Wrap(t.py[atom], B.g.L[1]);
+    c.cz[0] = Wrap(t.pz[atom], B.g.L[2]);
+  }
+  c.nc = t.box == 0 ? n : 0;
+  for (int i = c.nc; i < n; ++i) c.cE[i] = c.cR[i] = c.cO[i] = 0.0;
+}
+
+NOINLINE __device__ void SingleFinish(const DevBox &B, DevState &s, Ctl &c, DTrial &t, int atom,
+                                      bool isNew) {
+  const int n = B.cbmcFirst;
+  double stepWeight = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double inter = 0.0 + c.cE[i], real = 0.0 + c.cR[i];
+    c.w[i] = exp(-1 * B.beta * (inter + real));
+    stepWeight += c.w[i];
+  }
+  int win = 0;
+  if (isNew) {
+    win = PickWeightedD(s, c.w, n, stepWeight);
+    if (win < 0) {
+      s.status = kStWeights;
+      c.abort = 1;
+      win = 0;
+    }
+    t.px[atom] = c.cx[win];
+    t.py[atom] = c.cy[win];
+    t.pz[atom] = c.cz[win];
+  }
+  t.overlap = t.overlap || (c.cO[win] > 0.0);
+  t.weight *= stepWeight / n;
+  t.inter += 0.0 + c.cE[win];
+  t.real += 0.0 + c.cR[win];
+}
+
+template <int P>
+__device__ void SingleStage(const DevBox &B, DevState &s, Ctl &c, DTrial &t, int atom,
+                            bool isNew) {
+  if (threadIdx.x == 0) SinglePrep(B, s, c, t, atom, isNew);
+  __syncthreads();
+  ComboEnergies<P>(B, c);
+  if (threadIdx.x == 0) SingleFinish(B, s, c, t, atom, isNew);
+  __syncthreads();
+}
+
+// DCFreeHedron (DCFreeHedron.cpp:48-240) for a rigid three-site kind grown
+// from atom 0: the fixed angle's trials (all of weight 1, the winner still
+// drawn when growing), the seed atom, then cbmcNth spins of the rigid frame.
+NOINLINE __device__ void HedronTheta(const DevBox &B, DevState &s, Ctl &c, bool isNew) {
+  double tw = 0.0;
+  const int n = isNew ? B.cbmcAng : B.cbmcAng - 1;
+  for (int i = 0; i < n; ++i) tw += 1.0;
+  c.thetaW = tw;
+  if (isNew) (void)MtClosed(s);  // DCHedron::FreeAnglesNew's PickWeighted, equal weights
+}
+
+NOINLINE __device__ void HedronPrep(const DevBox &B, DevState &s, Ctl &c, const DTrial &t,
+                                    bool isNew) {
+  const int n = B.cbmcNth, kind = c.kind;
+  const double cx = t.px[0], cy = t.py[0], cz = t.pz[0];
+  // spun frames go to cx/cy/cz [0, n) (atom 2) and [n, 2n) (atom 1)
+  if (isNew) {
+    const double th = B.hedTh[kind];
+    const double sinTh = sin(th);
+    double pbx = B.hedB[kind], pby = B.hedB[kind], pbz = B.hedB[kind];
+    pbx *= sinTh * cos(0.0);
+    pby *= sinTh * sin(0.0);
+    pbz *= cos(th);
+    double ppx = B.hedP[kind], ppy = B.hedP[kind], ppz = B.hedP[kind];
+    ppx *= sin(0.0) * cos(0.0);
+    ppy *= sin(0.0) * sin(0.0);
+    ppz *= cos(0.0);
+    for (int lj = n; lj-- > 0;) {
+      const double u1 = MtClosed(s), u2 = MtClosed(s), u3 = MtClosed(s);
+      const DM3 spin = UniformRotationD(u1, u2, u3);
+      double ax, ay, az;
+      Apply(spin, pbx, pby, pbz, ax, ay, az);
+      c.cx[lj] = ax + cx;
+      c.cy[lj] = ay + cy;
+      c.cz[lj] = az + cz;
+      Apply(spin, ppx, ppy, ppz, ax, ay, az);
+      c.cx[n + lj] = ax + cx;
+      c.cy[n + lj] = ay + cy;
+      c.cz[n + lj] = az + cz;
+    }
+  } else {
+    c.thetaW += exp(-1 * B.beta * (0.0 + 0.0));  // DCHedron::IncorporateOld
+    const double bx = Unwrap(t.px[2], cx, B.g.L[0], B.g.half[0]) + (-cx);
+    const double by = Unwrap(t.py[2], cy, B.g.L[1], B.g.half[1]) + (-cy);
+    const double bz = Unwrap(t.pz[2], cz, B.g.L[2], B.g.half[2]) + (-cz);
+    const double qx = Unwrap(t.px[1], cx, B.g.L[0], B.g.half[0]) + (-cx);
+    const double qy = Unwrap(t.py[1], cy, B.g.L[1], B.g.half[1]) + (-cy);
+    const double qz = Unwrap(t.pz[1], cz, B.g.L[2], B.g.half[2]) + (-cz);
+    for (int lj = n; lj-- > 1;) {
+      const double u1 = MtClosed(s), u2 = MtClosed(s), u3 = MtClosed(s);
+      const DM3 spin = UniformRotationD(u1, u2, u3);
+      double ax, ay, az;
+      Apply(spin, bx, by, bz, ax, ay, az);
+      c.cx[lj] = ax + cx;
+      c.cy[lj] = ay + cy;
+      c.cz[lj] = az + cz;
+      Apply(spin, qx, qy, qz, ax, ay, az);
+      c.cx[n + lj] = ax + cx;
+      c.cy[n + lj] = ay + cy;
+      c.cz[n + lj] = az + cz;
+    }
+    c.cx[0] = bx + cx;
+    c.cy[0] = by + cy;
+    c.cz[0] = bz + cz;
+    c.cx[n] = qx + cx;
+    c.cy[n] = qy + cy;
+    c.cz[n] = qz + cz;
+  }
+  for (int i = 0; i < 2 * n; ++i) {
+    c.cx[i] = Wrap(c.cx[i], B.g.L[0]);
+    c.cy[i] = Wrap(c.cy[i], B.g.L[1]);
+    c.cz[i] = Wrap(c.cz[i], B.g.L[2]);
+    const int atom = i < n ? 2 : 1;
+    c.ct[i] = B.kindT[kind][atom];
+    c.cq[i] = B.kindQ[kind][atom];
+  }
+  c.nc = t.box == 0 ? 2 * n : 0;
+  for (int i = c.nc; i < 2 * n; ++i) c.cE[i] = c.cR[i] = c.cO[i] = 0.0;
+}
+
+NOINLINE __device__ void HedronFinish(const DevBox &B, DevState &s, Ctl &c, DTrial &t,
+                                      bool isNew) {
+  // atom 2's terms first, then atom 1's, as the host accumulates them
+  const int n = B.cbmcNth;
+  double stepWeight = 0.0;
+  for (int i = 0; i < n; ++i) {
+    c.sI[i] = (0.0 + c.cE[i]) + c.cE[n + i];
+    c.sR[i] = (0.0 + c.cR[i]) + c.cR[n + i];
+    c.w[i] = exp(-B.beta * (c.sI[i] + c.sR[i]));
+    stepWeight += c.w[i];
+  }
+  int win = 0;
+  if (isNew) {
+    win = PickWeightedD(s, c.w, n, stepWeight);
+    if (win < 0) {
+      s.status = kStWeights;
+      c.abort = 1;
+      win = 0;
+    }
+    t.px[2] = c.cx[win];
+    t.py[2] = c.cy[win];
+    t.pz[2] = c.cz[win];
+    t.px[1] = c.cx[n + win];
+    t.py[1] = c.cy[n + win];
+    t.pz[1] = c.cz[n + win];
+  }
+  t.overlap = t.overlap || (c.cO[win] > 0.0 || c.cO[n + win] > 0.0);
+  t.inter += c.sI[win];
+  t.real += c.sR[win];
+  t.weight *= c.thetaW * 1.0;
+  t.weight *= stepWeight / n;
+}
+
+template <int P>
+__device__ void HedronStage(const DevBox &B, DevState &s, Ctl &c, DTrial &t, bool isNew) {
+  if (threadIdx.x == 0) HedronTheta(B, s, c, isNew);
+  SingleStage<P>(B, s, c, t, 0, isNew);
+  if (c.abort) return;
+  if (threadIdx.x == 0) HedronPrep(B, s, c, t, isNew);
+  __syncthreads();
+  ComboEnergies<P>(B, c);
+  if (threadIdx.x == 0) HedronFinish(B, s, c, t, isNew);
+  __syncthreads();
+}
+
+// Ewald::SwapCorrection (src/Ewald.cpp:1311-1335) of a trial molecule.
+NOINLINE __device__ double SwapCorrectionD(const DevBox &B, const DTrial &t, int kind) {
+  const int len = B.kindLen[kind];
+  double correction = 0.0;
+  for (int i = 0; i < len; ++i)
+    for (int j = i + 1; j < len; ++j) {
+      double dx = t.px[i] - t.px[j], dy = t.py[i] - t.py[j], dz = t.pz[i] - t.pz[j];
+      if (dx > B.g.half[0]) dx -= B.g.L[0];
+      else if (dx < -B.g.half[0]) dx += B.g.L[0];
+      if (dy > B.g.half[1]) dy -= B.g.L[1];
+      else if (dy < -B.g.half[1]) dy += B.g.L[1];
+      if (dz > B.g.half[2]) dz -= B.g.L[2];
+      else if (dz < -B.g.half[2]) dz += B.g.L[2];
+      const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+      correction -= (B.kindQ[kind][i] * B.kindQ[kind][j] * erf(B.g.alpha * dist) / dist);
+    }
+  return kQQFact * correction;
+}
+
+// CalculateEnergy::MoleculeTailChange (CalculateEnergy.cpp:788-808).
+__device__ double TailChange(const DevBox &B, const DevState &s, int kind, bool add) {
+  const double sign = add ? 1.0 : -1.0;
+  double delta = 0.0;
+  for (int j = 0; j < B.nkinds; ++j)
+    delta += B.pairEnCorr[j * B.nkinds + kind] * (sign * 2.0 * double(s.kindCount[j]) * B.volInv);
+  delta += B.pairEnCorr[kind * B.nkinds + kind] * B.volInv;
+  return delta;
+}
+
+// MoleculeTransfer: PickMolAndBoxPair2 (PRNG.h:450-455) and Prep.
+NOINLINE __device__ void TransferPick(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+  const int K = B.nkinds;
+  c.op = kOpTransfer;
+  double subDraw = c.subDraw;
+  const double boxDiv = B.moveW[kMolTransfer] / 2;
+  uint32_t src = uint32_t(subDraw / boxDiv);
+  if (src >= 2) src--;
+  subDraw -= src * boxDiv;
+  const int dst = src == 0 ? 1 : 0;
+  const uint32_t mkTot = uint32_t(B.nSwapKinds);
+  const double molDiv = boxDiv / mkTot;
+  uint32_t k = uint32_t(subDraw / molDiv);
+  if (k == mkTot) k = mkTot - 1;
+  const int kind = B.swapKind[k];
+  c.src = int(src);
+  c.dst = dst;
+  c.kind = kind;
+  c.mol = -1;
+  if (NumKindInBox(B, s, kind, int(src)) == 0) {
+    if (src == 1) {
+      s.status = kStEmptyReservoir;
+      c.abort = 1;
+    } else {
+      s.swapTries[dst * K + kind]++;
+      Record(B, i, kMolTransfer, -1, 0, 0, 0.0, 0.0, 0.0);
+    }
+    c.op = kOpNone;
+    return;
+  }
+  const uint32_t off = MtBelow(s, NumKindInBox(B, s, kind, int(src)) - 1);
+  const int m = int(B.lookup[s.bkStart[int(src) * K + kind] + off]);
+  c.mol = m;
+  c.st = B.molStart[m];
+  c.len = B.molStart[m + 1] - c.st;
+  c.skip = m;
+  c.nm.box = dst;
+  c.om.box = int(src);
+  c.nm.weight = c.om.weight = 1.0;
+  c.nm.overlap = c.om.overlap = 0;
+  c.nm.inter = c.nm.real = c.om.inter = c.om.real = 0.0;
+  for (int p = 0; p < kDevKindAtoms; ++p) {
+    c.nm.px[p] = c.nm.py[p] = c.nm.pz[p] = 0.0;
+    c.om.px[p] = p < c.len ? B.x[c.st + p] : 0.0;
+    c.om.py[p] = p < c.len ? B.y[c.st + p] : 0.0;
+    c.om.pz[p] = p < c.len ? B.z[c.st + p] : 0.0;
+  }
+  if (c.len == 1)
+    (void)MtBelow(s, 1);  // DCLinear::Build: forward or backward
+  else
+    (void)MtBelow(s, 0);  // DCGraph::Build: its only node
+}
+
+// MoleculeTransfer::CalcEn (MoleculeTransfer.h:106-133) up to the
+// reciprocal change.
+NOINLINE __device__ void TransferCalc(const DevBox &B, DevState &s, Ctl &c) {
+  const int src = c.src, dst = c.dst, kind = c.kind;
+  c.tcLose = c.tcGain = 0.0;
+  c.wTc = 1.0;
+  if (B.lrc) {
+    c.tcLose = src == 0 ? TailChange(B, s, kind, false) : 0.0;
+    c.tcGain = dst == 0 ? TailChange(B, s, kind, true) : 0.0;
+    c.wTc = exp(-1.0 * B.beta * (c.tcGain + c.tcLose));
+  }
+  c.tried = c.nm.weight > kSmallWeight && !c.nm.overlap;
+  c.corrNew = c.corrOld = c.selfNew = c.selfOld = 0.0;
+  c.doRecip = c.tried && B.g.ewald;
+  if (c.doRecip) {
+    c.corrNew = dst == 0 ? SwapCorrectionD(B, c.nm, kind) : 0.0;
+    c.corrOld = src == 0 ? SwapCorrectionD(B, c.om, kind) : 0.0;
+    c.selfNew = dst == 0 ? B.selfKind[kind] : 0.0;
+    c.selfOld = src == 0 ? B.selfKind[kind] : 0.0;
+    const DTrial &t = dst == 0 ? c.nm : c.om;
+    for (int p = 0; p < kDevKindAtoms; ++p) {
+      c.nx[p] = t.px[p];
+      c.ny[p] = t.py[p];
+      c.nz[p] = t.pz[p];
+      c.mq[p] = p < c.len ? B.kindQ[kind][p] : 0.0;
+    }
+    c.sign = dst == 0 ? 1 : -1;
+  }
+}
+
+// MoleculeTransfer::Accept (MoleculeTransfer.h:167-246) with GetCoeff's
+// fugacities (143-165): the decision and the running energies.
+NOINLINE __device__ void TransferAccept(const DevBox &B, DevState &s, Ctl &c) {
+  const int K = B.nkinds, src = c.src, dst = c.dst, kind = c.kind;
+  c.recipLose = c.recipGain = 0.0;
+  double wRecip = 1.0;
+  if (c.doRecip) {
+    const double enew = c.res[0];
+    if (dst == 0)
+      c.recipGain = enew - s.pot[3];
+    else
+      c.recipLose = enew - s.pot[3];
+    wRecip = exp(-1.0 * B.beta *
+                 (c.recipGain + c.recipLose + c.corrNew - c.corrOld + c.selfNew - c.selfOld));
+  }
+  double coeff;
+  if (src == 0)
+    coeff = double(s.swapCount[0 * K + kind]) * B.volInv / (B.beta * B.fugacity[kind]);
+  else
+    coeff = B.volume / double(s.swapCount[dst * K + kind] + 1) * (B.beta * B.fugacity[kind]);
+  const double Wrat = c.nm.weight / c.om.weight * c.wTc * wRecip;
+  bool result = false;
+  if (c.tried) result = MtClosed(s) < coeff * Wrat;
+  c.commit = result;
+  if (!result) return;
+  if (src == 0) {
+    s.pot[1] += c.tcLose;
+    s.pot[0] -= c.om.inter;
+    s.pot[2] -= c.om.real;
+    s.pot[3] += c.recipLose;
+    s.pot[5] -= c.corrOld;
+    s.pot[4] -= c.selfOld;
+  } else {
+    s.pot[1] += c.tcGain;
+    s.pot[0] += c.nm.inter;
+    s.pot[2] += c.nm.real;
+    s.pot[3] += c.recipGain;
+    s.pot[5] += c.corrNew;
+    s.pot[4] += c.selfNew;
+  }
+  c.section = src * K + kind;
+  c.found = -1;
+}
+
+// The accepted transfer's bookkeeping: COM::SetNew (src/COM.h:63-88),
+// MoleculeLookup::Shift (src/MoleculeLookup.cpp:165-198), counts; and the
+// move's record.
+NOINLINE __device__ void TransferFinish(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+  const int K = B.nkinds, src = c.src, dst = c.dst, kind = c.kind, m = c.mol, len = c.len;
+  if (c.commit) {
+    const double rx = c.nm.px[0], ry = c.nm.py[0], rz = c.nm.pz[0];
+    double cx = 0, cy = 0, cz = 0;
+    for (int p = 0; p < len; ++p) {
+      cx += Unwrap(c.nm.px[p], rx, B.g.L[0], B.g.half[0]);
+      cy += Unwrap(c.nm.py[p], ry, B.g.L[1], B.g.half[1]);
+      cz += Unwrap(c.nm.pz[p], rz, B.g.L[2], B.g.half[2]);
+    }
+    const double inv = 1.0 / double(len);
+    B.com[m] = Wrap(cx * inv, B.g.L[0]);
+    B.com[B.nmols + m] = Wrap(cy * inv, B.g.L[1]);
+    B.com[2 * B.nmols + m] = Wrap(cz * inv, B.g.L[2]);
+    uint32_t section = uint32_t(c.section);
+    uint32_t oldIndex = uint32_t(c.found), newIndex;
+    const uint32_t target = uint32_t(dst * K + kind);
+    s.swapCount[section]--;
+    s.swapCount[target]++;
+    if (src >= dst) {
+      while (section != target) {
+        newIndex = s.bkStart[section]++;
+        const uint32_t tmp = B.lookup[oldIndex];
+        B.lookup[oldIndex] = B.lookup[newIndex];
+        B.lookup[newIndex] = tmp;
+        oldIndex = newIndex;
+        --section;
+      }
+    } else {
+      while (section != target) {
+        newIndex = --s.bkStart[++section];
+        const uint32_t tmp = B.lookup[oldIndex];
+        B.lookup[oldIndex] = B.lookup[newIndex];
+        B.lookup[newIndex] = tmp;
+        oldIndex = newIndex;
+      }
+    }
+    B.molBox[m] = dst;
+    if (dst == 0)
+      s.kindCount[kind]++;
+    else
+      s.kindCount[kind]--;
+    const uint32_t left = s.bkStart[K] - s.bkStart[0];
+    if (src == 0 && left == 0) {
+      for (int e = 0; e < 7; ++e) s.pot[e] = 0.0;
+    } else if (src == 0 && left == 1) {
+      s.pot[0] = 0;
+      s.pot[2] = 0;
+    }
+    Total(s);
+    s.accepted++;
+  }
+  s.swapTries[dst * K + kind]++;
+  if (c.commit) s.swapAcc[dst * K + kind]++;
+  const double e0 = dst == 0 ? c.nm.inter : -c.om.inter;
+  const double e1 = dst == 0 ? c.nm.real : -c.om.real;
+  Record(B, i, kMolTransfer, m, c.commit, c.nm.overlap, e0, e1, c.recipGain + c.recipLose);
+}
+
+// The MoleculeTransfer move after PickArbDist (gcmc.cpp TransferMove).
+template <int P>
+__device__ void TransferMoveD(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+  if (threadIdx.x == 0) TransferPick(B, s, c, i);
+  __syncthreads();
+  if (c.op != kOpTransfer) return;
+  if (c.len == 1) {
+    SingleStage<P>(B, s, c, c.nm, 0, true);
+    if (c.abort) return;
+    SingleStage<P>(B, s, c, c.om, 0, false);
+  } else {
+    HedronStage<P>(B, s, c, c.nm, true);
+    if (c.abort) return;
+    HedronStage<P>(B, s, c, c.om, false);
+  }
+  if (c.abort) return;
+  if (threadIdx.x == 0) TransferCalc(B, s, c);
+  __syncthreads();
+  if (c.doRecip) SwapRecipEnergy<P>(B, c);
+  if (threadIdx.x == 0) TransferAccept(B, s, c);
+  __syncthreads();
+  if (c.commit) {
+    // the molecule's index in its lookup section, and its new coordinates
+    const int m = c.mol, st = c.st;
+    const int lo = int(s.bkStart[c.section]), hi = int(s.bkStart[c.section + 1]);
+    for (int idx = lo + threadIdx.x; idx < hi; idx += kDevThreads)
+      if (int(B.lookup[idx]) == m) c.found = idx;
+    if (threadIdx.x < c.len) {
+      const int p = threadIdx.x;
+      B.x[st + p] = c.nm.px[p];
+      B.y[st + p] = c.nm.py[p];
+      B.z[st + p] = c.nm.pz[p];
+      B.mol[st + p] = c.dst == 0 ? m : -1;
+    }
+    if (c.recipLose != 0.0 || c.recipGain != 0.0) CommitRecip(B);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && c.commit && c.found < 0) {  // lookup out of step: stop here
+    s.status = kStErr;
+    c.abort = 1;
+  }
+  __syncthreads();
+  if (c.abort) return;
+  if (threadIdx.x == 0) TransferFinish(B, s, c, i);
+  __syncthreads();
+}
+
+// Translate / Rotate: PickMolAndBox, Prep and Transform (driver.cpp
+// OneMove; src/Coordinates.cpp:83-128).
+NOINLINE __device__ void MolPick(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+  c.op = kOpMol;
+  double subDraw = c.subDraw;
+  const int pick = c.pick;
+  const double boxDiv = B.movePerc[pick] / 1;
+  uint32_t box = uint32_t(subDraw / boxDiv);
+  if (box >= 1) box--;
+  subDraw -= box * boxDiv;
+  const uint32_t mkTot = uint32_t(B.nMoveKinds);
+  uint32_t k = uint32_t(subDraw / (boxDiv / mkTot));
+  if (k == mkTot) k = mkTot - 1;
+  const int mk = B.moveKind[k];
+  int mv = pick;
+  c.mk = mk;
+  c.mv = mv;
+  if (NumKindInBox(B, s, mk, 0) == 0) {
+    s.tries[mv][mk]++;
+    s.tTries[mv][mk]++;
+    Record(B, i, mv, -1, 0, 0, 0.0, 0.0, 0.0);
+    c.op = kOpNone;
+    return;
+  }
+  const uint32_t off = MtBelow(s, NumKindInBox(B, s, mk, 0) - 1);
+  const int m = int(B.lookup[s.bkStart[0 * B.nkinds + mk] + off]);
+  if (mv == kRotate && B.kindLen[mk] <= 1) mv = kDisplace;  // Rotate::Prep
+  c.mv = mv;
+  c.mol = m;
+  c.st = B.molStart[m];
+  c.len = B.molStart[m + 1] - c.st;
+  const double *L = B.g.L, *H = B.g.half;
+  c.ncom[0] = B.com[m];
+  c.ncom[1] = B.com[B.nmols + m];
+  c.ncom[2] = B.com[2 * B.nmols + m];
+  for (int p = 0; p < kDevKindAtoms; ++p) {
+    const bool in = p < c.len;
+    c.ox[p] = in ? B.x[c.st + p] : 0.0;
+    c.oy[p] = in ? B.y[c.st + p] : 0.0;
+    c.oz[p] = in ? B.z[c.st + p] : 0.0;
+    c.mq[p] = in ? B.q[c.st + p] : 0.0;
+    c.mt_[p] = in ? B.type[c.st + p] : 0;
+    c.nx[p] = c.ny[p] = c.nz[p] = 0.0;
+  }
+  if (mv == kDisplace) {
+    const double bb = s.scale[kDisplace][mk], b2 = 2.0 * bb;
+    const double sx = MtClosed(s) * b2 - bb;
+    const double sy = MtClosed(s) * b2 - bb;
+    const double sz = MtClosed(s) * b2 - bb;
+    for (int p = 0; p < c.len; ++p) {
+      c.nx[p] = Wrap(c.ox[p] + sx, L[0]);
+      c.ny[p] = Wrap(c.oy[p] + sy, L[1]);
+      c.nz[p] = Wrap(c.oz[p] + sz, L[2]);
+    }
+    c.ncom[0] = Wrap(c.ncom[0] + sx, L[0]);
+    c.ncom[1] = Wrap(c.ncom[1] + sy, L[1]);
+    c.ncom[2] = Wrap(c.ncom[2] + sz, L[2]);
+  } else {
+    const double bb = s.scale[kRotate][mk];
+    const double theta = MtClosed(s) * (2.0 * bb) - bb;
+    const double u = MtClosed(s) * (2.0 * 1.0) - 1.0;
+    const double phi = MtHalfOpen(s) * (2.0 * M_PI);
+    const double rr = sqrt(1.0 - u * u);
+    const DM3 R = AxisAngleD(theta, rr * cos(phi), rr * sin(phi), u);
+    const double cx = c.ncom[0], cy = c.ncom[1], cz = c.ncom[2];
+    for (int p = 0; p < c.len; ++p) {
+      const double vx = Unwrap(c.ox[p], cx, L[0], H[0]) + (-cx);
+      const double vy = Unwrap(c.oy[p], cy, L[1], H[1]) + (-cy);
+      const double vz = Unwrap(c.oz[p], cz, L[2], H[2]) + (-cz);
+      double ax, ay, az;
+      Apply(R, vx, vy, vz, ax, ay, az);
+      c.nx[p] = Wrap(ax + cx, L[0]);
+      c.ny[p] = Wrap(ay + cy, L[1]);
+      c.nz[p] = Wrap(az + cz, L[2]);
+    }
+  }
+  c.pr = MtClosed(s);  // Translate/Rotate::Accept draw
+  c.doRecip = B.g.ewald;
+}
+
+// Translate / Rotate::Accept and the move's record.
+NOINLINE __device__ void MolAccept(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+  const int mv = c.mv, mk = c.mk;
+  const bool overlap = c.res[3] > 0.0;
+  const double dlj = c.res[0], dreal = c.res[1];
+  const double drecip = (!overlap && B.g.ewald) ? c.res[2] - s.pot[3] : 0.0;
+  const bool res = c.pr < exp(-B.beta * (dlj + dreal + drecip));
+  const bool result = res && !overlap;
+  c.commit = result;
+  if (result) {
+    s.pot[0] += dlj;
+    s.pot[2] += dreal;
+    s.pot[3] += drecip;
+    if (mv == kDisplace) {
+      B.com[c.mol] = c.ncom[0];
+      B.com[B.nmols + c.mol] = c.ncom[1];
+      B.com[2 * B.nmols + c.mol] = c.ncom[2];
+    }
+    Total(s);
+    s.accepted++;
+  }
+  s.tries[mv][mk]++;
+  s.tTries[mv][mk]++;
+  if (result) {
+    s.tAcc[mv][mk]++;
+    s.acc[mv][mk]++;
+  }
+  Record(B, i, mv, c.mol, result, overlap, dlj, dreal, drecip);
+}
+
+template <int P>
+__device__ void MolMoveD(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+  if (threadIdx.x == 0) MolPick(B, s, c, i);
+  __syncthreads();
+  if (c.op != kOpMol) return;
+  MolTrialEnergies<P>(B, c);
+  if (threadIdx.x == 0) MolAccept(B, s, c, i);
+  __syncthreads();
+  if (c.commit) {
+    if (threadIdx.x < c.len) {
+      const int p = threadIdx.x;
+      B.x[c.st + p] = c.nx[p];
+      B.y[c.st + p] = c.ny[p];
+      B.z[c.st + p] = c.nz[p];
+    }
+    if (B.g.ewald) CommitRecip(B);
+  }
+  __syncthreads();
+}
+
+// One workgroup per box: the box's moves from step0, either nsteps moves
+// or, with cycles > 0, up to `cycles` cycles of N_box moves each (N_box
+// counted at the cycle's start, at least 1; never more than nsteps in all),
+// sampling the molecule count of kinds 0 and 1 after every cycle.
+template <int P>
+__global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *boxes) {
+  __shared__ DevBox B;
+  __shared__ DevState S;
+  __shared__ Ctl c;
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&boxes[blockIdx.x]);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&B);
+    for (int w = threadIdx.x; w < int(sizeof(DevBox) / 4); w += kDevThreads) dst[w] = src[w];
+  }
+  __syncthreads();
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(B.st);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&S);
+    for (int w = threadIdx.x; w < int(sizeof(DevState) / 4); w += kDevThreads) dst[w] = src[w];
+  }
+  if (threadIdx.x == 0) {
+    c.abort = 0;
+    c.done = 0;
+    c.cyclesDone = 0;
+    c.cycleLeft = -1;
+  }
+  __syncthreads();
+  while (true) {
+    if (threadIdx.x == 0) LoopTop(B, S, c);
+    __syncthreads();
+    const int op = c.op;
+    if (op == kOpStop) break;
+    const int64_t i = c.done;
+    if (op == kOpTransfer)
+      TransferMoveD<P>(B, S, c, i);
+    else
+      MolMoveD<P>(B, S, c, i);
+    if (threadIdx.x == 0 && !c.abort) {
+      c.done++;
+      S.moves++;
+      if (B.cycles > 0) c.cycleLeft--;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && B.cycles > 0 && c.cycleLeft == 0 && !c.abort && S.status == 0) {
+    S.sumN[0] += B.nkinds > 0 ? double(NumKindInBox(B, S, 0, 0)) : 0.0;
+    S.sumN[1] += B.nkinds > 1 ? double(NumKindInBox(B, S, 1, 0)) : 0.0;
+    S.samples++;
+  }
+  __syncthreads();
+  {
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(&S);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(B.st);
+    for (int w = threadIdx.x; w < int(sizeof(DevState) / 4); w += kDevThreads) dst[w] = src[w];
+  }
+}
+
+static_assert(sizeof(DevBox) % 4 == 0 && sizeof(DevState) % 4 == 0, "word copies");
+
+}  // namespace
+
+// Device buffers of a box's device move loop, allocated on first use.
+struct DevDriver {
+  DevBox hbox;                  // this box's description for the next launch
+  DevState *state = nullptr;    // its state (device)
+  int *molKind = nullptr, *molBox = nullptr;
+  uint32_t *lookup = nullptr;
+  double *com = nullptr;
+  int32_t *trace = nullptr;     // [4][cap] ints then [3][cap] doubles
+  int64_t traceCap = 0;
+  DevBox *list = nullptr;       // launch list when this box leads a launch
+  int listCap = 0;
+  DevState hstate;
+};
+
+void DevFree(gomc_box *b) {
+  DevDriver *d = b->dev;
+  if (!d) return;
+  for (void *p : {(void *)d->state, (void *)d->molKind, (void *)d->molBox,
+                  (void *)d->lookup, (void *)d->com, (void *)d->trace, (void *)d->list})
+    if (p) (void)hipFree(p);
+  delete d;
+  b->dev = nullptr;
+}
+
+namespace {
+
+int DevCheck(gomc_box *b) {
+  if (b->cached) return Fail(b, GOMC_ERR_STATE, "device move loop: EwaldCached mode is not supported");
+  if (b->moveW[2] > 0.0)
+    return Fail(b, GOMC_ERR_STATE, "device move loop: MultiParticle moves run on the host driver");
+  if (b->nkinds > kDevKinds) return Fail(b, GOMC_ERR_ARG, "device move loop: more than 4 molecule kinds");
+  for (int k = 0; k < b->nkinds; ++k) {
+    const int len = int(b->kindType[k].size());
+    if (len > kDevKindAtoms) return Fail(b, GOMC_ERR_ARG, "device move loop: molecules of more than 3 atoms");
+    if (b->ensemble == 1 && len != 1 && len != 3)
+      return Fail(b, GOMC_ERR_ARG, "device move loop: transferred kinds must have 1 or 3 atoms");
+  }
+  if (b->cbmcFirst > kDevTrials || b->cbmcNth > kDevTrials)
+    return Fail(b, GOMC_ERR_ARG, "device move loop: more than 32 CBMC trials per stage");
+  return GOMC_OK;
+}
+
+int DevEnsure(gomc_box *b, int64_t traceMoves) {
+  if (!b->dev) {
+    b->dev = new DevDriver();
+    DevDriver *d = b->dev;
+    HIP_TRY(b, hipMalloc(&d->state, sizeof(DevState)));
+    HIP_TRY(b, hipMalloc(&d->molKind, sizeof(int) * std::max(1, b->nmols)));
+    HIP_TRY(b, hipMalloc(&d->molBox, sizeof(int) * std::max(1, b->nmols)));
+    HIP_TRY(b, hipMalloc(&d->lookup, sizeof(uint32_t) * std::max(1, b->nmols)));
+    HIP_TRY(b, hipMalloc(&d->com, sizeof(double) * 3 * std::max(1, b->nmols)));
+    HIP_TRY(b, hipMemcpy(d->molKind, b->molKind.data(), sizeof(int) * b->nmols, hipMemcpyHostToDevice));
+  }
+  DevDriver *d = b->dev;
+  if (traceMoves > d->traceCap) {
+    if (d->trace) (void)hipFree(d->trace);
+    d->trace = nullptr;
+    d->traceCap = 0;
+    HIP_TRY(b, hipMalloc(&d->trace, size_t(traceMoves) * (4 * sizeof(int32_t) + 3 * sizeof(double))));
+    d->traceCap = traceMoves;
+  }
+  return GOMC_OK;
+}
+
+// Host state -> device: the box description and the move loop's state.
+int DevUpload(gomc_box *b, int64_t step0, int64_t nsteps, int cycles, bool trace) {
+  DevDriver *d = b->dev;
+  DevBox &h = d->hbox;
+  std::memset(&h, 0, sizeof(h));
+  h.g = b->geom;
+  h.beta = b->beta;
+  h.volume = b->volume;
+  h.volInv = b->volInv;
+  const int K = b->nkinds;
+  for (int i = 0; i < K * K; ++i) h.pairEnCorr[i] = b->pairEnCorr.empty() ? 0.0 : b->pairEnCorr[i];
+  for (int k = 0; k < K; ++k) {
+    h.fugacity[k] = b->fugacity.empty() ? 0.0 : b->fugacity[k];
+    double en = 0.0;  // Ewald::SwapSelf (src/Ewald.cpp:1375-1391)
+    for (double q : b->kindCharge[k]) en -= (q * q);
+    h.selfKind[k] = (en * b->alpha * kQQFact * M_2_SQRTPI * 0.5);
+    h.kindLen[k] = int(b->kindType[k].size());
+    for (int a = 0; a < h.kindLen[k]; ++a) {
+      h.kindT[k][a] = b->kindType[k][a];
+      h.kindQ[k][a] = b->kindCharge[k][a];
+    }
+    if (!b->hedron.empty()) {
+      h.hedB[k] = b->hedron[k].bBonded;
+      h.hedP[k] = b->hedron[k].bPrev;
+      h.hedTh[k] = b->hedron[k].theta;
+    }
+  }
+  for (int i = 0; i < 13; ++i) h.moveW[i] = b->moveW[i];
+  h.movePerc[0] = b->movePerc[0];
+  h.movePerc[1] = b->movePerc[1];
+  h.nMoveTypes = b->ensemble == 1 ? 13 : 9;
+  h.lrc = b->lrc;
+  h.nkinds = K;
+  h.natoms = b->natoms;
+  h.nmols = b->nmols;
+  h.ntypes = b->ntypes;
+  h.K = b->ewald ? b->K : 0;
+  h.cbmcFirst = b->cbmcFirst;
+  h.cbmcNth = b->cbmcNth;
+  h.cbmcAng = b->cbmcAng;
+  h.nSwapKinds = int(b->canSwapKind.size());
+  h.nMoveKinds = int(b->canMoveKind.size());
+  for (int i = 0; i < h.nSwapKinds; ++i) h.swapKind[i] = b->canSwapKind[i];
+  for (int i = 0; i < h.nMoveKinds; ++i) h.moveKind[i] = b->canMoveKind[i];
+  h.perAdjust = b->perAdjust;
+  h.x = b->x;
+  h.y = b->y;
+  h.z = b->z;
+  h.q = b->q;
+  h.type = b->type;
+  h.mol = b->mol;
+  h.sigSq = b->tSigSq;
+  h.epsCn = b->tEpsCn;
+  h.nHalf = b->tNHalf;
+  h.kx = b->kx;
+  h.ky = b->ky;
+  h.kz = b->kz;
+  h.pref = b->pref;
+  h.sumR = b->sumRref;
+  h.sumI = b->sumIref;
+  h.newR = b->sumRnew;
+  h.newI = b->sumInew;
+  h.molStart = b->molStartDev;
+  h.molKind = d->molKind;
+  h.molBox = d->molBox;
+  h.lookup = d->lookup;
+  h.com = d->com;
+  h.st = d->state;
+  h.step0 = step0;
+  h.nsteps = nsteps;
+  h.cycles = cycles;
+  if (trace) {
+    h.tKind = d->trace;
+    h.tMol = d->trace + d->traceCap;
+    h.tAcc = d->trace + 2 * d->traceCap;
+    h.tOvl = d->trace + 3 * d->traceCap;
+    h.tE = reinterpret_cast<double *>(d->trace + 4 * d->traceCap);
+  }
+  DevState &s = d->hstate;
+  std::memset(&s, 0, sizeof(s));
+  b->mt.GetState(s.mt, &s.mtIdx);
+  s.pot[0] = b->pot.inter;
+  s.pot[1] = b->pot.tc;
+  s.pot[2] = b->pot.real;
+  s.pot[3] = b->pot.recip;
+  s.pot[4] = b->pot.self;
+  s.pot[5] = b->pot.corr;
+  s.pot[6] = b->pot.total;
+  for (int mv = 0; mv < 2; ++mv)
+    for (int k = 0; k < K; ++k) {
+      s.scale[mv][k] = b->moves.scale[mv][k];
+      s.tries[mv][k] = b->moves.tries[mv][k];
+      s.acc[mv][k] = b->moves.accepted[mv][k];
+      s.tTries[mv][k] = b->moves.tempTries[mv][k];
+      s.tAcc[mv][k] = b->moves.tempAccepted[mv][k];
+    }
+  for (int i = 0; i < 2 * K; ++i) {
+    s.swapTries[i] = b->swapTries.empty() ? 0 : b->swapTries[i];
+    s.swapAcc[i] = b->swapAcc.empty() ? 0 : b->swapAcc[i];
+    s.swapCount[i] = b->swapCount[i];
+  }
+  for (int i = 0; i <= 2 * K; ++i) s.bkStart[i] = b->bkStart[i];
+  for (int k = 0; k < K; ++k) s.kindCount[k] = b->kindCount[k];
+  std::vector<double> com(3 * size_t(b->nmols));
+  for (int m = 0; m < b->nmols; ++m) {
+    com[m] = b->comx[m];
+    com[b->nmols + m] = b->comy[m];
+    com[2 * size_t(b->nmols) + m] = b->comz[m];
+  }
+  std::vector<int> molBox(b->nmols, 0);
+  for (int m = 0; m < b->nmols && !b->molBox.empty(); ++m) molBox[m] = b->molBox[m];
+  HIP_TRY(b, hipMemcpyAsync(d->state, &s, sizeof(s), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(d->com, com.data(), sizeof(double) * com.size(), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(d->molBox, molBox.data(), sizeof(int) * b->nmols, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(d->lookup, b->lookup.data(), sizeof(uint32_t) * b->nmols, hipMemcpyHostToDevice, b->stream));
+  // the copies read host vectors that go out of scope: complete them now
+  HIP_TRY(b, hipStreamSynchronize(b->stream));
+  return GOMC_OK;
+}
+
+// Device -> host after a launch: the driver's state, coordinates, COMs,
+// molecule boxes and lookup, so the host driver and every query continue
+// from where the device loop stopped.
+int DevDownload(gomc_box *b) {
+  DevDriver *d = b->dev;
+  DevState &s = d->hstate;
+  std::vector<double> com(3 * size_t(b->nmols));
+  std::vector<int> molBox(b->nmols);
+  const size_t bytes = sizeof(double) * b->natoms;
+  HIP_TRY(b, hipMemcpyAsync(&s, d->state, sizeof(s), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(com.data(), d->com, sizeof(double) * com.size(), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(molBox.data(), d->molBox, sizeof(int) * b->nmols, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->lookup.data(), d->lookup, sizeof(uint32_t) * b->nmols, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->hx.data(), b->x, bytes, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->hy.data(), b->y, bytes, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->hz.data(), b->z, bytes, hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipStreamSynchronize(b->stream));
+  const int K = b->nkinds;
+  b->mt.SetState(s.mt, s.mtIdx);
+  b->pot.inter = s.pot[0];
+  b->pot.tc = s.pot[1];
+  b->pot.real = s.pot[2];
+  b->pot.recip = s.pot[3];
+  b->pot.self = s.pot[4];
+  b->pot.corr = s.pot[5];
+  b->pot.total = s.pot[6];
+  for (int mv = 0; mv < 2; ++mv)
+    for (int k = 0; k < K; ++k) {
+      b->moves.scale[mv][k] = s.scale[mv][k];
+      b->moves.tries[mv][k] = s.tries[mv][k];
+      b->moves.accepted[mv][k] = s.acc[mv][k];
+      b->moves.tempTries[mv][k] = s.tTries[mv][k];
+      b->moves.tempAccepted[mv][k] = s.tAcc[mv][k];
+    }
+  for (int i = 0; i < 2 * K; ++i) {
+    if (!b->swapTries.empty()) b->swapTries[i] = s.swapTries[i];
+    if (!b->swapAcc.empty()) b->swapAcc[i] = s.swapAcc[i];
+    b->swapCount[i] = s.swapCount[i];
+  }
+  for (int i = 0; i <= 2 * K; ++i) b->bkStart[i] = s.bkStart[i];
+  for (int k = 0; k < K; ++k) b->kindCount[k] = s.kindCount[k];
+  for (int m = 0; m < b->nmols; ++m) {
+    b->comx[m] = com[m];
+    b->comy[m] = com[b->nmols + m];
+    b->comz[m] = com[2 * size_t(b->nmols) + m];
+    if (!b->molBox.empty()) b->molBox[m] = molBox[m];
+  }
+  b->mp.singleAccepted = true;  // forces are recomputed by a later MultiParticle move
+  switch (s.status) {
+    case 0:
+      return GOMC_OK;
+    case kStEmptyReservoir:
+      return Fail(b, GOMC_ERR_STATE, "no molecules of this kind left in the reservoir");
+    case kStWeights:
+      return Fail(b, GOMC_ERR_STATE, "CBMC weights inconsistent");
+    case kStMoveKind:
+      return Fail(b, GOMC_ERR_STATE, "move kind not served by this driver");
+    default:
+      return Fail(b, GOMC_ERR_STATE, "device move loop failed");
+  }
+}
+
+int Launch(gomc_box *lead, const std::vector<gomc_box *> &boxes) {
+  DevDriver *d = lead->dev;
+  const int n = int(boxes.size());
+  if (n > d->listCap) {
+    if (d->list) (void)hipFree(d->list);
+    d->list = nullptr;
+    d->listCap = 0;
+    HIP_TRY(lead, hipMalloc(&d->list, sizeof(DevBox) * n));
+    d->listCap = n;
+  }
+  std::vector<DevBox> v(n);
+  for (int i = 0; i < n; ++i) v[i] = boxes[i]->dev->hbox;
+  HIP_TRY(lead, hipMemcpyAsync(d->list, v.data(), sizeof(DevBox) * n, hipMemcpyHostToDevice,
+                               lead->stream));
+  if (lead->precision == 1)
+    hipLaunchKernelGGL(DeviceMovesKernel<1>, dim3(n), dim3(kDevThreads), 0, lead->stream, d->list);
+  else
+    hipLaunchKernelGGL(DeviceMovesKernel<0>, dim3(n), dim3(kDevThreads), 0, lead->stream, d->list);
+  HIP_TRY(lead, hipGetLastError());
+  HIP_TRY(lead, hipStreamSynchronize(lead->stream));
+  return GOMC_OK;
+}
+
+}  // namespace
+}  // namespace gomc
+
+using namespace gomc;
+
+extern "C" {
+
+int64_t gomc_run_moves_device(gomc_box *b, int64_t start_step, int64_t nsteps, int32_t *kind,
+                              int32_t *mol, int32_t *acc, int32_t *ovl, double *dlj,
+                              double *dreal, double *drecip) {
+  if (!b || nsteps < 0) return GOMC_ERR_ARG;
+  if (nsteps == 0) return 0;
+  (void)hipSetDevice(b->device);
+  int rc = DevCheck(b);
+  if (!rc) rc = FlushPending(b);
+  if (!rc) rc = DevEnsure(b, nsteps);
+  if (!rc) rc = DevUpload(b, start_step, nsteps, 0, true);
+  if (!rc) rc = Launch(b, {b});
+  if (!rc) rc = DevDownload(b);
+  if (rc) return rc;
+  DevDriver *d = b->dev;
+  const int64_t n = nsteps, cap = d->traceCap;
+  std::vector<int32_t> ti(4 * size_t(n));
+  std::vector<double> te(3 * size_t(n));
+  for (int r = 0; r < 4; ++r)
+    HIP_TRY(b, hipMemcpyAsync(ti.data() + r * n, d->trace + r * cap, sizeof(int32_t) * n,
+                              hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(te.data(), d->trace + 4 * cap, sizeof(double) * 3 * n,
+                            hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(b, hipStreamSynchronize(b->stream));
+  int64_t accepted = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    accepted += ti[2 * n + i];
+    if (kind) kind[i] = ti[i];
+    if (mol) mol[i] = ti[n + i];
+    if (acc) acc[i] = ti[2 * n + i];
+    if (ovl) ovl[i] = ti[3 * n + i];
+    if (dlj) dlj[i] = te[3 * i];
+    if (dreal) dreal[i] = te[3 * i + 1];
+    if (drecip) drecip[i] = te[3 * i + 2];
+  }
+  return accepted;
+}
+
+int gomc_run_moves_device_multi(gomc_box *const *boxes, int n, const int64_t *start_step,
+                                const int64_t *nsteps, const int32_t *cycles,
+                                int64_t *moves_done, int64_t *accepted, double *count_sum,
+                                int64_t *count_samples) {
+  if (!boxes || n <= 0 || !start_step || !nsteps) return GOMC_ERR_ARG;
+  gomc_box *lead = boxes[0];
+  if (!lead) return GOMC_ERR_ARG;
+  (void)hipSetDevice(lead->device);
+  std::vector<gomc_box *> list(boxes, boxes + n);
+  for (int i = 0; i < n; ++i) {
+    gomc_box *b = list[i];
+    if (!b || b->device != lead->device || nsteps[i] < 0) return GOMC_ERR_ARG;
+    if (b->precision != lead->precision)
+      return Fail(lead, GOMC_ERR_ARG, "boxes of one device launch must share a precision");
+    int rc = DevCheck(b);
+    if (!rc) rc = FlushPending(b);
+    if (!rc) rc = DevEnsure(b, 0);
+    if (!rc) rc = DevUpload(b, start_step[i], nsteps[i], cycles ? cycles[i] : 0, false);
+    if (rc) return rc;
+  }
+  int rc = Launch(lead, list);
+  if (rc) return rc;
+  int first = GOMC_OK;
+  for (int i = 0; i < n; ++i) {
+    gomc_box *b = list[i];
+    rc = DevDownload(b);
+    if (rc && first == GOMC_OK) first = rc;
+    const DevState &s = b->dev->hstate;  // counters start at zero in each launch
+    if (moves_done) moves_done[i] = s.moves;
+    if (accepted) accepted[i] = s.accepted;
+    if (count_sum) {
+      count_sum[2 * i] = s.sumN[0];
+      count_sum[2 * i + 1] = s.sumN[1];
+    }
+    if (count_samples) count_samples[i] = s.samples;
+  }
+  return first;
+}
+
+}  // extern "C"

@@ -39,6 +39,16 @@ class MTRand {
   }
   double Closed() { return double(Next()) * (1.0 / 4294967295.0); }    // rand()
   double HalfOpen() { return double(Next()) * (1.0 / 4294967296.0); }  // randExc()
+  // The generator's state (624 words and the read position), so that the
+  // same stream can continue elsewhere (the device move loop).
+  void GetState(uint32_t *words, int *index) const {
+    for (int i = 0; i < kN; ++i) words[i] = mt_[i];
+    *index = idx_;
+  }
+  void SetState(const uint32_t *words, int index) {
+    for (int i = 0; i < kN; ++i) mt_[i] = words[i];
+    idx_ = index;
+  }
 
  private:
   static constexpr int kN = 624, kM = 397;

@@ -227,6 +227,34 @@ int64_t gomc_run_moves(gomc_box *b, int64_t start_step, int64_t nsteps,
                        int32_t *kind, int32_t *mol, int32_t *acc,
                        int32_t *ovl, double *dlj, double *dreal,
                        double *drecip);
+/* The same move loop run on the device: one launch runs `nsteps` moves
+ * (Translate, Rotate and, in GCMC, MoleculeTransfer with its CBMC growth,
+ * src/System.cpp:262-302, src/moves/MoleculeTransfer.h:85-246,
+ * src/cbmc/DCSingle.cpp:23-100, src/cbmc/DCFreeHedron.cpp:48-240) with the
+ * box's MT19937 stream continued on the device, so the moves, their order
+ * and their outcomes are those of gomc_run_moves; records and return value
+ * as there.  Boxes with MultiParticle moves, the molecule cache, more than
+ * 4 kinds, molecules of more than 3 atoms or more than 32 CBMC trials per
+ * stage are refused (GOMC_ERR_STATE / GOMC_ERR_ARG).  No counterpart in the
+ * reference, which runs the move loop on the host. */
+int64_t gomc_run_moves_device(gomc_box *b, int64_t start_step, int64_t nsteps,
+                              int32_t *kind, int32_t *mol, int32_t *acc,
+                              int32_t *ovl, double *dlj, double *dreal,
+                              double *drecip);
+/* One launch of the device move loop over `n` boxes of one device and one
+ * precision (a workgroup per box).  Box i runs from start_step[i]:
+ * nsteps[i] moves, or with cycles (may be NULL) up to cycles[i] cycles of
+ * N_box moves each (N_box counted at the cycle's start, at least 1, never
+ * more than nsteps[i] moves in all), adding its kind-0 and kind-1 molecule
+ * counts to count_sum[2i], count_sum[2i+1] after each cycle
+ * (count_samples[i] cycles).  moves_done, accepted, count_sum and
+ * count_samples may be NULL.  Returns 0 or the first box's error code
+ * (gomc_last_error of that box).  No counterpart in the reference. */
+int gomc_run_moves_device_multi(gomc_box *const *boxes, int n,
+                                const int64_t *start_step,
+                                const int64_t *nsteps, const int32_t *cycles,
+                                int64_t *moves_done, int64_t *accepted,
+                                double *count_sum, int64_t *count_samples);
 /* GCMC: box of every molecule (0 simulation box, 1 reservoir) into `box`
  * (n_mols); returns the number of molecules in box 0 or an error code.
  * MoleculeLookup::NumInBox / IsMoleculeInBox (src/MoleculeLookup.h:98-130). */
