@@ -58,7 +58,7 @@ SIGNATURES = {
     "gomc_run_moves_device": (C.c_int64, [_vp, C.c_int64, C.c_int64, _ip, _ip, _ip, _ip, _dp, _dp,
                                           _dp]),
     "gomc_run_moves_device_multi": (C.c_int, [C.POINTER(_vp), C.c_int, _lp, _lp, _ip, _lp, _lp, _dp,
-                                              _lp]),
+                                              _lp, _ip]),
     "gomc_energies": (C.c_int, [_vp, _dp]),
     "gomc_mol_boxes": (C.c_int, [_vp, _ip]),
     "gomc_get_scales": (C.c_int, [_vp, _dp, _dp]),

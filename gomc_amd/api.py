@@ -240,11 +240,14 @@ def run_device(boxes, nsteps, start_steps=None, cycles=None):
           else np.broadcast_to(np.asarray(cycles, np.int32), (n,)).copy())
     moves, acc, samples = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
     sums = np.zeros((n, 2))
+    status = np.zeros(n, np.int32)
     rc = lib.gomc_run_moves_device_multi(hs, n, ptr(st), ptr(ns), None if cy is None else ptr(cy),
-                                         ptr(moves), ptr(acc), ptr(sums), ptr(samples))
+                                         ptr(moves), ptr(acc), ptr(sums), ptr(samples),
+                                         ptr(status))
     if rc != 0:
-        msgs = [lib.gomc_last_error(b.h).decode() for b in boxes]
-        raise GomcError(f"device move loop failed ({rc}): {[m for m in msgs if m][:1]}")
+        bad = [i for i in range(n) if status[i] != 0] or [0]
+        msg = lib.gomc_last_error(boxes[bad[0]].h).decode()
+        raise GomcError(f"device move loop failed ({rc}) for box {bad[0]}: {msg}")
     return moves, acc, sums, samples
 
 

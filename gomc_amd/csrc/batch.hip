@@ -228,27 +228,6 @@ struct LjReal {
   double lj, real, ov;
 };
 
-// erfc for 0 <= x <= 4 as exp(-x^2) erfcx(x), erfcx from the fitted
-// polynomial of erfcx_poly.hpp (relative error about 3e-15); libm beyond.
-__device__ __forceinline__ double ErfcPoly(double x) {
-  if (!(x <= kErfcxUse)) return erfc(x);
-  const double t = (x - kErfcxMid) * kErfcxInvHalf;
-  double p = kErfcxPoly[kErfcxDeg];
-#pragma unroll
-  for (int k = kErfcxDeg - 1; k >= 0; --k) p = fma(p, t, kErfcxPoly[k]);
-  return exp(-(x * x)) * p;
-}
-
-// 1/sqrt(d2) to double precision: the hardware estimate and two Newton
-// steps y <- y (3/2 - d2 y^2 / 2).
-__device__ __forceinline__ double Rsqrt64(double d2) {
-  const double h = 0.5 * d2;
-  double y = __builtin_amdgcn_rsq(d2);
-  y = y * fma(-h * y, y, 1.5);
-  y = y * fma(-h * y, y, 1.5);
-  return y;
-}
-
 // FP64 pair terms: one reciprocal square root serves the Coulomb distance
 // and 1/r^2 of the Lennard-Jones ratio.
 template <>

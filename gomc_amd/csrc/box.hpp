@@ -226,4 +226,9 @@ int SwapRecipImpl(gomc_box *b, int kind, int insert, const double *x, const doub
 int CacheBuild(gomc_box *b);
 // Frees the device move loop's buffers.
 void DevFree(gomc_box *b);
+// Half-space k-vectors of an orthogonal box in the reference's order
+// (box_setup.cpp); `packed` (may be null) gets their index triples.
+void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double volume,
+                  std::vector<double> &kx, std::vector<double> &ky, std::vector<double> &kz,
+                  std::vector<double> &pref, std::vector<int32_t> *packed);
 }  // namespace gomc

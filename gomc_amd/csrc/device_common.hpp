@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "erfcx_poly.hpp"
+
 namespace gomc {
 
 constexpr double kQQFact = 167103.208067979;  // lib/NumLib.h:22
@@ -182,6 +184,27 @@ __host__ __device__ __forceinline__ U64x4 Philox(uint64_t c0, uint64_t c1, uint6
 // x * 2^-64 + 2^-65, the product exact, one rounding in the add.
 __host__ __device__ __forceinline__ double U01(uint64_t v) {
   return double(v) * 0x1p-64 + 0x1p-65;
+}
+
+// erfc for 0 <= x <= 4 as exp(-x^2) erfcx(x), erfcx from the fitted
+// polynomial of erfcx_poly.hpp (relative error about 3e-15); libm beyond.
+__device__ __forceinline__ double ErfcPoly(double x) {
+  if (!(x <= kErfcxUse)) return erfc(x);
+  const double t = (x - kErfcxMid) * kErfcxInvHalf;
+  double p = kErfcxPoly[kErfcxDeg];
+#pragma unroll
+  for (int k = kErfcxDeg - 1; k >= 0; --k) p = fma(p, t, kErfcxPoly[k]);
+  return exp(-(x * x)) * p;
+}
+
+// 1/sqrt(d2) to double precision: the hardware estimate and two Newton
+// steps y <- y (3/2 - d2 y^2 / 2).
+__device__ __forceinline__ double Rsqrt64(double d2) {
+  const double h = 0.5 * d2;
+  double y = __builtin_amdgcn_rsq(d2);
+  y = y * fma(-h * y, y, 1.5);
+  y = y * fma(-h * y, y, 1.5);
+  return y;
 }
 
 // The pair terms and phases at precision P: 0 is FP64 throughout, 1 the

@@ -37,6 +37,9 @@ constexpr int kDevKindAtoms = 3;  // atoms per molecule (1- and 3-site rigid kin
 constexpr int kDevTrials = 32;    // CBMC trials per growth stage
 constexpr int kDevCombos = 2 * kDevTrials;
 constexpr int kMtN = 624, kMtM = 397;
+constexpr int kDevNMax = 40;             // largest |n| of a k-vector component (FP64 phases)
+constexpr int kTab = 2 * kDevNMax + 1;   // phase-table entries per axis
+constexpr int kDevPos = 2 * kDevKindAtoms;  // positions of one trial: new, then old
 constexpr int kDisplace = 0, kRotate = 1, kMolTransfer = 10;
 constexpr double kSmallWeight = 1.0e-38;  // lib/NumLib.h:19
 
@@ -60,8 +63,12 @@ struct DevBox {
   const double *sigSq, *epsCn, *nHalf;
   const double *kx, *ky, *kz, *pref;
   double *sumR, *sumI, *newR, *newI;
+  const int *kpack;  // k-vector index triples: ix | (iy + 128) << 8 | (iz + 128) << 16
+  int nmax[3];
   const int *molStart, *molKind;
   int *molBox;
+  int *live, *livePos;  // atoms of the box's molecules (any order), each atom's place or -1
+  int nLive0;
   uint32_t *lookup;
   double *com;  // [3][nmols]
   struct DevState *st;
@@ -110,6 +117,8 @@ struct Ctl {
   int ct[kDevCombos];
   double cE[kDevCombos], cR[kDevCombos], cO[kDevCombos];
   double w[kDevTrials], sI[kDevTrials], sR[kDevTrials];  // stage weights and sums
+  double u[3][kDevTrials], hc[3], hb[3], hp[3];          // hedron spins: draws, frame
+  int nLive;                                             // atoms in the box (live list)
   DTrial nm, om;
   int64_t done, cycleLeft;
   int cyclesDone;
@@ -120,6 +129,15 @@ struct Ctl {
 enum : int { kOpStop = 0, kOpNone = 1, kOpMol = 2, kOpTransfer = 3 };
 
 #define NOINLINE __attribute__((noinline))
+
+// Global-memory views of the box arrays, so the hot loops use global (not
+// flat) loads and stores.
+typedef const double __attribute__((address_space(1))) GD;
+typedef double __attribute__((address_space(1))) GDW;
+typedef const int __attribute__((address_space(1))) GI;
+__device__ __forceinline__ GD *G(const double *p) { return (GD *)p; }
+__device__ __forceinline__ GI *G(const int *p) { return (GI *)p; }
+__device__ __forceinline__ GDW *GW(double *p) { return (GDW *)p; }
 
 // ---------------------------------------------------------------- MT19937
 // The generator of host_rng.hpp over the state in LDS (thread 0 only).
@@ -134,7 +152,7 @@ NOINLINE __device__ void MtTwist(DevState &s) {
   }
   s.mtIdx = 0;
 }
-__device__ uint32_t MtNext(DevState &s) {
+__device__ __forceinline__ uint32_t MtNext(DevState &s) {
   if (s.mtIdx == kMtN) MtTwist(s);
   uint32_t v = s.mt[s.mtIdx++];
   v ^= v >> 11;
@@ -142,7 +160,7 @@ __device__ uint32_t MtNext(DevState &s) {
   v ^= (v << 15) & 0xefc60000u;
   return v ^ (v >> 18);
 }
-NOINLINE __device__ uint32_t MtBelow(DevState &s, uint32_t n) {
+__device__ __forceinline__ uint32_t MtBelow(DevState &s, uint32_t n) {
   uint32_t mask = n;
   for (int sh = 1; sh < 32; sh <<= 1) mask |= mask >> sh;
   uint32_t v;
@@ -151,10 +169,10 @@ NOINLINE __device__ uint32_t MtBelow(DevState &s, uint32_t n) {
   } while (v > n);
   return v;
 }
-NOINLINE __device__ double MtClosed(DevState &s) {
+__device__ __forceinline__ double MtClosed(DevState &s) {
   return double(MtNext(s)) * (1.0 / 4294967295.0);
 }
-NOINLINE __device__ double MtHalfOpen(DevState &s) {
+__device__ __forceinline__ double MtHalfOpen(DevState &s) {
   return double(MtNext(s)) * (1.0 / 4294967296.0);
 }
 
@@ -246,36 +264,181 @@ __device__ void BlockSum4(Ctl &c, double a, double b, double d, double e) {
 }
 
 // ------------------------------------------------- workgroup energy phases --
+__attribute__((noinline)) __device__ double ErfcOutOfLine(double x) { return erfc(x); }
+
+// The erfcx polynomial's coefficients, copied to LDS at the kernel's start:
+// read from there in the loop, they do not occupy registers.
+__shared__ double sErfcx[kErfcxDeg + 1];
+
+// ErfcPoly (device_common.hpp) with the coefficients from LDS and libm's
+// erfc out of line for x > 4.
+__device__ __forceinline__ double ErfcLds(double x) {
+  if (!(x <= kErfcxUse)) return ErfcOutOfLine(x);
+  const double t = (x - kErfcxMid) * kErfcxInvHalf;
+  double p = sErfcx[kErfcxDeg];
+#pragma unroll
+  for (int k = kErfcxDeg - 1; k >= 0; --k) p = fma(p, t, sErfcx[k]);
+  return exp(-(x * x)) * p;
+}
+
+// erfc-Coulomb at precision P; FP64 through the erfcx polynomial.
+template <int P>
+__device__ __forceinline__ double CoulDev(const BoxGeom &g, double d2, double qiqj) {
+  if (P) return CoulombEnergyF32(g, d2, qiqj);
+  const double dist = sqrt(d2);
+  if (g.ewald) return qiqj * ErfcLds(g.alpha * dist) / dist;
+  return qiqj / dist;
+}
+
+// The Mie repulsion for exponents other than 12, out of line so that its
+// pow does not weigh on the registers of the pair loops.
+__attribute__((noinline)) __device__ double PowOutOfLine(double x, double y) { return pow(x, y); }
+
+// LJEnergy / LJEnergyF32 (device_common.hpp) with that out-of-line pow.
+template <int P>
+__device__ __forceinline__ double LJDev(double d2, double sigmaSq, double epsCn, double nHalf) {
+  if (P) return LJEnergyF32(d2, sigmaSq, epsCn, nHalf);
+  const double rRat2 = sigmaSq / d2;
+  const double rRat4 = rRat2 * rRat2;
+  const double attract = rRat4 * rRat2;
+  const double repulse = (nHalf == 6.0) ? attract * attract : PowOutOfLine(rRat2, nHalf);
+  return epsCn * (repulse - attract);
+}
+
+// FP64 phases e^{i k.r} of a trial's positions as products of per-axis
+// tables e^{i (2 pi n / L_a) r_a}, |n| <= nmax_a: `np` positions given by
+// (px, py, pz) in LDS, tab[p][axis][n + kDevNMax] = (cos, sin).
+// With P = 1 the same LDS holds FP32 entries (float2, same indexing) and
+// the products are FP32 (the mixed mode's phases).
+typedef double2 PhaseTab[kDevPos][3][kTab];
+template <int P>
+__device__ void BuildPhases(const DevBox &B, const double *px, const double *py,
+                            const double *pz, int np, PhaseTab &tab) {
+  for (int e = threadIdx.x; e < np * 3 * kTab; e += kDevThreads) {
+    const int p = e / (3 * kTab), a = (e / kTab) % 3, n = e % kTab - kDevNMax;
+    if (n < -B.nmax[a] || n > B.nmax[a]) continue;
+    const double r = a == 0 ? px[p] : (a == 1 ? py[p] : pz[p]);
+    const double ca = (1.0 / B.g.L[a]) * (2.0 * M_PI);
+    double sn, cs;
+    sincos((ca * n) * r, &sn, &cs);
+    if (P)
+      reinterpret_cast<float2 *>(&tab[0][0][0])[e] = make_float2(float(cs), float(sn));
+    else
+      tab[p][a][n + kDevNMax] = make_double2(cs, sn);
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double2 CMulD(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 CMulF(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// e^{i k.r} of table position p for the k-vector with index triple (ix, iy, iz).
+template <int P>
+__device__ __forceinline__ double2 Phase(const PhaseTab &tab, int p, int ix, int iy, int iz) {
+  if (P) {
+    const float2 *t = reinterpret_cast<const float2 *>(&tab[0][0][0]) + p * 3 * kTab;
+    const float2 e = CMulF(CMulF(t[ix + kDevNMax], t[kTab + iy + kDevNMax]),
+                           t[2 * kTab + iz + kDevNMax]);
+    return make_double2(e.x, e.y);
+  }
+  return CMulD(CMulD(tab[p][0][ix + kDevNMax], tab[p][1][iy + kDevNMax]),
+               tab[p][2][iz + kDevNMax]);
+}
+// In-cutoff pairs found by a wave wait in its queue in LDS until 64 are
+// there; the wave then evaluates 64 pair terms at once, one per lane, so
+// the erfc path runs with full waves however sparse the cutoff sphere is.
+constexpr int kQCap = 2 * kWave;
+struct WaveQ {
+  double d2[kQCap], qq[kQCap];
+  int code[kQCap];  // LJ table index * 2 + (1 if the term is added, 0 if subtracted)
+};
+__shared__ WaveQ sQ[kDevWaves];
+
+template <int P>
+__device__ __forceinline__ void DrainQ(const BoxGeom &g, WaveQ &q, int n, int lane,
+                                       const GD *sigSq, const GD *epsCn, const GD *nHalf,
+                                       double &lj, double &real) {
+  if (lane < n) {
+    const double d2 = q.d2[lane], qq = q.qq[lane];
+    const int code = q.code[lane], idx = code >> 1;
+    const double sign = (code & 1) ? 1.0 : -1.0;
+    if (g.elect && qq != 0.0 && !(g.rCutCoulSq < d2)) real += sign * CoulDev<P>(g, d2, qq);
+    if (!(g.rCutLJSq < d2)) lj += sign * LJDev<P>(d2, sigSq[idx], epsCn[idx], nHalf[idx]);
+  }
+}
+
+// Appends this lane's pair (when `in`) to the wave's queue; drains 64 when
+// the queue holds that many.  Called by every lane of the wave.
+template <int P>
+__device__ __forceinline__ void PushQ(const BoxGeom &g, WaveQ &q, int &cnt, int lane, bool in,
+                                      double d2, double qq, int code, const GD *sigSq,
+                                      const GD *epsCn, const GD *nHalf, double &lj,
+                                      double &real) {
+  const uint64_t m = __ballot(in);
+  if (in) {
+    const int pos = cnt + __popcll(m & ((uint64_t(1) << lane) - 1));
+    q.d2[pos] = d2;
+    q.qq[pos] = qq;
+    q.code[pos] = code;
+  }
+  cnt += __popcll(m);
+  __builtin_amdgcn_wave_barrier();
+  if (cnt >= kWave) {
+    DrainQ<P>(g, q, kWave, lane, sigSq, epsCn, nHalf, lj, real);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < cnt - kWave) {
+      q.d2[lane] = q.d2[kWave + lane];
+      q.qq[lane] = q.qq[kWave + lane];
+      q.code[lane] = q.code[kWave + lane];
+    }
+    cnt -= kWave;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // ParticleInter (src/CalculateEnergy.cpp:727-785) for the stage's nc
 // (position, atom) combinations: one wave per combination, the lanes over
 // the atoms, then the wave's sum.
 template <int P>
 NOINLINE __device__ void ComboEnergies(const DevBox &B, Ctl &c) {
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-  const int nc = c.nc, skip = c.skip, natoms = B.natoms, ntypes = B.ntypes;
+  // live atoms only; the atoms of molecule `skip` (c.st, c.len) are left out
+  const int nc = c.nc, nLive = c.nLive, ntypes = B.ntypes;
+  const int skipSt = c.skip >= 0 ? c.st : 0, skipEnd = c.skip >= 0 ? c.st + c.len : 0;
+  const GI *live = G(B.live);
   const BoxGeom g = B.g;
-  const double *x = B.x, *y = B.y, *z = B.z, *q = B.q;
-  const int *type = B.type, *mol = B.mol;
+  const GD *x = G(B.x), *y = G(B.y), *z = G(B.z), *q = G(B.q);
+  const GI *type = G(B.type);
+  const GD *sigSq = G(B.sigSq), *epsCn = G(B.epsCn), *nHalf = G(B.nHalf);
   for (int ci = wave; ci < nc; ci += kDevWaves) {
     const double px = c.cx[ci], py = c.cy[ci], pz = c.cz[ci], charge = c.cq[ci];
     const int atype = c.ct[ci];
     double lj = 0.0, real = 0.0, ov = 0.0;
-    for (int j = lane; j < natoms; j += kWave) {
-      const int mj = mol[j];
-      if (mj == skip || mj < 0) continue;
-      const double d2 = DistSq(g, px, py, pz, x[j], y[j], z[j]);
-      if (g.rCutSq > d2) {
-        if (d2 < g.rCutLowSq) ov = 1.0;
-        if (!(g.rCutLJSq < d2)) {
-          const int idx = atype + type[j] * ntypes;
-          lj += LJ<P>(d2, B.sigSq[idx], B.epsCn[idx], B.nHalf[idx]);
-        }
-        if (g.elect) {
-          const double qiqj = q[j] * charge * kQQFact;
-          if (qiqj != 0.0 && !(g.rCutCoulSq < d2)) real += Coul<P>(g, d2, qiqj);
+    WaveQ &wq = sQ[wave];
+    int cnt = 0;
+    for (int base = 0; base < nLive; base += kWave) {
+      const int li = base + lane;
+      bool in = false;
+      double d2 = 0.0, qiqj = 0.0;
+      int code = 0;
+      if (li < nLive) {
+        const int j = live[li];
+        if (!(j >= skipSt && j < skipEnd)) {
+          d2 = DistSq(g, px, py, pz, x[j], y[j], z[j]);
+          in = g.rCutSq > d2;
+          if (in && d2 < g.rCutLowSq) ov = 1.0;
+          qiqj = q[j] * charge * kQQFact;
+          code = (atype + type[j] * ntypes) * 2 + 1;
         }
       }
+      PushQ<P>(g, wq, cnt, lane, in, d2, qiqj, code, sigSq, epsCn, nHalf, lj, real);
     }
+    DrainQ<P>(g, wq, cnt, lane, sigSq, epsCn, nHalf, lj, real);
+    __builtin_amdgcn_wave_barrier();
     lj = WaveSum(lj);
     real = WaveSum(real);
     ov = WaveSum(ov);
@@ -292,105 +455,133 @@ NOINLINE __device__ void ComboEnergies(const DevBox &B, Ctl &c) {
 // new positions (CalculateEnergy::MoleculeInter, CalculateEnergy.cpp:581-686)
 // and every k-vector's delta (Ewald::MolReciprocal, Ewald.cpp:409-473).
 template <int P>
-NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c) {
+NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c, PhaseTab &tab) {
   double lj = 0.0, real = 0.0, ov = 0.0, e = 0.0;
-  const int st = c.st, len = c.len, natoms = B.natoms, ntypes = B.ntypes;
+  const int st = c.st, len = c.len, ntypes = B.ntypes;
   const BoxGeom g = B.g;
-  double ox[kDevKindAtoms], oy[kDevKindAtoms], oz[kDevKindAtoms], nx[kDevKindAtoms],
-      ny[kDevKindAtoms], nz[kDevKindAtoms], mq[kDevKindAtoms];
-  int mt[kDevKindAtoms];
-  for (int p = 0; p < kDevKindAtoms; ++p) {
-    ox[p] = c.ox[p];
-    oy[p] = c.oy[p];
-    oz[p] = c.oz[p];
-    nx[p] = c.nx[p];
-    ny[p] = c.ny[p];
-    nz[p] = c.nz[p];
-    mq[p] = c.mq[p];
-    mt[p] = c.mt_[p];
-  }
-  for (int j = threadIdx.x; j < natoms; j += kDevThreads) {
-    if ((j >= st && j < st + len) || B.mol[j] < 0) continue;
-    const double xj = B.x[j], yj = B.y[j], zj = B.z[j], qj = B.q[j];
-    const int tj = B.type[j];
-#pragma unroll
-    for (int p = 0; p < kDevKindAtoms; ++p) {
-      if (p >= len) break;
-      const int idx = mt[p] + tj * ntypes;
-      const double qiqj = mq[p] * qj * kQQFact;
-      double d2 = DistSq(g, ox[p], oy[p], oz[p], xj, yj, zj);
-      if (g.rCutSq > d2) {
-        if (g.elect && qiqj != 0.0 && !(g.rCutCoulSq < d2)) real += -Coul<P>(g, d2, qiqj);
-        if (!(g.rCutLJSq < d2)) lj += -LJ<P>(d2, B.sigSq[idx], B.epsCn[idx], B.nHalf[idx]);
+  const GD *x = G(B.x), *y = G(B.y), *z = G(B.z), *q = G(B.q);
+  const GI *type = G(B.type);
+  const GD *sigSq = G(B.sigSq), *epsCn = G(B.epsCn), *nHalf = G(B.nHalf);
+  // (atom p, old) then (atom p, new) for each p, the molecule's data read
+  // from LDS; in-cutoff pairs go through the wave's queue
+  {
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    WaveQ &wq = sQ[wave];
+    int cnt = 0;
+    const int nLive = c.nLive;
+    const GI *live = G(B.live);
+    for (int base = wave * kWave; base < nLive; base += kDevThreads) {
+      const int li = base + lane;
+      const int j = li < nLive ? live[li] : 0;
+      const bool valid = li < nLive && !(j >= st && j < st + len);
+      double xj = 0.0, yj = 0.0, zj = 0.0, qj = 0.0;
+      int tj = 0;
+      if (valid) {
+        xj = x[j];
+        yj = y[j];
+        zj = z[j];
+        qj = q[j];
+        tj = type[j];
       }
-      d2 = DistSq(g, nx[p], ny[p], nz[p], xj, yj, zj);
-      if (g.rCutSq > d2) {
-        if (d2 < g.rCutLowSq) ov = 1.0;
-        if (g.elect && qiqj != 0.0 && !(g.rCutCoulSq < d2)) real += Coul<P>(g, d2, qiqj);
-        if (!(g.rCutLJSq < d2)) lj += LJ<P>(d2, B.sigSq[idx], B.epsCn[idx], B.nHalf[idx]);
+      for (int sp = 0; sp < 2 * len; ++sp) {
+        const int p = sp >> 1;
+        const bool isNew = sp & 1;
+        double d2 = 0.0;
+        bool in = false;
+        if (valid) {
+          d2 = isNew ? DistSq(g, c.nx[p], c.ny[p], c.nz[p], xj, yj, zj)
+                     : DistSq(g, c.ox[p], c.oy[p], c.oz[p], xj, yj, zj);
+          in = g.rCutSq > d2;
+          if (isNew && in && d2 < g.rCutLowSq) ov = 1.0;
+        }
+        PushQ<P>(g, wq, cnt, lane, in, d2, c.mq[p] * qj * kQQFact,
+                 (c.mt_[p] + tj * ntypes) * 2 + (isNew ? 1 : 0), sigSq, epsCn, nHalf, lj, real);
       }
     }
+    DrainQ<P>(g, wq, cnt, lane, sigSq, epsCn, nHalf, lj, real);
+    __builtin_amdgcn_wave_barrier();
   }
-  if (c.doRecip)
+  if (c.doRecip) {
+    const GD *pref = G(B.pref);
+    const GD *sumR = G(B.sumR), *sumI = G(B.sumI);
+    const GI *kpack = G(B.kpack);
+    GDW *newR = GW(B.newR), *newI = GW(B.newI);
+    {  // new positions are table rows [0, len), old ones [kDevKindAtoms, +len)
+      __shared__ double tpx[kDevPos], tpy[kDevPos], tpz[kDevPos];
+      if (threadIdx.x < kDevKindAtoms) {
+        const int p = threadIdx.x;
+        tpx[p] = c.nx[p];
+        tpy[p] = c.ny[p];
+        tpz[p] = c.nz[p];
+        tpx[kDevKindAtoms + p] = c.ox[p];
+        tpy[kDevKindAtoms + p] = c.oy[p];
+        tpz[kDevKindAtoms + p] = c.oz[p];
+      }
+      __syncthreads();
+      BuildPhases<P>(B, tpx, tpy, tpz, kDevPos, tab);
+    }
     for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
-      const double kxk = B.kx[k], kyk = B.ky[k], kzk = B.kz[k];
       double sRN = 0.0, sIN = 0.0, sRO = 0.0, sIO = 0.0;
+      const int code = kpack[k];
+      const int ix = code & 0xff, iy = ((code >> 8) & 0xff) - 128, iz = ((code >> 16) & 0xff) - 128;
+#pragma unroll 1
       for (int p = 0; p < len; ++p) {
-        if (fabs(mq[p]) < 0.000000001) continue;
-        const double dNew = nx[p] * kxk + ny[p] * kyk + nz[p] * kzk;
-        const double dOld = ox[p] * kxk + oy[p] * kyk + oz[p] * kzk;
-        double sn, cn, so, co;
-        Sincos<P>(dNew, &sn, &cn);
-        Sincos<P>(dOld, &so, &co);
-        sRN += mq[p] * cn;
-        sIN += mq[p] * sn;
-        sRO += mq[p] * co;
-        sIO += mq[p] * so;
+        const double qp = c.mq[p];
+        if (fabs(qp) < 0.000000001) continue;
+        const double2 en = Phase<P>(tab, p, ix, iy, iz);
+        const double2 eo = Phase<P>(tab, kDevKindAtoms + p, ix, iy, iz);
+        sRN += qp * en.x;
+        sIN += qp * en.y;
+        sRO += qp * eo.x;
+        sIO += qp * eo.y;
       }
-      const double R = B.sumR[k] + (sRN - sRO);
-      const double I = B.sumI[k] + (sIN - sIO);
-      B.newR[k] = R;
-      B.newI[k] = I;
-      e += (R * R + I * I) * B.pref[k];
+      const double R = sumR[k] + (sRN - sRO);
+      const double I = sumI[k] + (sIN - sIO);
+      newR[k] = R;
+      newI[k] = I;
+      e += (R * R + I * I) * pref[k];
     }
+  }
   BlockSum4(c, lj, real, e, ov);
 }
 
 // Ewald::SwapDestRecip / SwapSourceRecip (Ewald.cpp:478-531,657-710).
 template <int P>
-NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, Ctl &c) {
+NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, Ctl &c, PhaseTab &tab) {
   double e = 0.0;
   const int len = c.len, sign = c.sign;
-  double px[kDevKindAtoms], py[kDevKindAtoms], pz[kDevKindAtoms], mq[kDevKindAtoms];
-  for (int p = 0; p < kDevKindAtoms; ++p) {
-    px[p] = c.nx[p];
-    py[p] = c.ny[p];
-    pz[p] = c.nz[p];
-    mq[p] = c.mq[p];
-  }
+  const GD *pref = G(B.pref);
+  const GD *sumR = G(B.sumR), *sumI = G(B.sumI);
+  const GI *kpack = G(B.kpack);
+  GDW *newR = GW(B.newR), *newI = GW(B.newI);
+  BuildPhases<P>(B, c.nx, c.ny, c.nz, kDevKindAtoms, tab);
   for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
     double sR = 0.0, sI = 0.0;
+    const int code = kpack[k];
+    const int ix = code & 0xff, iy = ((code >> 8) & 0xff) - 128, iz = ((code >> 16) & 0xff) - 128;
+#pragma unroll 1
     for (int p = 0; p < len; ++p) {
-      if (fabs(mq[p]) < 0.000000001) continue;
-      const double dot = px[p] * B.kx[k] + py[p] * B.ky[k] + pz[p] * B.kz[k];
-      double sn, cs;
-      Sincos<P>(dot, &sn, &cs);
-      sR += mq[p] * cs;
-      sI += mq[p] * sn;
+      const double qp = c.mq[p];
+      if (fabs(qp) < 0.000000001) continue;
+      const double2 e1 = Phase<P>(tab, p, ix, iy, iz);
+      sR += qp * e1.x;
+      sI += qp * e1.y;
     }
-    const double R = (sign > 0) ? B.sumR[k] + sR : B.sumR[k] - sR;
-    const double I = (sign > 0) ? B.sumI[k] + sI : B.sumI[k] - sI;
-    B.newR[k] = R;
-    B.newI[k] = I;
-    e += (R * R + I * I) * B.pref[k];
+    const double R = (sign > 0) ? sumR[k] + sR : sumR[k] - sR;
+    const double I = (sign > 0) ? sumI[k] + sI : sumI[k] - sI;
+    newR[k] = R;
+    newI[k] = I;
+    e += (R * R + I * I) * pref[k];
   }
   BlockSum4(c, e, 0.0, 0.0, 0.0);
 }
 
 __device__ void CommitRecip(const DevBox &B) {
+  GDW *sumR = GW(B.sumR), *sumI = GW(B.sumI);
+  const GD *newR = G(B.newR), *newI = G(B.newI);
   for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
-    B.sumR[k] = B.newR[k];
-    B.sumI[k] = B.newI[k];
+    sumR[k] = newR[k];
+    sumI[k] = newI[k];
   }
 }
 
@@ -498,15 +689,27 @@ NOINLINE __device__ void SinglePrep(const DevBox &B, DevState &s, Ctl &c, const 
   for (int i = c.nc; i < n; ++i) c.cE[i] = c.cR[i] = c.cO[i] = 0.0;
 }
 
+// Boltzmann weights of a stage's n trials, one trial per thread.
+NOINLINE __device__ void StageWeights(const DevBox &B, Ctl &c, int n, bool hedron) {
+  const int i = threadIdx.x;
+  if (i < n) {
+    if (hedron) {  // atom 2's terms first, then atom 1's, as the host accumulates them
+      c.sI[i] = (0.0 + c.cE[i]) + c.cE[n + i];
+      c.sR[i] = (0.0 + c.cR[i]) + c.cR[n + i];
+      c.w[i] = exp(-B.beta * (c.sI[i] + c.sR[i]));
+    } else {
+      const double inter = 0.0 + c.cE[i], real = 0.0 + c.cR[i];
+      c.w[i] = exp(-1 * B.beta * (inter + real));
+    }
+  }
+  __syncthreads();
+}
+
 NOINLINE __device__ void SingleFinish(const DevBox &B, DevState &s, Ctl &c, DTrial &t, int atom,
                                       bool isNew) {
   const int n = B.cbmcFirst;
   double stepWeight = 0.0;
-  for (int i = 0; i < n; ++i) {
-    const double inter = 0.0 + c.cE[i], real = 0.0 + c.cR[i];
-    c.w[i] = exp(-1 * B.beta * (inter + real));
-    stepWeight += c.w[i];
-  }
+  for (int i = 0; i < n; ++i) stepWeight += c.w[i];
   int win = 0;
   if (isNew) {
     win = PickWeightedD(s, c.w, n, stepWeight);
@@ -531,6 +734,7 @@ __device__ void SingleStage(const DevBox &B, DevState &s, Ctl &c, DTrial &t, int
   if (threadIdx.x == 0) SinglePrep(B, s, c, t, atom, isNew);
   __syncthreads();
   ComboEnergies<P>(B, c);
+  StageWeights(B, c, B.cbmcFirst, false);
   if (threadIdx.x == 0) SingleFinish(B, s, c, t, atom, isNew);
   __syncthreads();
 }
@@ -546,11 +750,15 @@ NOINLINE __device__ void HedronTheta(const DevBox &B, DevState &s, Ctl &c, bool 
   if (isNew) (void)MtClosed(s);  // DCHedron::FreeAnglesNew's PickWeighted, equal weights
 }
 
-NOINLINE __device__ void HedronPrep(const DevBox &B, DevState &s, Ctl &c, const DTrial &t,
+// Thread 0: the frame to spin (the bonded and previous atoms relative to
+// atom 0) and the spins' draws, in the host's order (trial n-1 first).
+NOINLINE __device__ void HedronDraw(const DevBox &B, DevState &s, Ctl &c, const DTrial &t,
                                     bool isNew) {
   const int n = B.cbmcNth, kind = c.kind;
   const double cx = t.px[0], cy = t.py[0], cz = t.pz[0];
-  // spun frames go to cx/cy/cz [0, n) (atom 2) and [n, 2n) (atom 1)
+  c.hc[0] = cx;
+  c.hc[1] = cy;
+  c.hc[2] = cz;
   if (isNew) {
     const double th = B.hedTh[kind];
     const double sinTh = sin(th);
@@ -562,70 +770,69 @@ NOINLINE __device__ void HedronPrep(const DevBox &B, DevState &s, Ctl &c, const 
     ppx *= sin(0.0) * cos(0.0);
     ppy *= sin(0.0) * sin(0.0);
     ppz *= cos(0.0);
-    for (int lj = n; lj-- > 0;) {
-      const double u1 = MtClosed(s), u2 = MtClosed(s), u3 = MtClosed(s);
-      const DM3 spin = UniformRotationD(u1, u2, u3);
-      double ax, ay, az;
-      Apply(spin, pbx, pby, pbz, ax, ay, az);
-      c.cx[lj] = ax + cx;
-      c.cy[lj] = ay + cy;
-      c.cz[lj] = az + cz;
-      Apply(spin, ppx, ppy, ppz, ax, ay, az);
-      c.cx[n + lj] = ax + cx;
-      c.cy[n + lj] = ay + cy;
-      c.cz[n + lj] = az + cz;
-    }
+    c.hb[0] = pbx;
+    c.hb[1] = pby;
+    c.hb[2] = pbz;
+    c.hp[0] = ppx;
+    c.hp[1] = ppy;
+    c.hp[2] = ppz;
   } else {
     c.thetaW += exp(-1 * B.beta * (0.0 + 0.0));  // DCHedron::IncorporateOld
-    const double bx = Unwrap(t.px[2], cx, B.g.L[0], B.g.half[0]) + (-cx);
-    const double by = Unwrap(t.py[2], cy, B.g.L[1], B.g.half[1]) + (-cy);
-    const double bz = Unwrap(t.pz[2], cz, B.g.L[2], B.g.half[2]) + (-cz);
-    const double qx = Unwrap(t.px[1], cx, B.g.L[0], B.g.half[0]) + (-cx);
-    const double qy = Unwrap(t.py[1], cy, B.g.L[1], B.g.half[1]) + (-cy);
-    const double qz = Unwrap(t.pz[1], cz, B.g.L[2], B.g.half[2]) + (-cz);
-    for (int lj = n; lj-- > 1;) {
-      const double u1 = MtClosed(s), u2 = MtClosed(s), u3 = MtClosed(s);
-      const DM3 spin = UniformRotationD(u1, u2, u3);
-      double ax, ay, az;
-      Apply(spin, bx, by, bz, ax, ay, az);
-      c.cx[lj] = ax + cx;
-      c.cy[lj] = ay + cy;
-      c.cz[lj] = az + cz;
-      Apply(spin, qx, qy, qz, ax, ay, az);
-      c.cx[n + lj] = ax + cx;
-      c.cy[n + lj] = ay + cy;
-      c.cz[n + lj] = az + cz;
-    }
-    c.cx[0] = bx + cx;
-    c.cy[0] = by + cy;
-    c.cz[0] = bz + cz;
-    c.cx[n] = qx + cx;
-    c.cy[n] = qy + cy;
-    c.cz[n] = qz + cz;
+    c.hb[0] = Unwrap(t.px[2], cx, B.g.L[0], B.g.half[0]) + (-cx);
+    c.hb[1] = Unwrap(t.py[2], cy, B.g.L[1], B.g.half[1]) + (-cy);
+    c.hb[2] = Unwrap(t.pz[2], cz, B.g.L[2], B.g.half[2]) + (-cz);
+    c.hp[0] = Unwrap(t.px[1], cx, B.g.L[0], B.g.half[0]) + (-cx);
+    c.hp[1] = Unwrap(t.py[1], cy, B.g.L[1], B.g.half[1]) + (-cy);
+    c.hp[2] = Unwrap(t.pz[1], cz, B.g.L[2], B.g.half[2]) + (-cz);
   }
-  for (int i = 0; i < 2 * n; ++i) {
-    c.cx[i] = Wrap(c.cx[i], B.g.L[0]);
-    c.cy[i] = Wrap(c.cy[i], B.g.L[1]);
-    c.cz[i] = Wrap(c.cz[i], B.g.L[2]);
-    const int atom = i < n ? 2 : 1;
-    c.ct[i] = B.kindT[kind][atom];
-    c.cq[i] = B.kindQ[kind][atom];
+  const int lo = isNew ? 0 : 1;
+  for (int lj = n; lj-- > lo;) {
+    c.u[0][lj] = MtClosed(s);
+    c.u[1][lj] = MtClosed(s);
+    c.u[2][lj] = MtClosed(s);
   }
   c.nc = t.box == 0 ? 2 * n : 0;
-  for (int i = c.nc; i < 2 * n; ++i) c.cE[i] = c.cR[i] = c.cO[i] = 0.0;
+}
+
+// Trial i (one per thread): spin the frame, place atoms 2 and 1 at
+// c[i] and c[n + i]; the old build's trial 0 is the actual frame.
+NOINLINE __device__ void HedronSpin(const DevBox &B, Ctl &c, bool isNew) {
+  const int n = B.cbmcNth, kind = c.kind, i = threadIdx.x;
+  if (i < n) {
+    const double cx = c.hc[0], cy = c.hc[1], cz = c.hc[2];
+    double bx = c.hb[0], by = c.hb[1], bz = c.hb[2], qx = c.hp[0], qy = c.hp[1], qz = c.hp[2];
+    if (isNew || i > 0) {
+      const DM3 spin = UniformRotationD(c.u[0][i], c.u[1][i], c.u[2][i]);
+      double ax, ay, az;
+      Apply(spin, bx, by, bz, ax, ay, az);
+      bx = ax;
+      by = ay;
+      bz = az;
+      Apply(spin, qx, qy, qz, ax, ay, az);
+      qx = ax;
+      qy = ay;
+      qz = az;
+    }
+    c.cx[i] = Wrap(bx + cx, B.g.L[0]);
+    c.cy[i] = Wrap(by + cy, B.g.L[1]);
+    c.cz[i] = Wrap(bz + cz, B.g.L[2]);
+    c.ct[i] = B.kindT[kind][2];
+    c.cq[i] = B.kindQ[kind][2];
+    c.cx[n + i] = Wrap(qx + cx, B.g.L[0]);
+    c.cy[n + i] = Wrap(qy + cy, B.g.L[1]);
+    c.cz[n + i] = Wrap(qz + cz, B.g.L[2]);
+    c.ct[n + i] = B.kindT[kind][1];
+    c.cq[n + i] = B.kindQ[kind][1];
+    if (c.nc == 0) c.cE[i] = c.cR[i] = c.cO[i] = c.cE[n + i] = c.cR[n + i] = c.cO[n + i] = 0.0;
+  }
+  __syncthreads();
 }
 
 NOINLINE __device__ void HedronFinish(const DevBox &B, DevState &s, Ctl &c, DTrial &t,
                                       bool isNew) {
-  // atom 2's terms first, then atom 1's, as the host accumulates them
   const int n = B.cbmcNth;
   double stepWeight = 0.0;
-  for (int i = 0; i < n; ++i) {
-    c.sI[i] = (0.0 + c.cE[i]) + c.cE[n + i];
-    c.sR[i] = (0.0 + c.cR[i]) + c.cR[n + i];
-    c.w[i] = exp(-B.beta * (c.sI[i] + c.sR[i]));
-    stepWeight += c.w[i];
-  }
+  for (int i = 0; i < n; ++i) stepWeight += c.w[i];
   int win = 0;
   if (isNew) {
     win = PickWeightedD(s, c.w, n, stepWeight);
@@ -653,9 +860,11 @@ __device__ void HedronStage(const DevBox &B, DevState &s, Ctl &c, DTrial &t, boo
   if (threadIdx.x == 0) HedronTheta(B, s, c, isNew);
   SingleStage<P>(B, s, c, t, 0, isNew);
   if (c.abort) return;
-  if (threadIdx.x == 0) HedronPrep(B, s, c, t, isNew);
+  if (threadIdx.x == 0) HedronDraw(B, s, c, t, isNew);
   __syncthreads();
+  HedronSpin(B, c, isNew);
   ComboEnergies<P>(B, c);
+  StageWeights(B, c, B.cbmcNth, true);
   if (threadIdx.x == 0) HedronFinish(B, s, c, t, isNew);
   __syncthreads();
 }
@@ -861,6 +1070,27 @@ NOINLINE __device__ void TransferFinish(const DevBox &B, DevState &s, Ctl &c, in
       s.kindCount[kind]++;
     else
       s.kindCount[kind]--;
+    const int st = c.st;
+    for (int p = 0; p < len; ++p) {  // the live-atom list
+      const int a = st + p;
+      if (dst == 0) {
+        B.live[c.nLive] = a;
+        B.livePos[a] = c.nLive;
+        c.nLive++;
+      } else {
+        const int pos = B.livePos[a];
+        if (pos < 0 || c.nLive <= 0) {  // list out of step with the molecule: stop
+          s.status = kStErr;
+          c.abort = 1;
+          break;
+        }
+        const int last = B.live[c.nLive - 1];
+        B.live[pos] = last;
+        B.livePos[last] = pos;
+        B.livePos[a] = -1;
+        c.nLive--;
+      }
+    }
     const uint32_t left = s.bkStart[K] - s.bkStart[0];
     if (src == 0 && left == 0) {
       for (int e = 0; e < 7; ++e) s.pot[e] = 0.0;
@@ -880,7 +1110,7 @@ NOINLINE __device__ void TransferFinish(const DevBox &B, DevState &s, Ctl &c, in
 
 // The MoleculeTransfer move after PickArbDist (gcmc.cpp TransferMove).
 template <int P>
-__device__ void TransferMoveD(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+__device__ void TransferMoveD(const DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
   if (threadIdx.x == 0) TransferPick(B, s, c, i);
   __syncthreads();
   if (c.op != kOpTransfer) return;
@@ -896,7 +1126,7 @@ __device__ void TransferMoveD(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
   if (c.abort) return;
   if (threadIdx.x == 0) TransferCalc(B, s, c);
   __syncthreads();
-  if (c.doRecip) SwapRecipEnergy<P>(B, c);
+  if (c.doRecip) SwapRecipEnergy<P>(B, c, tab);
   if (threadIdx.x == 0) TransferAccept(B, s, c);
   __syncthreads();
   if (c.commit) {
@@ -929,6 +1159,7 @@ __device__ void TransferMoveD(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
 // OneMove; src/Coordinates.cpp:83-128).
 NOINLINE __device__ void MolPick(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
   c.op = kOpMol;
+  c.skip = -1;
   double subDraw = c.subDraw;
   const int pick = c.pick;
   const double boxDiv = B.movePerc[pick] / 1;
@@ -1036,11 +1267,11 @@ NOINLINE __device__ void MolAccept(const DevBox &B, DevState &s, Ctl &c, int64_t
 }
 
 template <int P>
-__device__ void MolMoveD(const DevBox &B, DevState &s, Ctl &c, int64_t i) {
+__device__ void MolMoveD(const DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
   if (threadIdx.x == 0) MolPick(B, s, c, i);
   __syncthreads();
   if (c.op != kOpMol) return;
-  MolTrialEnergies<P>(B, c);
+  MolTrialEnergies<P>(B, c, tab);
   if (threadIdx.x == 0) MolAccept(B, s, c, i);
   __syncthreads();
   if (c.commit) {
@@ -1064,6 +1295,7 @@ __global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *b
   __shared__ DevBox B;
   __shared__ DevState S;
   __shared__ Ctl c;
+  __shared__ PhaseTab tab;
   {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(&boxes[blockIdx.x]);
     uint32_t *dst = reinterpret_cast<uint32_t *>(&B);
@@ -1075,7 +1307,9 @@ __global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *b
     uint32_t *dst = reinterpret_cast<uint32_t *>(&S);
     for (int w = threadIdx.x; w < int(sizeof(DevState) / 4); w += kDevThreads) dst[w] = src[w];
   }
+  if (threadIdx.x <= kErfcxDeg) sErfcx[threadIdx.x] = kErfcxPoly[threadIdx.x];
   if (threadIdx.x == 0) {
+    c.nLive = B.nLive0;
     c.abort = 0;
     c.done = 0;
     c.cyclesDone = 0;
@@ -1089,9 +1323,9 @@ __global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *b
     if (op == kOpStop) break;
     const int64_t i = c.done;
     if (op == kOpTransfer)
-      TransferMoveD<P>(B, S, c, i);
+      TransferMoveD<P>(B, S, c, tab, i);
     else
-      MolMoveD<P>(B, S, c, i);
+      MolMoveD<P>(B, S, c, tab, i);
     if (threadIdx.x == 0 && !c.abort) {
       c.done++;
       S.moves++;
@@ -1120,9 +1354,11 @@ static_assert(sizeof(DevBox) % 4 == 0 && sizeof(DevState) % 4 == 0, "word copies
 struct DevDriver {
   DevBox hbox;                  // this box's description for the next launch
   DevState *state = nullptr;    // its state (device)
-  int *molKind = nullptr, *molBox = nullptr;
+  int *molKind = nullptr, *molBox = nullptr, *live = nullptr, *livePos = nullptr;
   uint32_t *lookup = nullptr;
   double *com = nullptr;
+  int32_t *kpack = nullptr;     // k-vector index triples
+  int nmax[3] = {0, 0, 0};
   int32_t *trace = nullptr;     // [4][cap] ints then [3][cap] doubles
   int64_t traceCap = 0;
   DevBox *list = nullptr;       // launch list when this box leads a launch
@@ -1133,7 +1369,8 @@ struct DevDriver {
 void DevFree(gomc_box *b) {
   DevDriver *d = b->dev;
   if (!d) return;
-  for (void *p : {(void *)d->state, (void *)d->molKind, (void *)d->molBox,
+  for (void *p : {(void *)d->state, (void *)d->kpack, (void *)d->molKind, (void *)d->molBox,
+                  (void *)d->live, (void *)d->livePos,
                   (void *)d->lookup, (void *)d->com, (void *)d->trace, (void *)d->list})
     if (p) (void)hipFree(p);
   delete d;
@@ -1167,7 +1404,26 @@ int DevEnsure(gomc_box *b, int64_t traceMoves) {
     HIP_TRY(b, hipMalloc(&d->molBox, sizeof(int) * std::max(1, b->nmols)));
     HIP_TRY(b, hipMalloc(&d->lookup, sizeof(uint32_t) * std::max(1, b->nmols)));
     HIP_TRY(b, hipMalloc(&d->com, sizeof(double) * 3 * std::max(1, b->nmols)));
+    HIP_TRY(b, hipMalloc(&d->live, sizeof(int) * std::max(1, b->natoms)));
+    HIP_TRY(b, hipMalloc(&d->livePos, sizeof(int) * std::max(1, b->natoms)));
     HIP_TRY(b, hipMemcpy(d->molKind, b->molKind.data(), sizeof(int) * b->nmols, hipMemcpyHostToDevice));
+    // index triples of the box's k-vectors, in their order
+    std::vector<double> kx, ky, kz, pref;
+    std::vector<int32_t> packed;
+    KVectorsOrth(b->geom.L, b->alpha * b->alpha, b->recipRcut, b->volume, kx, ky, kz, pref, &packed);
+    if (b->ewald && int(packed.size()) != b->K)
+      return Fail(b, GOMC_ERR_STATE, "device move loop: k-vector count mismatch");
+    for (int32_t code : packed) {
+      d->nmax[0] = std::max(d->nmax[0], code & 0xff);
+      d->nmax[1] = std::max(d->nmax[1], std::abs(((code >> 8) & 0xff) - 128));
+      d->nmax[2] = std::max(d->nmax[2], std::abs(((code >> 16) & 0xff) - 128));
+    }
+    if (d->nmax[0] > kDevNMax || d->nmax[1] > kDevNMax || d->nmax[2] > kDevNMax)
+      return Fail(b, GOMC_ERR_ARG, "device move loop: more than 40 k-vectors per axis");
+    HIP_TRY(b, hipMalloc(&d->kpack, sizeof(int32_t) * std::max<size_t>(1, packed.size())));
+    if (!packed.empty())
+      HIP_TRY(b, hipMemcpy(d->kpack, packed.data(), sizeof(int32_t) * packed.size(),
+                           hipMemcpyHostToDevice));
   }
   DevDriver *d = b->dev;
   if (traceMoves > d->traceCap) {
@@ -1242,9 +1498,13 @@ int DevUpload(gomc_box *b, int64_t step0, int64_t nsteps, int cycles, bool trace
   h.sumI = b->sumIref;
   h.newR = b->sumRnew;
   h.newI = b->sumInew;
+  h.kpack = d->kpack;
+  for (int a = 0; a < 3; ++a) h.nmax[a] = d->nmax[a];
   h.molStart = b->molStartDev;
   h.molKind = d->molKind;
   h.molBox = d->molBox;
+  h.live = d->live;
+  h.livePos = d->livePos;
   h.lookup = d->lookup;
   h.com = d->com;
   h.st = d->state;
@@ -1291,10 +1551,21 @@ int DevUpload(gomc_box *b, int64_t step0, int64_t nsteps, int cycles, bool trace
   }
   std::vector<int> molBox(b->nmols, 0);
   for (int m = 0; m < b->nmols && !b->molBox.empty(); ++m) molBox[m] = b->molBox[m];
+  std::vector<int> live, livePos(b->natoms, -1);
+  live.reserve(b->natoms);
+  for (int a = 0; a < b->natoms; ++a)
+    if (molBox[b->atomMol[a]] == 0) {
+      livePos[a] = int(live.size());
+      live.push_back(a);
+    }
+  h.nLive0 = int(live.size());
   HIP_TRY(b, hipMemcpyAsync(d->state, &s, sizeof(s), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(d->com, com.data(), sizeof(double) * com.size(), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(d->molBox, molBox.data(), sizeof(int) * b->nmols, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(d->lookup, b->lookup.data(), sizeof(uint32_t) * b->nmols, hipMemcpyHostToDevice, b->stream));
+  if (!live.empty())
+    HIP_TRY(b, hipMemcpyAsync(d->live, live.data(), sizeof(int) * live.size(), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(d->livePos, livePos.data(), sizeof(int) * b->natoms, hipMemcpyHostToDevice, b->stream));
   // the copies read host vectors that go out of scope: complete them now
   HIP_TRY(b, hipStreamSynchronize(b->stream));
   return GOMC_OK;
@@ -1432,12 +1703,14 @@ int64_t gomc_run_moves_device(gomc_box *b, int64_t start_step, int64_t nsteps, i
 int gomc_run_moves_device_multi(gomc_box *const *boxes, int n, const int64_t *start_step,
                                 const int64_t *nsteps, const int32_t *cycles,
                                 int64_t *moves_done, int64_t *accepted, double *count_sum,
-                                int64_t *count_samples) {
+                                int64_t *count_samples, int32_t *status) {
   if (!boxes || n <= 0 || !start_step || !nsteps) return GOMC_ERR_ARG;
   gomc_box *lead = boxes[0];
   if (!lead) return GOMC_ERR_ARG;
   (void)hipSetDevice(lead->device);
   std::vector<gomc_box *> list(boxes, boxes + n);
+  if (status)
+    for (int i = 0; i < n; ++i) status[i] = GOMC_OK;
   for (int i = 0; i < n; ++i) {
     gomc_box *b = list[i];
     if (!b || b->device != lead->device || nsteps[i] < 0) return GOMC_ERR_ARG;
@@ -1447,14 +1720,22 @@ int gomc_run_moves_device_multi(gomc_box *const *boxes, int n, const int64_t *st
     if (!rc) rc = FlushPending(b);
     if (!rc) rc = DevEnsure(b, 0);
     if (!rc) rc = DevUpload(b, start_step[i], nsteps[i], cycles ? cycles[i] : 0, false);
-    if (rc) return rc;
+    if (rc) {
+      if (status) status[i] = rc;
+      return rc;
+    }
   }
   int rc = Launch(lead, list);
-  if (rc) return rc;
+  if (rc) {
+    if (status)
+      for (int i = 0; i < n; ++i) status[i] = rc;
+    return rc;
+  }
   int first = GOMC_OK;
   for (int i = 0; i < n; ++i) {
     gomc_box *b = list[i];
     rc = DevDownload(b);
+    if (status) status[i] = rc;
     if (rc && first == GOMC_OK) first = rc;
     const DevState &s = b->dev->hstate;  // counters start at zero in each launch
     if (moves_done) moves_done[i] = s.moves;

@@ -17,13 +17,16 @@
 // launches or how its moves are split between launches.
 //
 // GCMC jobs (ensemble 1: insertion/deletion with CBMC growth, two molecule
-// kinds) run in a second lane: kGcmcWorkers host threads each take the next
-// queued GCMC job while the budget holds its bytes, build it as a device box
-// on its own stream and run the reference move loop (MT19937 draws) to the
-// end, sampling the molecules of each kind in the box after every MC cycle.
+// kinds) run in a second lane: builder threads take queued GCMC jobs while
+// the budget holds their bytes and build each as a device box; a runner
+// advances all built jobs together, one launch of the device move loop
+// (the reference move sequence, MT19937 draws continued on the device, a
+// workgroup per job) after another, sampling the molecules of each kind in
+// the box after every MC cycle.
 // The lanes share the byte budget (the batched lane's arenas are reserved
 // from it at creation; a GCMC job's device box is allocated on admission);
 // jobs of both lanes overlap on the device.
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -79,7 +82,7 @@ class Arena {
   std::map<int64_t, int64_t> free_;
 };
 
-constexpr int kGcmcWorkers = 8;  // GCMC jobs running at once, each on its own stream
+constexpr int kGcmcBuilders = 4;  // threads building GCMC jobs' device boxes
 constexpr int64_t kAtomBytes = 3 * sizeof(double);                          // x, y, z
 constexpr int64_t kMolBytes = 3 * sizeof(double);                           // centres
 constexpr int64_t kKBytes = sizeof(int32_t) + 5 * sizeof(double);           // kn, pref, S, dS
@@ -133,6 +136,7 @@ struct Job {
   // path), from a deep copy of its descriptor
   bool gcmc = false;
   std::unique_ptr<DescCopy> desc;
+  gomc_box *box = nullptr;
   int64_t gcmcBytes = 0;
   double sumN[2] = {0, 0};  // molecules of kinds 0, 1 in the box, summed over samples
   int64_t samples = 0;
@@ -162,9 +166,13 @@ struct gomc_server {
   std::thread loop;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   // GCMC lane
-  std::deque<Job *> gcmcQueue;
+  std::deque<Job *> gcmcQueue, gcmcReady;
+  std::deque<std::pair<gomc_box *, int64_t>> gcmcRetire;  // boxes to destroy, their bytes
   std::vector<std::thread> workers;
-  int gcmcRunning = 0;
+  std::thread gcmcRunner;
+  int gcmcBuilding = 0;
+  int64_t gcmcLaunches = 0;
+  double gcmcSec = 0.0;
 };
 
 namespace {
@@ -320,16 +328,25 @@ int64_t GcmcBytes(const gomc_system_desc &d) {
   return natoms * 96 + int64_t(d.n_mols) * 64 + int64_t(kv) * 64 + (int64_t(1) << 20);
 }
 
-// Worker of the GCMC lane: takes queued GCMC jobs in arrival order while
-// the byte budget allows, and runs each to completion on its own box
-// (gomc_run_moves in chunks of one MC cycle, N_box moves), sampling the
-// number of molecules of each kind in the box after every chunk.
-void GcmcWorker(gomc_server *sv) {
+// Builder of the GCMC lane: takes queued GCMC jobs in arrival order while
+// the byte budget allows, builds each as a device box (initial energies,
+// structure factors) and hands it to the runner; destroys the boxes of
+// finished jobs and returns their bytes.
+void GcmcBuilder(gomc_server *sv) {
   (void)hipSetDevice(sv->device);
   std::unique_lock<std::mutex> lk(sv->mu);
   while (true) {
     Job *j = nullptr;
-    while (!sv->stop) {
+    gomc_box *dead = nullptr;
+    int64_t deadBytes = 0;
+    while (true) {
+      if (!sv->gcmcRetire.empty()) {
+        dead = sv->gcmcRetire.front().first;
+        deadBytes = sv->gcmcRetire.front().second;
+        sv->gcmcRetire.pop_front();
+        break;
+      }
+      if (sv->stop) return;
       if (!sv->gcmcQueue.empty() &&
           sv->bytesUsed + sv->gcmcQueue.front()->gcmcBytes <= sv->budget) {
         j = sv->gcmcQueue.front();
@@ -338,69 +355,136 @@ void GcmcWorker(gomc_server *sv) {
       }
       sv->cv.wait_for(lk, std::chrono::milliseconds(2));
     }
-    if (!j) return;
+    if (dead) {
+      lk.unlock();
+      gomc_box_destroy(dead);
+      lk.lock();
+      sv->bytesUsed -= deadBytes;
+      sv->cv.notify_all();
+      continue;
+    }
     sv->bytesUsed += j->gcmcBytes;
     sv->bytesPeak = std::max(sv->bytesPeak, sv->bytesUsed);
-    ++sv->gcmcRunning;
+    ++sv->gcmcBuilding;
     j->status = 1;
     j->tAdmit = Now(sv->t0);
     lk.unlock();
     gomc_box *b = nullptr;
-    std::string failure;
     int rc = gomc_box_create(&j->desc->d, sv->device, &b);
-    if (rc) failure = "GCMC job setup failed";
     if (!rc && sv->precision == 1) rc = gomc_set_precision(b, 1);
-    const int nm = j->desc->d.n_mols;
-    std::vector<int32_t> where(nm);
-    int64_t moves = 0, accepted = 0;
-    while (!rc && moves < j->totalMoves) {
-      const int nIn = gomc_mol_boxes(b, where.data());
-      if (nIn < 0) {
-        rc = nIn;
-        break;
-      }
-      const int64_t chunk = std::min<int64_t>(std::max(1, nIn), j->totalMoves - moves);
-      const int64_t acc = gomc_run_moves(b, moves, chunk, nullptr, nullptr, nullptr, nullptr,
-                                         nullptr, nullptr, nullptr);
-      if (acc < 0) {
-        rc = int(acc);
-        break;
-      }
-      accepted += acc;
-      moves += chunk;
-      if (gomc_mol_boxes(b, where.data()) < 0) {
-        rc = GOMC_ERR_STATE;
-        break;
-      }
-      double n[2] = {0, 0};
-      for (int m = 0; m < nm; ++m)
-        if (where[m] == 0 && j->desc->molKind[m] < 2) n[j->desc->molKind[m]] += 1.0;
-      std::lock_guard<std::mutex> g(sv->mu);
-      j->sumN[0] += n[0];
-      j->sumN[1] += n[1];
-      ++j->samples;
-      j->done = moves;
-      sv->movesDone += chunk;
-    }
-    double e[7] = {0};
-    if (!rc) rc = gomc_energies(b, e);
-    if (rc && failure.empty()) failure = b ? gomc_last_error(b) : "GCMC job failed";
-    if (b) gomc_box_destroy(b);
     lk.lock();
-    sv->bytesUsed -= j->gcmcBytes;
-    --sv->gcmcRunning;
+    --sv->gcmcBuilding;
     if (rc) {
+      if (b) {
+        sv->gcmcRetire.emplace_back(b, j->gcmcBytes);
+      } else {
+        sv->bytesUsed -= j->gcmcBytes;
+      }
       j->status = 3;
-      sv->err = failure;
+      sv->err = "GCMC job setup failed";
+      j->tDone = Now(sv->t0);
+      j->desc.reset();
+      --sv->outstanding;
+      sv->doneCv.notify_all();
     } else {
-      for (int i = 0; i < 7; ++i) j->energy[i] = e[i];
-      j->accepted = accepted;
-      j->status = 2;
+      j->box = b;
+      sv->gcmcReady.push_back(j);
     }
-    j->tDone = Now(sv->t0);
-    j->desc.reset();
-    --sv->outstanding;
-    sv->doneCv.notify_all();
+    sv->cv.notify_all();
+  }
+}
+
+// Runner of the GCMC lane: every launch of the device move loop advances
+// all running GCMC jobs together (one workgroup each), each by whole MC
+// cycles of N_box moves -- about as many moves per job as the largest
+// job's one cycle, so small jobs are not held behind large ones -- and
+// samples the molecules of each kind in the box after every cycle.  Jobs
+// join at the next launch after their box is built and leave when their
+// moves are done.
+void GcmcRunner(gomc_server *sv) {
+  (void)hipSetDevice(sv->device);
+  std::vector<Job *> run;
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (true) {
+    while (!sv->gcmcReady.empty()) {
+      run.push_back(sv->gcmcReady.front());
+      sv->gcmcReady.pop_front();
+    }
+    if (sv->stop) {  // running jobs are dropped after the launch in flight
+      for (Job *j : run) {
+        sv->gcmcRetire.emplace_back(j->box, j->gcmcBytes);
+        j->box = nullptr;
+      }
+      return;
+    }
+    if (run.empty()) {
+      if (sv->stop) return;
+      sv->cv.wait_for(lk, std::chrono::milliseconds(2));
+      continue;
+    }
+    lk.unlock();
+    const int n = int(run.size());
+    std::vector<gomc_box *> boxes(n);
+    std::vector<int64_t> start(n), steps(n), moves(n), acc(n), samples(n), nIn(n);
+    std::vector<int32_t> cycles(n), status(n);
+    std::vector<double> sums(2 * n);
+    std::vector<int32_t> where;
+    int64_t nMax = 1;
+    for (int i = 0; i < n; ++i) {
+      Job *j = run[i];
+      boxes[i] = j->box;
+      nIn[i] = std::max<int64_t>(1, j->box->bkStart[j->box->nkinds] - j->box->bkStart[0]);
+      nMax = std::max(nMax, nIn[i]);
+    }
+    for (int i = 0; i < n; ++i) {
+      start[i] = run[i]->done;
+      steps[i] = run[i]->totalMoves - run[i]->done;
+      cycles[i] = int32_t(std::max<int64_t>(1, nMax / nIn[i]));
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    const int rc = gomc_run_moves_device_multi(boxes.data(), n, start.data(), steps.data(),
+                                               cycles.data(), moves.data(), acc.data(),
+                                               sums.data(), samples.data(), status.data());
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<std::array<double, 7>> energy(n);
+    for (int i = 0; i < n; ++i) {
+      const Job *j = run[i];
+      if (!status[i] && j->done + moves[i] >= j->totalMoves) gomc_energies(j->box, energy[i].data());
+    }
+    lk.lock();
+    sv->gcmcSec += sec;
+    ++sv->gcmcLaunches;
+    std::vector<Job *> keep;
+    for (int i = 0; i < n; ++i) {
+      Job *j = run[i];
+      const bool failed = rc != GOMC_OK && status[i] != GOMC_OK;
+      if (!failed) {
+        j->done += moves[i];
+        j->accepted += acc[i];
+        j->sumN[0] += sums[2 * i];
+        j->sumN[1] += sums[2 * i + 1];
+        j->samples += samples[i];
+        sv->movesDone += moves[i];
+      }
+      if (failed || j->done >= j->totalMoves) {
+        if (failed) {
+          j->status = 3;
+          sv->err = gomc_last_error(j->box);
+        } else {
+          for (int e = 0; e < 7; ++e) j->energy[e] = energy[i][e];
+          j->status = 2;
+        }
+        j->tDone = Now(sv->t0);
+        sv->gcmcRetire.emplace_back(j->box, j->gcmcBytes);
+        j->box = nullptr;
+        j->desc.reset();
+        --sv->outstanding;
+        sv->doneCv.notify_all();
+      } else {
+        keep.push_back(j);
+      }
+    }
+    run.swap(keep);
     sv->cv.notify_all();
   }
 }
@@ -458,7 +542,8 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
   sv->slotJob.assign(slots, nullptr);
   gomc_server *raw = sv.release();
   raw->loop = std::thread(Loop, raw);
-  for (int w = 0; w < kGcmcWorkers; ++w) raw->workers.emplace_back(GcmcWorker, raw);
+  for (int w = 0; w < kGcmcBuilders; ++w) raw->workers.emplace_back(GcmcBuilder, raw);
+  raw->gcmcRunner = std::thread(GcmcRunner, raw);
   *out = raw;
   return GOMC_OK;
 }
@@ -593,8 +678,12 @@ int gomc_server_destroy(gomc_server *sv) {
     sv->cv.notify_all();
   }
   if (sv->loop.joinable()) sv->loop.join();
-  for (auto &w : sv->workers)  // a running GCMC job finishes first
+  if (sv->gcmcRunner.joinable()) sv->gcmcRunner.join();  // its current launch finishes first
+  for (auto &w : sv->workers)
     if (w.joinable()) w.join();
+  for (auto &r : sv->gcmcRetire) gomc_box_destroy(r.first);
+  for (Job *j : sv->gcmcReady)
+    if (j->box) gomc_box_destroy(j->box);
   if (sv->bt) gomc_batch_destroy(sv->bt);
   delete sv;
   return GOMC_OK;

@@ -247,14 +247,16 @@ int64_t gomc_run_moves_device(gomc_box *b, int64_t start_step, int64_t nsteps,
  * N_box moves each (N_box counted at the cycle's start, at least 1, never
  * more than nsteps[i] moves in all), adding its kind-0 and kind-1 molecule
  * counts to count_sum[2i], count_sum[2i+1] after each cycle
- * (count_samples[i] cycles).  moves_done, accepted, count_sum and
- * count_samples may be NULL.  Returns 0 or the first box's error code
- * (gomc_last_error of that box).  No counterpart in the reference. */
+ * (count_samples[i] cycles); status[i] is box i's return code.
+ * moves_done, accepted, count_sum, count_samples and status may be NULL.
+ * Returns 0 or the first failing box's error code (gomc_last_error of
+ * that box).  No counterpart in the reference. */
 int gomc_run_moves_device_multi(gomc_box *const *boxes, int n,
                                 const int64_t *start_step,
                                 const int64_t *nsteps, const int32_t *cycles,
                                 int64_t *moves_done, int64_t *accepted,
-                                double *count_sum, int64_t *count_samples);
+                                double *count_sum, int64_t *count_samples,
+                                int32_t *status);
 /* GCMC: box of every molecule (0 simulation box, 1 reservoir) into `box`
  * (n_mols); returns the number of molecules in box 0 or an error code.
  * MoleculeLookup::NumInBox / IsMoleculeInBox (src/MoleculeLookup.h:98-130). */
