@@ -90,7 +90,8 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=60, help="0 skips the serving run")
-    ap.add_argument("--serve-gcmc-jobs", type=int, default=12,
+    ap.add_argument("--serve-gcmc-cycles", type=int, default=100)
+    ap.add_argument("--serve-gcmc-jobs", type=int, default=60,
                     help="GCMC state-point jobs in the served run (0 skips)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (a known-size reference "
@@ -183,7 +184,7 @@ def main() -> None:
         from gomc_amd.serve import Server, poisson_stream_gcmc, run_stream
         from oracle.pyoracle import Oracle
         jobs = poisson_stream_gcmc(args.serve_gcmc_jobs, rate=20.0, seed=17, lmin=20.0,
-                                   lmax=30.0, cycles=5)
+                                   lmax=60.0, cycles=args.serve_gcmc_cycles)
         sv = Server(precision=1, slots=64, budget_bytes=8 << 30, tick_ms=20.0)
         rep = run_stream(sv, jobs, timeout=600.0)
         sv.close()
@@ -202,9 +203,9 @@ def main() -> None:
             samples += 1
         served_gcmc = dict(
             workload=f"Poisson(20/s) stream of {rep['jobs']} GCMC methane/CO2 adsorption jobs, "
-                     "L 20-30 A, T 250-350 K, fugacities log-uniform 1e3-1e7 Pa, 5 cycles each; "
-                     "GCMC lane (plain path in mixed precision, 8 concurrent jobs on their own "
-                     "streams)",
+                     f"L 20-60 A, T 250-350 K, fugacities log-uniform 1e3-1e7 Pa, "
+                     f"{args.serve_gcmc_cycles} cycles each; GCMC lane (device move loop in "
+                     "mixed precision, every running job in each launch)",
             moves_per_s=rep["moves_per_s"], p50_s=rep["p50_s"], p99_s=rep["p99_s"],
             moves=rep["moves"], wall_s=rep["wall_s"],
             mean_n_job0=r0["mean_n"].tolist(), mean_n_job0_oracle=(sums / samples).tolist())
