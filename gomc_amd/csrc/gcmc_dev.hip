@@ -80,8 +80,10 @@ struct DevBox {
 
 // Mutable state of a box's move loop, copied to LDS for a launch.
 struct DevState {
-  uint32_t mt[kMtN];
-  int mtIdx, status;
+  // MT19937: the current block of 624 state words is mt[mtCur]; the next
+  // block, once the workgroup has computed it (nextReady), is mt[mtCur ^ 1]
+  uint32_t mt[2][kMtN];
+  int mtIdx, mtCur, nextReady, status;
   double pot[7];  // inter, tc, real, recip, self, corr, total
   double scale[2][kDevKinds];
   uint64_t tries[2][kDevKinds], acc[2][kDevKinds], tTries[2][kDevKinds], tAcc[2][kDevKinds];
@@ -118,6 +120,7 @@ struct Ctl {
   double cE[kDevCombos], cR[kDevCombos], cO[kDevCombos];
   double w[kDevTrials], sI[kDevTrials], sR[kDevTrials];  // stage weights and sums
   double u[3][kDevTrials], hc[3], hb[3], hp[3];          // hedron spins: draws, frame
+  int uPeek;                                             // spins read ahead in the stream
   int nLive;                                             // atoms in the box (live list)
   DTrial nm, om;
   int64_t done, cycleLeft;
@@ -140,25 +143,85 @@ __device__ __forceinline__ GI *G(const int *p) { return (GI *)p; }
 __device__ __forceinline__ GDW *GW(double *p) { return (GDW *)p; }
 
 // ---------------------------------------------------------------- MT19937
-// The generator of host_rng.hpp over the state in LDS (thread 0 only).
-NOINLINE __device__ void MtTwist(DevState &s) {
-  for (int i = 0; i < kMtN; ++i) {
-    const int i1 = i + 1 < kMtN ? i + 1 : i + 1 - kMtN;
-    const int im = i + kMtM < kMtN ? i + kMtM : i + kMtM - kMtN;
-    const uint32_t y = (s.mt[i] & 0x80000000u) | (s.mt[i1] & 0x7fffffffu);
-    uint32_t v = s.mt[im] ^ (y >> 1);
-    if (s.mt[i1] & 1u) v ^= 0x9908b0dfu;
-    s.mt[i] = v;
-  }
-  s.mtIdx = 0;
+// The generator of host_rng.hpp over the state in LDS.  Thread 0 draws;
+// the workgroup computes the next block of state words ahead of need
+// (MtPrepareNext), so thread 0 does not twist serially.
+__device__ __forceinline__ uint32_t MtTwistWord(uint32_t a, uint32_t b, uint32_t m) {
+  const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+  uint32_t v = m ^ (y >> 1);
+  if (b & 1u) v ^= 0x9908b0dfu;
+  return v;
 }
-__device__ __forceinline__ uint32_t MtNext(DevState &s) {
-  if (s.mtIdx == kMtN) MtTwist(s);
-  uint32_t v = s.mt[s.mtIdx++];
+__device__ __forceinline__ uint32_t MtTemper(uint32_t v) {
   v ^= v >> 11;
   v ^= (v << 7) & 0x9d2c5680u;
   v ^= (v << 15) & 0xefc60000u;
   return v ^ (v >> 18);
+}
+// Serial twist of the current block in place (thread 0; only when the next
+// block was not prepared).
+NOINLINE __device__ void MtTwist(DevState &s) {
+  uint32_t *mt = s.mt[s.mtCur];
+  for (int i = 0; i < kMtN; ++i) {
+    const int i1 = i + 1 < kMtN ? i + 1 : i + 1 - kMtN;
+    const int im = i + kMtM < kMtN ? i + kMtM : i + kMtM - kMtN;
+    mt[i] = MtTwistWord(mt[i], mt[i1], mt[im]);
+  }
+}
+// The in-place twist, as three data-parallel passes into the other block:
+// word i < 227 reads old words i, i+1, i+397; 227 <= i < 623 reads old i,
+// i+1 and new i-227; word 623 reads old 623, new 0 and new 396.
+__device__ void MtPrepareNext(DevState &s) {
+  const uint32_t *o = s.mt[s.mtCur];
+  uint32_t *nw = s.mt[s.mtCur ^ 1];
+  const int t = threadIdx.x;
+  if (t < kMtN - kMtM) nw[t] = MtTwistWord(o[t], o[t + 1], o[t + kMtM]);
+  __syncthreads();
+  if (t < kMtN - kMtM) {
+    const int i = t + (kMtN - kMtM);
+    nw[i] = MtTwistWord(o[i], o[i + 1], nw[i - (kMtN - kMtM)]);
+  }
+  __syncthreads();
+  const int i = t + 2 * (kMtN - kMtM);
+  if (i < kMtN - 1) nw[i] = MtTwistWord(o[i], o[i + 1], nw[i - (kMtN - kMtM)]);
+  if (i == kMtN - 1) nw[i] = MtTwistWord(o[i], nw[0], nw[kMtM - 1]);
+  __syncthreads();
+  if (t == 0) s.nextReady = 1;
+  __syncthreads();
+}
+__device__ __forceinline__ uint32_t MtNext(DevState &s) {
+  if (s.mtIdx == kMtN) {
+    if (s.nextReady) {
+      s.mtCur ^= 1;
+      s.nextReady = 0;
+    } else {
+      MtTwist(s);
+    }
+    s.mtIdx = 0;
+  }
+  return MtTemper(s.mt[s.mtCur][s.mtIdx++]);
+}
+// Words still unread, counting a prepared next block.
+__device__ __forceinline__ int MtAvail(const DevState &s) {
+  return kMtN - s.mtIdx + (s.nextReady ? kMtN : 0);
+}
+// The k-th next tempered word without consuming it (k < MtAvail).
+__device__ __forceinline__ uint32_t MtPeek(const DevState &s, int k) {
+  const int i = s.mtIdx + k;
+  return MtTemper(i < kMtN ? s.mt[s.mtCur][i] : s.mt[s.mtCur ^ 1][i - kMtN]);
+}
+// Consumes k words (k <= MtAvail), as k MtNext calls would.
+__device__ __forceinline__ void MtSkip(DevState &s, int k) {
+  for (int r = k; r > 0;) {
+    if (s.mtIdx == kMtN) {
+      s.mtCur ^= 1;
+      s.nextReady = 0;
+      s.mtIdx = 0;
+    }
+    const int step = min(r, kMtN - s.mtIdx);
+    s.mtIdx += step;
+    r -= step;
+  }
 }
 __device__ __forceinline__ uint32_t MtBelow(DevState &s, uint32_t n) {
   uint32_t mask = n;
@@ -169,6 +232,8 @@ __device__ __forceinline__ uint32_t MtBelow(DevState &s, uint32_t n) {
   } while (v > n);
   return v;
 }
+__device__ __forceinline__ double U32Closed(uint32_t v) { return double(v) * (1.0 / 4294967295.0); }
+__device__ __forceinline__ double U32HalfOpen(uint32_t v) { return double(v) * (1.0 / 4294967296.0); }
 __device__ __forceinline__ double MtClosed(DevState &s) {
   return double(MtNext(s)) * (1.0 / 4294967295.0);
 }
@@ -728,10 +793,38 @@ NOINLINE __device__ void SingleFinish(const DevBox &B, DevState &s, Ctl &c, DTri
   t.real += 0.0 + c.cR[win];
 }
 
+// SinglePrep with the trial positions computed one per thread from words
+// read ahead in the stream (when the stream holds them).
+NOINLINE __device__ void SinglePrepParallel(const DevBox &B, DevState &s, Ctl &c,
+                                            const DTrial &t, int atom, bool isNew) {
+  const int n = B.cbmcFirst, kind = c.kind, i = threadIdx.x;
+  if (i < n) {
+    c.cx[i] = U32HalfOpen(MtPeek(s, 3 * i)) * B.g.L[0];
+    c.cy[i] = U32HalfOpen(MtPeek(s, 3 * i + 1)) * B.g.L[1];
+    c.cz[i] = U32HalfOpen(MtPeek(s, 3 * i + 2)) * B.g.L[2];
+    c.ct[i] = B.kindT[kind][atom];
+    c.cq[i] = B.kindQ[kind][atom];
+  }
+  __syncthreads();
+  if (i == 0) {
+    MtSkip(s, 3 * n);
+    if (!isNew) {
+      c.cx[0] = Wrap(t.px[atom], B.g.L[0]);
+      c.cy[0] = Wrap(t.py[atom], B.g.L[1]);
+      c.cz[0] = Wrap(t.pz[atom], B.g.L[2]);
+    }
+    c.nc = t.box == 0 ? n : 0;
+    for (int k = c.nc; k < n; ++k) c.cE[k] = c.cR[k] = c.cO[k] = 0.0;
+  }
+}
+
 template <int P>
 __device__ void SingleStage(const DevBox &B, DevState &s, Ctl &c, DTrial &t, int atom,
                             bool isNew) {
-  if (threadIdx.x == 0) SinglePrep(B, s, c, t, atom, isNew);
+  if (MtAvail(s) >= 3 * B.cbmcFirst)
+    SinglePrepParallel(B, s, c, t, atom, isNew);
+  else if (threadIdx.x == 0)
+    SinglePrep(B, s, c, t, atom, isNew);
   __syncthreads();
   ComboEnergies<P>(B, c);
   StageWeights(B, c, B.cbmcFirst, false);
@@ -786,23 +879,32 @@ NOINLINE __device__ void HedronDraw(const DevBox &B, DevState &s, Ctl &c, const 
     c.hp[2] = Unwrap(t.pz[1], cz, B.g.L[2], B.g.half[2]) + (-cz);
   }
   const int lo = isNew ? 0 : 1;
-  for (int lj = n; lj-- > lo;) {
-    c.u[0][lj] = MtClosed(s);
-    c.u[1][lj] = MtClosed(s);
-    c.u[2][lj] = MtClosed(s);
-  }
+  c.uPeek = MtAvail(s) >= 3 * (n - lo);
+  if (!c.uPeek)
+    for (int lj = n; lj-- > lo;) {
+      c.u[0][lj] = MtClosed(s);
+      c.u[1][lj] = MtClosed(s);
+      c.u[2][lj] = MtClosed(s);
+    }
   c.nc = t.box == 0 ? 2 * n : 0;
 }
 
 // Trial i (one per thread): spin the frame, place atoms 2 and 1 at
 // c[i] and c[n + i]; the old build's trial 0 is the actual frame.
-NOINLINE __device__ void HedronSpin(const DevBox &B, Ctl &c, bool isNew) {
+NOINLINE __device__ void HedronSpin(const DevBox &B, DevState &s, Ctl &c, bool isNew) {
   const int n = B.cbmcNth, kind = c.kind, i = threadIdx.x;
   if (i < n) {
     const double cx = c.hc[0], cy = c.hc[1], cz = c.hc[2];
     double bx = c.hb[0], by = c.hb[1], bz = c.hb[2], qx = c.hp[0], qy = c.hp[1], qz = c.hp[2];
     if (isNew || i > 0) {
-      const DM3 spin = UniformRotationD(c.u[0][i], c.u[1][i], c.u[2][i]);
+      double u1 = c.u[0][i], u2 = c.u[1][i], u3 = c.u[2][i];
+      if (c.uPeek) {  // trial n-1 draws first, three words each
+        const int k = 3 * (n - 1 - i);
+        u1 = U32Closed(MtPeek(s, k));
+        u2 = U32Closed(MtPeek(s, k + 1));
+        u3 = U32Closed(MtPeek(s, k + 2));
+      }
+      const DM3 spin = UniformRotationD(u1, u2, u3);
       double ax, ay, az;
       Apply(spin, bx, by, bz, ax, ay, az);
       bx = ax;
@@ -826,6 +928,7 @@ NOINLINE __device__ void HedronSpin(const DevBox &B, Ctl &c, bool isNew) {
     if (c.nc == 0) c.cE[i] = c.cR[i] = c.cO[i] = c.cE[n + i] = c.cR[n + i] = c.cO[n + i] = 0.0;
   }
   __syncthreads();
+  if (i == 0 && c.uPeek) MtSkip(s, 3 * (isNew ? n : n - 1));
 }
 
 NOINLINE __device__ void HedronFinish(const DevBox &B, DevState &s, Ctl &c, DTrial &t,
@@ -858,11 +961,12 @@ NOINLINE __device__ void HedronFinish(const DevBox &B, DevState &s, Ctl &c, DTri
 template <int P>
 __device__ void HedronStage(const DevBox &B, DevState &s, Ctl &c, DTrial &t, bool isNew) {
   if (threadIdx.x == 0) HedronTheta(B, s, c, isNew);
+  __syncthreads();  // the stream position is read by every thread next
   SingleStage<P>(B, s, c, t, 0, isNew);
   if (c.abort) return;
   if (threadIdx.x == 0) HedronDraw(B, s, c, t, isNew);
   __syncthreads();
-  HedronSpin(B, c, isNew);
+  HedronSpin(B, s, c, isNew);
   ComboEnergies<P>(B, c);
   StageWeights(B, c, B.cbmcNth, true);
   if (threadIdx.x == 0) HedronFinish(B, s, c, t, isNew);
@@ -1317,6 +1421,8 @@ __global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *b
   }
   __syncthreads();
   while (true) {
+    // keep the next block of the MT19937 stream ready (a move draws < 400 words)
+    if (!S.nextReady && kMtN - S.mtIdx < 400) MtPrepareNext(S);
     if (threadIdx.x == 0) LoopTop(B, S, c);
     __syncthreads();
     const int op = c.op;
@@ -1520,7 +1626,9 @@ int DevUpload(gomc_box *b, int64_t step0, int64_t nsteps, int cycles, bool trace
   }
   DevState &s = d->hstate;
   std::memset(&s, 0, sizeof(s));
-  b->mt.GetState(s.mt, &s.mtIdx);
+  b->mt.GetState(s.mt[0], &s.mtIdx);
+  s.mtCur = 0;
+  s.nextReady = 0;
   s.pot[0] = b->pot.inter;
   s.pot[1] = b->pot.tc;
   s.pot[2] = b->pot.real;
@@ -1589,7 +1697,7 @@ int DevDownload(gomc_box *b) {
   HIP_TRY(b, hipMemcpyAsync(b->hz.data(), b->z, bytes, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(b, hipStreamSynchronize(b->stream));
   const int K = b->nkinds;
-  b->mt.SetState(s.mt, s.mtIdx);
+  b->mt.SetState(s.mt[s.mtCur], s.mtIdx);
   b->pot.inter = s.pot[0];
   b->pot.tc = s.pot[1];
   b->pot.real = s.pot[2];
