@@ -180,6 +180,14 @@ class Box:
         self._check(self.lib.gomc_torque(self.h, *(ptr(v) for v in t)))
         return np.stack(t, 1)
 
+    def virial(self) -> dict:
+        """Molecular virial (K) and pressure (bar) of the current configuration:
+        {inter, tail, real, recip, total, pressure, tensor (xx, yy, zz)}."""
+        out = np.zeros(9)
+        self._check(self.lib.gomc_box_virial(self.h, ptr(out)))
+        return dict(inter=out[0], tail=out[1], real=out[2], recip=out[3], total=out[4],
+                    pressure=out[5], tensor=out[6:9].copy())
+
     def mp_trial(self, move_type: int, step: int, max_step: float) -> np.ndarray:
         out = np.zeros(5)
         self._check(self.lib.gomc_mp_trial(self.h, move_type, step, max_step, ptr(out)))

@@ -745,6 +745,7 @@ static void FreeAll(gomc_box *b) {
   MPFree(b);
   DevFree(b);
   if (b->trialBuf) (void)hipFree(b->trialBuf);
+  if (b->virBuf) (void)hipFree(b->virBuf);
   for (void *p : {(void *)b->cacheC, (void *)b->cacheS, (void *)b->stageC, (void *)b->stageS})
     if (p) (void)hipFree(p);
   if (b->hostRes) (void)hipHostFree(b->hostRes);

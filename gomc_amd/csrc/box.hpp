@@ -111,7 +111,7 @@ struct gomc_box {
   std::vector<int> molKind, molStart, molLen, atomType, atomMol, kindCount;
   std::vector<double> atomCharge;
   std::vector<double> sigSq, epsCn, epsCn6, nOver6, nHalf, nRep;  // [ntypes^2]
-  std::vector<double> pairEnCorr;                                 // [nkinds^2]
+  std::vector<double> pairEnCorr, pairVirCorr;                    // [nkinds^2]
   double rCutLJ = 0, rCutCoul = 0, rCutLow = 0, tol = 0, T = 0, beta = 0;
   double alpha = 0, recipRcut = 0, recipRcutSq = 0, volume = 0, volInv = 0;
   bool ewald = true, elect = true, lrc = true;
@@ -155,6 +155,9 @@ struct gomc_box {
   // staging for trial positions and their energies (ParticleInter)
   double *trialBuf = nullptr;
   int trialCap = 0;
+  // per-block partial sums of the virial kernels
+  double *virBuf = nullptr;
+  size_t virCap = 0;
 
   // ---- GCMC (ensemble 1): box 0 interacts, box 1 is the reservoir ----
   // Reservoir atoms carry molecule id -1 on the device and are skipped by

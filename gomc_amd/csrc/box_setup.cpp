@@ -243,6 +243,7 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   // LRC table over molecule kinds (src/Molecules.cpp:157-175 with
   // FFParticle::EnergyLRC, src/FFParticle.cpp:96-116).
   b->pairEnCorr.assign(b->nkinds * b->nkinds, 0.0);
+  b->pairVirCorr.assign(b->nkinds * b->nkinds, 0.0);
   if (b->lrc) {
     for (int i = 0; i < b->nkinds; ++i)
       for (int j = i; j < b->nkinds; ++j) {
@@ -257,8 +258,15 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
             tc *= 2.0 * M_PI * b->epsCn[idx] / (N - 3.0);
             tc *= (std::pow(rRat, b->nRep[idx] - 3.0) - ((N - 3.0) / 3.0) * rRat * rRat * rRat);
             b->pairEnCorr[i * b->nkinds + j] += tc;
+            // FFParticle::VirialLRC (src/FFParticle.cpp:118-138)
+            double tv = 1.0;
+            tv *= sigma * b->sigSq[idx];
+            tv *= 2.0 * M_PI * b->epsCn[idx] / (N - 3.0);
+            tv *= (N * std::pow(rRat, b->nRep[idx] - 3.0) - (N - 3.0) * 2.0 * rRat * rRat * rRat);
+            b->pairVirCorr[i * b->nkinds + j] += tv;
           }
         b->pairEnCorr[j * b->nkinds + i] = b->pairEnCorr[i * b->nkinds + j];
+        b->pairVirCorr[j * b->nkinds + i] = b->pairVirCorr[i * b->nkinds + j];
       }
   }
   // ---- cell grid (CellList::ResizeGrid/RebuildNeighbors) ----

@@ -1,0 +1,286 @@
+// Molecular virial and pressure of a box (CalculateEnergy::VirialCalc,
+// src/CalculateEnergy.cpp:411-579; VirialCorrection 1317-1336;
+// Ewald::VirialReciprocal, src/Ewald.cpp:1168-1305; the pressure of
+// OutputVars.cpp:147-172), diagonal tensor terms as in the reference.
+//
+// PairVirialKernel: one workgroup per (cell, neighbour slot), the same
+// enumeration as BoxInterKernel: for every inter-molecular pair inside the
+// box cutoff, F/r of the LJ and erfc-Coulomb terms times the atom
+// separation's and the minimum-image centre separation's components.
+// RecipVirialKernel: blocks [0, kBlocks) one k-vector per thread (the
+// k-space term); the others one atom per thread over LDS-staged k data (the
+// intramolecular term: q 2 pref (S_I cos - S_R sin) k_d times the atom's
+// offset from its molecule's centre).  Each block writes its partial sums;
+// the host adds them in block order.
+#include <algorithm>
+#include <cmath>
+
+#include "box.hpp"
+
+namespace {
+using namespace gomc;
+
+constexpr int kVirSlots = 8;
+
+struct PairVirArgs {
+  BoxGeom g;
+  const double *x, *y, *z, *q;
+  const int *type, *mol;
+  const double *sigSq, *epsCn6, *nOver6, *nHalf;
+  int ntypes;
+  const int *cellStart, *cellVec, *nbr;
+  const double *cx, *cy, *cz;  // molecule centres
+  double *part;                // [blocks][kVirSlots]
+};
+
+// FFParticle::CalcCoulombVir with the Coulomb cutoff (FFParticle.cpp:401-446).
+__device__ __forceinline__ double CoulVirGated(const BoxGeom &g, double d2, double qiqj) {
+  if (g.rCutCoulSq < d2) return 0.0;
+  return CoulombVirial(g, d2, qiqj);
+}
+
+__global__ void __launch_bounds__(kBlock) PairVirialKernel(const PairVirArgs a) {
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
+  __shared__ int sidx[kBlock], stype[kBlock], smol[kBlock];
+  __shared__ double red[kBlock / kWave];
+  const int cell = blockIdx.x / 27, nb = a.nbr[blockIdx.x];
+  const int cs = a.cellStart[cell], ce = a.cellStart[cell + 1];
+  const int ns = a.cellStart[nb], ne = a.cellStart[nb + 1];
+  const int ni = ce - cs;
+  double v[6] = {0, 0, 0, 0, 0, 0};  // LJ xx yy zz, real xx yy zz
+  for (int tile = ns; tile < ne; tile += kBlock) {
+    const int tn = min(kBlock, ne - tile);
+    if (threadIdx.x < tn) {
+      const int j = a.cellVec[tile + threadIdx.x];
+      sidx[threadIdx.x] = j;
+      sx[threadIdx.x] = a.x[j];
+      sy[threadIdx.x] = a.y[j];
+      sz[threadIdx.x] = a.z[j];
+      sq[threadIdx.x] = a.q[j];
+      stype[threadIdx.x] = a.type[j];
+      smol[threadIdx.x] = a.mol[j];
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < ni * tn; p += kBlock) {
+      const int i = a.cellVec[cs + p / tn], jj = p % tn, j = sidx[jj];
+      const int mi = a.mol[i], mj = smol[jj];
+      if (i < j && mi != mj) {
+        const double dx = MinImage(a.x[i] - sx[jj], a.g.L[0], a.g.half[0]);
+        const double dy = MinImage(a.y[i] - sy[jj], a.g.L[1], a.g.half[1]);
+        const double dz = MinImage(a.z[i] - sz[jj], a.g.L[2], a.g.half[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (a.g.rCutSq > d2) {
+          const double cdx = MinImage(a.cx[mi] - a.cx[mj], a.g.L[0], a.g.half[0]);
+          const double cdy = MinImage(a.cy[mi] - a.cy[mj], a.g.L[1], a.g.half[1]);
+          const double cdz = MinImage(a.cz[mi] - a.cz[mj], a.g.L[2], a.g.half[2]);
+          if (a.g.elect) {
+            const double qiqj = a.q[i] * sq[jj];
+            if (qiqj != 0.0) {
+              const double pRF = CoulVirGated(a.g, d2, qiqj);
+              v[3] += pRF * (dx * cdx);
+              v[4] += pRF * (dy * cdy);
+              v[5] += pRF * (dz * cdz);
+            }
+          }
+          if (!(a.g.rCutLJSq < d2)) {
+            const int idx = a.type[i] + stype[jj] * a.ntypes;
+            const double pVF = LJVirial(d2, a.sigSq[idx], a.epsCn6[idx], a.nOver6[idx], a.nHalf[idx]);
+            v[0] += pVF * (dx * cdx);
+            v[1] += pVF * (dy * cdy);
+            v[2] += pVF * (dz * cdz);
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int c = 0; c < 6; ++c) {
+    const double s = BlockSum(v[c], red);
+    if (threadIdx.x == 0) a.part[size_t(blockIdx.x) * kVirSlots + c] = s;
+  }
+}
+
+struct RecipVirArgs {
+  BoxGeom g;
+  const double *x, *y, *z, *q;
+  const int *mol;
+  int natoms, K, kBlocks;
+  const double *kx, *ky, *kz, *pref, *sumR, *sumI;
+  const double *cx, *cy, *cz;
+  double constVal;  // 1 / (4 alpha^2)
+  double *part;
+};
+
+__global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a) {
+  __shared__ double red[kBlock / kWave];
+  __shared__ double sk[6][kBlock];
+  double w[3] = {0, 0, 0};
+  if (int(blockIdx.x) < a.kBlocks) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < a.K) {
+      const double kx = a.kx[i], ky = a.ky[i], kz = a.kz[i];
+      const double hsqr = kx * kx + ky * ky + kz * kz;
+      const double factor = a.pref[i] * (a.sumR[i] * a.sumR[i] + a.sumI[i] * a.sumI[i]);
+      w[0] = factor * (1.0 - 2.0 * (a.constVal + 1.0 / hsqr) * kx * kx);
+      w[1] = factor * (1.0 - 2.0 * (a.constVal + 1.0 / hsqr) * ky * ky);
+      w[2] = factor * (1.0 - 2.0 * (a.constVal + 1.0 / hsqr) * kz * kz);
+    }
+  } else {
+    const int p = (blockIdx.x - a.kBlocks) * kBlock + threadIdx.x;
+    bool on = false;
+    double xp = 0, yp = 0, zp = 0, qp = 0, ddx = 0, ddy = 0, ddz = 0;
+    if (p < a.natoms) {
+      qp = a.q[p];
+      const int m = a.mol[p];
+      on = m >= 0 && !(fabs(qp) < 0.000000001);
+      if (on) {
+        xp = a.x[p];
+        yp = a.y[p];
+        zp = a.z[p];
+        const double cx = a.cx[m], cy = a.cy[m], cz = a.cz[m];
+        ddx = Unwrap(xp, cx, a.g.L[0], a.g.half[0]) - cx;
+        ddy = Unwrap(yp, cy, a.g.L[1], a.g.half[1]) - cy;
+        ddz = Unwrap(zp, cz, a.g.L[2], a.g.half[2]) - cz;
+      }
+    }
+    for (int t = 0; t < a.K; t += kBlock) {
+      const int tn = min(kBlock, a.K - t);
+      __syncthreads();
+      if (int(threadIdx.x) < tn) {
+        const int k = t + threadIdx.x;
+        sk[0][threadIdx.x] = a.kx[k];
+        sk[1][threadIdx.x] = a.ky[k];
+        sk[2][threadIdx.x] = a.kz[k];
+        sk[3][threadIdx.x] = a.pref[k];
+        sk[4][threadIdx.x] = a.sumR[k];
+        sk[5][threadIdx.x] = a.sumI[k];
+      }
+      __syncthreads();
+      if (on)
+        for (int i = 0; i < tn; ++i) {
+          const double arg = xp * sk[0][i] + yp * sk[1][i] + zp * sk[2][i];
+          double s, c;
+          sincos(arg, &s, &c);
+          const double factor = sk[3][i] * 2.0 * (sk[5][i] * c - sk[4][i] * s) * qp;
+          w[0] += factor * (sk[0][i] * ddx);
+          w[1] += factor * (sk[1][i] * ddy);
+          w[2] += factor * (sk[2][i] * ddz);
+        }
+    }
+  }
+  for (int c = 0; c < 3; ++c) {
+    const double s = BlockSum(w[c], red);
+    if (threadIdx.x == 0) a.part[size_t(blockIdx.x) * kVirSlots + c] = s;
+  }
+}
+
+}  // namespace
+
+extern "C" int gomc_box_virial(gomc_box *b, double out[9]) {
+  if (!b || !out) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);
+  if (!rc) rc = MPEnsure(b);
+  if (!rc) rc = MPUploadCom(b);  // molecule centres of the current coordinates
+  if (!rc) rc = BuildCells(b);
+  if (rc) return rc;
+  const int pairBlocks = b->ncell * 27;
+  const int kBlocks = b->ewald ? (b->K + kBlock - 1) / kBlock : 0;
+  const int atomBlocks = b->ewald ? (b->natoms + kBlock - 1) / kBlock : 0;
+  const int nblocks = pairBlocks + kBlocks + atomBlocks;
+  const size_t need = size_t(kVirSlots) * std::max(nblocks, 1);
+  if (need > b->virCap) {  // grows only with the box's block count
+    if (b->virBuf) (void)hipFree(b->virBuf);
+    b->virBuf = nullptr;
+    b->virCap = 0;
+    HIP_TRY(b, hipMalloc(&b->virBuf, sizeof(double) * need));
+    b->virCap = need;
+  }
+  double *part = b->virBuf;
+  PairVirArgs a;
+  a.g = b->geom;
+  a.x = b->x;
+  a.y = b->y;
+  a.z = b->z;
+  a.q = b->q;
+  a.type = b->type;
+  a.mol = b->mol;
+  a.sigSq = b->tSigSq;
+  a.epsCn6 = b->tEpsCn6;
+  a.nOver6 = b->tNOver6;
+  a.nHalf = b->tNHalf;
+  a.ntypes = b->ntypes;
+  a.cellStart = b->cellStart;
+  a.cellVec = b->cellVec;
+  a.nbr = b->nbr;
+  a.cx = b->mp.com[0][0];
+  a.cy = b->mp.com[0][1];
+  a.cz = b->mp.com[0][2];
+  a.part = part;
+  hipLaunchKernelGGL(PairVirialKernel, dim3(pairBlocks), dim3(kBlock), 0, b->stream, a);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && b->ewald) {
+    RecipVirArgs r;
+    r.g = b->geom;
+    r.x = b->x;
+    r.y = b->y;
+    r.z = b->z;
+    r.q = b->q;
+    r.mol = b->mol;
+    r.natoms = b->natoms;
+    r.K = b->K;
+    r.kBlocks = kBlocks;
+    r.kx = b->kx;
+    r.ky = b->ky;
+    r.kz = b->kz;
+    r.pref = b->pref;
+    r.sumR = b->sumRref;
+    r.sumI = b->sumIref;
+    r.cx = a.cx;
+    r.cy = a.cy;
+    r.cz = a.cz;
+    r.constVal = 1.0 / (4.0 * (b->alpha * b->alpha));
+    r.part = part + size_t(pairBlocks) * kVirSlots;
+    hipLaunchKernelGGL(RecipVirialKernel, dim3(kBlocks + atomBlocks), dim3(kBlock), 0, b->stream, r);
+    e = hipGetLastError();
+  }
+  std::vector<double> h(size_t(kVirSlots) * std::max(nblocks, 1), 0.0);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h.data(), part, sizeof(double) * kVirSlots * nblocks, hipMemcpyDeviceToHost,
+                       b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return HipFail(b, e, "virial kernels");
+  double vT[3] = {0, 0, 0}, rT[3] = {0, 0, 0}, wT[3] = {0, 0, 0};
+  for (int blk = 0; blk < pairBlocks; ++blk)
+    for (int d = 0; d < 3; ++d) {
+      vT[d] += h[size_t(blk) * kVirSlots + d];
+      rT[d] += h[size_t(blk) * kVirSlots + 3 + d];
+    }
+  for (int blk = pairBlocks; blk < nblocks; ++blk)
+    for (int d = 0; d < 3; ++d) wT[d] += h[size_t(blk) * kVirSlots + d];
+  const double inter = vT[0] + vT[1] + vT[2];
+  const double real = b->elect ? (rT[0] + rT[1] + rT[2]) * kQQFact : 0.0;
+  double tail = 0.0;  // VirialCorrection
+  if (b->lrc)
+    for (int i = 0; i < b->nkinds; ++i)
+      for (int j = 0; j < b->nkinds; ++j)
+        tail += b->pairVirCorr[i * b->nkinds + j] * unsigned(b->kindCount[i]) *
+                unsigned(b->kindCount[j]) * b->volInv;
+  const double recip = wT[0] + wT[1] + wT[2];
+  out[0] = inter;
+  out[1] = tail;
+  out[2] = real;
+  out[3] = recip;
+  out[4] = inter + tail + real + recip;
+  // the pressure: each term over the dimensionality and the volume, the
+  // ideal part from the number densities, K molecule / A^3 to bar
+  const double V = b->volume;
+  const double total = inter / 3.0 / V + tail / 3.0 / V + real / 3.0 / V + recip / 3.0 / V;
+  double raw = 0.0;
+  for (int k = 0; k < b->nkinds; ++k) raw += b->kindCount[k] * b->volInv;
+  raw *= b->T;
+  raw += total;
+  out[5] = raw * 138.0649;
+  for (int d = 0; d < 3; ++d) out[6 + d] = vT[d] + (b->elect ? rT[d] * kQQFact : 0.0) + wT[d];
+  return GOMC_OK;
+}

@@ -120,6 +120,16 @@ int gomc_particle_inter(gomc_box *b, int atom_type, double charge,
                         const double *ty, const double *tz, double *en,
                         double *real, int32_t *overlap);
 
+/* Molecular virial and pressure of the current configuration:
+ * CalculateEnergy::VirialCalc (CalculateEnergy.cpp:411-579) with
+ * VirialCorrection (1317-1336) and Ewald::VirialReciprocal
+ * (Ewald.cpp:1168-1305), the pressure as OutputVars.cpp:147-172 forms it;
+ * replaces CallBoxInterForceGPU and CallVirialReciprocalGPU
+ * (src/GPU/CalculateForceCUDAKernel.cuh:17-50).  out = {inter (LJ), tail,
+ * real, recip, total, pressure in bar, total tensor xx, yy, zz} (virials
+ * in K). */
+int gomc_box_virial(gomc_box *b, double out[9]);
+
 /* ---- Ewald reciprocal space (src/Ewald.cpp) -------------------------- */
 /* RecipInitOrth + BoxReciprocalSetup + SetRecipRef (Ewald.cpp:847-903,
  * 193-274, 1021-1053); replaces CallBoxReciprocalSetupGPU

@@ -184,6 +184,15 @@ struct ForceField {
   double CalcCoulomb(double distSq, double qiqj) const;        // 361-399
   double CalcCoulombVir(double distSq, double qiqj) const;     // 401-446
   double EnergyLRC(int k1, int k2) const;                      // 96-116
+  double VirialLRC(int k1, int k2) const;                      // 118-138
+};
+
+// The diagonal of the virial tensors and the scalar virials (EnergyTypes.h
+// Virial; the off-diagonal terms are left out there too,
+// CalculateEnergy.cpp:408-410).
+struct VirialD {
+  double inter = 0, tc = 0, real = 0, recip = 0, total = 0;
+  double interT[3] = {0, 0, 0}, realT[3] = {0, 0, 0}, recipT[3] = {0, 0, 0}, totalT[3] = {0, 0, 0};
 };
 
 // ---------------------------------------------------------- molecules -----
@@ -256,6 +265,7 @@ class Ewald {
                           const std::vector<double> &z,
                           std::vector<XYZ> &atomForceRec,
                           std::vector<XYZ> &molForceRec);  // 1496-1596
+  void VirialReciprocal(VirialD &v) const;                 // 1168-1305 (CPU branch)
   int imageSize = 0, imageSizeRef = 0, kmax = 0;
   std::vector<double> kx, ky, kz, hsqr, prefact;
   std::vector<double> kxRef, kyRef, kzRef, hsqrRef, prefactRef;
@@ -310,6 +320,8 @@ class System {
                 std::vector<XYZ> &molForce);               // 268-406
   void CalculateTorque(const std::vector<XYZ> &aF, const std::vector<XYZ> &aFR,
                        std::vector<XYZ> &torque) const;    // 1365-1406
+  VirialD VirialCalc();                                    // 411-579
+  double Pressure(const VirialD &v) const;                 // OutputVars.cpp:120-185
   // Moves.
   void CalcCOM();                                          // COM.h:41-62
   int RunStepMT(uint64_t step, MoveRecord *rec);           // System.cpp:262-302
@@ -340,6 +352,7 @@ class System {
   std::vector<double> x, y, z, comx, comy, comz;
   std::vector<int> kindCount;          // molecules of each kind in the box
   std::vector<double> pairEnCorrections;  // Molecules.cpp:157-175
+  std::vector<double> pairVirCorrections;
   CellList cells;
   Ewald ewald{*this};
   Energy pot;

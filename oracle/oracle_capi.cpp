@@ -251,6 +251,20 @@ void orc_torque(void *h, double *tx, double *ty, double *tz) {
   }
 }
 
+// CalculateEnergy::VirialCalc and the pressure of OutputVars:
+// out = {inter, tail, real, recip, total, pressure (bar), total tensor xx, yy, zz}.
+void orc_virial(void *h, double *out) {
+  System *s = S(h);
+  const orc::VirialD v = s->VirialCalc();
+  out[0] = v.inter;
+  out[1] = v.tc;
+  out[2] = v.real;
+  out[3] = v.recip;
+  out[4] = v.total;
+  out[5] = s->Pressure(v);
+  for (int d = 0; d < 3; ++d) out[6 + d] = v.totalT[d];
+}
+
 int orc_mol_boxes(void *h, int32_t *box) {
   System *s = S(h);
   for (int m = 0; m < s->NumMols(); ++m) box[m] = s->molBox[m];

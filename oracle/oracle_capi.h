@@ -97,6 +97,11 @@ double orc_uneg11(uint64_t v);
 void orc_r123(uint64_t seed, uint64_t step, uint32_t counter, double bound, double out[8]);
 void orc_rotation(double theta, double ax, double ay, double az, double m[9]);
 void orc_uniform_rotation(double u1, double u2, double u3, double m[9]);
+/* CalculateEnergy::VirialCalc (CalculateEnergy.cpp:411-579, with
+ * VirialCorrection and Ewald::VirialReciprocal) and the pressure of
+ * OutputVars.cpp:147-172: out = {inter, tail, real, recip, total,
+ * pressure in bar, total tensor xx, yy, zz}. */
+void orc_virial(void *h, double out[9]);
 /* Pair terms on their own: [lj, coulomb, ljvir, coulvir, lrc]. */
 void orc_pair_terms(const orc_system_desc *d, int t1, int t2, double distSq,
                     double qiqj, double out[5]);

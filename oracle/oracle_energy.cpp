@@ -116,6 +116,19 @@ double ForceField::EnergyLRC(int k1, int k2) const {
   return tc;
 }
 
+// src/FFParticle.cpp:118-138.
+double ForceField::VirialLRC(int k1, int k2) const {
+  int idx = Flat(k1, k2);
+  double tc = 1.0;
+  double sigma = std::sqrt(sigmaSq[idx]);
+  double rRat = sigma / rCut;
+  double N = n[idx];
+  tc *= sigma * sigmaSq[idx];
+  tc *= 2.0 * M_PI * epsilon_cn[idx] / (N - 3.0);
+  tc *= (N * std::pow(rRat, n[idx] - 3.0) - (N - 3.0) * 2.0 * rRat * rRat * rRat);
+  return tc;
+}
+
 // ----------------------------------------------------------------- BoxDims --
 void BoxDims::Set(double lx, double ly, double lz, double rcut) {
   axis = {lx, ly, lz};
@@ -795,6 +808,136 @@ void Ewald::BoxForceReciprocal(const std::vector<double> &x, const std::vector<d
       mFR[m] = mFR[m] + XYZ(X, Y, Z);
     }
   }
+}
+
+}  // namespace orc
+
+namespace orc {
+
+// src/CalculateEnergy.cpp:411-579 (CPU branch, lambda = 1): the molecular
+// virial of every inter-molecular pair inside the box cutoff, the atom
+// separation times the minimum-image centre separation, diagonal terms;
+// then VirialCorrection (1317-1336) and VirialReciprocal.
+VirialD System::VirialCalc() {
+  VirialD v;
+  std::vector<int> cellVector, cellStart, map;
+  cells.GetCellListNeighbor(NumAtoms(), cellVector, cellStart, map);
+  double vT11 = 0.0, vT22 = 0.0, vT33 = 0.0, rT11 = 0.0, rT22 = 0.0, rT33 = 0.0;
+  const int nv = int(cellVector.size());
+#pragma omp parallel for num_threads(threads) schedule(static) \
+    reduction(+ : vT11, vT22, vT33, rT11, rT22, rT33)
+  for (int ci = 0; ci < nv; ci++) {
+    int cur = cellVector[ci];
+    int curCell = map[cur];
+    for (int nc = 0; nc < kNeighborCells; nc++) {
+      int nb = cells.neighbors[curCell][nc];
+      for (int ni = cellStart[nb]; ni < cellStart[nb + 1]; ni++) {
+        int n = cellVector[ni];
+        if (cur < n && particleMol[cur] != particleMol[n]) {
+          double distSq;
+          XYZ virC;
+          if (box.InRcut(distSq, virC, Pos(cur), Pos(n))) {
+            const int mc = particleMol[cur], mn = particleMol[n];
+            XYZ comC = box.MinImage(XYZ(comx[mc] - comx[mn], comy[mc] - comy[mn],
+                                        comz[mc] - comz[mn]));
+            if (ff.electrostatic) {
+              double qi_qj = particleCharge[cur] * particleCharge[n];
+              if (qi_qj != 0.0) {
+                double pRF = ff.CalcCoulombVir(distSq, qi_qj);
+                rT11 += pRF * (virC.x * comC.x);
+                rT22 += pRF * (virC.y * comC.y);
+                rT33 += pRF * (virC.z * comC.z);
+              }
+            }
+            double pVF = ff.CalcVir(distSq, particleType[cur], particleType[n]);
+            vT11 += pVF * (virC.x * comC.x);
+            vT22 += pVF * (virC.y * comC.y);
+            vT33 += pVF * (virC.z * comC.z);
+          }
+        }
+      }
+    }
+  }
+  v.interT[0] = vT11;
+  v.interT[1] = vT22;
+  v.interT[2] = vT33;
+  if (ff.electrostatic) {
+    v.realT[0] = rT11 * kQQFact;
+    v.realT[1] = rT22 * kQQFact;
+    v.realT[2] = rT33 * kQQFact;
+  }
+  v.inter = vT11 + vT22 + vT33;
+  v.real = (rT11 + rT22 + rT33) * kQQFact;
+  if (ff.useLRC) {  // VirialCorrection
+    const int K = int(kinds.size());
+    double vir = 0.0;
+    for (int i = 0; i < K; ++i) {
+      const unsigned numI = unsigned(kindCount[i]);
+      for (int j = 0; j < K; ++j) {
+        const unsigned numJ = unsigned(kindCount[j]);
+        vir += pairVirCorrections[i * K + j] * numI * numJ * box.volInv;
+      }
+    }
+    v.tc = vir;
+  }
+  if (ff.ewald) ewald.VirialReciprocal(v);
+  // Virial::Total (EnergyTypes.h:181-191)
+  v.total = v.inter + v.tc + v.real + v.recip;
+  for (int d = 0; d < 3; ++d) v.totalT[d] = v.interT[d] + v.realT[d] + v.recipT[d];
+  return v;
+}
+
+// OutputVars::CalcAndConvert (OutputVars.cpp:147-172): every virial term
+// divided by the dimensionality and the volume, the ideal part from the
+// number densities, K molecule / A^3 to bar.
+double System::Pressure(const VirialD &v) const {
+  const double inter = v.inter / 3.0 / box.volume, tc = v.tc / 3.0 / box.volume;
+  const double real = v.real / 3.0 / box.volume, recip = v.recip / 3.0 / box.volume;
+  const double total = inter + tc + real + recip;
+  double raw = 0.0;
+  for (size_t k = 0; k < kinds.size(); ++k) raw += kindCount[k] * box.volInv;
+  raw *= ff.T;
+  raw += total;
+  return raw * 138.0649;  // UnitConst.h:31
+}
+
+// src/Ewald.cpp:1168-1305 (CPU branch): the k-space term of every k, then
+// the intramolecular term of every charged atom (its offset from the
+// molecule's centre, the atom unwrapped about it).
+void Ewald::VirialReciprocal(VirialD &v) const {
+  const ForceField &ff = sys.ff;
+  double wT11 = 0.0, wT22 = 0.0, wT33 = 0.0;
+  const double constVal = 1.0 / (4.0 * ff.alphaSq);
+  for (int i = 0; i < imageSizeRef; i++) {
+    double factor = prefactRef[i] * (sumRref[i] * sumRref[i] + sumIref[i] * sumIref[i]);
+    wT11 += factor * (1.0 - 2.0 * (constVal + 1.0 / hsqrRef[i]) * kxRef[i] * kxRef[i]);
+    wT22 += factor * (1.0 - 2.0 * (constVal + 1.0 / hsqrRef[i]) * kyRef[i] * kyRef[i]);
+    wT33 += factor * (1.0 - 2.0 * (constVal + 1.0 / hsqrRef[i]) * kzRef[i] * kzRef[i]);
+  }
+  for (int m = 0; m < sys.NumMols(); m++) {
+    if (!sys.InBox0(m)) continue;
+    const XYZ comC(sys.comx[m], sys.comy[m], sys.comz[m]);
+    for (int p = sys.molStart[m]; p < sys.molStart[m] + sys.molLen[m]; p++) {
+      if (sys.particleHasNoCharge[p]) continue;
+      XYZ atomC(BoxDims::Unwrap1(sys.x[p], comC.x, sys.box.axis.x, sys.box.halfAx.x),
+                BoxDims::Unwrap1(sys.y[p], comC.y, sys.box.axis.y, sys.box.halfAx.y),
+                BoxDims::Unwrap1(sys.z[p], comC.z, sys.box.axis.z, sys.box.halfAx.z));
+      const XYZ diffC(atomC.x - comC.x, atomC.y - comC.y, atomC.z - comC.z);
+      const double charge = sys.particleCharge[p] * 1.0;
+      for (int i = 0; i < imageSizeRef; i++) {
+        double arg = sys.x[p] * kxRef[i] + sys.y[p] * kyRef[i] + sys.z[p] * kzRef[i];
+        double factor = prefactRef[i] * 2.0 *
+                        (sumIref[i] * std::cos(arg) - sumRref[i] * std::sin(arg)) * charge;
+        wT11 += factor * (kxRef[i] * diffC.x);
+        wT22 += factor * (kyRef[i] * diffC.y);
+        wT33 += factor * (kzRef[i] * diffC.z);
+      }
+    }
+  }
+  v.recipT[0] = wT11;
+  v.recipT[1] = wT22;
+  v.recipT[2] = wT33;
+  v.recip = wT11 + wT22 + wT33;
 }
 
 }  // namespace orc

@@ -69,14 +69,19 @@ void System::Build(const SystemDesc &d) {
   // Molecules.cpp:157-175: LRC pair table over molecule kinds.
   const int K = int(kinds.size());
   pairEnCorrections.assign(K * K, 0.0);
+  pairVirCorrections.assign(K * K, 0.0);
   if (ff.useLRC) {
     for (int i = 0; i < K; ++i)
       for (int j = i; j < K; ++j) {
         for (int pI = 0; pI < kinds[i].NumAtoms(); ++pI)
-          for (int pJ = 0; pJ < kinds[j].NumAtoms(); ++pJ)
+          for (int pJ = 0; pJ < kinds[j].NumAtoms(); ++pJ) {
             pairEnCorrections[i * K + j] +=
                 ff.EnergyLRC(kinds[i].atomType[pI], kinds[j].atomType[pJ]);
+            pairVirCorrections[i * K + j] +=
+                ff.VirialLRC(kinds[i].atomType[pI], kinds[j].atomType[pJ]);
+          }
         pairEnCorrections[j * K + i] = pairEnCorrections[i * K + j];
+        pairVirCorrections[j * K + i] = pairVirCorrections[i * K + j];
       }
   }
   // MoleculeLookup::Init (MoleculeLookup.cpp:24-119): [box][kind] lists in

@@ -48,6 +48,7 @@ _SIG = {
     "orc_box_force": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "orc_box_force_recip": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "orc_torque": (None, [_vp, _dp, _dp, _dp]),
+    "orc_virial": (None, [_vp, _dp]),
     "orc_mp_settings": (None, [_vp, _dp]),
     "orc_mol_boxes": (C.c_int, [_vp, _ip]),
     "orc_batch_run": (C.c_double, [C.c_int, C.POINTER(_vp), _lp, C.c_int]),
@@ -238,6 +239,13 @@ class Oracle:
         t = [np.zeros(self.nmols) for _ in range(3)]
         self.L.orc_torque(self.h, *(_p(v) for v in t))
         return np.stack(t, 1)
+
+    def virial(self):
+        """{inter, tail, real, recip, total, pressure (bar), total tensor diagonal}."""
+        out = np.zeros(9)
+        self.L.orc_virial(self.h, _p(out))
+        return dict(inter=out[0], tail=out[1], real=out[2], recip=out[3], total=out[4],
+                    pressure=out[5], tensor=out[6:9].copy())
 
     def mol_boxes(self):
         b = np.zeros(self.nmols, np.int32)
