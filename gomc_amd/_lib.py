@@ -30,6 +30,7 @@ SIGNATURES = {
     "gomc_box_destroy": (C.c_int, [_vp]),
     "gomc_last_error": (C.c_char_p, [_vp]),
     "gomc_box_info": (C.c_int, [_vp, _lp]),
+    "gomc_box_dims": (C.c_int, [_vp, _dp]),
     "gomc_set_coords": (C.c_int, [_vp, _dp, _dp, _dp]),
     "gomc_get_coords": (C.c_int, [_vp, _dp, _dp, _dp]),
     "gomc_commit_mol": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),

@@ -180,6 +180,12 @@ class Box:
         self._check(self.lib.gomc_torque(self.h, *(ptr(v) for v in t)))
         return np.stack(t, 1)
 
+    def dims(self) -> np.ndarray:
+        """Current edge lengths and volume: [Lx, Ly, Lz, V]."""
+        out = np.zeros(4)
+        self._check(self.lib.gomc_box_dims(self.h, ptr(out)))
+        return out
+
     def virial(self) -> dict:
         """Molecular virial (K) and pressure (bar) of the current configuration:
         {inter, tail, real, recip, total, pressure, tensor (xx, yy, zz)}."""

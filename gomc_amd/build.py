@@ -37,7 +37,8 @@ FLAGS = [
     "-Wno-unused-variable",
 ]
 SOURCES = ["box.hip", "box_setup.cpp", "driver.cpp", "mp.hip", "gcmc.cpp", "batch.hip", "batch_setup.cpp", "server.cpp",
-           "placement.cpp", "gcmc_dev.hip", "virial.hip"]
+           "placement.cpp", "gcmc_dev.hip", "virial.hip",
+           "volume.cpp"]
 
 
 def _compile(src: Path) -> Path:

@@ -772,6 +772,13 @@ int gomc_box_info(const gomc_box *b, int64_t out[7]) {
   return GOMC_OK;
 }
 
+int gomc_box_dims(const gomc_box *b, double out[4]) {
+  if (!b || !out) return GOMC_ERR_ARG;
+  for (int i = 0; i < 3; ++i) out[i] = b->geom.L[i];
+  out[3] = b->volume;
+  return GOMC_OK;
+}
+
 int gomc_set_coords(gomc_box *b, const double *x, const double *y, const double *z) {
   if (!b || !x || !y || !z) return GOMC_ERR_ARG;
   (void)hipSetDevice(b->device);

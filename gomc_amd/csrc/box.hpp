@@ -85,6 +85,10 @@ struct MPState {
 struct DriverMoves {
   std::vector<double> scale[2];  // [displace|rotate][kind]
   std::vector<uint64_t> tries[2], accepted[2], tempTries[2], tempAccepted[2];
+  // VOL_TRANSFER (NPT): the reference keeps identical counts for every
+  // kind (MoveSettings.cpp:130-145); one set stands for all of them
+  double volScale = 500.0;  // MoveSettings.cpp:58-64
+  uint64_t volTries = 0, volAccepted = 0, volTempTries = 0, volTempAccepted = 0;
 };
 
 // Running energies, src/EnergyTypes.h:75-129 order.
@@ -187,6 +191,7 @@ struct gomc_box {
   double movePerc[3] = {0.5, 0.5, 0.0};
   uint64_t perAdjust = 1000;
   gomc::DevDriver *dev = nullptr;  // device move loop, allocated on first use
+  double pressure = 0.0;             // NPT: K / A^3
 };
 
 namespace gomc {
@@ -229,6 +234,11 @@ int SwapRecipImpl(gomc_box *b, int kind, int insert, const double *x, const doub
 int CacheBuild(gomc_box *b);
 // Frees the device move loop's buffers.
 void DevFree(gomc_box *b);
+// k-vectors of the current geometry, structure factors of the current
+// coordinates, made the reference sums; *erecip = BoxReciprocal.
+int RecipInitImpl(gomc_box *b, double *erecip);
+// VolumeTransfer of an NPT box (volume.cpp).
+int VolumeMove(gomc_box *b, int &accepted, double dE[3]);
 // Half-space k-vectors of an orthogonal box in the reference's order
 // (box_setup.cpp); `packed` (may be null) gets their index triples.
 void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double volume,

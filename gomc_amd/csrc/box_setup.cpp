@@ -65,7 +65,7 @@ void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double vo
 
 // Upload k-vectors, structure factors from the coordinates, and make them
 // the reference sums (UpdateVectorsAndRecipTerms, Ewald.cpp:129-139).
-static int RecipInitImpl(gomc_box *b, double *erecip) {
+int RecipInitImpl(gomc_box *b, double *erecip) {
   std::vector<double> kx, ky, kz, pref;
   KVectorsOrth(b->geom.L, b->alpha * b->alpha, b->recipRcut, b->volume, kx, ky, kz, pref, nullptr);
   const int K = int(kx.size());
@@ -213,8 +213,8 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   }
   b->nmols = d->n_mols;
   b->kindCount.assign(b->nkinds, 0);
-  b->ensemble = d->ensemble == 1 ? 1 : 0;
-  if (d->ensemble != 0 && d->ensemble != 1) return fail(GOMC_ERR_ARG, "ensemble must be 0 or 1");
+  b->ensemble = d->ensemble;
+  if (d->ensemble < 0 || d->ensemble > 2) return fail(GOMC_ERR_ARG, "ensemble must be 0, 1 or 2");
   b->molBox.assign(b->nmols, 0);
   if (b->ensemble == 1 && d->mol_box)
     for (int m = 0; m < b->nmols; ++m) {
@@ -367,7 +367,8 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   b->pot.Total();
   // ---- driver: move mix (StaticVals.cpp:37-96), MoveSettings::Init ----
   const double swapW = b->ensemble == 1 ? d->move_swap : 0.0;
-  const double tot = d->move_disp + d->move_rot + d->move_mp + swapW;
+  const double volW = b->ensemble == 2 ? d->move_vol : 0.0;
+  const double tot = d->move_disp + d->move_rot + d->move_mp + swapW + volW;
   if (!(tot > 0)) return fail(GOMC_ERR_ARG, "no moves enabled");
   if (b->ensemble == 1 && d->move_mp > 0)
     return fail(GOMC_ERR_ARG, "MultiParticle moves are not supported with GCMC");
@@ -379,6 +380,11 @@ int gomc_box_create(const gomc_system_desc *d, int device, gomc_box **out) {
   b->moveW[1] = b->movePerc[1];
   b->moveW[2] = b->movePerc[2];
   b->moveW[10] = swapW / tot;
+  if (b->ensemble == 2) {  // NPT: VOL_TRANSFER is move kind 9 (MoveConst.h:52-54)
+    if (b->cached) return fail(GOMC_ERR_ARG, "volume moves are not supported with the molecule cache");
+    b->moveW[9] = volW / tot;
+  }
+  b->pressure = d->pressure * 0.007242971;  // bar -> K / A^3 (ConfigSetup.cpp:387-389)
   // GCMC: fugacities (bar -> K/A^3, UnitConst.h:44), CBMC trial counts, the
   // molecule lookup and the growth geometry of three-site kinds.
   b->fugacity.assign(b->nkinds, 0.0);

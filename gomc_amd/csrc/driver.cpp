@@ -13,7 +13,7 @@ using namespace gomc;
 
 namespace {
 
-constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2, kMolTransfer = 10;
+constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2, kVolTransfer = 9, kMolTransfer = 10;
 
 // MoveSettings::AdjustMoves / Adjust (src/MoveSettings.cpp:148-162,205-258).
 void AdjustMoves(gomc_box *b, uint64_t step) {
@@ -34,6 +34,21 @@ void AdjustMoves(gomc_box *b, uint64_t step) {
       b->moves.tempAccepted[mv][k] = 0;
       b->moves.tempTries[mv][k] = 0;
     }
+  if (b->ensemble == 2) {  // MoveSettings::Adjust, VOL_TRANSFER (MoveSettings.cpp:241-259)
+    DriverMoves &mv = b->moves;
+    if (mv.volTempTries > 0) {
+      const double acc = double(mv.volTempAccepted) / double(mv.volTempTries);
+      const double fract = acc / 0.5;
+      mv.volScale *= (fract > 0.0) ? fract : 0.5;
+    }
+    const double rc = std::max(b->rCutLJ, b->rCutCoul);
+    const double minVol = 8.0 * (rc * rc) * rc + 0.001;  // BoxDimensions.cpp:19
+    const double hi = 0.34 * (b->volume - minVol);
+    if (mv.volScale < 0.001) mv.volScale = 0.001;
+    if (mv.volScale > hi) mv.volScale = hi;
+    mv.volTempAccepted = 0;
+    mv.volTempTries = 0;
+  }
   // MoveSettings::AdjustMultiParticle (src/MoveSettings.cpp:164-200): the
   // MultiParticle target is mp::TARGET_ACCEPT_FRACT = 0.3 (MoveSettings.h:22),
   // the tolerance 0.1 and the averaging weight 0.2.
@@ -68,7 +83,7 @@ int OneMove(gomc_box *b, uint64_t step, Record &r) {
   // PRNG::PickArbDist over the move kinds (src/PRNG.h:153-163): nine in
   // NVT, thirteen in GCMC (src/MoveConst.h:31-40).
   const double *w = b->moveW;
-  const uint32_t nKinds = b->ensemble == 1 ? 13 : 9;
+  const uint32_t nKinds = b->ensemble == 1 ? 13 : (b->ensemble == 2 ? 10 : 9);
   const double draw0 = b->mt.Closed() * 1.0;
   double sum = 0.0, prev = 0.0;
   uint32_t pick = 0;
@@ -88,6 +103,20 @@ int OneMove(gomc_box *b, uint64_t step, Record &r) {
     r.mol = type;
     r.acc = acc;
     r.ovl = failed;
+    r.dlj = dE[0];
+    r.dreal = dE[1];
+    r.drecip = dE[2];
+    return 0;
+  }
+  if (pick == kVolTransfer && b->ensemble == 2) {
+    int acc = 0;
+    double dE[3];
+    int rc = VolumeMove(b, acc, dE);
+    if (rc) return rc;
+    r.kind = kVolTransfer;
+    r.mol = -1;
+    r.acc = acc;
+    r.ovl = 0;
     r.dlj = dE[0];
     r.dreal = dE[1];
     r.drecip = dE[2];

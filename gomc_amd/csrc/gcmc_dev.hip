@@ -1489,6 +1489,8 @@ int DevCheck(gomc_box *b) {
   if (b->cached) return Fail(b, GOMC_ERR_STATE, "device move loop: EwaldCached mode is not supported");
   if (b->moveW[2] > 0.0)
     return Fail(b, GOMC_ERR_STATE, "device move loop: MultiParticle moves run on the host driver");
+  if (b->ensemble == 2)
+    return Fail(b, GOMC_ERR_STATE, "device move loop: NPT volume moves run on the host driver");
   if (b->nkinds > kDevKinds) return Fail(b, GOMC_ERR_ARG, "device move loop: more than 4 molecule kinds");
   for (int k = 0; k < b->nkinds; ++k) {
     const int len = int(b->kindType[k].size());

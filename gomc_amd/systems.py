@@ -63,6 +63,8 @@ class SystemDesc(C.Structure):
         ("mol_box", _ip),
         ("kind_fugacity", _dp),
         ("move_swap", C.c_double),
+        ("pressure", C.c_double),
+        ("move_vol", C.c_double),
     ]
 
 
@@ -108,6 +110,8 @@ class System:
     cbmc_first: int = 16
     cbmc_nth: int = 8
     cbmc_ang: int = 100
+    pressure: float = 1.0  # bar (NPT)
+    move_vol: float = 0.0  # VolFreq (NPT)
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -164,6 +168,8 @@ class System:
         d.mol_box = mb.ctypes.data_as(_ip)
         d.kind_fugacity = fug.ctypes.data_as(_dp)
         d.move_swap = float(self.move_swap)
+        d.pressure = float(self.pressure)
+        d.move_vol = float(self.move_vol)
         return d
 
 

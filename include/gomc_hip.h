@@ -39,6 +39,7 @@ extern "C" {
  *   adj_steps -- AdjSteps
  *   ensemble, cbmc_*, mol_box, kind_fugacity, move_swap -- GCMC settings
  *                 (Fugacity, CBMC_First/Nth/Ang, SwapFreq)
+ *   pressure, move_vol -- NPT settings (Pressure in bar, VolFreq)
  * The same layout is used by the CPU oracle's orc_system_desc. */
 typedef struct gomc_system_desc {
   double box[3];
@@ -59,11 +60,13 @@ typedef struct gomc_system_desc {
   /* GCMC (ensemble 1): molecules with mol_box[m] == 0 are in the simulation
    * box; mol_box[m] == 1 puts a molecule in the non-interacting reservoir
    * (GOMC's second GCMC box), given the same edge lengths. */
-  int32_t ensemble;                        /* 0 NVT, 1 GCMC */
+  int32_t ensemble;                        /* 0 NVT, 1 GCMC, 2 NPT */
   int32_t cbmc_first, cbmc_nth, cbmc_ang;  /* CBMC_First, CBMC_Nth, CBMC_Ang */
   const int32_t *mol_box;                  /* [n_mols]; NULL: all in box 0 */
   const double *kind_fugacity;             /* [n_kinds] Fugacity, bar */
   double move_swap;                        /* SwapFreq */
+  /* NPT (ensemble 2): Pressure (bar) and VolFreq. */
+  double pressure, move_vol;
 } gomc_system_desc;
 
 typedef struct gomc_box gomc_box;
@@ -79,6 +82,9 @@ int gomc_box_destroy(gomc_box *b);
 const char *gomc_last_error(const gomc_box *b);
 /* Sizes: [n_atoms, n_mols, n_kvec, n_cells, cells_per_edge_x, y, z]. */
 int gomc_box_info(const gomc_box *b, int64_t out[7]);
+/* Current edge lengths and volume, A and A^3 (BoxDimensions::axis, volume;
+ * they change with NPT volume moves): out = {Lx, Ly, Lz, V}. */
+int gomc_box_dims(const gomc_box *b, double out[4]);
 
 /* ---- coordinates ----------------------------------------------------- */
 /* Replaces the per-call coordinate uploads inside every Call*GPU

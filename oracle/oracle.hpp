@@ -290,6 +290,10 @@ struct MoveSettings {  // src/MoveSettings.cpp:19-258 (displace/rotate)
   uint64_t mp_tries[2] = {0, 0}, mp_accepted[2] = {0, 0};
   uint64_t mp_int_tries[2] = {0, 0}, mp_int_acc[2] = {0, 0};
   bool singleMoveAccepted = true;
+  // VOL_TRANSFER (NPT): the same counts are kept for every kind
+  // (MoveSettings.cpp:130-145), so one set stands for all of them.
+  double volScale = 500.0;  // MoveSettings.cpp:58-64
+  uint64_t volTries = 0, volAccepted = 0, volTempTries = 0, volTempAccepted = 0;
 };
 
 // Record of one MC move, for per-move parity.
@@ -396,6 +400,8 @@ class System {
   void ShiftMolBox(int m, int from, int to, int k);  // MoleculeLookup.cpp:165-198
   void SetNewCOM(int m);                              // COM.h:63-88
   int RunMoleculeTransfer(double subDraw, MoveRecord *rec);  // MoleculeTransfer.h
+  int RunVolumeMove(MoveRecord *rec);                         // VolumeTransfer.h (NPT)
+  double pressure = 0.0;  // K / A^3 (bar * 0.007242971, ConfigSetup.cpp:387-389)
   uint64_t perAdjust = 1000;
   int threads = 1;
 

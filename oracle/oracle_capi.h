@@ -30,11 +30,13 @@ typedef struct orc_system_desc {
   /* GCMC (ensemble 1): molecules with mol_box[m] == 0 are in the simulation
    * box; mol_box[m] == 1 puts a molecule in the non-interacting reservoir
    * (GOMC's second GCMC box), given the same edge lengths. */
-  int32_t ensemble;                        /* 0 NVT, 1 GCMC */
+  int32_t ensemble;                        /* 0 NVT, 1 GCMC, 2 NPT */
   int32_t cbmc_first, cbmc_nth, cbmc_ang;  /* CBMC_First, CBMC_Nth, CBMC_Ang */
   const int32_t *mol_box;                  /* [n_mols]; NULL: all in box 0 */
   const double *kind_fugacity;             /* [n_kinds] Fugacity, bar */
   double move_swap;                        /* SwapFreq */
+  /* NPT (ensemble 2): Pressure (bar) and VolFreq. */
+  double pressure, move_vol;
 } orc_system_desc;
 
 void *orc_create(const orc_system_desc *d);

@@ -9,6 +9,7 @@ namespace orc {
 
 namespace {
 constexpr int kDisplace = 0, kRotate = 1, kMultiParticle = 2, kMolTransfer = 10;
+constexpr int kVolTransfer = 9;  // NPT (MoveConst.h:52-54)
 constexpr double kTinyAmount = 1.0e-7;      // MoveSettings.cpp:14
 constexpr double kTargetAccept = 0.5;       // MoveSettings.cpp:13
 constexpr uint64_t kPhiloxTag = 0x474F4D43ull;  // key word 1 of the batched streams
@@ -142,7 +143,8 @@ void System::Build(const SystemDesc &d) {
   // StaticVals.cpp:37-96: move percentages normalised to one, summed in
   // move-kind order.
   const double swapW = d.ensemble == 1 ? d.move_swap : 0.0;
-  double tot = d.move_disp + d.move_rot + d.move_mp + swapW;
+  const double volW = d.ensemble == 2 ? d.move_vol : 0.0;
+  double tot = d.move_disp + d.move_rot + d.move_mp + swapW + volW;
   movePerc[0] = d.move_disp / tot;
   movePerc[1] = d.move_rot / tot;
   movePerc[2] = d.move_mp / tot;
@@ -151,6 +153,8 @@ void System::Build(const SystemDesc &d) {
   moveW[1] = movePerc[1];
   moveW[2] = movePerc[2];
   moveW[10] = swapW / tot;
+  if (d.ensemble == 2) moveW[kVolTransfer] = volW / tot;
+  pressure = d.pressure * 0.007242971;
   ms.swapTries.assign(2 * K, 0);
   ms.swapAccepted.assign(2 * K, 0);
   perAdjust = d.adj_steps > 0 ? uint64_t(d.adj_steps) : 1000;
@@ -239,6 +243,20 @@ void System::AdjustMoves(uint64_t step) {
       ms.tempAccepted[mv][k] = 0;
       ms.tempTries[mv][k] = 0;
     }
+  if (ensemble == 2) {  // MoveSettings::Adjust, VOL_TRANSFER (MoveSettings.cpp:241-259)
+    if (ms.volTempTries > 0) {
+      double currentAccept = double(ms.volTempAccepted) / double(ms.volTempTries);
+      double fract = currentAccept / kTargetAccept;
+      if (fract > 0.0)
+        ms.volScale *= fract;
+      else
+        ms.volScale *= 0.5;
+    }
+    const double minVol = 8.0 * box.rCutSq * box.rCut + 0.001;  // BoxDimensions.cpp:19
+    Bound(ms.volScale, 0.001, 0.34 * (box.volume - minVol));
+    ms.volTempAccepted = 0;
+    ms.volTempTries = 0;
+  }
   // MoveSettings::AdjustMultiParticle (MoveSettings.cpp:164-200); the
   // MultiParticle target acceptance is mp::TARGET_ACCEPT_FRACT = 0.3
   // (MoveSettings.h:22), the tolerance 0.1 and both alphas 0.2.
@@ -348,10 +366,11 @@ int System::RunStepMT(uint64_t step, MoveRecord *rec) {
   (void)step;
   uint32_t kind = 0;
   double draw = 0;
-  // MOVE_KINDS_TOTAL: 9 kinds in NVT, 13 in GCMC (MoveConst.h:31-40).
-  prng.PickArbDist(kind, draw, moveW, 1.0, ensemble == 1 ? 13 : 9);
+  // MOVE_KINDS_TOTAL: 9 kinds in NVT, 13 in GCMC, 10 in NPT (MoveConst.h:31-56).
+  prng.PickArbDist(kind, draw, moveW, 1.0, ensemble == 1 ? 13 : (ensemble == 2 ? 10 : 9));
   if (kind == kMultiParticle) return ensemble == 1 ? -1 : RunMultiParticle(step, rec);
   if (kind == kMolTransfer) return RunMoleculeTransfer(draw, rec);
+  if (kind == kVolTransfer && ensemble == 2) return RunVolumeMove(rec);
   if (kind != kDisplace && kind != kRotate) return -1;
   // PickMolAndBox with one box: boxDiv = movPerc, subDraw unchanged.
   double subPerc = movePerc[kind];
@@ -627,6 +646,125 @@ int System::RunStepPhilox(uint64_t step, MoveRecord *rec) {
   XYZ axis(root * std::cos(phi), root * std::sin(phi), cu);
   RotateCoords(*this, m, theta, axis, newPos);
   return EvaluateAndAccept(*this, kRotate, m, newPos, nullptr, u[5], rec);
+}
+
+}  // namespace orc
+
+namespace orc {
+
+// VolumeTransfer for NPT (src/moves/VolumeTransfer.h:74-310) with one box:
+// PickBox draws nothing; Transform draws delta = Sym(scale) and scales the
+// box (BoxDimensions::ShiftVolume/SetVolume, BoxDimensions.cpp:101-123,
+// 215-227) and every molecule's centre (Coordinates::TranslateOneBox,
+// Coordinates.cpp:149-180); CalcEn regrids, rebuilds the k-vectors and the
+// structure factors for the new box and recomputes BoxInter and the tail
+// term; Accept uses (V'/V)^N exp(-beta P dV) exp(-beta dU).
+int System::RunVolumeMove(MoveRecord *rec) {
+  const double delta = prng.Sym(ms.volScale);
+  const double oldVol = box.volume, newVol = box.volume + delta;
+  BoxDims nb = box;
+  const double ratio = std::cbrt(newVol / box.volume);
+  nb.axis = {box.axis.x * ratio, box.axis.y * ratio, box.axis.z * ratio};
+  nb.halfAx = {box.halfAx.x * ratio, box.halfAx.y * ratio, box.halfAx.z * ratio};
+  nb.volume = newVol;
+  nb.volInv = 1.0 / newVol;
+  const double minVol = 8.0 * box.rCutSq * box.rCut + 0.001;
+  if (nb.halfAx.x < box.rCut || nb.halfAx.y < box.rCut || nb.halfAx.z < box.rCut ||
+      newVol < minVol)
+    return -1;  // the reference stops: box shrunk below 2 Rcut
+  const XYZ scale(nb.axis.x / box.axis.x, nb.axis.y / box.axis.y, nb.axis.z / box.axis.z);
+  // keep the current state for a rejection
+  const std::vector<double> ox = x, oy = y, oz = z, ocx = comx, ocy = comy, ocz = comz;
+  const BoxDims ob = box;
+  const std::vector<double> okx = ewald.kx, oky = ewald.ky, okz = ewald.kz, oh = ewald.hsqr,
+                            op = ewald.prefact, osr = ewald.sumRnew, osi = ewald.sumInew;
+  const int oImage = ewald.imageSize, oKmax = ewald.kmax;
+  for (int m = 0; m < NumMols(); ++m) {
+    if (!InBox0(m)) continue;
+    const XYZ oldC(comx[m], comy[m], comz[m]);
+    const XYZ newC(oldC.x * scale.x, oldC.y * scale.y, oldC.z * scale.z);
+    const XYZ shift(newC.x - oldC.x, newC.y - oldC.y, newC.z - oldC.z);
+    comx[m] = newC.x;
+    comy[m] = newC.y;
+    comz[m] = newC.z;
+    for (int p = molStart[m]; p < molStart[m] + molLen[m]; ++p) {
+      XYZ u = ob.Unwrap(Pos(p), oldC);
+      u = {u.x + shift.x, u.y + shift.y, u.z + shift.z};
+      const XYZ w = nb.Wrap(u);
+      x[p] = w.x;
+      y[p] = w.y;
+      z[p] = w.z;
+    }
+  }
+  box = nb;
+  cells.Resize(box, box.rCut);
+  GridAll();
+  Energy potNew = pot;
+  if (ff.ewald) {
+    ewald.RecipInitOrth();
+    ewald.BoxReciprocalSetup();
+    potNew.recip = ewald.BoxReciprocal(true);
+  }
+  BoxInter(potNew.inter, potNew.real);
+  if (ff.useLRC) potNew.tailCorrection = EnergyCorrection();
+  potNew.Total();
+  const double coeff = std::pow(newVol / oldVol, double(NumInBox(0))) *
+                       std::exp(-ff.beta * pressure * (newVol - oldVol));
+  const double uBoltz = std::exp(-ff.beta * (potNew.Total() - pot.Total()));
+  const double accept = coeff * uBoltz;
+  const bool result = prng() < accept;
+  if (rec) {
+    rec->kind = kVolTransfer;
+    rec->mol = -1;
+    rec->accepted = result;
+    rec->overlap = 0;
+    rec->dLJ = potNew.inter - pot.inter;
+    rec->dReal = potNew.real - pot.real;
+    rec->dRecip = potNew.recip - pot.recip;
+  }
+  if (result) {
+    pot = potNew;
+    if (ff.ewald) {  // UpdateRecip + UpdateRecipVec
+      ewald.UpdateRecip();
+      ewald.kxRef = ewald.kx;
+      ewald.kyRef = ewald.ky;
+      ewald.kzRef = ewald.kz;
+      ewald.hsqrRef = ewald.hsqr;
+      ewald.prefactRef = ewald.prefact;
+      ewald.imageSizeRef = ewald.imageSize;
+      // the reference sizes these arrays for the largest k count
+      // (Ewald.cpp:166); here the trial arrays take the new count
+      ewald.sumRnew.resize(ewald.imageSizeRef);
+      ewald.sumInew.resize(ewald.imageSizeRef);
+    }
+    ms.singleMoveAccepted = true;
+  } else {
+    x = ox;
+    y = oy;
+    z = oz;
+    comx = ocx;
+    comy = ocy;
+    comz = ocz;
+    box = ob;
+    cells.Resize(box, box.rCut);
+    GridAll();
+    ewald.kx = okx;
+    ewald.ky = oky;
+    ewald.kz = okz;
+    ewald.hsqr = oh;
+    ewald.prefact = op;
+    ewald.sumRnew = osr;
+    ewald.sumInew = osi;
+    ewald.imageSize = oImage;
+    ewald.kmax = oKmax;
+  }
+  ms.volTries++;
+  ms.volTempTries++;
+  if (result) {
+    ms.volTempAccepted++;
+    ms.volAccepted++;
+  }
+  return int(result);
 }
 
 }  // namespace orc

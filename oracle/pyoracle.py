@@ -240,6 +240,11 @@ class Oracle:
         self.L.orc_torque(self.h, *(_p(v) for v in t))
         return np.stack(t, 1)
 
+    def volume(self):
+        out = np.zeros(6)
+        self.L.orc_cutoffs(self.h, _p(out))
+        return out[4]
+
     def virial(self):
         """{inter, tail, real, recip, total, pressure (bar), total tensor diagonal}."""
         out = np.zeros(9)

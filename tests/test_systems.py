@@ -54,4 +54,6 @@ def test_desc_layout():
     assert systems.SystemDesc.ensemble.offset == 216
     assert systems.SystemDesc.mol_box.offset == 232
     assert systems.SystemDesc.move_swap.offset == 248
-    assert ctypes.sizeof(systems.SystemDesc) == 256
+    assert systems.SystemDesc.pressure.offset == 256
+    assert systems.SystemDesc.move_vol.offset == 264
+    assert ctypes.sizeof(systems.SystemDesc) == 272
