@@ -71,6 +71,7 @@ SIGNATURES = {
     "gomc_batch_last_error": (C.c_char_p, [_vp]),
     "gomc_batch_run": (C.c_int, [_vp, _lp]),
     "gomc_batch_stats": (C.c_int, [_vp, _dp, _lp, _lp]),
+    "gomc_batch_exchange": (C.c_int, [_vp, _ip, C.c_int, C.c_int64, C.c_uint32, _ip]),
     "gomc_batch_get_coords": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "gomc_batch_box_layout": (C.c_int, [_vp, C.c_int, _ip]),
     "gomc_batch_box_times": (C.c_int, [_vp, _dp]),

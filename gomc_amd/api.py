@@ -345,6 +345,16 @@ class Batch:
         self._check(self.lib.gomc_batch_stats(self.h, ptr(e), ptr(acc), ptr(done)))
         return e, acc, done
 
+    def exchange(self, ladder: np.ndarray, attempt: int, seed: int = 0) -> np.ndarray:
+        """One replica-exchange attempt over the boxes listed by temperature
+        slot in `ladder` (int32, updated in place); returns accepted[s] for
+        the pairs (s-1, s)."""
+        assert ladder.dtype == np.int32 and ladder.flags.c_contiguous
+        acc = np.zeros(len(ladder), np.int32)
+        self._check(self.lib.gomc_batch_exchange(self.h, ptr(ladder), len(ladder), attempt, seed,
+                                                 ptr(acc)))
+        return acc
+
     def coords(self, box: int) -> np.ndarray:
         n = self.systems[box].natoms
         x, y, z = (np.zeros(n) for _ in range(3))

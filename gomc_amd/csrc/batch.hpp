@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/gomc_hip.h"
+#include "host_rng.hpp"
 
 namespace gomc {
 
@@ -150,6 +151,8 @@ struct gomc_batch {
   // one MC cycle (N_i moves per box) captured as a graph, replayed per cycle
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
+  // replica exchange: the stream of exchange draws (seeded at attempt 0)
+  gomc::MTRand rexMt{5489u};
 };
 
 namespace gomc {

@@ -617,6 +617,73 @@ int gomc_batch_stats(const gomc_batch *bt_, double *energies, int64_t *accepted,
   return GOMC_OK;
 }
 
+// Replica exchange between boxes of one batch: the neighbour exchange of
+// ParallelTemperingUtilities::evaluateExchangeCriteria
+// (src/ParallelTemperingUtilities.cpp:108-240, the non-multi branch), the
+// MPI all-reduce of energies and the coordinate/centre exchange
+// (exchangePositions/COMsNonBlocking, 406-448) replaced by an exchange of
+// the temperature-bound state between the two boxes: beta, the move scales
+// and counters and the Philox stream, so each configuration keeps its
+// coordinates, structure factors and energies in HBM and moves to the other
+// temperature without a copy.  ladder[s] is the box running the s-th
+// temperature; pairs (s-1, s) with s % 2 == attempt % 2 are tried, each with
+// one draw u < exp((beta_s - beta_{s-1}) (E_s - E_{s-1})), E the total
+// energy of the configuration at that temperature.
+int gomc_batch_exchange(gomc_batch *bt, int32_t *ladder, int n, int64_t attempt, uint32_t seed,
+                        int32_t *accepted) {
+  if (!bt || !ladder || n < 2 || attempt < 0) return GOMC_ERR_ARG;
+  BHIP(bt, hipSetDevice(bt->device));
+  std::vector<char> seen(bt->nboxes, 0);
+  for (int s = 0; s < n; ++s) {
+    const int b = ladder[s];
+    if (b < 0 || b >= bt->nboxes || seen[b]) return BFail(bt, GOMC_ERR_ARG, "ladder must list distinct boxes");
+    seen[b] = 1;
+    const gomc::BatchBox &P = bt->hboxes[b], &Q = bt->hboxes[ladder[0]];
+    if (P.natoms != Q.natoms || P.nmols != Q.nmols || P.L[0] != Q.L[0] || P.L[1] != Q.L[1] ||
+        P.L[2] != Q.L[2])
+      return BFail(bt, GOMC_ERR_ARG, "replicas must have the same molecules and box");
+  }
+  if (attempt == 0) bt->rexMt.Seed(seed);
+  std::vector<gomc::BatchState> st(bt->nboxes);
+  BHIP(bt, hipMemcpy(st.data(), bt->state, sizeof(gomc::BatchState) * bt->nboxes,
+                     hipMemcpyDeviceToHost));
+  std::vector<double> beta(n), energy(n);
+  for (int s = 0; s < n; ++s) {
+    beta[s] = bt->hboxes[ladder[s]].beta;
+    energy[s] = st[ladder[s]].energy[6];
+  }
+  const int parity = int(attempt % 2);
+  bool any = false;
+  for (int s = 1; s < n; ++s) {
+    int ok = 0;
+    if (s % 2 == parity) {
+      const double uBoltz = std::exp((beta[s] - beta[s - 1]) * (energy[s] - energy[s - 1]));
+      ok = bt->rexMt.Closed() < uBoltz;
+    }
+    if (accepted) accepted[s] = ok;
+    if (!ok) continue;
+    any = true;
+    gomc::BatchBox &A = bt->hboxes[ladder[s - 1]], &B = bt->hboxes[ladder[s]];
+    std::swap(A.beta, B.beta);
+    std::swap(A.seed, B.seed);
+    gomc::BatchState &SA = st[ladder[s - 1]], &SB = st[ladder[s]];
+    std::swap(SA.scale, SB.scale);
+    std::swap(SA.tries, SB.tries);
+    std::swap(SA.accepted, SB.accepted);
+    std::swap(SA.tempTries, SB.tempTries);
+    std::swap(SA.tempAccepted, SB.tempAccepted);
+    std::swap(ladder[s - 1], ladder[s]);
+  }
+  if (accepted) accepted[0] = 0;
+  if (any) {
+    BHIP(bt, hipMemcpy(bt->boxes, bt->hboxes.data(), sizeof(gomc::BatchBox) * bt->nboxes,
+                       hipMemcpyHostToDevice));
+    BHIP(bt, hipMemcpy(bt->state, st.data(), sizeof(gomc::BatchState) * bt->nboxes,
+                       hipMemcpyHostToDevice));
+  }
+  return GOMC_OK;
+}
+
 int gomc_batch_get_coords(const gomc_batch *bt_, int box, double *x, double *y, double *z) {
   gomc_batch *bt = const_cast<gomc_batch *>(bt_);
   if (!bt || box < 0 || box >= bt->nboxes) return GOMC_ERR_ARG;

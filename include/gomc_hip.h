@@ -320,6 +320,19 @@ int gomc_batch_run(gomc_batch *bt, const int64_t *moves);
  * With use_graph the cycle is captured once as a HIP graph and replayed;
  * gomc_batch_last_kernel_ms then covers all cycles. */
 int gomc_batch_run_cycles(gomc_batch *bt, int ncycles, int use_graph);
+/* Replica exchange between boxes of a batch (parallel tempering on one
+ * device): ParallelTemperingUtilities::evaluateExchangeCriteria's
+ * neighbour exchange (src/ParallelTemperingUtilities.cpp:108-240) with the
+ * MPI exchange of coordinates (406-448) replaced by an exchange of the
+ * temperature-bound state (beta, move scales and counters, Philox stream)
+ * between the two boxes.  ladder[0..n) lists the boxes running temperature
+ * slots 0..n-1 (same molecules and box edges); pairs (s-1, s) with
+ * s % 2 == attempt % 2 are tried with one draw each from the batch's
+ * exchange stream (MT19937 seeded with `seed` at attempt 0).  accepted[s]
+ * (may be NULL) is 1 when slots s-1 and s exchanged; ladder is updated so
+ * that ladder[s] is again the box at slot s's temperature. */
+int gomc_batch_exchange(gomc_batch *bt, int32_t *ladder, int n, int64_t attempt,
+                        uint32_t seed, int32_t *accepted);
 /* Per-box running energies [nboxes][7], accepted counts [nboxes], moves
  * made [nboxes]; any NULL. */
 int gomc_batch_stats(const gomc_batch *bt, double *energies,

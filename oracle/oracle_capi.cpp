@@ -265,6 +265,51 @@ void orc_virial(void *h, double *out) {
   for (int d = 0; d < 3; ++d) out[6 + d] = v.totalT[d];
 }
 
+// ParallelTemperingUtilities::evaluateExchangeCriteria, neighbour branch
+// (src/ParallelTemperingUtilities.cpp:108-240), over systems in one process:
+// pairs (s-1, s) with s % 2 == attempt % 2, one draw each from MT19937
+// seeded with `seed` after `drawsBefore` earlier draws; an accepted pair
+// exchanges the temperature-bound state (beta and T, move scales and
+// counters, the Philox key) of the two systems, which is the same as
+// exchanging their configurations, and swaps the ladder entries.  Returns
+// the draws made.
+int orc_exchange(void *const *hs, int32_t *ladder, int n, int64_t attempt, uint32_t seed,
+                 int64_t drawsBefore, int32_t *accepted) {
+  orc::MTRand g(seed);
+  for (int64_t i = 0; i < drawsBefore; ++i) (void)g.randInt();
+  std::vector<double> beta(n), energy(n);
+  for (int s = 0; s < n; ++s) {
+    beta[s] = S(hs[ladder[s]])->ff.beta;
+    energy[s] = S(hs[ladder[s]])->pot.total;
+  }
+  const int parity = int(attempt % 2);
+  int draws = 0;
+  for (int s = 1; s < n; ++s) {
+    int ok = 0;
+    if (s % 2 == parity) {
+      const double uBoltz = std::exp((beta[s] - beta[s - 1]) * (energy[s] - energy[s - 1]));
+      ok = g.rand() < uBoltz;
+      ++draws;
+    }
+    accepted[s] = ok;
+    if (!ok) continue;
+    System *a = S(hs[ladder[s - 1]]), *b = S(hs[ladder[s]]);
+    std::swap(a->ff.beta, b->ff.beta);
+    std::swap(a->ff.T, b->ff.T);
+    std::swap(a->philoxSeed, b->philoxSeed);
+    for (int mv = 0; mv < 2; ++mv) {
+      std::swap(a->ms.scale[mv], b->ms.scale[mv]);
+      std::swap(a->ms.tries[mv], b->ms.tries[mv]);
+      std::swap(a->ms.accepted[mv], b->ms.accepted[mv]);
+      std::swap(a->ms.tempTries[mv], b->ms.tempTries[mv]);
+      std::swap(a->ms.tempAccepted[mv], b->ms.tempAccepted[mv]);
+    }
+    std::swap(ladder[s - 1], ladder[s]);
+  }
+  accepted[0] = 0;
+  return draws;
+}
+
 int orc_mol_boxes(void *h, int32_t *box) {
   System *s = S(h);
   for (int m = 0; m < s->NumMols(); ++m) box[m] = s->molBox[m];

@@ -104,6 +104,10 @@ void orc_uniform_rotation(double u1, double u2, double u3, double m[9]);
  * OutputVars.cpp:147-172: out = {inter, tail, real, recip, total,
  * pressure in bar, total tensor xx, yy, zz}. */
 void orc_virial(void *h, double out[9]);
+/* Replica exchange over systems by temperature slot (ParallelTempering-
+ * Utilities.cpp:108-240, neighbour branch): see oracle_capi.cpp. */
+int orc_exchange(void *const *hs, int32_t *ladder, int n, int64_t attempt, uint32_t seed,
+                 int64_t drawsBefore, int32_t *accepted);
 /* Pair terms on their own: [lj, coulomb, ljvir, coulvir, lrc]. */
 void orc_pair_terms(const orc_system_desc *d, int t1, int t2, double distSq,
                     double qiqj, double out[5]);

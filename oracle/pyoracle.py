@@ -49,6 +49,7 @@ _SIG = {
     "orc_box_force_recip": (None, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "orc_torque": (None, [_vp, _dp, _dp, _dp]),
     "orc_virial": (None, [_vp, _dp]),
+    "orc_exchange": (C.c_int, [C.POINTER(_vp), _ip, C.c_int, C.c_int64, C.c_uint32, C.c_int64, _ip]),
     "orc_mp_settings": (None, [_vp, _dp]),
     "orc_mol_boxes": (C.c_int, [_vp, _ip]),
     "orc_batch_run": (C.c_double, [C.c_int, C.POINTER(_vp), _lp, C.c_int]),
@@ -321,3 +322,13 @@ def pair_terms(system: System, t1, t2, dist_sq, qiqj):
     o = np.zeros(5)
     lib().orc_pair_terms(C.byref(d), t1, t2, dist_sq, qiqj, _p(o))
     return o
+
+
+def exchange(oracles: list, ladder: np.ndarray, attempt: int, seed: int, draws_before: int):
+    """Replica exchange over oracle systems (see orc_exchange); returns
+    (accepted per slot, draws made); `ladder` (int32) is updated in place."""
+    L = lib()
+    hs = (C.c_void_p * len(oracles))(*[o.h for o in oracles])
+    acc = np.zeros(len(ladder), np.int32)
+    d = L.orc_exchange(hs, _p(ladder), len(ladder), attempt, seed, draws_before, _p(acc))
+    return acc, d
