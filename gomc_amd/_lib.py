@@ -53,6 +53,8 @@ SIGNATURES = {
     "gomc_box_force_recip": (C.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "gomc_torque": (C.c_int, [_vp, _dp, _dp, _dp]),
     "gomc_box_virial": (C.c_int, [_vp, _dp]),
+    "gomc_box_checkpoint": (C.c_int64, [_vp, C.c_int64, _vp, C.c_int64]),
+    "gomc_box_restore": (C.c_int, [_vp, _vp, C.c_int64, _lp]),
     "gomc_mp_trial": (C.c_int, [_vp, C.c_int, C.c_uint64, C.c_double, _dp]),
     "gomc_mp_accept": (C.c_int, [_vp, C.c_int, _dp]),
     "gomc_mp_settings": (C.c_int, [_vp, _dp]),

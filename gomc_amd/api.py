@@ -194,6 +194,35 @@ class Box:
         return dict(inter=out[0], tail=out[1], real=out[2], recip=out[3], total=out[4],
                     pressure=out[5], tensor=out[6:9].copy())
 
+    def checkpoint(self, step: int = 0) -> bytes:
+        """The box's Monte Carlo state (coordinates, MT19937 state, move
+        settings, molecule lookup, running energies, structure factors) and
+        `step`, as bytes that `restore` takes back."""
+        n = self.lib.gomc_box_checkpoint(self.h, int(step), None, 0)
+        if n < 0:
+            self._check(int(n))
+        buf = C.create_string_buffer(int(n))
+        m = self.lib.gomc_box_checkpoint(self.h, int(step), C.cast(buf, C.c_void_p), int(n))
+        if m != n:
+            self._check(int(m) if m < 0 else -3)
+        return buf.raw
+
+    def restore(self, data: bytes) -> int:
+        """Restore a checkpoint of a box of the same system; returns its step."""
+        step = np.zeros(1, np.int64)
+        buf = C.create_string_buffer(bytes(data), len(data))
+        self._check(self.lib.gomc_box_restore(self.h, C.cast(buf, C.c_void_p), len(data),
+                                              ptr(step)))
+        return int(step[0])
+
+    def save(self, path: str, step: int = 0) -> None:
+        with open(path, "wb") as f:
+            f.write(self.checkpoint(step))
+
+    def load(self, path: str) -> int:
+        with open(path, "rb") as f:
+            return self.restore(f.read())
+
     def mp_trial(self, move_type: int, step: int, max_step: float) -> np.ndarray:
         out = np.zeros(5)
         self._check(self.lib.gomc_mp_trial(self.h, move_type, step, max_step, ptr(out)))

@@ -38,7 +38,7 @@ FLAGS = [
 ]
 SOURCES = ["box.hip", "box_setup.cpp", "driver.cpp", "mp.hip", "gcmc.cpp", "batch.hip", "batch_setup.cpp", "server.cpp",
            "placement.cpp", "gcmc_dev.hip", "virial.hip",
-           "volume.cpp"]
+           "volume.cpp", "checkpoint.cpp"]
 
 
 def _compile(src: Path) -> Path:

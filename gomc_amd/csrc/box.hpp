@@ -239,6 +239,8 @@ void DevFree(gomc_box *b);
 int RecipInitImpl(gomc_box *b, double *erecip);
 // VolumeTransfer of an NPT box (volume.cpp).
 int VolumeMove(gomc_box *b, int &accepted, double dE[3]);
+// Replaces the cell grid by one for the box's current geometry (volume.cpp).
+int Regrid(gomc_box *b);
 // Half-space k-vectors of an orthogonal box in the reference's order
 // (box_setup.cpp); `packed` (may be null) gets their index triples.
 void KVectorsOrth(const double L[3], double alphaSq, double recipRcut, double volume,

@@ -125,6 +125,18 @@ int UploadCoords(gomc_box *b) {
 
 }  // namespace
 
+int Regrid(gomc_box *b) {
+  const Grid old = SaveGrid(b);
+  int rc = NewGrid(b);
+  if (rc) {
+    FreeGridArrays(SaveGrid(b));
+    PutGrid(b, old);
+    return rc;
+  }
+  FreeGridArrays(old);
+  return GOMC_OK;
+}
+
 int VolumeMove(gomc_box *b, int &accepted, double dE[3]) {
   accepted = 0;
   dE[0] = dE[1] = dE[2] = 0.0;

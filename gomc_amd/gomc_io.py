@@ -404,6 +404,11 @@ def _molecules(natom: int, bonds: np.ndarray) -> list[list[int]]:
     return sorted(groups.values(), key=lambda g: g[0])
 
 
+def _wrap(xyz: np.ndarray, L: np.ndarray) -> np.ndarray:
+    w = xyz - L * np.floor(xyz / L)
+    return np.where(w >= L, w - L, w)
+
+
 def _read_box(conf: GomcConfig, b: int):
     if b not in conf.structure or b not in conf.coordinates:
         raise GomcFormatError(f"{conf.path}: no Structure/Coordinates for box {b}")
@@ -508,10 +513,11 @@ def system_from_gomc(conf_path: str, ensemble: str | None = None, threads: int =
             else:
                 raise GomcFormatError(f"no ChemPot or Fugacity for residue {k.name}")
 
-    xyz_all = np.concatenate(xyz) if xyz else np.zeros((0, 3))
-    # box-0 atoms are wrapped into [0, L) as the reference's coordinate set-up does
-    n0 = sum(len(kinds[k].atom_types) for k, b in zip(mol_kind, mol_box) if b == 0)
-    xyz_all[:n0] = np.mod(xyz_all[:n0], np.array(L))
+    # Every atom is wrapped into box 0's [0, L): box-0 atoms as the
+    # reference's coordinate set-up wraps them; a reservoir molecule's
+    # position carries only its shape (an insertion regrows it), and its
+    # intramolecular vectors survive the wrap under the minimum image.
+    xyz_all = _wrap(np.concatenate(xyz) if xyz else np.zeros((0, 3)), np.array(L))
     f = conf.freq
     electro = conf.electrostatic
     elec_ewald = conf.ewald and electro

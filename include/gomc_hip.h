@@ -136,6 +136,22 @@ int gomc_particle_inter(gomc_box *b, int atom_type, double charge,
  * in K). */
 int gomc_box_virial(gomc_box *b, double out[9]);
 
+/* Checkpoint of the box's Monte Carlo state -- what Checkpoint::serialize
+ * keeps (src/Checkpoint.h:118-167, written by CheckpointOutput.cpp:69-90,
+ * read back by CheckpointSetup): the caller's step number, the MT19937
+ * state, MoveSettings, MultiParticle settings, MoleculeLookup, coordinates
+ * -- plus the geometry, centres, running energies and structure factors,
+ * so that a restored box continues move for move.  The bytes are this
+ * library's own versioned format, not a cereal archive.  Returns the size
+ * in bytes (writing them only when cap >= size; buf may be NULL to ask for
+ * the size) or a negative error code. */
+int64_t gomc_box_checkpoint(gomc_box *b, int64_t step, void *buf, int64_t cap);
+/* Restores a checkpoint taken from a box of the same system and returns
+ * its step in *step (may be NULL).  A checkpoint of another system (other
+ * atom, molecule or kind counts, ensemble or molecule kinds) or a truncated
+ * one is refused with GOMC_ERR_ARG before the box is touched. */
+int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step);
+
 /* ---- Ewald reciprocal space (src/Ewald.cpp) -------------------------- */
 /* RecipInitOrth + BoxReciprocalSetup + SetRecipRef (Ewald.cpp:847-903,
  * 193-274, 1021-1053); replaces CallBoxReciprocalSetupGPU

@@ -104,6 +104,10 @@ def test_ar_kr_gcmc_reservoir_and_chemical_potential():
         beta_f = f * gomc_io.BAR_TO_K_MOLECULE_PER_A3 / 143.15
         assert beta_f == pytest.approx(math.exp(-968.0 / 143.15), rel=1e-14)
     assert s.move_swap == 0.001 and s.cbmc_first == 10
+    # reservoir atoms (box 1 is 65 A) are wrapped into box 0's cell
+    assert np.all((s.xyz >= 0) & (s.xyz < 34.0))
+    assert np.allclose(s.xyz[3:], np.array([[16.323, 61.619, 62.256], [26.139, 44.297, 52.883],
+                                            [30.844, 0.499, 9.084]]) % 34.0)
     o = Oracle(s)
     assert o.mol_boxes()[0] == 3
 

@@ -1,0 +1,90 @@
+"""Checkpoint and restore of a box (gomc_box_checkpoint / gomc_box_restore),
+tested as the reference's CheckpointTest tests its own: a run interrupted
+by a checkpoint and continued in a new box equals the uninterrupted run.
+
+Bar: after restoring into a freshly created box of the same system, the
+next moves have the same kind, molecule, overlap and accept sequences and
+the same per-move energy changes (bitwise), and end with the same
+coordinates, energies and move settings as the box the checkpoint was
+taken from -- for NVT water with Ewald (host driver), GCMC methane/CO2 on
+the device move loop, NPT water whose volume changed before the checkpoint
+(the restored box regrids and rebuilds its k-vectors) and water with
+MultiParticle moves.  A checkpoint of another system, or a truncated one,
+is refused and leaves the box as it was.
+"""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from gomc_amd import systems
+from gomc_amd.api import Box, GomcError
+
+pytestmark = pytest.mark.gpu
+
+
+def same(ra, rb):
+    for key in ("kind", "mol", "ovl", "acc", "dlj", "dreal", "drecip"):
+        assert np.array_equal(ra[key], rb[key]), (key, np.nonzero(ra[key] != rb[key])[0][:5])
+
+
+def continued_equals_uninterrupted(s, n1, n2, device=False):
+    a = Box(s)
+    a.run(n1, device=device)
+    ck = a.checkpoint(step=n1)
+    na, ra = a.run(n2, start_step=n1, device=device)
+    b = Box(s)
+    assert b.restore(ck) == n1
+    nb, rb = b.run(n2, start_step=n1, device=device)
+    same(ra, rb)
+    assert na == nb
+    assert np.array_equal(a.coords(), b.coords())
+    assert np.array_equal(a.energies(), b.energies())
+    assert np.array_equal(a.scales()[0], b.scales()[0])
+    assert np.array_equal(a.dims(), b.dims())
+    return a, b, ra
+
+
+def test_nvt_water_host_driver():
+    a, b, ra = continued_equals_uninterrupted(systems.water_box(300, seed=11), 1500, 1500)
+    assert ra["acc"].sum() > 300
+
+
+def test_gcmc_device_loop():
+    s = systems.gcmc_mixture(60, 60, 25.0, fugacity_bar=(200.0, 60.0), reservoir=120, seed=3,
+                             adj_steps=250)
+    a, b, ra = continued_equals_uninterrupted(s, 1500, 1500, device=True)
+    assert ra["acc"][ra["kind"] == 10].sum() > 10
+    assert np.array_equal(a.mol_boxes()[1], b.mol_boxes()[1])
+
+
+def test_npt_volume_changed_before_the_checkpoint():
+    s = replace(systems.water_box(300, seed=11, rcut=8.0), ensemble=2, pressure=1.0,
+                move_vol=0.05, adj_steps=150, _keep=[])
+    v0 = Box(s).dims()[3]
+    a, b, ra = continued_equals_uninterrupted(s, 1200, 1000)
+    assert a.dims()[3] != v0 and (ra["kind"] == 9).sum() > 20
+
+
+def test_multiparticle_moves():
+    s = systems.water_box(200, seed=21, adj_steps=100, move_disp=0.25, move_rot=0.25,
+                          move_mp=0.5)
+    a, b, ra = continued_equals_uninterrupted(s, 300, 300)
+    assert (ra["kind"] == 2).sum() > 50
+
+
+def test_foreign_and_truncated_checkpoints_are_refused():
+    s = systems.water_box(200, seed=5)
+    other = Box(systems.water_box(201, seed=5)).checkpoint()
+    b = Box(s)
+    e0, x0 = b.energies(), b.coords()
+    ck = b.checkpoint(step=7)
+    with pytest.raises(GomcError, match="different system"):
+        b.restore(other)
+    with pytest.raises(GomcError):
+        b.restore(ck[:-9])
+    with pytest.raises(GomcError, match="not a box checkpoint"):
+        b.restore(b"x" * 64)
+    assert np.array_equal(b.energies(), e0) and np.array_equal(b.coords(), x0)
+    assert b.restore(ck) == 7
+    b.run(200)
