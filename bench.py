@@ -76,6 +76,89 @@ def copy_reference(nbytes: int) -> None:
     hip.hipFree(dst)
 
 
+def next_rows(cpu_threads: int, with_cpu: bool) -> dict:
+    """Timings of the rows around the batched path, each beside the oracle on
+    the host: the molecular virial and pressure (config-1 and config-2
+    water), NPT water with volume moves, a replica-exchange attempt over a
+    6-temperature batch, and a checkpoint/restore round trip."""
+    from dataclasses import replace
+
+    from gomc_amd import systems
+    from gomc_amd.api import Batch, Box
+    from oracle.pyoracle import Oracle
+
+    def best_ms(fn, reps=5):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t) * 1e3)
+        return float(np.median(ts))
+
+    out = {}
+    for name, s in (("config1_1000_water", systems.config1()),
+                    ("config2_10000_water", systems.config2())):
+        b = Box(s)
+        row = dict(gpu_ms=best_ms(b.virial), pressure_bar=b.virial()["pressure"])
+        if with_cpu:
+            o = Oracle(s, threads=cpu_threads)
+            t = time.perf_counter()
+            vo = o.virial()
+            row["cpu_ms"] = (time.perf_counter() - t) * 1e3
+            row["cpu_threads"] = cpu_threads
+            row["rel_diff_pressure"] = abs(row["pressure_bar"] - vo["pressure"]) / abs(vo["pressure"])
+            o.close()
+        b.close()
+        out[f"virial_{name}"] = row
+        print(f"# virial {name}: {row}", file=sys.stderr, flush=True)
+
+    s = replace(systems.water_box(300, seed=11, rcut=8.0), ensemble=2, pressure=1.0,
+                move_vol=0.05, adj_steps=150, _keep=[])
+    b = Box(s)
+    b.run(200, record=False)
+    t = time.perf_counter()
+    b.run(2000, start_step=200, record=False)
+    row = dict(workload="300 waters, rc 8 A, 1 bar, VolFreq 0.05 of translate/rotate",
+               gpu_moves_per_s=2000 / (time.perf_counter() - t))
+    b.close()
+    if with_cpu:
+        o = Oracle(s, threads=cpu_threads)
+        o.run(200, record=False)
+        t = time.perf_counter()
+        o.run(2000, start_step=200, mode=0, record=False)
+        row["cpu_moves_per_s"] = 2000 / (time.perf_counter() - t)
+        row["cpu_threads"] = cpu_threads
+        o.close()
+    out["npt_water"] = row
+    print(f"# npt {row}", file=sys.stderr, flush=True)
+
+    temps = [280.0, 300.0, 320.0, 345.0, 370.0, 400.0]
+    bt = Batch([systems.water_box(260, temperature=tt, seed=100 + i, place_seed=7)
+                for i, tt in enumerate(temps)])
+    ladder = np.arange(len(temps), dtype=np.int32)
+    bt.run([260] * len(temps))
+    n_att = 200
+    t = time.perf_counter()
+    for a in range(n_att):
+        bt.exchange(ladder, a, seed=3)
+    out["replica_exchange"] = dict(replicas=len(temps),
+                                   us_per_attempt=(time.perf_counter() - t) / n_att * 1e6)
+    bt.close()
+
+    b = Box(systems.config1())
+    b.run(500, record=False)
+    ck = b.checkpoint(500)
+    b2 = Box(systems.config1())
+    out["checkpoint_config1"] = dict(bytes=len(ck), save_ms=best_ms(lambda: b.checkpoint(500)),
+                                     restore_ms=best_ms(lambda: b2.restore(ck)))
+    b.close()
+    b2.close()
+    print(f"# replica/checkpoint {out['replica_exchange']} {out['checkpoint_config1']}",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +172,8 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-next", action="store_true",
+                    help="skip the virial / NPT / replica-exchange / checkpoint timings")
     ap.add_argument("--serve-jobs", type=int, default=60, help="0 skips the serving run")
     ap.add_argument("--serve-gcmc-cycles", type=int, default=100)
     ap.add_argument("--serve-gcmc-jobs", type=int, default=60,
@@ -250,6 +335,10 @@ def main() -> None:
         print(f"# cpu oracle {cpu['value']:.0f} moves/s on {threads} threads", file=sys.stderr,
               flush=True)
 
+    nxt = None
+    if not args.no_next:
+        nxt = next_rows(min(16, os.cpu_count() or 1), not args.no_cpu)
+
     if args.calibrate_copy > 0:
         copy_reference(args.calibrate_copy << 20)
         print(f"# copy reference: {args.calibrate_copy} MiB read + written per copy, 3 copies",
@@ -300,6 +389,7 @@ def main() -> None:
             "flops_per_launch": flops_step,
         },
         "cpu_baseline": cpu,
+        "next_rows": nxt,
     }
     print(json.dumps(out))
 

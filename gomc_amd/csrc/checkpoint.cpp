@@ -72,8 +72,13 @@ int Serialize(gomc_box *b, int64_t step, Writer &w) {
   w.Put<int32_t>(b->nkinds);
   w.Put<int32_t>(b->ensemble);
   w.PutVec(b->molKind);
-  // geometry
+  // geometry: edges, half edges and volume as the volume moves left them
+  // (VolumeTransfer scales the half edges and adds dV to the volume, so
+  // neither is recomputed from the edges)
   for (int i = 0; i < 3; ++i) w.Put(b->geom.L[i]);
+  for (int i = 0; i < 3; ++i) w.Put(b->geom.half[i]);
+  w.Put(b->volume);
+  w.Put(b->volInv);
   // coordinates and centres
   w.PutVec(b->hx);
   w.PutVec(b->hy);
@@ -167,8 +172,10 @@ int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step) 
   if (!r.ok || natoms != b->natoms || nmols != b->nmols || nkinds != b->nkinds ||
       ens != b->ensemble || molKind != b->molKind)
     return Fail(b, GOMC_ERR_ARG, "checkpoint belongs to a different system");
-  double L[3];
+  double L[3], H[3];
   for (double &v : L) v = r.Get<double>();
+  for (double &v : H) v = r.Get<double>();
+  const double vol = r.Get<double>(), volInv = r.Get<double>();
   const uint64_t na = uint64_t(natoms), nm = uint64_t(nmols), nk = uint64_t(nkinds);
   auto hx = r.GetVec<double>(na), hy = r.GetVec<double>(na), hz = r.GetVec<double>(na);
   auto cx = r.GetVec<double>(nm), cy = r.GetVec<double>(nm), cz = r.GetVec<double>(nm);
@@ -218,8 +225,9 @@ int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step) 
     if (mv.scale[m].size() != nk || mv.tries[m].size() != nk || mv.accepted[m].size() != nk ||
         mv.tempTries[m].size() != nk || mv.tempAccepted[m].size() != nk)
       return Fail(b, GOMC_ERR_ARG, "checkpoint move settings do not match the system");
-  for (double v : L)
-    if (!(v > 0.0)) return Fail(b, GOMC_ERR_ARG, "checkpoint box length not positive");
+  for (int i = 0; i < 3; ++i)
+    if (!(L[i] > 0.0) || !(H[i] > 0.0) || !(vol > 0.0))
+      return Fail(b, GOMC_ERR_ARG, "checkpoint box length not positive");
   if (2.0 * std::max(b->rCutLJ, b->rCutCoul) > std::min(L[0], std::min(L[1], L[2])))
     return Fail(b, GOMC_ERR_ARG, "checkpoint box is smaller than twice the cutoff");
 
@@ -229,14 +237,15 @@ int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step) 
   b->pendingMol = -1;
   b->stageMol = -1;
   int rc = GOMC_OK;
-  const bool newGeom = L[0] != b->geom.L[0] || L[1] != b->geom.L[1] || L[2] != b->geom.L[2];
+  bool newGeom = vol != b->volume || volInv != b->volInv;
+  for (int i = 0; i < 3; ++i) newGeom = newGeom || L[i] != b->geom.L[i] || H[i] != b->geom.half[i];
   if (newGeom) {
     for (int i = 0; i < 3; ++i) {
       b->geom.L[i] = L[i];
-      b->geom.half[i] = 0.5 * L[i];
+      b->geom.half[i] = H[i];
     }
-    b->volume = L[0] * L[1] * L[2];
-    b->volInv = 1.0 / b->volume;
+    b->volume = vol;
+    b->volInv = volInv;
     rc = Regrid(b);
     if (rc) return rc;
   }

@@ -189,7 +189,7 @@ int OneMove(gomc_box *b, uint64_t step, Record &r) {
   }
   const double pr = b->mt.Closed();  // Translate/Rotate::Accept draw
   double out[4];
-  int rc = LaunchTrial(b, m, &b->hx[s], &b->hy[s], &b->hz[s], nx, ny, nz, true, true, out);
+  int rc = LaunchTrial(b, m, &b->hx[s], &b->hy[s], &b->hz[s], nx, ny, nz, true, b->ewald, out);
   if (rc) return rc;
   const bool overlap = out[3] > 0.0;
   const double dlj = out[0], dreal = out[1];
