@@ -8,9 +8,9 @@
 // box cutoff, F/r of the LJ and erfc-Coulomb terms times the atom
 // separation's and the minimum-image centre separation's components.
 // RecipVirialKernel: blocks [0, kBlocks) one k-vector per thread (the
-// k-space term); the others one atom per thread over LDS-staged k data (the
-// intramolecular term: q 2 pref (S_I cos - S_R sin) k_d times the atom's
-// offset from its molecule's centre).  Each block writes its partial sums;
+// k-space term); the others one atom per thread over one chunk of LDS-staged
+// k data (the intramolecular term: q 2 pref (S_I cos - S_R sin) k_d times
+// the atom's offset from its molecule's centre).  Each block writes its partial sums;
 // the host adds them in block order.
 #include <algorithm>
 #include <cmath>
@@ -105,6 +105,7 @@ struct RecipVirArgs {
   const double *x, *y, *z, *q;
   const int *mol;
   int natoms, K, kBlocks;
+  int kChunk, kChunks;  // k-vectors per chunk (whole tiles), chunks per atom block
   const double *kx, *ky, *kz, *pref, *sumR, *sumI;
   const double *cx, *cy, *cz;
   double constVal;  // 1 / (4 alpha^2)
@@ -126,7 +127,7 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
       w[2] = factor * (1.0 - 2.0 * (a.constVal + 1.0 / hsqr) * kz * kz);
     }
   } else {
-    const int p = (blockIdx.x - a.kBlocks) * kBlock + threadIdx.x;
+    const int p = (blockIdx.x - a.kBlocks) / a.kChunks * kBlock + threadIdx.x;
     bool on = false;
     double xp = 0, yp = 0, zp = 0, qp = 0, ddx = 0, ddy = 0, ddz = 0;
     if (p < a.natoms) {
@@ -143,8 +144,10 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
         ddz = Unwrap(zp, cz, a.g.L[2], a.g.half[2]) - cz;
       }
     }
-    for (int t = 0; t < a.K; t += kBlock) {
-      const int tn = min(kBlock, a.K - t);
+    const int chunk = (blockIdx.x - a.kBlocks) % a.kChunks;
+    const int k0 = chunk * a.kChunk, k1 = min(a.K, k0 + a.kChunk);
+    for (int t = k0; t < k1; t += kBlock) {
+      const int tn = min(kBlock, k1 - t);
       __syncthreads();
       if (int(threadIdx.x) < tn) {
         const int k = t + threadIdx.x;
@@ -187,7 +190,18 @@ extern "C" int gomc_box_virial(gomc_box *b, double out[9]) {
   const int pairBlocks = b->ncell * 27;
   const int kBlocks = b->ewald ? (b->K + kBlock - 1) / kBlock : 0;
   const int atomBlocks = b->ewald ? (b->natoms + kBlock - 1) / kBlock : 0;
-  const int nblocks = pairBlocks + kBlocks + atomBlocks;
+  // The intramolecular term splits the k-vectors into chunks as well as the
+  // atoms into blocks, so that a large box fills the chip (about 2048
+  // workgroups); chunks are whole k tiles.
+  int kChunks = 1, kChunk = std::max(b->K, 1);
+  if (atomBlocks > 0) {
+    const int tiles = std::max(kBlocks, 1);
+    kChunks = std::min(tiles, std::max(1, (2048 + atomBlocks - 1) / atomBlocks));
+    const int tilesPerChunk = (tiles + kChunks - 1) / kChunks;
+    kChunk = tilesPerChunk * kBlock;
+    kChunks = (tiles + tilesPerChunk - 1) / tilesPerChunk;
+  }
+  const int nblocks = pairBlocks + kBlocks + atomBlocks * kChunks;
   const size_t need = size_t(kVirSlots) * std::max(nblocks, 1);
   if (need > b->virCap) {  // grows only with the box's block count
     if (b->virBuf) (void)hipFree(b->virBuf);
@@ -230,6 +244,8 @@ extern "C" int gomc_box_virial(gomc_box *b, double out[9]) {
     r.natoms = b->natoms;
     r.K = b->K;
     r.kBlocks = kBlocks;
+    r.kChunk = kChunk;
+    r.kChunks = kChunks;
     r.kx = b->kx;
     r.ky = b->ky;
     r.kz = b->kz;
@@ -241,7 +257,8 @@ extern "C" int gomc_box_virial(gomc_box *b, double out[9]) {
     r.cz = a.cz;
     r.constVal = 1.0 / (4.0 * (b->alpha * b->alpha));
     r.part = part + size_t(pairBlocks) * kVirSlots;
-    hipLaunchKernelGGL(RecipVirialKernel, dim3(kBlocks + atomBlocks), dim3(kBlock), 0, b->stream, r);
+    hipLaunchKernelGGL(RecipVirialKernel, dim3(kBlocks + atomBlocks * kChunks), dim3(kBlock), 0,
+                       b->stream, r);
     e = hipGetLastError();
   }
   std::vector<double> h(size_t(kVirSlots) * std::max(nblocks, 1), 0.0);
