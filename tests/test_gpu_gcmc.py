@@ -114,3 +114,26 @@ def test_gcmc_mixed_precision_mean_counts_within_error():
         se = np.sqrt(sum(x.var(ddof=1) / len(x) for x in blocks))
         z = abs(blocks[0].mean() - blocks[1].mean()) / max(se, 1e-12)
         assert z < 3.0, (k, blocks[0].mean(), blocks[1].mean(), se)
+
+
+@pytest.mark.parametrize("device", [False, True])
+def test_gcmc_starting_from_an_empty_box(device):
+    """The usual GCMC start: every molecule in the reservoir, box 0 empty
+    (no live atoms, zero structure factors); insertions fill it with the
+    oracle's sequence on both drivers."""
+    s = systems.gcmc_mixture(0, 0, 25.0, fugacity_bar=(200.0, 60.0), reservoir=120, seed=3,
+                             adj_steps=250)
+    b, o = Box(s), Oracle(s, threads=8)
+    assert b.mol_boxes()[0] == 0
+    assert np.array_equal(b.system_total(), o.system_total())
+    ng, rg = b.run(3000, device=device)
+    no, ro = o.run(3000, mode=0)
+    e0 = o.energies()
+    for key in ("kind", "mol", "ovl", "acc"):
+        assert np.array_equal(rg[key], ro[key]), (key, np.nonzero(rg[key] != ro[key])[0][:5])
+    assert ng == no and ro["acc"][ro["kind"] == 10].sum() > 100
+    assert dE_close(rg["dlj"], ro["dlj"], e0[0])
+    assert dE_close(rg["dreal"], ro["dreal"], e0[2])
+    assert np.array_equal(b.mol_boxes()[1], o.mol_boxes()[1])
+    assert np.allclose(b.coords(), o.coords(), rtol=0, atol=1e-9)
+    assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=1e-6)

@@ -55,3 +55,17 @@ def test_running_energy_equals_recompute_after_transfers():
     assert rec["acc"][k == 10].sum() > 100
     e, tot = o.energies(), o.system_total()
     assert np.allclose(e, tot, rtol=1e-10, atol=1e-6), (e, tot)
+
+
+def test_empty_box_fills_and_running_energy_equals_recompute():
+    """GCMC from an empty box (every molecule in the reservoir): zero
+    energies at the start, insertions accepted, and the running energies
+    equal a from-scratch recomputation afterwards."""
+    s = systems.gcmc_mixture(0, 0, 25.0, fugacity_bar=(200.0, 60.0), reservoir=120, seed=3,
+                             adj_steps=250)
+    o = Oracle(s, threads=4)
+    assert o.mol_boxes()[0] == 0 and np.all(o.energies() == 0.0)
+    n, rec = o.run(3000)
+    k = rec["kind"]
+    assert rec["acc"][k == 10].sum() > 100 and o.mol_boxes()[0] > 50
+    assert np.allclose(o.energies(), o.system_total(), rtol=1e-10, atol=1e-6)
