@@ -79,8 +79,10 @@ def copy_reference(nbytes: int) -> None:
 def next_rows(cpu_threads: int, with_cpu: bool) -> dict:
     """Timings of the rows around the batched path, each beside the oracle on
     the host: the molecular virial and pressure (config-1 and config-2
-    water), NPT water with volume moves, a replica-exchange attempt over a
-    6-temperature batch, and a checkpoint/restore round trip."""
+    water), MultiParticle moves (config-1 and config-2 water; the oracle on
+    config 1 only, a config-2 move takes it tens of seconds), NPT water with
+    volume moves, a replica-exchange attempt over a 6-temperature batch, and
+    a checkpoint/restore round trip."""
     from dataclasses import replace
 
     from gomc_amd import systems
@@ -112,6 +114,27 @@ def next_rows(cpu_threads: int, with_cpu: bool) -> dict:
         b.close()
         out[f"virial_{name}"] = row
         print(f"# virial {name}: {row}", file=sys.stderr, flush=True)
+
+    # MultiParticle moves only: structure factors, reciprocal and pair forces,
+    # torques and the force-biased transform of every molecule per move
+    for name, s, n_gpu, n_cpu in (("config1_1000_water", systems.config1(), 20, 4),
+                                  ("config2_10000_water", systems.config2(), 10, 0)):
+        s = replace(s, move_disp=0.0, move_rot=0.0, move_mp=1.0, _keep=[])
+        b = Box(s)
+        b.run(2, record=False)
+        t = time.perf_counter()
+        b.run(n_gpu, start_step=2, record=False)
+        row = dict(gpu_ms_per_move=(time.perf_counter() - t) / n_gpu * 1e3)
+        b.close()
+        if with_cpu and n_cpu:
+            o = Oracle(s, threads=cpu_threads)
+            t = time.perf_counter()
+            o.run(n_cpu, mode=0, record=False)
+            row["cpu_ms_per_move"] = (time.perf_counter() - t) / n_cpu * 1e3
+            row["cpu_threads"] = cpu_threads
+            o.close()
+        out[f"multiparticle_{name}"] = row
+        print(f"# multiparticle {name}: {row}", file=sys.stderr, flush=True)
 
     s = replace(systems.water_box(300, seed=11, rcut=8.0), ensemble=2, pressure=1.0,
                 move_vol=0.05, adj_steps=150, _keep=[])

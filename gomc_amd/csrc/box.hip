@@ -311,18 +311,21 @@ struct SumsArgs {
   Partial4 *partials;
   unsigned *counter;
   double *result;
+  // Atom chunks: with aChunks > 1, RecipSumsPartKernel writes the sums of
+  // each chunk of aChunk atoms to part[chunk][k][2] and RecipSumsKernel adds
+  // the chunks in order, so that a box with few k blocks still fills the chip.
+  int kBlocks, aChunk, aChunks;
+  double *part;
 };
 
-__global__ void __launch_bounds__(kBlock) RecipSumsKernel(const SumsArgs a) {
-  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
-  __shared__ double red[kBlock / kWave];
-  const int k = blockIdx.x * kBlock + threadIdx.x;
-  const bool active = k < a.K;
-  const double kxk = active ? a.kx[k] : 0.0, kyk = active ? a.ky[k] : 0.0,
-               kzk = active ? a.kz[k] : 0.0;
-  double sR = 0.0, sI = 0.0;
-  for (int tile = 0; tile < a.natoms; tile += kBlock) {
-    const int tn = min(kBlock, a.natoms - tile);
+// S(k) += sum over atoms [a0, a1) of q (cos, sin)(k.r), atoms staged through
+// LDS one tile at a time; every thread of the block takes part in staging.
+__device__ __forceinline__ void SumsOverAtoms(const SumsArgs &a, bool active, double kxk,
+                                              double kyk, double kzk, int a0, int a1,
+                                              double *sx, double *sy, double *sz, double *sq,
+                                              double &sR, double &sI) {
+  for (int tile = a0; tile < a1; tile += kBlock) {
+    const int tn = min(kBlock, a1 - tile);
     if (threadIdx.x < tn) {
       int j = tile + threadIdx.x;
       sx[threadIdx.x] = a.x[j];
@@ -341,6 +344,41 @@ __global__ void __launch_bounds__(kBlock) RecipSumsKernel(const SumsArgs a) {
         sI += sq[jj] * s;
       }
     __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) RecipSumsPartKernel(const SumsArgs a) {
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
+  const int kb = blockIdx.x % a.kBlocks, chunk = blockIdx.x / a.kBlocks;
+  const int k = kb * kBlock + threadIdx.x;
+  const bool active = k < a.K;
+  const double kxk = active ? a.kx[k] : 0.0, kyk = active ? a.ky[k] : 0.0,
+               kzk = active ? a.kz[k] : 0.0;
+  const int a0 = chunk * a.aChunk, a1 = min(a.natoms, a0 + a.aChunk);
+  double sR = 0.0, sI = 0.0;
+  SumsOverAtoms(a, active, kxk, kyk, kzk, a0, a1, sx, sy, sz, sq, sR, sI);
+  if (active) {
+    a.part[(size_t(chunk) * a.K + k) * 2] = sR;
+    a.part[(size_t(chunk) * a.K + k) * 2 + 1] = sI;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) RecipSumsKernel(const SumsArgs a) {
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
+  __shared__ double red[kBlock / kWave];
+  const int k = blockIdx.x * kBlock + threadIdx.x;
+  const bool active = k < a.K;
+  const double kxk = active ? a.kx[k] : 0.0, kyk = active ? a.ky[k] : 0.0,
+               kzk = active ? a.kz[k] : 0.0;
+  double sR = 0.0, sI = 0.0;
+  if (a.aChunks > 1) {
+    if (active)
+      for (int c = 0; c < a.aChunks; ++c) {
+        sR += a.part[(size_t(c) * a.K + k) * 2];
+        sI += a.part[(size_t(c) * a.K + k) * 2 + 1];
+      }
+  } else {
+    SumsOverAtoms(a, active, kxk, kyk, kzk, 0, a.natoms, sx, sy, sz, sq, sR, sI);
   }
   double e = 0.0;
   if (active) {
@@ -676,6 +714,28 @@ int RecipSums(gomc_box *b, double *erecip) {
   a.partials = b->partials;
   a.counter = b->counter;
   a.result = b->devRes;
+  // atom chunks of whole tiles, about 2048 workgroups in the first pass
+  const int tiles = std::max(1, (b->natoms + kBlock - 1) / kBlock);
+  int chunks = std::min(tiles, std::max(1, (2048 + nblocks - 1) / nblocks));
+  const int tilesPerChunk = (tiles + chunks - 1) / chunks;
+  chunks = (tiles + tilesPerChunk - 1) / tilesPerChunk;
+  a.kBlocks = nblocks;
+  a.aChunk = tilesPerChunk * kBlock;
+  a.aChunks = chunks;
+  a.part = nullptr;
+  if (chunks > 1) {
+    const size_t need = size_t(chunks) * size_t(std::max(b->K, 1)) * 2;
+    if (need > b->sumPartCap) {
+      if (b->sumPart) (void)hipFree(b->sumPart);
+      b->sumPart = nullptr;
+      b->sumPartCap = 0;
+      HIP_TRY(b, hipMalloc(&b->sumPart, sizeof(double) * need));
+      b->sumPartCap = need;
+    }
+    a.part = b->sumPart;
+    hipLaunchKernelGGL(RecipSumsPartKernel, dim3(nblocks * chunks), dim3(kBlock), 0, b->stream, a);
+    HIP_TRY(b, hipGetLastError());
+  }
   hipLaunchKernelGGL(RecipSumsKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   rc = Sync(b);
@@ -746,6 +806,9 @@ static void FreeAll(gomc_box *b) {
   DevFree(b);
   if (b->trialBuf) (void)hipFree(b->trialBuf);
   if (b->virBuf) (void)hipFree(b->virBuf);
+  if (b->sumPart) (void)hipFree(b->sumPart);
+  b->sumPart = nullptr;
+  b->sumPartCap = 0;
   for (void *p : {(void *)b->cacheC, (void *)b->cacheS, (void *)b->stageC, (void *)b->stageS})
     if (p) (void)hipFree(p);
   if (b->hostRes) (void)hipHostFree(b->hostRes);

@@ -79,6 +79,8 @@ struct MPState {
   int nchunks = 0, maxChunks = 0;
   double *wpart = nullptr;                              // per-block products
   unsigned *wcounter = nullptr;
+  double *fpart = nullptr;  // per-k-chunk reciprocal forces [chunk][xyz][natoms]
+  size_t fpartCap = 0;
 };
 
 // MoveSettings state of the driver (src/MoveSettings.cpp:19-258).
@@ -162,6 +164,9 @@ struct gomc_box {
   // per-block partial sums of the virial kernels
   double *virBuf = nullptr;
   size_t virCap = 0;
+  // per-chunk structure factors of RecipSums [chunks][K][2]
+  double *sumPart = nullptr;
+  size_t sumPartCap = 0;
 
   // ---- GCMC (ensemble 1): box 0 interacts, box 1 is the reservoir ----
   // Reservoir atoms carry molecule id -1 on the device and are skipped by

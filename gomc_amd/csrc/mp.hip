@@ -121,7 +121,67 @@ struct RecipForceArgs {
   int natoms, K;
   const double *kx, *ky, *kz, *pref, *sumR, *sumI;
   double *fx, *fy, *fz;
+  // k chunks: with kChunks > 1, RecipForcePartKernel writes each chunk's
+  // k-space force of every atom to part[chunk][xyz][natoms] and
+  // RecipForceKernel adds the chunks in order after the intramolecular
+  // term, so that a box with few atom blocks still fills the chip.
+  int kChunk, kChunks;
+  double *part;
 };
+
+// k-space force on one atom from k-vectors [k0, k1), staged through LDS.
+__device__ __forceinline__ void ForceOverK(const RecipForceArgs &a, bool on, double xp,
+                                           double yp, double zp, double qp, int k0, int k1,
+                                           double (*sk)[kBlock], double &X, double &Y,
+                                           double &Z) {
+  for (int t = k0; t < k1; t += kBlock) {
+    const int tn = min(kBlock, k1 - t);
+    __syncthreads();
+    if (int(threadIdx.x) < tn) {
+      const int k = t + threadIdx.x;
+      sk[0][threadIdx.x] = a.kx[k];
+      sk[1][threadIdx.x] = a.ky[k];
+      sk[2][threadIdx.x] = a.kz[k];
+      sk[3][threadIdx.x] = a.pref[k];
+      sk[4][threadIdx.x] = a.sumR[k];
+      sk[5][threadIdx.x] = a.sumI[k];
+    }
+    __syncthreads();
+    if (on)
+      for (int i = 0; i < tn; ++i) {
+        const double dot = xp * sk[0][i] + yp * sk[1][i] + zp * sk[2][i];
+        double s, c;
+        sincos(dot, &s, &c);
+        const double factor = 2.0 * qp * sk[3][i] * 1.0 * (s * sk[4][i] - c * sk[5][i]);
+        X += factor * sk[0][i];
+        Y += factor * sk[1][i];
+        Z += factor * sk[2][i];
+      }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceArgs a) {
+  __shared__ double sk[6][kBlock];
+  const int chunk = blockIdx.x % a.kChunks;
+  const int p = blockIdx.x / a.kChunks * kBlock + threadIdx.x;
+  const bool active = p < a.natoms;
+  double xp = 0.0, yp = 0.0, zp = 0.0, qp = 0.0;
+  if (active) {
+    xp = a.x[p];
+    yp = a.y[p];
+    zp = a.z[p];
+    qp = a.q[p];
+  }
+  const bool on = active && !(fabs(qp) < 0.000000001);
+  double X = 0.0, Y = 0.0, Z = 0.0;
+  const int k0 = chunk * a.kChunk, k1 = min(a.K, k0 + a.kChunk);
+  ForceOverK(a, on, xp, yp, zp, qp, k0, k1, sk, X, Y, Z);
+  if (active) {
+    a.part[(size_t(chunk) * 3 + 0) * a.natoms + p] = X;
+    a.part[(size_t(chunk) * 3 + 1) * a.natoms + p] = Y;
+    a.part[(size_t(chunk) * 3 + 2) * a.natoms + p] = Z;
+  }
+}
 
 __global__ void __launch_bounds__(kBlock) RecipForceKernel(const RecipForceArgs a) {
   __shared__ double sk[6][kBlock];
@@ -156,29 +216,15 @@ __global__ void __launch_bounds__(kBlock) RecipForceKernel(const RecipForceArgs 
       Z -= intraForce * dz;
     }
   }
-  for (int t = 0; t < a.K; t += kBlock) {
-    const int tn = min(kBlock, a.K - t);
-    __syncthreads();
-    if (int(threadIdx.x) < tn) {
-      const int k = t + threadIdx.x;
-      sk[0][threadIdx.x] = a.kx[k];
-      sk[1][threadIdx.x] = a.ky[k];
-      sk[2][threadIdx.x] = a.kz[k];
-      sk[3][threadIdx.x] = a.pref[k];
-      sk[4][threadIdx.x] = a.sumR[k];
-      sk[5][threadIdx.x] = a.sumI[k];
-    }
-    __syncthreads();
-    if (active && charged)
-      for (int i = 0; i < tn; ++i) {
-        const double dot = xp * sk[0][i] + yp * sk[1][i] + zp * sk[2][i];
-        double s, c;
-        sincos(dot, &s, &c);
-        const double factor = 2.0 * qp * sk[3][i] * 1.0 * (s * sk[4][i] - c * sk[5][i]);
-        X += factor * sk[0][i];
-        Y += factor * sk[1][i];
-        Z += factor * sk[2][i];
+  if (a.kChunks > 1) {
+    if (active)
+      for (int c = 0; c < a.kChunks; ++c) {
+        X += a.part[(size_t(c) * 3 + 0) * a.natoms + p];
+        Y += a.part[(size_t(c) * 3 + 1) * a.natoms + p];
+        Z += a.part[(size_t(c) * 3 + 2) * a.natoms + p];
       }
+  } else {
+    ForceOverK(a, active && charged, xp, yp, zp, qp, 0, a.K, sk, X, Y, Z);
   }
   if (active) {
     a.fx[p] = X;
@@ -408,7 +454,7 @@ namespace gomc {
 void MPFree(gomc_box *b) {
   MPState &mp = b->mp;
   std::vector<void *> ptrs = {mp.xn, mp.yn, mp.zn, mp.inRange, mp.fail, mp.chunks, mp.wpart,
-                              mp.wcounter};
+                              mp.wcounter, mp.fpart};
   for (int s = 0; s < 2; ++s)
     for (int d = 0; d < 3; ++d) {
       ptrs.push_back(mp.com[s][d]);
@@ -558,6 +604,27 @@ int MPRecipForce(gomc_box *b, int set) {
   a.fy = mp.aFR[set][1];
   a.fz = mp.aFR[set][2];
   const int g = (b->natoms + kBlock - 1) / kBlock;
+  // k chunks of whole tiles, about 2048 workgroups in the first pass
+  const int tiles = std::max(1, (b->K + kBlock - 1) / kBlock);
+  int chunks = std::min(tiles, std::max(1, (2048 + g - 1) / g));
+  const int tilesPerChunk = (tiles + chunks - 1) / chunks;
+  chunks = (tiles + tilesPerChunk - 1) / tilesPerChunk;
+  a.kChunk = tilesPerChunk * kBlock;
+  a.kChunks = chunks;
+  a.part = nullptr;
+  if (chunks > 1) {
+    const size_t need = size_t(chunks) * 3 * size_t(std::max(b->natoms, 1));
+    if (need > mp.fpartCap) {
+      if (mp.fpart) (void)hipFree(mp.fpart);
+      mp.fpart = nullptr;
+      mp.fpartCap = 0;
+      HIP_TRY(b, hipMalloc(&mp.fpart, sizeof(double) * need));
+      mp.fpartCap = need;
+    }
+    a.part = mp.fpart;
+    hipLaunchKernelGGL(RecipForcePartKernel, dim3(g * chunks), dim3(kBlock), 0, b->stream, a);
+    HIP_TRY(b, hipGetLastError());
+  }
   hipLaunchKernelGGL(RecipForceKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   return Sync(b);
