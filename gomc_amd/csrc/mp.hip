@@ -17,8 +17,7 @@ using namespace gomc;
 
 namespace {
 
-constexpr int kChunk = 64;                  // home atoms per force workgroup
-constexpr int kGroups = kBlock / kChunk;    // waves sharing the 27 neighbour cells
+constexpr int kChunk = 64;  // home atoms per force workgroup
 constexpr double kMinForce = 1E-12, kMaxForce = 30;  // MultiParticle.h:18-19
 constexpr double kLambda = 0.5;                      // MultiParticle.h:95
 
@@ -33,9 +32,11 @@ __device__ __forceinline__ double Uneg11(uint64_t v) {
 
 // ------------------------------------------------------------ pair force --
 // CalculateEnergy::BoxForce (CalculateEnergy.cpp:268-406).  One workgroup
-// per (cell, up to 64 atoms of it); its four waves take the 27 neighbour
-// cells round robin and their per-atom partial forces are summed in wave
-// order through LDS.  Energies count each pair once (i < j).
+// per (cell, up to 64 atoms of it); its G waves take the 27 neighbour cells
+// round robin and their per-atom partial forces are summed in wave order
+// through LDS.  Energies count each pair once (i < j).  A box with few
+// workgroups uses G = 16 (at most two neighbour cells per wave), a larger
+// one G = 4 (more workgroups resident per CU).
 struct PairForceArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
@@ -50,9 +51,10 @@ struct PairForceArgs {
   double *result;
 };
 
-__global__ void __launch_bounds__(kBlock) PairForceKernel(const PairForceArgs a) {
+template <int kGroups>
+__global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairForceArgs a) {
   __shared__ double sf[kGroups][3][kChunk];
-  __shared__ double red[kBlock / kWave];
+  __shared__ double red[kGroups * kChunk / kWave];
   const int lane = threadIdx.x % kChunk, grp = threadIdx.x / kChunk;
   const int cell = a.chunks[3 * blockIdx.x], off = a.chunks[3 * blockIdx.x + 1],
             cnt = a.chunks[3 * blockIdx.x + 2];
@@ -567,7 +569,10 @@ int MPPairForce(gomc_box *b, int set, double *lj, double *real) {
   a.partials = b->partials;
   a.counter = b->counter;
   a.result = b->devRes;
-  hipLaunchKernelGGL(PairForceKernel, dim3(n), dim3(kBlock), 0, b->stream, a);
+  if (n < 256)
+    hipLaunchKernelGGL(PairForceKernel<16>, dim3(n), dim3(16 * kChunk), 0, b->stream, a);
+  else
+    hipLaunchKernelGGL(PairForceKernel<4>, dim3(n), dim3(4 * kChunk), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   rc = Sync(b);
   if (rc) return rc;
