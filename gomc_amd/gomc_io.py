@@ -546,7 +546,10 @@ def write_gomc(outdir: str, s: System, type_names: list[str] | None = None,
                prefix: str = "system") -> str:
     """Write `s` as GOMC input files (control file, PDB and PSF per box, a
     Mie-style parameter file) under `outdir`; returns the control file's path.
-    Box 1 holds the reservoir molecules of a GCMC system."""
+    Box 1 holds the reservoir molecules of a GCMC system.  Atom masses are
+    written as 1.0 (a System carries none; the energies do not use them) and
+    each molecule's atoms are bonded as a chain so that it stays one
+    connected component when read back."""
     os.makedirs(outdir, exist_ok=True)
     nt = len(s.sigma)
     names = type_names or [f"T{i}" for i in range(nt)]
