@@ -34,8 +34,8 @@ beta * f for a fugacity f and exp(beta * mu) for a chemical potential, so
 f = T exp(mu / T) K/A^3, divided by BAR_TO_K_MOLECULE_PER_A3 to give bar.
 
 Limits (raised as errors, not approximated): orthogonal cells only,
-`Potential VDW` only, molecules of at most the sizes the energy path takes,
-GEMC not built.
+`Potential VDW` only, no NBFIX pair overrides among the system's types,
+molecules of at most the sizes the energy path takes, GEMC not built.
 """
 from __future__ import annotations
 
@@ -373,6 +373,30 @@ def read_parameters(paths: list[str] | str, charmm: bool) -> dict[str, NonBonded
     return out
 
 
+def read_nbfix_pairs(paths: list[str] | str, charmm: bool) -> list[tuple[str, str]]:
+    """Atom-type pairs that an NBFIX (CHARMM) or NBFIX_MIE section overrides
+    (src/FFSetup.cpp NBfix::Read).  The energy path blends every pair from
+    the per-type parameters, so a system that uses such a pair is refused."""
+    if isinstance(paths, str):
+        paths = [paths]
+    want = "NBFIX" if charmm else "NBFIX_MIE"
+    pairs = []
+    for path in paths:
+        section = None
+        with open(path) as f:
+            for line in f:
+                s = line.split("!", 1)[0].strip()
+                if not s or s.startswith("*"):
+                    continue
+                t = s.split()
+                if t[0].upper() in _SECTIONS:
+                    section = t[0].upper()
+                    continue
+                if section == want and len(t) >= 4:
+                    pairs.append((t[0], t[1]))
+    return pairs
+
+
 def write_parameters(path: str, params: dict[str, NonBonded]) -> None:
     with open(path, "w") as f:
         f.write("* written by gomc_amd\n\nNONBONDED_MIE\n!atom eps(K) sigma(A) n\n")
@@ -496,6 +520,9 @@ def system_from_gomc(conf_path: str, ensemble: str | None = None, threads: int =
             mol_kind.append(kind_of[rn])
             mol_box.append(bi)
             xyz.append(p)
+    for a, b in read_nbfix_pairs(conf.parameters, conf.charmm):
+        if a in type_names and b in type_names:
+            raise GomcFormatError(f"NBFIX override of the {a}-{b} pair is not built")
     if max(len(k.atom_types) for k in kinds) > 3 and ensemble == "GCMC":
         raise GomcFormatError("transfers are built for molecules of at most 3 atoms")
     if conf.exclude == "1-2" and any(len(k.atom_types) >= 3 for k in kinds):

@@ -240,6 +240,13 @@ def test_refusals(tmp_path):
     open(conf, "w").write(text.replace("Potential VDW", "Potential SHIFT"))
     with pytest.raises(gomc_io.GomcFormatError, match="VDW"):
         gomc_io.system_from_gomc(conf)
+    par = tmp_path / "fix.par"
+    par.write_text("NONBONDED\nOT 0.0 -0.1521 1.7682\nHT 0.0 -0.046 0.2245\n\n"
+                   "NBFIX\nOT HT -0.1 2.0\n")
+    open(conf, "w").write(text.replace(os.path.join(TIP3, "GOMC_toppar_water_ions_namd.str"),
+                                       str(par)))
+    with pytest.raises(gomc_io.GomcFormatError, match="NBFIX"):
+        gomc_io.system_from_gomc(conf)
     lines = open(tmp_path / "w.pdb").read().splitlines()
     open(tmp_path / "w.pdb", "w").write("\n".join(lines[:-2] + lines[-1:]) + "\n")
     open(conf, "w").write(text)
