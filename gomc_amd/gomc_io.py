@@ -546,8 +546,10 @@ def system_from_gomc(conf_path: str, ensemble: str | None = None, threads: int =
     # intramolecular vectors survive the wrap under the minimum image.
     xyz_all = _wrap(np.concatenate(xyz) if xyz else np.zeros((0, 3)), np.array(L))
     f = conf.freq
-    electro = conf.electrostatic
-    elec_ewald = conf.ewald and electro
+    # ConfigSetup::fillDefaults (src/ConfigSetup.cpp:1446-1449): Ewald turns
+    # the electrostatics on whatever ElectroStatic says
+    electro = conf.electrostatic or conf.ewald
+    elec_ewald = conf.ewald
     rc = conf.rcut
     return System(
         box=L, temperature=conf.temperature, rcut=rc, rcut_low=conf.rcut_low,

@@ -186,6 +186,11 @@ def test_tip3_lattice_system(tmp_path):
     assert s.move_disp == 0.6 and s.move_rot == 0.4 and s.seed == 77
     e = Oracle(s).system_total()
     assert np.all(np.isfinite(e)) and e[0] != 0.0 and e[2] != 0.0 and e[3] > 0.0
+    # Ewald on implies electrostatics on (ConfigSetup::fillDefaults)
+    conf = tmp_path / "in.conf"
+    conf.write_text(conf.read_text().replace("ElectroStatic true", "ElectroStatic false"))
+    s2 = gomc_io.system_from_gomc(str(conf))
+    assert s2.ewald and s2.electrostatic
 
 
 def _rounded(s):
