@@ -82,7 +82,7 @@ class GomcConfig:
     rcut: float | None = None
     rcut_low: float = 0.0
     rcut_coulomb: dict[int, float] = field(default_factory=dict)
-    lrc: bool = False
+    lrc: bool = True  # ConfigSetup.cpp:73, doTailCorr defaults on
     ewald: bool = False
     electrostatic: bool = False
     cached_fourier: bool = False

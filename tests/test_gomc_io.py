@@ -191,6 +191,9 @@ def test_tip3_lattice_system(tmp_path):
     conf.write_text(conf.read_text().replace("ElectroStatic true", "ElectroStatic false"))
     s2 = gomc_io.system_from_gomc(str(conf))
     assert s2.ewald and s2.electrostatic
+    # the tail correction is on unless LRC says otherwise (ConfigSetup.cpp:73)
+    conf.write_text(conf.read_text().replace("LRC true\n", ""))
+    assert gomc_io.system_from_gomc(str(conf)).lrc
 
 
 def _rounded(s):
