@@ -330,6 +330,12 @@ class Batch:
                 t.team_min[i] = int(v)
             t.pair12_max_k = int(tuning.get("pair12_max_k", t.pair12_max_k))
             tp = C.byref(t)
+        t_used = _lib.BatchTuning()
+        self.lib.gomc_batch_default_tuning(C.byref(t_used))
+        if isinstance(tuning, dict):
+            t_used = t
+        self.tuning = {"team_min": [int(v) for v in t_used.team_min],
+                       "pair12_max_k": int(t_used.pair12_max_k)}
         h = C.c_void_p()
         rc = self.lib.gomc_batch_create_tuned(arr, len(systems), device, precision, tp, C.byref(h))
         if rc != 0:

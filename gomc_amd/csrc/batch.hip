@@ -886,6 +886,19 @@ static hipError_t LaunchOne(int grid, const BatchArgs &a, hipStream_t s, bool se
   return hipGetLastError();
 }
 
+template <int PREC>
+static hipError_t ResidentOne(int ldsMols, int *perCU) {
+  hipError_t e = SetLds<PREC>(ldsMols);
+  if (e != hipSuccess) return e;
+  const size_t dyn = sizeof(double) * 3 * size_t(ldsMols);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(perCU, McBatchKernel<PREC>, kBatchThreads,
+                                                      dyn);
+}
+
+hipError_t BatchResidentPerCU(int precision, int ldsMols, int *perCU) {
+  return precision == 1 ? ResidentOne<1>(ldsMols, perCU) : ResidentOne<0>(ldsMols, perCU);
+}
+
 hipError_t PrepareBatch(int precision, int ldsMols) {
   return precision == 1 ? SetLds<1>(ldsMols) : SetLds<0>(ldsMols);
 }

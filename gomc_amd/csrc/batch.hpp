@@ -162,6 +162,8 @@ namespace gomc {
 hipError_t LaunchBatch(int precision, int grid, const BatchArgs &a, hipStream_t s,
                        bool setAttr = true);
 hipError_t PrepareBatch(int precision, int ldsMols);
+// Workgroups of the MC kernel that one CU holds at once with this LDS size.
+hipError_t BatchResidentPerCU(int precision, int ldsMols, int *perCU);
 // LDS centre capacity for a batch whose largest box has maxMols molecules
 // (boxes beyond it read centres from HBM).
 int BatchLdsMols(int maxMols);

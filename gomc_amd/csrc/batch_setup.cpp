@@ -526,6 +526,19 @@ int gomc_batch_create_tuned(const gomc_system_desc *descs, int nboxes, int devic
   }
   // Boxes up to the cap keep their molecule centres in LDS.
   bt->ldsMols = BatchLdsMols(maxMols);
+  // Team members wait for one another, so every workgroup of a launch must be
+  // resident at once: the launch has at most gridCap <= CU-count workgroups,
+  // which needs one resident workgroup per CU on an otherwise idle device.
+  // (Another process's kernels on the same device can still hold CUs; a
+  // team wait is bounded and then reports an error instead of hanging.)
+  {
+    int perCU = 0;
+    if (BatchResidentPerCU(precision, bt->ldsMols, &perCU) != hipSuccess)
+      return fail(GOMC_ERR_NODEV);
+    bool teams = false;
+    for (auto &pb : packed) teams = teams || pb.P.team > 1;
+    if (teams && int64_t(perCU) * cus < gridCap) return fail(GOMC_ERR_STATE);
+  }
   int rc = AllocBatch(bt, nboxes, atoms, mols, kvec);
   if (rc) return fail(rc);
   bt->hboxes.resize(nboxes);

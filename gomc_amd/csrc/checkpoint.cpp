@@ -11,6 +11,7 @@
 // binary archive: the reference archive also carries MolSetup and PDB atom
 // records of its own classes, which the energy path does not hold.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "box.hpp"
@@ -19,7 +20,80 @@ namespace gomc {
 namespace {
 
 constexpr char kMagic[8] = {'G', 'O', 'M', 'C', 'H', 'C', 'K', 'P'};
-constexpr uint32_t kVersion = 1;
+constexpr uint32_t kVersion = 2;
+
+// FNV-1a over the bytes of what fixes the box's Hamiltonian and ensemble:
+// cutoffs, Ewald splitting, temperature, the switches, the pair tables, the
+// atoms' types and charges, the molecule layout, fugacities, pressure and
+// the CBMC trial counts.  A checkpoint restores running energies, so one
+// taken under another Hamiltonian must be refused, not continued.
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void Bytes(const void *p, size_t n) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  }
+  template <class T> void Put(const T &v) { Bytes(&v, sizeof(T)); }
+  template <class T> void PutVec(const std::vector<T> &v) {
+    Put<uint64_t>(v.size());
+    Bytes(v.data(), sizeof(T) * v.size());
+  }
+};
+
+uint64_t Fingerprint(const gomc_box *b) {
+  Fnv f;
+  for (double v : {b->rCutLJ, b->rCutCoul, b->rCutLow, b->tol, b->alpha, b->recipRcut, b->T,
+                   b->pressure})
+    f.Put(v);
+  for (bool v : {b->ewald, b->elect, b->lrc}) f.Put<uint8_t>(v);
+  f.Put<int32_t>(b->ntypes);
+  f.PutVec(b->sigSq);
+  f.PutVec(b->epsCn);
+  f.PutVec(b->nRep);
+  f.PutVec(b->atomType);
+  f.PutVec(b->atomCharge);
+  f.PutVec(b->molStart);
+  f.PutVec(b->fugacity);
+  for (int v : {b->cbmcFirst, b->cbmcNth, b->cbmcAng}) f.Put<int32_t>(v);
+  return f.h;
+}
+
+// The MoleculeLookup read from a checkpoint must describe the system's
+// molecules: section starts from 0 to n_mols in order, every molecule listed
+// once, each in the section of its box and kind, box 0 or 1 (1 only in
+// GCMC), swappable counts within their sections and box-0 kind counts that
+// agree.  Later moves index the box's arrays with these values.
+bool LookupConsistent(const gomc_box *b, const std::vector<int> &molBox,
+                      const std::vector<uint32_t> &lookup, const std::vector<uint32_t> &bkStart,
+                      const std::vector<uint32_t> &swapCount, const std::vector<int> &kindCount) {
+  const int K = b->nkinds, N = b->nmols;
+  if (bkStart.empty() || bkStart.front() != 0 || bkStart.back() != uint32_t(N)) return false;
+  if (lookup.size() != size_t(N) || molBox.size() != size_t(N)) return false;
+  if (bkStart.size() != size_t(2 * K + 1) || swapCount.size() != size_t(2 * K) ||
+      kindCount.size() != size_t(K))
+    return false;
+  for (size_t s = 1; s < bkStart.size(); ++s)
+    if (bkStart[s] < bkStart[s - 1]) return false;
+  std::vector<char> seen(size_t(N), 0);
+  std::vector<int> box0(size_t(K), 0);
+  for (int m = 0; m < N; ++m) {
+    if (molBox[m] != 0 && molBox[m] != 1) return false;
+    if (molBox[m] == 1 && b->ensemble != 1) return false;
+    if (molBox[m] == 0) box0[b->molKind[m]]++;
+  }
+  for (int s = 0; s < 2 * K; ++s) {
+    const int bx = s / K, k = s % K;
+    if (swapCount[s] > bkStart[s + 1] - bkStart[s]) return false;
+    for (uint32_t i = bkStart[s]; i < bkStart[s + 1]; ++i) {
+      const uint32_t m = lookup[i];
+      if (m >= uint32_t(N) || seen[m] || molBox[m] != bx || b->molKind[m] != k) return false;
+      seen[m] = 1;
+    }
+  }
+  for (int k = 0; k < K; ++k)
+    if (kindCount[k] != box0[k]) return false;
+  return true;
+}
 
 struct Writer {
   std::vector<unsigned char> buf;
@@ -72,6 +146,7 @@ int Serialize(gomc_box *b, int64_t step, Writer &w) {
   w.Put<int32_t>(b->nkinds);
   w.Put<int32_t>(b->ensemble);
   w.PutVec(b->molKind);
+  w.Put<uint64_t>(Fingerprint(b));
   // geometry: edges, half edges and volume as the volume moves left them
   // (VolumeTransfer scales the half edges and adds dV to the volume, so
   // neither is recomputed from the edges)
@@ -169,9 +244,14 @@ int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step) 
   const int32_t natoms = r.Get<int32_t>(), nmols = r.Get<int32_t>(), nkinds = r.Get<int32_t>(),
                 ens = r.Get<int32_t>();
   const std::vector<int> molKind = r.GetVec<int>(uint64_t(b->nmols));
+  const uint64_t print = r.Get<uint64_t>();
   if (!r.ok || natoms != b->natoms || nmols != b->nmols || nkinds != b->nkinds ||
       ens != b->ensemble || molKind != b->molKind)
     return Fail(b, GOMC_ERR_ARG, "checkpoint belongs to a different system");
+  if (print != Fingerprint(b))
+    return Fail(b, GOMC_ERR_ARG,
+                "checkpoint was taken under another force field, cutoff, Ewald, charge or "
+                "ensemble setting");
   double L[3], H[3];
   for (double &v : L) v = r.Get<double>();
   for (double &v : H) v = r.Get<double>();
@@ -230,6 +310,28 @@ int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step) 
       return Fail(b, GOMC_ERR_ARG, "checkpoint box length not positive");
   if (2.0 * std::max(b->rCutLJ, b->rCutCoul) > std::min(L[0], std::min(L[1], L[2])))
     return Fail(b, GOMC_ERR_ARG, "checkpoint box is smaller than twice the cutoff");
+  for (int i = 0; i < 3; ++i)
+    if (!(H[i] == 0.5 * L[i] || std::abs(H[i] - 0.5 * L[i]) <= 1e-9 * L[i]))
+      return Fail(b, GOMC_ERR_ARG, "checkpoint half edges do not match its edges");
+  for (const auto *v : {&hx, &hy, &hz, &cx, &cy, &cz})
+    for (double c : *v)
+      if (!std::isfinite(c)) return Fail(b, GOMC_ERR_ARG, "checkpoint coordinate not finite");
+  for (int m = 0; m < 2; ++m)
+    for (double sc : mv.scale[m])
+      if (!(sc > 0.0) || !std::isfinite(sc))
+        return Fail(b, GOMC_ERR_ARG, "checkpoint move scale not positive");
+  if (!LookupConsistent(b, molBox, lookup, bkStart, swapCount, kindCount))
+    return Fail(b, GOMC_ERR_ARG, "checkpoint molecule lookup does not describe the system");
+  // the k-vector count of the checkpoint's geometry, computed before the box
+  // is touched
+  if (b->ewald) {
+    std::vector<double> kx, ky, kz, pref;
+    KVectorsOrth(L, b->alpha * b->alpha, b->recipRcut, vol, kx, ky, kz, pref, nullptr);
+    if (sr.size() != kx.size())
+      return Fail(b, GOMC_ERR_ARG, "checkpoint k-vector count does not match the box");
+  } else if (!sr.empty()) {
+    return Fail(b, GOMC_ERR_ARG, "checkpoint has structure factors but Ewald is off");
+  }
 
   // A device move loop keeps a copy of the state: drop it, the next device
   // run uploads the restored one.
@@ -300,8 +402,8 @@ int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step) 
     double erecip = 0.0;
     rc = newGeom ? RecipInitImpl(b, &erecip) : GOMC_OK;
     if (rc) return rc;
-    if (sr.size() != size_t(b->K))
-      return Fail(b, GOMC_ERR_ARG, "checkpoint k-vector count does not match the box");
+    if (sr.size() != size_t(b->K))  // checked above against the same geometry
+      return Fail(b, GOMC_ERR_STATE, "k-vector count changed during restore");
     const size_t kb = sizeof(double) * sr.size();
     HIP_TRY(b, hipMemcpyAsync(b->sumRref, sr.data(), kb, hipMemcpyHostToDevice, b->stream));
     HIP_TRY(b, hipMemcpyAsync(b->sumIref, si.data(), kb, hipMemcpyHostToDevice, b->stream));

@@ -181,17 +181,50 @@ int VolumeMove(gomc_box *b, int &accepted, double dE[3]) {
   b->geom = ng;
   b->volume = newVol;
   b->volInv = 1.0 / newVol;
-  const bool regrid = true;
-  rc = UploadCoords(b);
-  if (!rc) rc = NewGrid(b);
+  bool gridReplaced = false, kTaken = false;
   KArrays oldK{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // Puts back the kept geometry, coordinates, grid and k-space arrays: the
+  // rejection of the move, and the way out of a failure part-way through
+  // its energy so that the box stays as it was before the move.
+  auto putBack = [&]() -> int {
+    b->geom = og;
+    b->volume = oldVol;
+    b->volInv = oldVolInv;
+    b->hx = ox;
+    b->hy = oy;
+    b->hz = oz;
+    b->comx = ocx;
+    b->comy = ocy;
+    b->comz = ocz;
+    if (gridReplaced) {
+      FreeGridArrays(SaveGrid(b));
+      PutGrid(b, oldGrid);
+    }
+    if (kTaken) {
+      FreeK(TakeK(b));
+      PutK(b, oldK);
+    }
+    return UploadCoords(b);
+  };
+  rc = UploadCoords(b);
+  if (!rc) {
+    gridReplaced = true;  // NewGrid replaces the grid pointers even when it fails
+    rc = NewGrid(b);
+  }
   Energies potNew = b->pot;
   if (!rc && b->ewald) {
     oldK = TakeK(b);
+    kTaken = true;
     rc = RecipInitImpl(b, &potNew.recip);
   }
   if (!rc) rc = BoxInter(b, &potNew.inter, &potNew.real);
-  if (rc) return rc;
+  if (rc) {
+    const std::string why = b->err;
+    putBack();
+    b->err = why;
+    if (!why.empty()) b->err += " (volume move undone)";
+    return rc;
+  }
   if (b->lrc) {  // EnergyCorrection of the new volume
     double tc = 0.0;
     for (int i = 0; i < b->nkinds; ++i)
@@ -213,29 +246,12 @@ int VolumeMove(gomc_box *b, int &accepted, double dE[3]) {
   dE[2] = potNew.recip - b->pot.recip;
   if (result) {
     b->pot = potNew;
-    if (b->ewald) FreeK(oldK);
-    if (regrid) FreeGridArrays(oldGrid);
+    if (kTaken) FreeK(oldK);
+    FreeGridArrays(oldGrid);
     DevFree(b);  // the device loop's k-vector tables belong to the old box
     b->mp.singleAccepted = true;
   } else {
-    b->geom = og;
-    b->volume = oldVol;
-    b->volInv = oldVolInv;
-    b->hx = ox;
-    b->hy = oy;
-    b->hz = oz;
-    b->comx = ocx;
-    b->comy = ocy;
-    b->comz = ocz;
-    Grid newGrid = SaveGrid(b);
-    FreeGridArrays(newGrid);
-    PutGrid(b, oldGrid);
-    if (b->ewald) {
-      const KArrays trial = TakeK(b);
-      FreeK(trial);
-      PutK(b, oldK);
-    }
-    rc = UploadCoords(b);
+    rc = putBack();
     if (rc) return rc;
   }
   b->moves.volTries++;

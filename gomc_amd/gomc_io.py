@@ -35,7 +35,11 @@ f = T exp(mu / T) K/A^3, divided by BAR_TO_K_MOLECULE_PER_A3 to give bar.
 
 Limits (raised as errors, not approximated): orthogonal cells only,
 `Potential VDW` only, no NBFIX pair overrides among the system's types,
-molecules of at most the sizes the energy path takes, GEMC not built.
+molecules of at most the sizes the energy path takes, GEMC not built, no
+nonzero frequency of a move that is not built (regrowth, intra-swap,
+crankshaft, MEMC, targeted swap, NeMTMC, Brownian MultiParticle).  The move
+frequencies the ensemble reads must sum to one within 0.001, as the
+reference requires.
 """
 from __future__ import annotations
 
@@ -101,13 +105,45 @@ class GomcConfig:
 
 _FREQ_KEYS = {"DisFreq": "disp", "RotFreq": "rot", "MultiParticleFreq": "mp",
               "SwapFreq": "swap", "VolFreq": "vol"}
+# Move frequencies the reference reads (src/ConfigSetup.cpp:785-935) for
+# moves this build does not run; a nonzero one is refused.  The MEMC keys
+# share one frequency there (sys.moves.memc / intraMemc).
+_UNBUILT_FREQ_KEYS = {
+    "MultiParticleBrownianFreq": "mp_brownian", "IntraSwapFreq": "intra_swap",
+    "RegrowthFreq": "regrowth", "CrankShaftFreq": "crankshaft",
+    "IntraTargetedSwapFreq": "intra_targeted_swap",
+    "IntraMEMC-1Freq": "intra_memc", "IntraMEMC-2Freq": "intra_memc",
+    "IntraMEMC-3Freq": "intra_memc", "MEMC-1Freq": "memc", "MEMC-2Freq": "memc",
+    "MEMC-3Freq": "memc", "MEMC-2-LiqFreq": "memc", "MEMC-3-LiqFreq": "memc",
+    "TargetedSwapFreq": "targeted_swap", "NeMTMCFreq": "nemtmc"}
+# The frequencies each ensemble's binary reads and sums to one
+# (src/ConfigSetup.cpp:2036-2085): VolFreq is read only where the volume
+# varies and SwapFreq, MEMC, targeted swap and NeMTMC only in GCMC/GEMC.
+_ENSEMBLE_FREQS = {
+    "NVT": ("disp", "rot", "mp", "mp_brownian", "intra_swap", "regrowth", "crankshaft",
+            "intra_targeted_swap", "intra_memc"),
+    "NPT": ("disp", "rot", "mp", "mp_brownian", "intra_swap", "regrowth", "crankshaft",
+            "intra_targeted_swap", "intra_memc", "vol"),
+    "GCMC": ("disp", "rot", "mp", "mp_brownian", "intra_swap", "regrowth", "crankshaft",
+             "intra_targeted_swap", "intra_memc", "swap", "memc", "targeted_swap", "nemtmc"),
+}
 _CBMC_KEYS = {"CBMC_First": "first", "CBMC_Nth": "nth", "CBMC_Ang": "ang", "CBMC_Dih": "dih"}
 
 
+_KNOWN_KEYS = ["Parameters", "ParaTypeCHARMM", "ParaTypeEXOTIC", "ParaTypeMie",
+               "ParaTypeMARTINI", "Coordinates", "Structure", "CellBasisVector1",
+               "CellBasisVector2", "CellBasisVector3", "Temperature", "Rcut", "RcutLow",
+               "RcutCoulomb", "LRC", "Ewald", "ElectroStatic", "CachedFourier", "Tolerance",
+               "Potential", "Exclude", "Random_Seed", "AdjSteps", "Pressure", "Fugacity",
+               "ChemPot", *_FREQ_KEYS, *_UNBUILT_FREQ_KEYS, *_CBMC_KEYS]
+_CANON = {k.upper(): k for k in _KNOWN_KEYS}
+
+
 def read_conf(path: str) -> GomcConfig:
-    """Parse a GOMC control file.  Keys are matched case-sensitively as the
-    reference's CheckString does; a key GOMC requires that is missing is
-    reported when a system is built from the file."""
+    """Parse a GOMC control file.  Keys are matched without regard to case,
+    as the reference's CheckString does (src/ConfigSetup.cpp:194-210); a key
+    GOMC requires that is missing is reported when a system is built from
+    the file."""
     base = os.path.dirname(os.path.abspath(path))
     c = GomcConfig(path=os.path.abspath(path))
     here = lambda p: os.path.normpath(os.path.join(base, p))  # noqa: E731
@@ -116,7 +152,7 @@ def read_conf(path: str) -> GomcConfig:
             tok = line.split("#", 1)[0].split()
             if not tok:
                 continue
-            k, v = tok[0], tok[1:]
+            k, v = _CANON.get(tok[0].upper(), tok[0]), tok[1:]
             c.raw.append(tok)
             try:
                 if k == "Parameters":
@@ -161,6 +197,10 @@ def read_conf(path: str) -> GomcConfig:
                     c.seed = int(v[0])
                 elif k in _FREQ_KEYS:
                     c.freq[_FREQ_KEYS[k]] = float(v[0])
+                elif k in _UNBUILT_FREQ_KEYS:
+                    # the MEMC keys set their shared frequency only when > 0
+                    if not (k.startswith(("MEMC", "IntraMEMC")) and float(v[0]) <= 0.0):
+                        c.freq[_UNBUILT_FREQ_KEYS[k]] = float(v[0])
                 elif k == "AdjSteps":
                     c.adj_steps = int(v[0])
                 elif k in _CBMC_KEYS:
@@ -483,6 +523,7 @@ def system_from_gomc(conf_path: str, ensemble: str | None = None, threads: int =
         raise GomcFormatError(f"Potential {conf.potential} is not built (VDW only)")
     if conf.ewald and conf.tolerance is None:
         raise GomcFormatError(f"{conf.path}: Tolerance is not specified")
+    _check_freqs(conf, ensemble)
     params = read_parameters(conf.parameters, conf.charmm)
 
     boxes = [0, 1] if ensemble == "GCMC" else [0]
@@ -569,6 +610,29 @@ def system_from_gomc(conf_path: str, ensemble: str | None = None, threads: int =
         cbmc_ang=conf.cbmc.get("ang", 100),
         pressure=conf.pressure if conf.pressure is not None else 1.0,
         move_vol=f.get("vol", 0.0) if ensemble == "NPT" else 0.0)
+
+
+def _check_freqs(conf: GomcConfig, ensemble: str) -> None:
+    """The reference's move-frequency checks for `ensemble`
+    (src/ConfigSetup.cpp:1975-2085): DisFreq must be given (and SwapFreq in
+    GCMC, VolFreq in NPT), and the frequencies the ensemble reads must sum
+    to one within 0.001.  A nonzero frequency of a move this build does not
+    run is refused rather than dropped, which would change the move mix."""
+    f = conf.freq
+    need = {"NVT": ("disp",), "NPT": ("disp", "vol"), "GCMC": ("disp", "swap")}[ensemble]
+    names = {v: k for k, v in {**_FREQ_KEYS, **_UNBUILT_FREQ_KEYS}.items()}
+    for key in need:
+        if key not in f:
+            raise GomcFormatError(f"{conf.path}: {names[key]} is not specified")
+    read = _ENSEMBLE_FREQS[ensemble]
+    for key in set(_UNBUILT_FREQ_KEYS.values()) & set(read):
+        if f.get(key, 0.0) != 0.0:
+            raise GomcFormatError(f"{conf.path}: {names[key]} {f[key]} asks for a move that is "
+                                  "not built (Translate, Rotate, MultiParticle, "
+                                  "MoleculeTransfer and VolumeTransfer are)")
+    total = sum(f.get(key, 0.0) for key in read)
+    if abs(total - 1.0) > 0.001:
+        raise GomcFormatError(f"{conf.path}: the sum of move frequencies is {total:g}, not one")
 
 
 def write_gomc(outdir: str, s: System, type_names: list[str] | None = None,

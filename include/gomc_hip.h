@@ -382,11 +382,14 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
                        double tick_ms, gomc_server **out);
 int gomc_server_destroy(gomc_server *sv);
 const char *gomc_server_last_error(const gomc_server *sv);
-/* Builds the job's initial state (on the caller's thread) and queues it for
- * `moves` MC moves; *job_id identifies it.  A GCMC descriptor (ensemble 1)
- * goes to the GCMC lane: up to 8 such jobs run at once, each on its own
- * device box and stream (the plain path, FP64, MT19937 draws), admitted in
- * arrival order while the budget holds their bytes. */
+/* Queues a job for `moves` MC moves; *job_id identifies it.  An NVT
+ * descriptor's initial state is built on the caller's thread and the job
+ * joins the resident batch.  A GCMC descriptor (ensemble 1) goes to the
+ * GCMC lane: builder threads make each such job a device box in arrival
+ * order while the budget holds its bytes, and every built job is advanced,
+ * by whole MC cycles, in each launch of the device move loop
+ * (gomc_run_moves_device_multi: one workgroup per job, the job's own
+ * MT19937 stream) at the server's precision. */
 int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d,
                        int64_t moves, int64_t *job_id);
 /* status 0 queued, 1 running, 2 done, 3 failed; times = {submitted,

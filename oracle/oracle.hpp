@@ -17,6 +17,13 @@
 
 namespace orc {
 
+// Planted mistakes, for the tests that show a statistical check detects
+// them (0, the default, plants nothing): 1 swaps the new and old forces in
+// MultiParticle's weight ratio, 2 makes a CBMC builder keep the trial after
+// the one PickWeighted chose, 3 drops the +1 of N+1 in the insertion
+// coefficient of MoleculeTransfer.  Set only through orc_plant.
+extern int g_plant;
+
 // lib/NumLib.h:22 -- Coulomb constant in K*A/e^2.
 constexpr double kQQFact = 167103.208067979;
 constexpr int kNeighborCells = 27;  // CellList.cpp:199-214 (3x3x3 stencil)

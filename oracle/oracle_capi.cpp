@@ -40,6 +40,7 @@ void *orc_create(const orc_system_desc *d) {
 }
 
 void orc_destroy(void *h) { delete S(h); }
+void orc_plant(int which) { orc::g_plant = which; }
 int orc_natoms(void *h) { return S(h)->NumAtoms(); }
 int orc_nmols(void *h) { return S(h)->NumMols(); }
 int orc_nkvec(void *h) { return S(h)->ewald.imageSizeRef; }
@@ -155,6 +156,17 @@ void orc_get_scales(void *h, double *disp, double *rot) {
     disp[k] = s->ms.scale[0][k];
     rot[k] = s->ms.scale[1][k];
   }
+}
+
+void orc_set_scales(void *h, const double *disp, const double *rot, double mp_t_max,
+                    double mp_r_max) {
+  System *s = S(h);
+  for (size_t k = 0; k < s->kinds.size(); ++k) {
+    s->ms.scale[0][k] = disp[k];
+    s->ms.scale[1][k] = rot[k];
+  }
+  s->ms.mp_t_max = mp_t_max;
+  s->ms.mp_r_max = mp_r_max;
 }
 
 int orc_cell_csr(void *h, int32_t *cellVector, int32_t *cellStart, int32_t *map,

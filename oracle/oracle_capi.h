@@ -41,6 +41,14 @@ typedef struct orc_system_desc {
 
 void *orc_create(const orc_system_desc *d);
 void orc_destroy(void *h);
+/* Plants a known mistake in the oracle for every box (see orc::g_plant);
+ * 0 removes it.  Used by tests that show a statistical check fails on it. */
+void orc_plant(int which);
+/* Sets the displacement and rotation scales of every molecule kind and the
+ * MultiParticle step sizes (test set-up: a run whose AdjSteps exceeds its
+ * length then samples with fixed steps). */
+void orc_set_scales(void *h, const double *disp, const double *rot, double mp_t_max,
+                    double mp_r_max);
 int orc_natoms(void *h);
 int orc_nmols(void *h);
 int orc_nkvec(void *h);

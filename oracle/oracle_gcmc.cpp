@@ -109,6 +109,7 @@ void SingleNew(System &s, TrialMol &t, int kind, int atom) {
   }
   int winner = s.prng.PickWeighted(w.data(), n, stepWeight);
   if (winner < 0) winner = 0;
+  if (g_plant == 2) winner = (winner + 1) % n;
   t.overlap = t.overlap || ov[winner];
   t.weight *= stepWeight / n;
   t.inter += inter[winner];
@@ -185,6 +186,7 @@ void FreeHedronNew(System &s, TrialMol &t, int kind) {
   }
   int winner = s.prng.PickWeighted(w.data(), n, stepWeight);
   if (winner < 0) winner = 0;
+  if (g_plant == 2) winner = (winner + 1) % n;
   t.pos[2] = pb[winner];
   t.pos[1] = pp[winner];
   t.overlap = t.overlap || ov[winner];
@@ -313,7 +315,8 @@ int System::RunMoleculeTransfer(double subDraw, MoveRecord *rec) {
   if (src == 0)
     coeff = double(swapCount[0 * K + kind]) * box.volInv / (ff.beta * fugacity[kind]);
   else
-    coeff = box.volume / double(swapCount[dst * K + kind] + 1) * (ff.beta * fugacity[kind]);
+    coeff = box.volume / double(swapCount[dst * K + kind] + (g_plant == 3 ? 0 : 1)) *
+            (ff.beta * fugacity[kind]);
   const double Wrat = newMol.weight / oldMol.weight * W_tc * W_recip;
   bool result = false;
   if (tried) result = prng() < coeff * Wrat;

@@ -569,11 +569,12 @@ int System::RunMultiParticle(uint64_t step, MoveRecord *rec) {
     const double lBeta = kMPLambda * ff.beta;
     for (int m = 0; m < nm; ++m) {
       if (!inRange[m]) continue;
+      const System::MPForces &fn = g_plant == 1 ? mpRef : mpNew, &fo = g_plant == 1 ? mpNew : mpRef;
       if (moveType == 1)
-        wRatio *= CalculateWRatio(mpNew.torque[m] * lBeta, mpRef.torque[m] * lBeta, rk[m], rmax);
+        wRatio *= CalculateWRatio(fn.torque[m] * lBeta, fo.torque[m] * lBeta, rk[m], rmax);
       else
-        wRatio *= CalculateWRatio((mpNew.mF[m] + mpNew.mFR[m]) * lBeta,
-                                  (mpRef.mF[m] + mpRef.mFR[m]) * lBeta, tk[m], tmax);
+        wRatio *= CalculateWRatio((fn.mF[m] + fn.mFR[m]) * lBeta,
+                                  (fo.mF[m] + fo.mFR[m]) * lBeta, tk[m], tmax);
     }
     if (!std::isfinite(wRatio)) wRatio = 0.0;
   }

@@ -4,6 +4,9 @@
 
 namespace orc {
 
+int g_plant = 0;
+
+
 // src/MersenneTwister.h:194-207 (initialize) then 226-231 (seed -> reload).
 void MTRand::seed(uint32_t s) {
   state_[0] = s;

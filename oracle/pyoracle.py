@@ -23,6 +23,8 @@ _vp = C.c_void_p
 _SIG = {
     "orc_create": (_vp, [C.POINTER(SystemDesc)]),
     "orc_destroy": (None, [_vp]),
+    "orc_plant": (None, [C.c_int]),
+    "orc_set_scales": (None, [_vp, _dp, _dp, C.c_double, C.c_double]),
     "orc_natoms": (C.c_int, [_vp]),
     "orc_nmols": (C.c_int, [_vp]),
     "orc_nkvec": (C.c_int, [_vp]),
@@ -202,6 +204,12 @@ class Oracle:
         self.L.orc_get_scales(self.h, _p(d), _p(r))
         return d, r
 
+    def set_scales(self, disp, rot, mp_t_max, mp_r_max):
+        """Fixed move scales per kind and MultiParticle step sizes."""
+        d = np.ascontiguousarray(disp, dtype=np.float64)
+        r = np.ascontiguousarray(rot, dtype=np.float64)
+        self.L.orc_set_scales(self.h, _p(d), _p(r), float(mp_t_max), float(mp_r_max))
+
     def cell_csr(self):
         edge = np.zeros(3, np.int32)
         ncell = self.L.orc_cell_csr(self.h, None, None, None, _p(edge))
@@ -262,6 +270,24 @@ class Oracle:
         out = np.zeros(2)
         self.L.orc_mp_settings(self.h, _p(out))
         return out
+
+
+class planted:
+    """Context manager that plants mistake `which` in the oracle (1: MP
+    weight ratio with new and old forces swapped; 2: CBMC keeps the trial
+    after the weighted pick; 3: N instead of N + 1 in the insertion
+    coefficient) and removes it on exit."""
+
+    def __init__(self, which: int):
+        self.which = which
+
+    def __enter__(self):
+        lib().orc_plant(self.which)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_plant(0)
+        return False
 
 
 def batch_run(oracles: list[Oracle], moves, threads: int) -> float:

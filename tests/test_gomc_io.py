@@ -67,8 +67,27 @@ def test_mie_parameter_file():
                                 charmm=True)
 
 
-def test_ar_kr_nvt_energy_is_the_lj_tail_only():
-    s = gomc_io.system_from_gomc(CONF, ensemble="NVT")
+def arkr_nvt(tmp_path, rcut=None):
+    """The AR_KR example set up for the NVT binary: GOMC_CPU_NVT does not
+    read SwapFreq, so the example's displacement frequency takes its 0.001
+    for the frequencies to sum to one (ConfigSetup.cpp:2077-2085)."""
+    base = tmp_path / "Base"
+    shutil.copytree(os.path.join(ARKR, "Base"), base)
+    shutil.copy(os.path.join(ARKR, "Par_Vrabec01_Noble_Gases_Exotic.inp"), tmp_path)
+    text = (base / "in.conf").read_text().replace("DisFreq               0.87",
+                                                  "DisFreq               0.871")
+    if rcut is not None:
+        text = text.replace("Rcut\t\t10", f"Rcut\t\t{rcut}")
+    (base / "in.conf").write_text(text)
+    return str(base / "in.conf")
+
+
+def test_ar_kr_nvt_energy_is_the_lj_tail_only(tmp_path):
+    # read as the GCMC example it is, the frequencies sum to one; the NVT
+    # binary does not read SwapFreq, so there they fall short
+    with pytest.raises(gomc_io.GomcFormatError, match="sum of move frequencies"):
+        gomc_io.system_from_gomc(CONF, ensemble="NVT")
+    s = gomc_io.system_from_gomc(arkr_nvt(tmp_path), ensemble="NVT")
     assert s.ensemble == 0 and s.nmols == 3 and s.box == (34.0, 34.0, 34.0)
     assert [k.name for k in s.kinds] == ["AR", "KR"] and list(s.mol_kind) == [0, 1, 0]
     e = Oracle(s).system_total()
@@ -82,12 +101,7 @@ def test_ar_kr_nvt_energy_is_the_lj_tail_only():
 
 
 def test_ar_kr_one_pair_inside_a_longer_cutoff(tmp_path):
-    base = tmp_path / "Base"
-    shutil.copytree(os.path.join(ARKR, "Base"), base)
-    shutil.copy(os.path.join(ARKR, "Par_Vrabec01_Noble_Gases_Exotic.inp"), tmp_path)
-    text = (base / "in.conf").read_text().replace("Rcut\t\t10", "Rcut\t\t13")
-    (base / "in.conf").write_text(text)
-    s = gomc_io.system_from_gomc(str(base / "in.conf"), ensemble="NVT")
+    s = gomc_io.system_from_gomc(arkr_nvt(tmp_path, rcut=13), ensemble="NVT")
     assert s.rcut == 13.0
     d = np.array([8.340, 31.348, 31.671]) - np.array([13.326, 21.201, 26.910])
     r = float(np.sqrt(d @ d))  # 12.27 A; the other two pairs are 13.30 and 20.98 A
@@ -260,3 +274,54 @@ def test_refusals(tmp_path):
     open(conf, "w").write(text)
     with pytest.raises(gomc_io.GomcFormatError, match="PDB atoms"):
         gomc_io.system_from_gomc(conf)
+
+
+def test_move_frequencies_checked_as_the_reference_does(tmp_path):
+    """ConfigSetup.cpp:2036-2085: the frequencies an ensemble reads must sum
+    to one, and a move that is not built must not be silently dropped."""
+    conf = tmp_path / "in.conf"
+    base = gomc_io.system_from_gomc(tip3_lattice(tmp_path))
+    text = conf.read_text()
+    # a nonzero frequency of an unbuilt move (the BPTI example sets this one)
+    conf.write_text(text.replace("DisFreq 0.6", "DisFreq 0.5\nRegrowthFreq 0.1"))
+    with pytest.raises(gomc_io.GomcFormatError, match="RegrowthFreq"):
+        gomc_io.system_from_gomc(str(conf))
+    conf.write_text(text.replace("DisFreq 0.6", "DisFreq 0.5\nIntraSwapFreq 0.1"))
+    with pytest.raises(gomc_io.GomcFormatError, match="IntraSwapFreq"):
+        gomc_io.system_from_gomc(str(conf))
+    # a zero frequency of an unbuilt move is harmless
+    conf.write_text(text.replace("DisFreq 0.6", "DisFreq 0.6\nCrankShaftFreq 0.0"))
+    assert gomc_io.system_from_gomc(str(conf)).move_disp == base.move_disp
+    # frequencies that do not sum to one
+    conf.write_text(text.replace("DisFreq 0.6", "DisFreq 0.7"))
+    with pytest.raises(gomc_io.GomcFormatError, match="sum of move frequencies"):
+        gomc_io.system_from_gomc(str(conf))
+    # within the reference's 0.001 tolerance
+    conf.write_text(text.replace("DisFreq 0.6", "DisFreq 0.6005"))
+    assert gomc_io.system_from_gomc(str(conf)).move_disp == 0.6005
+    # DisFreq is required
+    conf.write_text(text.replace("DisFreq 0.6\n", "RotFreq 0.6\n"))
+    with pytest.raises(gomc_io.GomcFormatError, match="DisFreq"):
+        gomc_io.system_from_gomc(str(conf))
+    # a volume frequency is read only by an ensemble whose volume varies
+    conf.write_text(text.replace("DisFreq 0.6", "DisFreq 0.6\nVolFreq 0.2"))
+    assert gomc_io.system_from_gomc(str(conf), ensemble="NVT").move_vol == 0.0
+    with pytest.raises(gomc_io.GomcFormatError, match="sum of move frequencies"):
+        gomc_io.system_from_gomc(str(conf), ensemble="NPT")
+
+
+def test_keys_match_without_case(tmp_path):
+    """CheckString upper-cases both strings (ConfigSetup.cpp:194-210)."""
+    conf = tmp_path / "in.conf"
+    base = gomc_io.system_from_gomc(tip3_lattice(tmp_path))
+    text = conf.read_text()
+    conf.write_text(text.replace("DisFreq 0.6", "DISFREQ 0.6").replace("Rcut 8.0", "rcut 8.0"))
+    s = gomc_io.system_from_gomc(str(conf))
+    assert s.move_disp == base.move_disp and s.rcut == base.rcut
+
+
+def test_ar_kr_example_frequencies_sum_to_one():
+    """The AR_KR example (DisFreq 0.87, RotFreq 0.129, SwapFreq 0.001; its
+    VolFreq line is not read by the GCMC binary) reads as given."""
+    s = gomc_io.system_from_gomc(os.path.join(ARKR, "Base", "in.conf"), ensemble="GCMC")
+    assert (s.move_disp, s.move_rot, s.move_swap) == (0.87, 0.129, 0.001)

@@ -123,3 +123,32 @@ def test_foreign_and_truncated_checkpoints_are_refused():
     assert np.array_equal(b.energies(), e0) and np.array_equal(b.coords(), x0)
     assert b.restore(ck) == 7
     b.run(200)
+
+
+def test_other_hamiltonian_and_corrupt_lookup_are_refused_before_the_box_changes():
+    """A checkpoint of the same molecules under another cutoff (another
+    Hamiltonian: its running energies do not belong to this box) and one
+    whose molecule lookup lists a molecule twice are both refused, and the
+    box continues exactly as a box that never saw them."""
+    s = systems.water_box(200, seed=5)
+    b, ref = Box(s), Box(s)
+    b.run(300)
+    ref.run(300)
+    e0, x0 = b.energies(), b.coords()
+    other = Box(replace(s, rcut=8.0, _keep=[]))
+    other.run(300)
+    with pytest.raises(GomcError, match="another force field, cutoff"):
+        b.restore(other.checkpoint(step=300))
+    ck = bytearray(b.checkpoint(step=300))
+    ident = np.arange(s.nmols, dtype=np.uint32).tobytes()
+    at = bytes(ck).find(ident)
+    assert at > 0 and bytes(ck).find(ident, at + 1) < 0
+    bad = bytearray(ck)
+    bad[at + 4:at + 8] = np.uint32(0).tobytes()   # molecule 0 listed twice, 1 never
+    with pytest.raises(GomcError, match="molecule lookup"):
+        b.restore(bytes(bad))
+    assert np.array_equal(b.energies(), e0) and np.array_equal(b.coords(), x0)
+    _, rb = b.run(500, start_step=300)
+    _, rr = ref.run(500, start_step=300)
+    same(rb, rr)
+    assert np.array_equal(b.coords(), ref.coords())

@@ -1,8 +1,9 @@
 """The plain workloads at their full sizes on the device path.
 
 * 1000 waters in a 31.07 A box (plain NVT, rc 10 A + LRC, Ewald 1e-5):
-  the accept sequence of 2000 moves equals the oracle's under the same
-  MT19937 stream; energies within 1e-10 relative.
+  the kind, molecule, overlap and accept sequences of the first 10^4 moves
+  equal the oracle's under the same MT19937 stream; per-move energy
+  changes within 1e-10 relative; energies within 1e-10 relative.
 * 10,000 waters in a 66.9 A box (rc 12 A, Ewald 1e-6, about 10^4
   k-vectors; MultiParticle 1:1 with translate/rotate): the whole-box
   totals and the pair, reciprocal and molecule forces and torques against
@@ -12,9 +13,17 @@
   size-independent check of the MP transform and acceptance bookkeeping.
 * GCMC methane / CO2 in a 40 A box (1200 molecules seeded, fugacities for
   about 2000, a reservoir of 3000): totals with the reservoir excluded, and
-  400 moves (transfers with CBMC growth,
-  translations, rotations) with the oracle's kind, molecule, overlap and
-  accept sequences; final membership identical.
+  10^4 moves (transfers with CBMC growth, translations, rotations) with
+  the oracle's kind, molecule, overlap and accept sequences; final
+  membership identical.
+* The 256-box batch the bench times (N log-uniform 200-4000, rc clamped
+  to L/2 below 20 A, teams of 2/4/8 workgroups, some halved because the
+  device cannot hold every team at full size): one graph-replayed MC cycle,
+  after which every box's running energies equal a from-scratch
+  recomputation of its final coordinates (1e-10 relative); then one
+  traced cycle each of a halved-team box, a team-8 box and an rc-clamped
+  box, whose accept sequences and energy changes equal the oracle's
+  Philox stream.
 """
 import numpy as np
 import pytest
@@ -31,14 +40,22 @@ def rel_close(a, b, tol=1e-9):
     return float(np.abs(a - b).max()) <= tol * max(float(np.abs(b).max()), 1e-300)
 
 
+def same_sequence(rg, ro, totals):
+    for key in ("kind", "mol", "ovl", "acc"):
+        assert np.array_equal(rg[key], ro[key]), (key, np.nonzero(rg[key] != ro[key])[0][:5])
+    for key, e in (("dlj", totals[0]), ("dreal", totals[2]), ("drecip", totals[3])):
+        a, b = rg[key], ro[key]
+        assert np.all(np.abs(a - b) <= 1e-10 * np.maximum(np.abs(b), abs(e))), key
+
+
 def test_plain_nvt_1000_waters_same_sequence_as_oracle():
     s = systems.config1()
     b, o = Box(s), Oracle(s, threads=8)
     assert np.allclose(b.system_total(), o.system_total(), rtol=1e-11, atol=0)
-    n = 2000
+    n = 10000
     ng, rg = b.run(n)
     no, ro = o.run(n, mode=0)
-    assert np.array_equal(rg["acc"], ro["acc"]), np.nonzero(rg["acc"] != ro["acc"])[0][:5]
+    same_sequence(rg, ro, o.energies())
     assert ng == no
     assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=0)
     assert np.allclose(b.coords(), o.coords(), rtol=0, atol=1e-9)
@@ -67,14 +84,52 @@ def test_gcmc_40A_box_same_sequence_as_oracle():
     s = systems.config3()
     b, o = Box(s), Oracle(s, threads=8)
     assert np.allclose(b.system_total(), o.system_total(), rtol=1e-11, atol=1e-9)
-    n = 400
+    n = 10000
     ng, rg = b.run(n)
     no, ro = o.run(n, mode=0)
     assert (ro["kind"] == 10).sum() > n // 5
-    for key in ("kind", "mol", "ovl", "acc"):
-        assert np.array_equal(rg[key], ro[key]), key
+    same_sequence(rg, ro, o.energies())
     assert ng == no
     n1, box1 = b.mol_boxes()
     n2, box2 = o.mol_boxes()
     assert n1 == n2 and np.array_equal(box1, box2)
     assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=1e-6)
+
+
+def test_config4_batch_cycle_and_traced_boxes():
+    from dataclasses import replace
+
+    from gomc_amd.api import Batch
+    sy = systems.config4()
+    bt = Batch(sy)  # the launch shape the bench uses
+    tmin = bt.tuning["team_min"]
+    full = [8 if s.nmols >= tmin[2] else 4 if s.nmols >= tmin[1] else 2 if s.nmols >= tmin[0]
+            else 1 for s in sy]
+    team = [bt.layout(i)["team"] for i in range(len(sy))]
+    halved = [i for i in range(len(sy)) if team[i] < full[i]]
+    eight = [i for i in range(len(sy)) if team[i] == 8]
+    clamped = [i for i in range(len(sy)) if sy[i].rcut < 10.0]
+    assert halved and eight and clamped, (len(halved), len(eight), len(clamped))
+    # one MC cycle replayed from the captured graph: running energies of
+    # every box equal a recomputation from its final coordinates
+    bt.run_cycles(1, graph=True)
+    e, _, done = bt.stats()
+    assert np.array_equal(done, [s.nmols for s in sy])
+    for i, s in enumerate(sy):
+        fresh = Box(replace(s, xyz=bt.coords(i), _keep=[])).system_total()
+        assert np.allclose(e[i], fresh, rtol=1e-10, atol=1e-6), (i, e[i], fresh)
+    # one traced cycle per chosen box, against the oracle's Philox stream
+    for c, i in enumerate((halved[-1], eight[0], clamped[0]), start=1):
+        s, n = sy[i], sy[i].nmols
+        bt.set_trace(i, n)
+        bt.run()
+        tr = bt.trace()
+        o = Oracle(s, threads=8)
+        o.run(c * n, mode=1, record=False)
+        e0 = o.energies()
+        no, ro = o.run(n, start_step=c * n, mode=1)
+        assert np.array_equal(tr["acc"], ro["acc"]), (i, np.nonzero(tr["acc"] != ro["acc"])[0][:5])
+        for key, tot in (("dlj", e0[0]), ("dreal", e0[2]), ("drecip", e0[3])):
+            assert np.all(np.abs(tr[key] - ro[key]) <= 1e-10 * np.maximum(np.abs(ro[key]), abs(tot)))
+        assert np.allclose(bt.stats()[0][i], o.energies(), rtol=1e-10, atol=0)
+        assert np.allclose(bt.coords(i), o.coords(), rtol=0, atol=1e-9)

@@ -71,7 +71,10 @@ def ar_kr_gcmc(tmp_path, n_res=1000):
     gomc_io.write_psf(str(base / "START_BOX_1.psf"), res, ids, res, res, res,
                       [0.0] * len(xyz), mass, np.zeros((0, 2), np.int64))
     conf = base / "in.conf"
-    conf.write_text(conf.read_text().replace("SwapFreq\t      0.001", "SwapFreq\t      0.3"))
+    # more transfers than the example's 0.1 %, displacements lowered so that
+    # the frequencies still sum to one
+    txt = conf.read_text().replace("SwapFreq\t      0.001", "SwapFreq\t      0.3")
+    conf.write_text(txt.replace("DisFreq               0.87", "DisFreq               0.571"))
     return str(conf)
 
 
