@@ -9,9 +9,15 @@ Reported beside it:
   * the plain FP64 GPU path (one fused launch per move, host Metropolis,
     MT19937 draws -- the reference's call pattern with state kept in HBM),
     on a bounded sample of moves per box;
-  * a bounded serving run: a Poisson stream of water state-point jobs through
-    the continuous-batching server (moves/s, p50/p99 latency);
-  * the CPU oracle (C++/OpenMP, one box per thread) on a bounded sample.
+  * the single-box workloads (1000 waters; 10,000 waters with MultiParticle
+    moves 1:1; GCMC methane/CO2 in a 40 A box): GPU moves/s on the host
+    driver and the device move loop, and the CPU oracle on all cores and on
+    one core, on bounded samples;
+  * serving: a Poisson stream of water state-point jobs through the
+    continuous-batching server, and the GCMC adsorption stream (60 jobs of
+    2000 MC cycles, 200 GB budget) through its GCMC lane (moves/s, p50/p99);
+  * the CPU oracle (C++/OpenMP, one box per thread) on a bounded sample of
+    the 256-box workload, on all cores and on one.
 
 Prints one JSON line.
 """
@@ -33,6 +39,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = ("MC trial moves/s (1 GPU), batched+served vs plain at ≤1e-6 rel ΔE; "
           "p99 latency")
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per launch of McBatchKernel<0> on the default 256-box workload,
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
 # (profiles/McBatchKernel_water256_fp64_hbm.md); reported for that workload.
@@ -57,6 +64,27 @@ def move_flops(s, nkvec: int) -> float:
     Lj = 2 * 1 * ((s.nmols - 1) * frac)          # O-O pairs in cutoff
     S = 2 * a * nkvec                            # sincos (old + new)
     return D * W_DIST + C * W_COUL + Lj * W_LJ + S * W_SINCOS + nkvec * W_KENERGY
+
+
+def move_bytes(s, nkvec: int) -> float:
+    """Algorithmic (compulsory) HBM bytes of one single-molecule trial:
+    64 K for the k-vector data (kx, ky, kz, prefactor and the two
+    structure-factor sums read, the two new sums written, 8 B each) plus
+    40 M for the M atoms of the 27-cell neighbourhood (x, y, z, kind,
+    charge, molecule id)."""
+    L = s.box[0]
+    edge = max(int(L / max(s.rcut, s.rcut_coulomb)), 3)
+    M = s.natoms * min(1.0, 27.0 / edge ** 3)
+    return 64.0 * nkvec + 40.0 * M
+
+
+def cpu_threads() -> tuple[int, int]:
+    """(threads used, cores visible): every core this process may run on,
+    limited by OMP_NUM_THREADS where the host sets it (a GPU box's share
+    of its host)."""
+    seen = len(os.sched_getaffinity(0))
+    lim = os.environ.get("OMP_NUM_THREADS", "")
+    return (min(seen, int(lim)) if lim.isdigit() and int(lim) > 0 else seen), seen
 
 
 def copy_reference(nbytes: int) -> None:
@@ -182,6 +210,88 @@ def next_rows(cpu_threads: int, with_cpu: bool) -> dict:
     return out
 
 
+def config_rows(threads: int, with_cpu: bool) -> dict:
+    """Moves/s on the single-box workloads: 1000 waters (translate/rotate
+    1:1), 10,000 waters (MultiParticle 1:1 with translate/rotate) and GCMC
+    methane/CO2 in a 40 A box (40 % transfers with CBMC), on the GPU (host
+    driver: one fused launch per energy call; device move loop: whole moves
+    in one launch) and on the CPU oracle with `threads` threads and with one,
+    each on a bounded sample of the same move stream."""
+    from dataclasses import replace
+
+    from gomc_amd import systems
+    from gomc_amd.api import Box
+    from oracle.pyoracle import Oracle
+
+    def gpu_rate(s, n, device=False, warm=20):
+        b = Box(s)
+        b.run(warm, record=False, device=device)
+        t = time.perf_counter()
+        _, rec = b.run(n, start_step=warm, device=device)
+        dt = time.perf_counter() - t
+        b.close()
+        kinds = {int(k): int((rec["kind"] == k).sum()) for k in np.unique(rec["kind"])}
+        return n / dt, kinds
+
+    def cpu_rate(s, n, th, warm=5):
+        o = Oracle(s, threads=th)
+        o.run(warm, record=False)
+        t = time.perf_counter()
+        o.run(n, start_step=warm, record=False)
+        dt = time.perf_counter() - t
+        o.close()
+        return n / dt
+
+    out = {}
+    s1 = systems.config1()
+    r, k = gpu_rate(s1, 10000)
+    rd, _ = gpu_rate(s1, 10000, device=True)
+    row = dict(workload="1000 SPC/E waters, L 31.07 A, rc 10 A + LRC, Ewald 1e-5, "
+                        "translate/rotate 1:1", gpu_host_driver_moves_per_s=r,
+               gpu_device_loop_moves_per_s=rd, moves_by_kind=k, gpu_sample="10000 moves")
+    if with_cpu:
+        row.update(cpu_moves_per_s=cpu_rate(s1, 2000, threads), cpu_threads=threads,
+                   cpu_1thread_moves_per_s=cpu_rate(s1, 600, 1), cpu_sample="2000 / 600 moves")
+    out["config1"] = row
+    print(f"# config1 {row}", file=sys.stderr, flush=True)
+
+    s2 = systems.config2()
+    r, k = gpu_rate(s2, 200, warm=4)
+    row = dict(workload="10,000 SPC/E waters, L 66.9 A, rc 12 A, Ewald 1e-6 (30,902 "
+                        "k-vectors), MultiParticle 1:1 with translate/rotate",
+               gpu_host_driver_moves_per_s=r, moves_by_kind=k, gpu_sample="200 moves")
+    if with_cpu:
+        # a MultiParticle move takes the oracle seconds: its time and the
+        # single-move time are measured apart and combined 1:1
+        single = cpu_rate(replace(s2, move_disp=0.5, move_rot=0.5, move_mp=0.0, _keep=[]), 100,
+                          threads)
+        o = Oracle(replace(s2, move_disp=0.0, move_rot=0.0, move_mp=1.0, _keep=[]),
+                   threads=threads)
+        t = time.perf_counter()
+        o.run(1, record=False)
+        t_mp = time.perf_counter() - t
+        o.close()
+        row.update(cpu_moves_per_s=1.0 / (0.5 / single + 0.5 * t_mp), cpu_threads=threads,
+                   cpu_single_moves_per_s=single, cpu_s_per_mp_move=t_mp,
+                   cpu_sample="100 single moves and 1 MultiParticle move, combined 1:1")
+    out["config2"] = row
+    print(f"# config2 {row}", file=sys.stderr, flush=True)
+
+    s3 = systems.config3()
+    r, k = gpu_rate(s3, 10000)
+    rd, _ = gpu_rate(s3, 10000, device=True)
+    row = dict(workload="GCMC methane/CO2, L 40 A, 300 K, rc 10 A, Ewald 1e-5, 1200 "
+                        "molecules seeded, 40 % transfers (CBMC 16/8), 30 % translate, "
+                        "30 % rotate", gpu_host_driver_moves_per_s=r,
+               gpu_device_loop_moves_per_s=rd, moves_by_kind=k, gpu_sample="10000 moves")
+    if with_cpu:
+        row.update(cpu_moves_per_s=cpu_rate(s3, 3000, threads), cpu_threads=threads,
+                   cpu_1thread_moves_per_s=cpu_rate(s3, 800, 1), cpu_sample="3000 / 800 moves")
+    out["config3"] = row
+    print(f"# config3 {row}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,7 +308,10 @@ def main() -> None:
     ap.add_argument("--no-next", action="store_true",
                     help="skip the virial / NPT / replica-exchange / checkpoint timings")
     ap.add_argument("--serve-jobs", type=int, default=60, help="0 skips the serving run")
-    ap.add_argument("--serve-gcmc-cycles", type=int, default=100)
+    ap.add_argument("--serve-gcmc-cycles", type=int, default=2000)
+    ap.add_argument("--serve-gcmc-budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the single-box workload rows (configs 1-3)")
     ap.add_argument("--serve-gcmc-jobs", type=int, default=60,
                     help="GCMC state-point jobs in the served run (0 skips)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -233,6 +346,7 @@ def main() -> None:
         ks = (c * c) * (X * X + Y * Y + Z * Z)
         nk.append(int(np.count_nonzero(half & (ks < kc * kc))))
     flops_step = float(sum(nmol[i] * move_flops(boxes[i], nk[i]) for i in range(len(boxes))))
+    bytes_step = float(sum(nmol[i] * move_bytes(boxes[i], nk[i]) for i in range(len(boxes))))
     print(f"# batch ready ({time.time() - t0:.1f}s); warmup {args.warmup}", file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
@@ -248,6 +362,7 @@ def main() -> None:
     value = moves / elapsed
     avg_ms = float(np.mean(kernel_ms))
     achieved_tflops = flops_step / (avg_ms * 1e-3) / 1e12
+    achieved_gbs = bytes_step / (avg_ms * 1e-3) / 1e9
     e, acc, done = bt.stats()
     accept_rate = float(acc.sum() / done.sum())
 
@@ -290,25 +405,12 @@ def main() -> None:
     served_gcmc = None
     if args.serve_gcmc_jobs > 0:
         from gomc_amd.serve import Server, poisson_stream_gcmc, run_stream
-        from oracle.pyoracle import Oracle
         jobs = poisson_stream_gcmc(args.serve_gcmc_jobs, rate=20.0, seed=17, lmin=20.0,
                                    lmax=60.0, cycles=args.serve_gcmc_cycles)
-        sv = Server(precision=1, slots=64, budget_bytes=8 << 30, tick_ms=20.0)
-        rep = run_stream(sv, jobs, timeout=600.0)
+        sv = Server(precision=1, slots=64, budget_bytes=int(args.serve_gcmc_budget_gb * 2**30),
+                    tick_ms=20.0)
+        rep = run_stream(sv, jobs, timeout=480.0, progress=30.0)
         sv.close()
-        # <N> of the first job against the FP64 oracle under the same draws
-        j0, r0 = jobs[0], rep["results"][0]
-        o = Oracle(j0.system, threads=min(16, os.cpu_count() or 1))
-        done, sums, samples = 0, np.zeros(2), 0
-        while done < j0.moves:
-            n_in, _ = o.mol_boxes()
-            chunk = min(max(1, n_in), j0.moves - done)
-            o.run(chunk, start_step=done, mode=0, record=False)
-            done += chunk
-            _, where = o.mol_boxes()
-            for k in (0, 1):
-                sums[k] += np.count_nonzero((where == 0) & (j0.system.mol_kind == k))
-            samples += 1
         served_gcmc = dict(
             workload=f"Poisson(20/s) stream of {rep['jobs']} GCMC methane/CO2 adsorption jobs, "
                      f"L 20-60 A, T 250-350 K, fugacities log-uniform 1e3-1e7 Pa, "
@@ -316,11 +418,12 @@ def main() -> None:
                      "mixed precision, every running job in each launch)",
             moves_per_s=rep["moves_per_s"], p50_s=rep["p50_s"], p99_s=rep["p99_s"],
             moves=rep["moves"], wall_s=rep["wall_s"],
-            mean_n_job0=r0["mean_n"].tolist(), mean_n_job0_oracle=(sums / samples).tolist())
+            budget_bytes=rep["stats"]["budget_bytes"], peak_bytes=rep["stats"]["peak_bytes"],
+            mean_n_range=[float(min(r["mean_n"].sum() for r in rep["results"])),
+                          float(max(r["mean_n"].sum() for r in rep["results"]))])
         print(f"# served GCMC {served_gcmc['moves_per_s']:.0f} moves/s p50 "
-              f"{served_gcmc['p50_s']:.2f} s p99 {served_gcmc['p99_s']:.2f} s; <N> job 0 "
-              f"{served_gcmc['mean_n_job0']} oracle {served_gcmc['mean_n_job0_oracle']}",
-              file=sys.stderr, flush=True)
+              f"{served_gcmc['p50_s']:.2f} s p99 {served_gcmc['p99_s']:.2f} s; <N> per job "
+              f"{served_gcmc['mean_n_range']}", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain:
@@ -341,9 +444,9 @@ def main() -> None:
         print(f"# plain path {plain_rate:.0f} moves/s", file=sys.stderr, flush=True)
 
     cpu = None
+    threads, seen = cpu_threads()
     if not args.no_cpu:
         from oracle.pyoracle import Oracle, batch_run
-        threads = min(16, os.cpu_count() or 1)
         oracles = [Oracle(b, threads=1) for b in boxes]
         # calibrate: a short run, then scale to ~cpu-seconds of work
         probe = np.maximum(1, nmol // 200)
@@ -352,15 +455,24 @@ def main() -> None:
         frac = max(1e-3, min(1.0, args.cpu_seconds * rate / nmol.sum()))
         sample = np.maximum(1, (nmol * frac).astype(np.int64))
         tc = batch_run(oracles, sample, threads)
+        # the same work model on one core: a smaller sample of the same boxes
+        s1 = np.maximum(1, (sample // max(threads, 1)).astype(np.int64))
+        t1 = batch_run(oracles, s1, 1)
         cpu = dict(value=float(sample.sum() / tc), unit="moves/s", cores=threads, kind="port",
                    sample=f"{int(sample.sum())} Philox-stream moves over the same {len(boxes)} "
-                          f"boxes ({frac:.3f} of one MC cycle per box), one box per thread")
-        print(f"# cpu oracle {cpu['value']:.0f} moves/s on {threads} threads", file=sys.stderr,
-              flush=True)
+                          f"boxes ({frac:.3f} of one MC cycle per box), one box per thread",
+                   cores_visible=seen, value_1thread=float(s1.sum() / t1),
+                   sample_1thread=f"{int(s1.sum())} moves over the same boxes on one thread")
+        print(f"# cpu oracle {cpu['value']:.0f} moves/s on {threads} threads, "
+              f"{cpu['value_1thread']:.0f} on one", file=sys.stderr, flush=True)
+
+    cfg = None
+    if not args.no_configs:
+        cfg = config_rows(threads, not args.no_cpu)
 
     nxt = None
     if not args.no_next:
-        nxt = next_rows(min(16, os.cpu_count() or 1), not args.no_cpu)
+        nxt = next_rows(threads, not args.no_cpu)
 
     if args.calibrate_copy > 0:
         copy_reference(args.calibrate_copy << 20)
@@ -401,17 +513,27 @@ def main() -> None:
         "mixed_speedup_vs_plain": (mixed["moves_per_s"] / plain["moves_per_s"])
         if (plain and mixed) else None,
         "roofline": {
+            "bound": "hbm",
+            "achieved": achieved_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved_gbs / HBM_PEAK_GBS,
+            "traffic": PROFILED_TRAFFIC["bytes"] if default_workload else None,
+            "traffic_source": PROFILED_TRAFFIC["source"] if default_workload else None,
+            "kernel": "McBatchKernel",
+            "bytes_per_launch": bytes_step,
+            "byte_model": "per move 64 K (k-vector data) + 40 M (27-cell neighbourhood atoms)",
+        },
+        "roofline_valu": {
             "bound": "valu",
             "achieved": achieved_tflops,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-            "traffic": PROFILED_TRAFFIC["bytes"] if default_workload else None,
-            "traffic_source": PROFILED_TRAFFIC["source"] if default_workload else None,
-            "kernel": "McBatchKernel",
             "flops_per_launch": flops_step,
         },
         "cpu_baseline": cpu,
+        "configs_1_3": cfg,
         "next_rows": nxt,
     }
     print(json.dumps(out))

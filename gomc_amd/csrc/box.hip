@@ -5,7 +5,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "box.hpp"
 
@@ -299,89 +302,157 @@ __global__ void __launch_bounds__(kBlock) BoxInterKernel(const BoxInterArgs a) {
 }
 
 // ------------------------------------------------------ structure factor --
-// S(k) = sum_j q_j (cos, sin)(k . r_j) over all charged atoms, one k per
-// thread, atoms staged through LDS (Ewald.cpp:193-274), then the per-k
-// energy prefact (Re^2 + Im^2) (Ewald.cpp:375-406).
+// S(k) = sum_j q_j (cos, sin)(k . r_j) over all charged atoms
+// (Ewald.cpp:193-274), then the per-k energy prefact (Re^2 + Im^2)
+// (Ewald.cpp:375-406).
+//
+// The k list comes in GOMC's x, y, z loop order, so it is made of runs of
+// consecutive nz at fixed (nx, ny).  KSegmentsKernel cuts it into segments:
+// pieces of a run of at most kSeg vectors.  Along a segment the phase of an
+// atom advances by one complex product, e^{i k_{n+1} . r} =
+// e^{i k_n . r} e^{i dz z}, so a segment costs two sincos per atom (its
+// first vector and the step) instead of one per vector.  The first pass
+// takes one segment per thread and one chunk of atoms per work item
+// (per-chunk sums into part[chunk][k][2]); the second adds the chunks in
+// order, one k per thread.  Both passes read the segment count from device
+// memory, so a launch never waits on the host.
 struct SumsArgs {
   const double *x, *y, *z, *q;
-  const int *mol;
+  const int *mol;  // may be null: every atom counts
   int natoms, K;
   const double *kx, *ky, *kz, *pref;
+  const int *kseg;   // segment starts [nseg + 1], kseg[nseg] = K
+  const int *nseg;   // device
   double *sumR, *sumI;
+  double *part;      // [chunks][K][2]
   Partial4 *partials;
   unsigned *counter;
   double *result;
-  // Atom chunks: with aChunks > 1, RecipSumsPartKernel writes the sums of
-  // each chunk of aChunk atoms to part[chunk][k][2] and RecipSumsKernel adds
-  // the chunks in order, so that a box with few k blocks still fills the chip.
-  int kBlocks, aChunk, aChunks;
-  double *part;
 };
 
-// S(k) += sum over atoms [a0, a1) of q (cos, sin)(k.r), atoms staged through
-// LDS one tile at a time; every thread of the block takes part in staging.
-__device__ __forceinline__ void SumsOverAtoms(const SumsArgs &a, bool active, double kxk,
-                                              double kyk, double kzk, int a0, int a1,
-                                              double *sx, double *sy, double *sz, double *sq,
-                                              double &sR, double &sI) {
-  for (int tile = a0; tile < a1; tile += kBlock) {
-    const int tn = min(kBlock, a1 - tile);
-    if (threadIdx.x < tn) {
-      int j = tile + threadIdx.x;
-      sx[threadIdx.x] = a.x[j];
-      sy[threadIdx.x] = a.y[j];
-      sz[threadIdx.x] = a.z[j];
-      sq[threadIdx.x] = a.mol[j] >= 0 ? a.q[j] : 0.0;  // reservoir atoms add nothing
-    }
+__device__ __forceinline__ bool RunStart(const double *kx, const double *ky, const double *kz,
+                                         int k) {
+  return k == 0 || kx[k] != kx[k - 1] || ky[k] != ky[k - 1] || !(kz[k] > kz[k - 1]);
+}
+
+// One workgroup of kSegThreads: each thread takes a contiguous range of the
+// K vectors; a max-scan carries the run start in effect into each range, a
+// sum-scan places each range's segments.
+__global__ void __launch_bounds__(kSegThreads) KSegmentsKernel(const double *kx, const double *ky,
+                                                               const double *kz, int K, int *kseg,
+                                                               int *nseg) {
+  __shared__ int sRun[kSegThreads], sCnt[kSegThreads];
+  const int t = threadIdx.x;
+  const int per = (K + kSegThreads - 1) / kSegThreads;
+  const int lo = min(K, t * per), hi = min(K, lo + per);
+  int last = -1;
+  for (int k = lo; k < hi; ++k)
+    if (RunStart(kx, ky, kz, k)) last = k;
+  sRun[t] = last;
+  __syncthreads();
+  for (int off = 1; off < kSegThreads; off <<= 1) {  // inclusive max-scan
+    const int v = t >= off ? sRun[t - off] : -1;
     __syncthreads();
-    if (active)
-      for (int jj = 0; jj < tn; ++jj) {
-        if (fabs(sq[jj]) < 0.000000001) continue;
-        double dot = sx[jj] * kxk + sy[jj] * kyk + sz[jj] * kzk;
-        double s, c;
-        sincos(dot, &s, &c);
-        sR += sq[jj] * c;
-        sI += sq[jj] * s;
-      }
+    sRun[t] = max(sRun[t], v);
     __syncthreads();
+  }
+  const int runAtLo = t > 0 ? sRun[t - 1] : -1;
+  int run = runAtLo, cnt = 0;
+  for (int k = lo; k < hi; ++k) {
+    if (RunStart(kx, ky, kz, k)) run = k;
+    if ((k - run) % kSeg == 0) ++cnt;
+  }
+  sCnt[t] = cnt;
+  __syncthreads();
+  for (int off = 1; off < kSegThreads; off <<= 1) {  // inclusive sum-scan
+    const int v = t >= off ? sCnt[t - off] : 0;
+    __syncthreads();
+    sCnt[t] += v;
+    __syncthreads();
+  }
+  int out = sCnt[t] - cnt;
+  run = runAtLo;
+  for (int k = lo; k < hi; ++k) {
+    if (RunStart(kx, ky, kz, k)) run = k;
+    if ((k - run) % kSeg == 0) kseg[out++] = k;
+  }
+  if (t == kSegThreads - 1) {
+    kseg[sCnt[t]] = K;
+    *nseg = sCnt[t];
   }
 }
 
 __global__ void __launch_bounds__(kBlock) RecipSumsPartKernel(const SumsArgs a) {
   __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
-  const int kb = blockIdx.x % a.kBlocks, chunk = blockIdx.x / a.kBlocks;
-  const int k = kb * kBlock + threadIdx.x;
-  const bool active = k < a.K;
-  const double kxk = active ? a.kx[k] : 0.0, kyk = active ? a.ky[k] : 0.0,
-               kzk = active ? a.kz[k] : 0.0;
-  const int a0 = chunk * a.aChunk, a1 = min(a.natoms, a0 + a.aChunk);
-  double sR = 0.0, sI = 0.0;
-  SumsOverAtoms(a, active, kxk, kyk, kzk, a0, a1, sx, sy, sz, sq, sR, sI);
-  if (active) {
-    a.part[(size_t(chunk) * a.K + k) * 2] = sR;
-    a.part[(size_t(chunk) * a.K + k) * 2 + 1] = sI;
+  const int nseg = *a.nseg;
+  const int segBlocks = max(1, (nseg + kBlock - 1) / kBlock);
+  const int chunks = SumsChunks(nseg, a.natoms, a.K);
+  const int aChunk = (a.natoms + chunks - 1) / chunks;
+  for (int item = blockIdx.x; item < segBlocks * chunks; item += gridDim.x) {
+    const int seg = (item % segBlocks) * kBlock + threadIdx.x, chunk = item / segBlocks;
+    const bool active = seg < nseg;
+    const int k0 = active ? a.kseg[seg] : 0;
+    const int len = active ? a.kseg[seg + 1] - k0 : 0;
+    const double kx0 = active ? a.kx[k0] : 0.0, ky0 = active ? a.ky[k0] : 0.0,
+                 kz0 = active ? a.kz[k0] : 0.0;
+    const double dz = len > 1 ? a.kz[k0 + 1] - kz0 : 0.0;
+    double sR[kSeg], sI[kSeg];
+#pragma unroll
+    for (int i = 0; i < kSeg; ++i) sR[i] = sI[i] = 0.0;
+    const int a0 = min(a.natoms, chunk * aChunk), a1 = min(a.natoms, a0 + aChunk);
+    for (int tile = a0; tile < a1; tile += kBlock) {
+      const int tn = min(kBlock, a1 - tile);
+      __syncthreads();
+      if (int(threadIdx.x) < tn) {
+        const int j = tile + threadIdx.x;
+        sx[threadIdx.x] = a.x[j];
+        sy[threadIdx.x] = a.y[j];
+        sz[threadIdx.x] = a.z[j];
+        sq[threadIdx.x] = (!a.mol || a.mol[j] >= 0) ? a.q[j] : 0.0;  // reservoir adds nothing
+      }
+      __syncthreads();
+      if (active)
+        for (int jj = 0; jj < tn; ++jj) {
+          const double qj = sq[jj];
+          if (fabs(qj) < 0.000000001) continue;
+          double im, re, ss, cs;
+          sincos(sx[jj] * kx0 + sy[jj] * ky0 + sz[jj] * kz0, &im, &re);
+          sincos(dz * sz[jj], &ss, &cs);
+#pragma unroll
+          for (int i = 0; i < kSeg; ++i) {
+            if (i < len) {
+              sR[i] += qj * re;
+              sI[i] += qj * im;
+            }
+            const double nr = re * cs - im * ss;
+            im = re * ss + im * cs;
+            re = nr;
+          }
+        }
+    }
+    if (active) {
+      double *out = a.part + (size_t(chunk) * a.K + k0) * 2;
+#pragma unroll
+      for (int i = 0; i < kSeg; ++i)
+        if (i < len) {
+          out[2 * i] = sR[i];
+          out[2 * i + 1] = sI[i];
+        }
+    }
   }
 }
 
 __global__ void __launch_bounds__(kBlock) RecipSumsKernel(const SumsArgs a) {
-  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
   __shared__ double red[kBlock / kWave];
   const int k = blockIdx.x * kBlock + threadIdx.x;
   const bool active = k < a.K;
-  const double kxk = active ? a.kx[k] : 0.0, kyk = active ? a.ky[k] : 0.0,
-               kzk = active ? a.kz[k] : 0.0;
-  double sR = 0.0, sI = 0.0;
-  if (a.aChunks > 1) {
-    if (active)
-      for (int c = 0; c < a.aChunks; ++c) {
-        sR += a.part[(size_t(c) * a.K + k) * 2];
-        sI += a.part[(size_t(c) * a.K + k) * 2 + 1];
-      }
-  } else {
-    SumsOverAtoms(a, active, kxk, kyk, kzk, 0, a.natoms, sx, sy, sz, sq, sR, sI);
-  }
-  double e = 0.0;
+  const int chunks = SumsChunks(*a.nseg, a.natoms, a.K);
+  double sR = 0.0, sI = 0.0, e = 0.0;
   if (active) {
+    for (int c = 0; c < chunks; ++c) {
+      sR += a.part[(size_t(c) * a.K + k) * 2];
+      sI += a.part[(size_t(c) * a.K + k) * 2 + 1];
+    }
     a.sumR[k] = sR;
     a.sumI[k] = sI;
     e = (sR * sR + sI * sI) * a.pref[k];
@@ -691,6 +762,35 @@ int BoxInter(gomc_box *b, double *lj, double *real) {
   return GOMC_OK;
 }
 
+// Both passes on stream s.  a carries the arrays: the segments of the k
+// list, part [SumsChunks(nseg) x K x 2], partials [ceil(K / kBlock)],
+// counter (zero, re-armed by the last block) and result [4] (result[2]
+// receives the energy).
+hipError_t LaunchRecipSums(const SumsArgs &a, hipStream_t s) {
+  hipLaunchKernelGGL(RecipSumsPartKernel, dim3(kSumsGrid), dim3(kBlock), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int nblocks = std::max(1, (a.K + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL(RecipSumsKernel, dim3(nblocks), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+int BuildKSegments(gomc_box *b) {
+  if (b->K + 1 > b->ksegCap) {
+    if (b->kseg) (void)hipFree(b->kseg);
+    b->kseg = nullptr;
+    b->ksegCap = 0;
+    HIP_TRY(b, hipMalloc(&b->kseg, sizeof(int) * (b->K + 1)));
+    b->ksegCap = b->K + 1;
+  }
+  if (!b->nsegDev) HIP_TRY(b, hipMalloc(&b->nsegDev, sizeof(int)));
+  hipLaunchKernelGGL(KSegmentsKernel, dim3(1), dim3(kSegThreads), 0, b->stream, b->kx, b->ky,
+                     b->kz, b->K, b->kseg, b->nsegDev);
+  HIP_TRY(b, hipGetLastError());
+  HIP_TRY(b, hipMemcpyAsync(&b->nseg, b->nsegDev, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  return Sync(b);
+}
+
 // BoxReciprocalSums into the "new" sums; returns the reciprocal energy.
 int RecipSums(gomc_box *b, double *erecip) {
   if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
@@ -706,6 +806,8 @@ int RecipSums(gomc_box *b, double *erecip) {
   a.ky = b->ky;
   a.kz = b->kz;
   a.pref = b->pref;
+  a.kseg = b->kseg;
+  a.nseg = b->nsegDev;
   a.sumR = b->sumRnew;
   a.sumI = b->sumInew;
   const int nblocks = std::max(1, (b->K + kBlock - 1) / kBlock);
@@ -714,35 +816,23 @@ int RecipSums(gomc_box *b, double *erecip) {
   a.partials = b->partials;
   a.counter = b->counter;
   a.result = b->devRes;
-  // atom chunks of whole tiles, about 2048 workgroups in the first pass
-  const int tiles = std::max(1, (b->natoms + kBlock - 1) / kBlock);
-  int chunks = std::min(tiles, std::max(1, (2048 + nblocks - 1) / nblocks));
-  const int tilesPerChunk = (tiles + chunks - 1) / chunks;
-  chunks = (tiles + tilesPerChunk - 1) / tilesPerChunk;
-  a.kBlocks = nblocks;
-  a.aChunk = tilesPerChunk * kBlock;
-  a.aChunks = chunks;
-  a.part = nullptr;
-  if (chunks > 1) {
-    const size_t need = size_t(chunks) * size_t(std::max(b->K, 1)) * 2;
-    if (need > b->sumPartCap) {
-      if (b->sumPart) (void)hipFree(b->sumPart);
-      b->sumPart = nullptr;
-      b->sumPartCap = 0;
-      HIP_TRY(b, hipMalloc(&b->sumPart, sizeof(double) * need));
-      b->sumPartCap = need;
-    }
-    a.part = b->sumPart;
-    hipLaunchKernelGGL(RecipSumsPartKernel, dim3(nblocks * chunks), dim3(kBlock), 0, b->stream, a);
-    HIP_TRY(b, hipGetLastError());
+  const size_t need = size_t(SumsChunks(b->nseg, b->natoms, b->K)) * size_t(b->K) * 2;
+  if (need > b->sumPartCap) {
+    if (b->sumPart) (void)hipFree(b->sumPart);
+    b->sumPart = nullptr;
+    b->sumPartCap = 0;
+    HIP_TRY(b, hipMalloc(&b->sumPart, sizeof(double) * need));
+    b->sumPartCap = need;
   }
-  hipLaunchKernelGGL(RecipSumsKernel, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
-  HIP_TRY(b, hipGetLastError());
+  a.part = b->sumPart;
+  HIP_TRY(b, LaunchRecipSums(a, b->stream));
   rc = Sync(b);
   if (rc) return rc;
   *erecip = b->hostRes[2];
   return GOMC_OK;
 }
+
+
 
 static double BoxHostMinImage(double d, int axis, const gomc_box *b) {
   const double ax = b->geom.L[axis], half = b->geom.half[axis];
@@ -799,7 +889,7 @@ static void FreeAll(gomc_box *b) {
                   b->tSigSq,   b->tEpsCn,  b->tNHalf,   b->tEpsCn6,  b->tNOver6, b->cellCount,
                   b->cellCursor, b->cellStart, b->cellVec, b->atomCell, b->nbr,   b->kx,
                   b->ky,       b->kz,      b->pref,     b->sumRref,  b->sumIref, b->sumRnew,
-                  b->sumInew,  b->partials, b->counter};
+                  b->sumInew,  b->partials, b->counter, b->kseg, b->nsegDev};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   MPFree(b);
@@ -1049,6 +1139,21 @@ int SetMolLive(gomc_box *b, int m, bool live) {
 }  // namespace gomc
 
 extern "C" {
+
+int gomc_set_mol_box(gomc_box *b, int mol, int box) {
+  if (!b || mol < 0 || mol >= b->nmols || (box != 0 && box != 1)) return GOMC_ERR_ARG;
+  if (box == 1 && b->ensemble != 1)
+    return Fail(b, GOMC_ERR_STATE, "only a GCMC box has a reservoir");
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);
+  if (rc) return rc;
+  if (b->molBox[mol] == box) return GOMC_OK;
+  DevFree(b);  // the device move loop keeps its own copy of the lookup
+  b->kindCount[b->molKind[mol]] += box == 0 ? 1 : -1;
+  b->molBox[mol] = box;
+  LookupInit(b);
+  return SetMolLive(b, mol, box == 0);
+}
 
 int gomc_build_cells(gomc_box *b, int32_t *cell_vector, int32_t *cell_start, int32_t *atom_cell) {
   if (!b) return GOMC_ERR_ARG;

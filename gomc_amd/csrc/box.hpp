@@ -23,6 +23,29 @@ struct DevDriver;  // buffers of the device move loop (gcmc_dev.hip)
 constexpr int kMaxMolAtoms = 16;   // atoms per molecule in a trial launch
 constexpr int kBlock = 256;        // threads per workgroup, plain kernels
 
+// Structure factors and reciprocal forces walk the k list in segments:
+// pieces of at most kSeg consecutive-nz vectors of one (nx, ny) run, built
+// on the device by one workgroup of kSegThreads (box.hip).
+constexpr int kSeg = 16;
+constexpr int kSegThreads = 1024;
+// First structure-factor pass: a fixed grid of kSumsGrid workgroups loops
+// over (segment block, atom chunk) items; chunks x K stays within
+// kSumsPartCap so the per-chunk sums fit a bounded buffer.
+constexpr int kSumsGrid = 1024;
+constexpr size_t kSumsPartCap = size_t(1) << 21;
+
+// Atom chunks of the first structure-factor pass: about kSumsGrid items
+// over the segment blocks, chunks of at least 32 atoms, within the cap.
+__host__ __device__ inline int SumsChunks(int nseg, int natoms, int K) {
+  const int segBlocks = nseg > kBlock ? (nseg + kBlock - 1) / kBlock : 1;
+  int c = (kSumsGrid + segBlocks - 1) / segBlocks;
+  const int byAtoms = natoms > 32 ? (natoms + 31) / 32 : 1;
+  if (c > byAtoms) c = byAtoms;
+  const int byCap = K > 0 ? int(kSumsPartCap / size_t(K)) : 1;
+  if (c > byCap) c = byCap;
+  return c > 1 ? c : 1;
+}
+
 // Per-block partial of a fused trial: LJ, real, reciprocal, overlap.
 struct Partial4 {
   double lj, real, recip, overlap;
@@ -167,6 +190,10 @@ struct gomc_box {
   // per-chunk structure factors of RecipSums [chunks][K][2]
   double *sumPart = nullptr;
   size_t sumPartCap = 0;
+  // k segments (box.hip): starts [nseg + 1] and the count, on the device;
+  // nseg also on the host for launch shapes
+  int *kseg = nullptr, *nsegDev = nullptr;
+  int ksegCap = 0, nseg = 0;
 
   // ---- GCMC (ensemble 1): box 0 interacts, box 1 is the reservoir ----
   // Reservoir atoms carry molecule id -1 on the device and are skipped by
@@ -208,6 +235,8 @@ int LaunchTrial(gomc_box *b, int mol, const double *ox, const double *oy, const 
                 bool doRecip, double out[4]);
 void HostCorrectionSelfLRC(gomc_box *b, double &self, double &corr, double &lrc);
 int RecipSums(gomc_box *b, double *erecip);
+// Cuts the box's k list into segments (after the k-vectors change).
+int BuildKSegments(gomc_box *b);
 int BoxInter(gomc_box *b, double *lj, double *real);
 int EnsurePartials(gomc_box *b, int nblocks);
 int Sync(gomc_box *b);

@@ -84,8 +84,10 @@ int RecipInitImpl(gomc_box *b, double *erecip) {
   HIP_TRY(b, hipMemcpyAsync(b->ky, ky.data(), n, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->kz, kz.data(), n, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->pref, pref.data(), n, hipMemcpyHostToDevice, b->stream));
+  int rc = BuildKSegments(b);
+  if (rc) return rc;
   b->recipReady = true;
-  int rc = RecipSums(b, erecip);
+  rc = RecipSums(b, erecip);
   if (rc) return rc;
   HIP_TRY(b, hipMemcpyAsync(b->sumRref, b->sumRnew, n, hipMemcpyDeviceToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->sumIref, b->sumInew, n, hipMemcpyDeviceToDevice, b->stream));

@@ -114,56 +114,35 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
 
 // ------------------------------------------------------ reciprocal force --
 // Ewald::BoxForceReciprocal (Ewald.cpp:1496-1596), one atom per thread:
-// the intramolecular erf correction is subtracted first, then the k-sum
-// over the current structure factors, k-vectors staged through LDS.
+// the k-space force F_j = sum_k 2 q_j prefact_k (sin ReS - cos ImS) k over
+// the current structure factors, walked segment by segment (pieces of at
+// most kSeg consecutive-nz k-vectors, built with the box's k-vectors):
+// two sincos per segment (its first vector and the nz step), then one
+// complex product per vector.  The k data of a tile of segments is staged
+// through LDS.  Then the intramolecular erf correction is subtracted and
+// the chunks added in order.
+constexpr int kForceSegTile = 64;  // segments per LDS tile (at most 1024 k)
+
 struct RecipForceArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
   const int *mol, *molStart;
   int natoms, K;
   const double *kx, *ky, *kz, *pref, *sumR, *sumI;
+  const int *kseg;  // segment starts [nseg + 1]
+  int nseg;
   double *fx, *fy, *fz;
-  // k chunks: with kChunks > 1, RecipForcePartKernel writes each chunk's
-  // k-space force of every atom to part[chunk][xyz][natoms] and
-  // RecipForceKernel adds the chunks in order after the intramolecular
-  // term, so that a box with few atom blocks still fills the chip.
-  int kChunk, kChunks;
+  // segment chunks: RecipForcePartKernel writes chunk c's k-space force of
+  // every atom (segments [c segChunk, (c + 1) segChunk)) to
+  // part[c][xyz][natoms]; RecipForceKernel adds the chunks in order after
+  // the intramolecular term.
+  int segChunk, kChunks;
   double *part;
 };
 
-// k-space force on one atom from k-vectors [k0, k1), staged through LDS.
-__device__ __forceinline__ void ForceOverK(const RecipForceArgs &a, bool on, double xp,
-                                           double yp, double zp, double qp, int k0, int k1,
-                                           double (*sk)[kBlock], double &X, double &Y,
-                                           double &Z) {
-  for (int t = k0; t < k1; t += kBlock) {
-    const int tn = min(kBlock, k1 - t);
-    __syncthreads();
-    if (int(threadIdx.x) < tn) {
-      const int k = t + threadIdx.x;
-      sk[0][threadIdx.x] = a.kx[k];
-      sk[1][threadIdx.x] = a.ky[k];
-      sk[2][threadIdx.x] = a.kz[k];
-      sk[3][threadIdx.x] = a.pref[k];
-      sk[4][threadIdx.x] = a.sumR[k];
-      sk[5][threadIdx.x] = a.sumI[k];
-    }
-    __syncthreads();
-    if (on)
-      for (int i = 0; i < tn; ++i) {
-        const double dot = xp * sk[0][i] + yp * sk[1][i] + zp * sk[2][i];
-        double s, c;
-        sincos(dot, &s, &c);
-        const double factor = 2.0 * qp * sk[3][i] * 1.0 * (s * sk[4][i] - c * sk[5][i]);
-        X += factor * sk[0][i];
-        Y += factor * sk[1][i];
-        Z += factor * sk[2][i];
-      }
-  }
-}
-
 __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceArgs a) {
-  __shared__ double sk[6][kBlock];
+  __shared__ double sk[6][kForceSegTile * kSeg];
+  __shared__ int ss[kForceSegTile + 1];
   const int chunk = blockIdx.x % a.kChunks;
   const int p = blockIdx.x / a.kChunks * kBlock + threadIdx.x;
   const bool active = p < a.natoms;
@@ -176,8 +155,42 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
   }
   const bool on = active && !(fabs(qp) < 0.000000001);
   double X = 0.0, Y = 0.0, Z = 0.0;
-  const int k0 = chunk * a.kChunk, k1 = min(a.K, k0 + a.kChunk);
-  ForceOverK(a, on, xp, yp, zp, qp, k0, k1, sk, X, Y, Z);
+  const int s0 = min(a.nseg, chunk * a.segChunk), s1 = min(a.nseg, s0 + a.segChunk);
+  for (int t = s0; t < s1; t += kForceSegTile) {
+    const int tn = min(kForceSegTile, s1 - t);
+    const int kA = a.kseg[t], kn = a.kseg[t + tn] - kA;
+    __syncthreads();
+    for (int i = threadIdx.x; i <= tn; i += kBlock) ss[i] = a.kseg[t + i] - kA;
+    for (int i = threadIdx.x; i < kn; i += kBlock) {
+      sk[0][i] = a.kx[kA + i];
+      sk[1][i] = a.ky[kA + i];
+      sk[2][i] = a.kz[kA + i];
+      sk[3][i] = a.pref[kA + i];
+      sk[4][i] = a.sumR[kA + i];
+      sk[5][i] = a.sumI[kA + i];
+    }
+    __syncthreads();
+    if (on)
+      for (int j = 0; j < tn; ++j) {
+        const int k0 = ss[j], len = ss[j + 1] - k0;
+        double im, re, sn, cn;
+        sincos(xp * sk[0][k0] + yp * sk[1][k0] + zp * sk[2][k0], &im, &re);
+        sincos((len > 1 ? sk[2][k0 + 1] - sk[2][k0] : 0.0) * zp, &sn, &cn);
+#pragma unroll
+        for (int i = 0; i < kSeg; ++i) {
+          if (i < len) {
+            const int k = k0 + i;
+            const double factor = 2.0 * qp * sk[3][k] * 1.0 * (im * sk[4][k] - re * sk[5][k]);
+            X += factor * sk[0][k];
+            Y += factor * sk[1][k];
+            Z += factor * sk[2][k];
+          }
+          const double nr = re * cn - im * sn;
+          im = re * sn + im * cn;
+          re = nr;
+        }
+      }
+  }
   if (active) {
     a.part[(size_t(chunk) * 3 + 0) * a.natoms + p] = X;
     a.part[(size_t(chunk) * 3 + 1) * a.natoms + p] = Y;
@@ -186,19 +199,11 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
 }
 
 __global__ void __launch_bounds__(kBlock) RecipForceKernel(const RecipForceArgs a) {
-  __shared__ double sk[6][kBlock];
   const int p = blockIdx.x * kBlock + threadIdx.x;
-  const bool active = p < a.natoms;
-  double X = 0.0, Y = 0.0, Z = 0.0, xp = 0.0, yp = 0.0, zp = 0.0, qp = 0.0;
-  bool charged = false;
-  if (active) {
-    xp = a.x[p];
-    yp = a.y[p];
-    zp = a.z[p];
-    qp = a.q[p];
-    charged = !(fabs(qp) < 0.000000001);
-  }
-  if (active && charged) {
+  if (p >= a.natoms) return;
+  double X = 0.0, Y = 0.0, Z = 0.0;
+  const double xp = a.x[p], yp = a.y[p], zp = a.z[p], qp = a.q[p];
+  if (!(fabs(qp) < 0.000000001)) {
     const int m = a.mol[p], s = a.molStart[m], e = a.molStart[m + 1];
     const double constValue = a.g.alpha * M_2_SQRTPI;
     const double alphaSq = a.g.alpha * a.g.alpha;
@@ -218,21 +223,14 @@ __global__ void __launch_bounds__(kBlock) RecipForceKernel(const RecipForceArgs 
       Z -= intraForce * dz;
     }
   }
-  if (a.kChunks > 1) {
-    if (active)
-      for (int c = 0; c < a.kChunks; ++c) {
-        X += a.part[(size_t(c) * 3 + 0) * a.natoms + p];
-        Y += a.part[(size_t(c) * 3 + 1) * a.natoms + p];
-        Z += a.part[(size_t(c) * 3 + 2) * a.natoms + p];
-      }
-  } else {
-    ForceOverK(a, active && charged, xp, yp, zp, qp, 0, a.K, sk, X, Y, Z);
+  for (int c = 0; c < a.kChunks; ++c) {
+    X += a.part[(size_t(c) * 3 + 0) * a.natoms + p];
+    Y += a.part[(size_t(c) * 3 + 1) * a.natoms + p];
+    Z += a.part[(size_t(c) * 3 + 2) * a.natoms + p];
   }
-  if (active) {
-    a.fx[p] = X;
-    a.fy[p] = Y;
-    a.fz[p] = Z;
-  }
+  a.fx[p] = X;
+  a.fy[p] = Y;
+  a.fz[p] = Z;
 }
 
 // ------------------------------------------------ molecule terms, torque --
@@ -608,28 +606,25 @@ int MPRecipForce(gomc_box *b, int set) {
   a.fx = mp.aFR[set][0];
   a.fy = mp.aFR[set][1];
   a.fz = mp.aFR[set][2];
+  a.kseg = b->kseg;
+  a.nseg = b->nseg;
   const int g = (b->natoms + kBlock - 1) / kBlock;
-  // k chunks of whole tiles, about 2048 workgroups in the first pass
-  const int tiles = std::max(1, (b->K + kBlock - 1) / kBlock);
-  int chunks = std::min(tiles, std::max(1, (2048 + g - 1) / g));
-  const int tilesPerChunk = (tiles + chunks - 1) / chunks;
-  chunks = (tiles + tilesPerChunk - 1) / tilesPerChunk;
-  a.kChunk = tilesPerChunk * kBlock;
+  // segment chunks, about 1024 workgroups in the first pass
+  int chunks = std::min(std::max(1, b->nseg), std::max(1, (1024 + g - 1) / g));
+  a.segChunk = std::max(1, (b->nseg + chunks - 1) / chunks);
+  chunks = std::max(1, (b->nseg + a.segChunk - 1) / a.segChunk);
   a.kChunks = chunks;
-  a.part = nullptr;
-  if (chunks > 1) {
-    const size_t need = size_t(chunks) * 3 * size_t(std::max(b->natoms, 1));
-    if (need > mp.fpartCap) {
-      if (mp.fpart) (void)hipFree(mp.fpart);
-      mp.fpart = nullptr;
-      mp.fpartCap = 0;
-      HIP_TRY(b, hipMalloc(&mp.fpart, sizeof(double) * need));
-      mp.fpartCap = need;
-    }
-    a.part = mp.fpart;
-    hipLaunchKernelGGL(RecipForcePartKernel, dim3(g * chunks), dim3(kBlock), 0, b->stream, a);
-    HIP_TRY(b, hipGetLastError());
+  const size_t need = size_t(chunks) * 3 * size_t(std::max(b->natoms, 1));
+  if (need > mp.fpartCap) {
+    if (mp.fpart) (void)hipFree(mp.fpart);
+    mp.fpart = nullptr;
+    mp.fpartCap = 0;
+    HIP_TRY(b, hipMalloc(&mp.fpart, sizeof(double) * need));
+    mp.fpartCap = need;
   }
+  a.part = mp.fpart;
+  hipLaunchKernelGGL(RecipForcePartKernel, dim3(g * chunks), dim3(kBlock), 0, b->stream, a);
+  HIP_TRY(b, hipGetLastError());
   hipLaunchKernelGGL(RecipForceKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   return Sync(b);

@@ -508,12 +508,16 @@ int gomc_server_create(int device, int precision, int slots, int64_t budget_byte
   sv->nslots = slots;
   sv->tickMs = tick_ms;
   sv->budget = budget_bytes;
-  // split the budget in the proportions of a water box: 3 atoms and about
-  // 3.1 k-vectors per molecule, k-vector slots rounded up to whole chunks
-  // of 1024 (about 1.4 slots per k-vector over 200-2000 molecules)
-  sv->atomCap = int64_t(0.25 * budget_bytes) / kAtomBytes;
-  sv->molCap = int64_t(0.085 * budget_bytes) / kMolBytes;
-  sv->kCap = int64_t(0.665 * budget_bytes) / kKBytes;
+  // The batched lane's arenas take the budget, up to 64 MiB per slot (a
+  // water job of about 20,000 molecules), so that a large budget meant for
+  // GCMC jobs does not reserve device memory the NVT lane cannot use.  They
+  // are split in the proportions of a water box: 3 atoms and about 3.1
+  // k-vectors per molecule, k-vector slots rounded up to whole chunks of
+  // 1024 (about 1.4 slots per k-vector over 200-2000 molecules).
+  const double arena = double(std::min<int64_t>(budget_bytes, int64_t(slots) * (int64_t(64) << 20)));
+  sv->atomCap = int64_t(0.25 * arena) / kAtomBytes;
+  sv->molCap = int64_t(0.085 * arena) / kMolBytes;
+  sv->kCap = int64_t(0.665 * arena) / kKBytes;
   sv->atoms = Arena(sv->atomCap);
   sv->mols = Arena(sv->molCap);
   sv->kvec = Arena(sv->kCap);

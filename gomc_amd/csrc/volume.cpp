@@ -89,21 +89,33 @@ int NewGrid(gomc_box *b) {
 struct KArrays {
   int K;
   double *kx, *ky, *kz, *pref, *sumRref, *sumIref, *sumRnew, *sumInew;
+  int *kseg, *nsegDev;  // the k list's segments (box.hip)
+  int ksegCap, nseg;
 };
 
 KArrays TakeK(gomc_box *b) {
-  KArrays k{b->K, b->kx, b->ky, b->kz, b->pref, b->sumRref, b->sumIref, b->sumRnew, b->sumInew};
+  KArrays k{b->K,       b->kx,       b->ky,      b->kz,      b->pref,    b->sumRref,
+            b->sumIref, b->sumRnew,  b->sumInew, b->kseg,    b->nsegDev, b->ksegCap,
+            b->nseg};
   b->K = 0;
   b->kx = b->ky = b->kz = b->pref = b->sumRref = b->sumIref = b->sumRnew = b->sumInew = nullptr;
+  b->kseg = b->nsegDev = nullptr;
+  b->ksegCap = b->nseg = 0;
   return k;
 }
 
 void FreeK(const KArrays &k) {
   for (double *p : {k.kx, k.ky, k.kz, k.pref, k.sumRref, k.sumIref, k.sumRnew, k.sumInew})
     if (p) (void)hipFree(p);
+  for (int *p : {k.kseg, k.nsegDev})
+    if (p) (void)hipFree(p);
 }
 
 void PutK(gomc_box *b, const KArrays &k) {
+  b->kseg = k.kseg;
+  b->nsegDev = k.nsegDev;
+  b->ksegCap = k.ksegCap;
+  b->nseg = k.nseg;
   b->K = k.K;
   b->kx = k.kx;
   b->ky = k.ky;
@@ -182,7 +194,8 @@ int VolumeMove(gomc_box *b, int &accepted, double dE[3]) {
   b->volume = newVol;
   b->volInv = 1.0 / newVol;
   bool gridReplaced = false, kTaken = false;
-  KArrays oldK{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  KArrays oldK{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+               nullptr, nullptr, 0, 0};
   // Puts back the kept geometry, coordinates, grid and k-space arrays: the
   // rejection of the move, and the way out of a failure part-way through
   // its energy so that the box stays as it was before the move.

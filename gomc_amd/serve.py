@@ -134,9 +134,13 @@ def poisson_stream_gcmc(njobs: int, rate: float = 20.0, seed: int = 17, lmin: fl
     return jobs
 
 
-def run_stream(server: Server, jobs: list[Job], timeout: float = 600.0) -> dict:
+def run_stream(server: Server, jobs: list[Job], timeout: float = 600.0,
+               progress: float = 0.0) -> dict:
     """Submits each job at its arrival time (relative to now), waits for all
-    of them, and returns throughput and latency figures."""
+    of them, and returns throughput and latency figures.  With progress > 0
+    a line with the jobs still outstanding is printed to stderr every
+    `progress` seconds while it waits."""
+    import sys
     t0 = time.perf_counter()
     ids = []
     for j in jobs:
@@ -144,7 +148,20 @@ def run_stream(server: Server, jobs: list[Job], timeout: float = 600.0) -> dict:
         if delay > 0:
             time.sleep(delay)
         ids.append(server.submit(j.system, j.moves))
-    server.drain(timeout)
+    if progress > 0:
+        while True:
+            left = timeout - (time.perf_counter() - t0)
+            if left <= 0:
+                raise GomcError("served jobs did not finish in time")
+            rc = server.lib.gomc_server_drain(server.h, min(progress, left))
+            if rc == 0:
+                break
+            st = server.stats()
+            print(f"# serving: {int(st['outstanding'])} jobs outstanding after "
+                  f"{time.perf_counter() - t0:.0f} s, {st['moves']:.3g} moves done",
+                  file=sys.stderr, flush=True)
+    else:
+        server.drain(timeout)
     wall = time.perf_counter() - t0
     res = [server.poll(i) for i in ids]
     if any(r["status"] != 2 for r in res):

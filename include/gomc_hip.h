@@ -289,6 +289,12 @@ int gomc_run_moves_device_multi(gomc_box *const *boxes, int n,
                                 int64_t *moves_done, int64_t *accepted,
                                 double *count_sum, int64_t *count_samples,
                                 int32_t *status);
+/* GCMC with the moves driven by the caller (MoleculeTransfer::Accept,
+ * src/moves/MoleculeTransfer.h:168-246, through MoleculeLookup::ShiftMolBox):
+ * puts molecule `mol` in box 0 (it interacts) or the reservoir (1), with its
+ * coordinates as last set.  The box's own move driver's lookup is rebuilt in
+ * molecule order.  GOMC_ERR_STATE for 1 outside GCMC. */
+int gomc_set_mol_box(gomc_box *b, int mol, int box);
 /* GCMC: box of every molecule (0 simulation box, 1 reservoir) into `box`
  * (n_mols); returns the number of molecules in box 0 or an error code.
  * MoleculeLookup::NumInBox / IsMoleculeInBox (src/MoleculeLookup.h:98-130). */
@@ -374,9 +380,10 @@ int gomc_batch_last_kernel_ms(const gomc_batch *bt, double *ms);
  * workgroup slots; submitted jobs (NVT boxes as for gomc_batch_create) are
  * admitted in arrival order when a slot is free and their coordinates,
  * centres, k-vectors and structure factors fit the `budget_bytes` device
- * arenas; every launch advances each resident job by the moves that cost
- * about `tick_ms`; finished jobs release their slot and memory.  A job's
- * result does not depend on the other jobs in the batch. */
+ * arenas (the NVT lane's arenas take the budget up to 64 MiB per slot);
+ * every launch advances each resident job by the moves that cost about
+ * `tick_ms`; finished jobs release their slot and memory.  A job's result
+ * does not depend on the other jobs in the batch. */
 typedef struct gomc_server gomc_server;
 int gomc_server_create(int device, int precision, int slots, int64_t budget_bytes,
                        double tick_ms, gomc_server **out);

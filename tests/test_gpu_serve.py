@@ -127,3 +127,61 @@ def test_gcmc_job_same_alone_and_in_mix_and_matches_oracle():
     assert n_acc == mixed["accepted"]
     assert np.allclose(mixed["energies"], o.energies(), rtol=1e-10, atol=1e-6)
     assert np.array_equal(mixed["mean_n"], sums / samples)
+
+
+def _mean_counts(run_cycle, counts, ncyc, burn):
+    """<N_k> over cycles burn..ncyc and its standard error (10 blocks)."""
+    c = []
+    for _ in range(ncyc):
+        run_cycle()
+        c.append(counts())
+    c = np.array(c[burn:], dtype=float)
+    b = c[: len(c) // 10 * 10].reshape(10, -1, 2).mean(1)
+    return b.mean(0), b.std(0, ddof=1) / np.sqrt(10)
+
+
+@pytest.mark.parametrize("L,T,fug", [(20.0, 300.0, (5.0, 2.0)), (22.0, 280.0, (60.0, 30.0)),
+                                     (24.0, 320.0, (20.0, 80.0))])
+def test_mixed_precision_gcmc_occupation_matches_independent_fp64_oracle(L, T, fug):
+    """The served GCMC lane's mixed precision (FP32 pair terms, CBMC trial
+    energies and phases; FP64 sums and Metropolis) samples the FP64
+    Hamiltonian's grand-canonical <N_k>: the device move loop at precision 1
+    on one stream against the FP64 oracle on an independent stream, at a
+    dilute, a dense (high-fugacity) and a CO2-rich state point.  Same start,
+    300 MC cycles each, the first 60 discarded; bar 3 standard errors of the
+    difference for each kind (block errors of 10 blocks)."""
+    from dataclasses import replace
+
+    from gomc_amd.api import Box
+    s = systems.gcmc_mixture(6, 6, L, temperature=T, fugacity_bar=fug, reservoir=400, seed=23,
+                             adj_steps=500)
+    mk = s.mol_kind
+
+    def counter(obj):
+        def counts():
+            _, b = obj.mol_boxes()
+            return [np.sum((b == 0) & (mk == 0)), np.sum((b == 0) & (mk == 1))]
+        return counts
+
+    g = Box(s)
+    g.set_precision(1)
+    gstep = [0]
+
+    def gcycle():
+        n = max(1, int(sum(counter(g)())))
+        g.run(n, start_step=gstep[0], record=False, device=True)
+        gstep[0] += n
+
+    mg, eg = _mean_counts(gcycle, counter(g), 300, 60)
+    o = Oracle(replace(s, seed=9001, _keep=[]), threads=8)
+    ostep = [0]
+
+    def ocycle():
+        n = max(1, int(sum(counter(o)())))
+        o.run(n, start_step=ostep[0], record=False)
+        ostep[0] += n
+
+    mo, eo = _mean_counts(ocycle, counter(o), 300, 60)
+    z = (mg - mo) / np.hypot(eg, eo)
+    assert np.all(np.abs(z) < 3.0), (mg, eg, mo, eo, z)
+    print(f"L {L} T {T} fug {fug}: device mixed {mg} +- {eg}, oracle FP64 {mo} +- {eo}, z {z}")
