@@ -294,6 +294,35 @@ class PsfStructure:
     charge: np.ndarray
     mass: np.ndarray
     bonds: np.ndarray  # (nbond, 2), 0-based
+    # the other topology sections GOMC's kind map keeps (MolSetup.cpp:
+    # 1214-1480), 0-based, in file order: !NTHETA, !NPHI, !NIMPHI, !NDON, !NACC
+    angles: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int64))
+    dihedrals: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int64))
+    impropers: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int64))
+    donors: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int64))
+    acceptors: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int64))
+
+
+def _psf_section(lines: list[str], tag: str, width: int, path: str) -> np.ndarray:
+    """The index tuples of one PSF section ("!NTHETA" ...), 0-based; empty
+    when the file has no such section."""
+    i = 0
+    while i < len(lines) and tag not in lines[i]:
+        i += 1
+    if i == len(lines):
+        return np.zeros((0, width), np.int64)
+    n = int(lines[i].split()[0])
+    vals: list[int] = []
+    i += 1
+    while len(vals) < width * n and i < len(lines):
+        t = lines[i].split()
+        i += 1
+        if t and t[0].startswith("!"):
+            break
+        vals.extend(int(x) - 1 for x in t)
+    if len(vals) < width * n:
+        raise GomcFormatError(f"{path}: found {len(vals) // width} of {n} entries of {tag}")
+    return np.array(vals[:width * n], np.int64).reshape(-1, width)
 
 
 def read_psf(path: str) -> PsfStructure:
@@ -338,7 +367,12 @@ def read_psf(path: str) -> PsfStructure:
             raise GomcFormatError(f"{path}: found {len(bonds) // 2} of {nb} bonds")
     return PsfStructure(seg, np.array(resid, np.int64), res, name, typ, np.array(q),
                         np.array(m), np.array(bonds[:2 * nb] if bonds else [],
-                                              np.int64).reshape(-1, 2))
+                                              np.int64).reshape(-1, 2),
+                        _psf_section(lines, "!NTHETA", 3, path),
+                        _psf_section(lines, "!NPHI", 4, path),
+                        _psf_section(lines, "!NIMPHI", 4, path),
+                        _psf_section(lines, "!NDON", 2, path),
+                        _psf_section(lines, "!NACC", 2, path))
 
 
 def write_psf(path: str, segment: list[str], resid, resname, name, typ, charge, mass,
@@ -635,24 +669,19 @@ def _check_freqs(conf: GomcConfig, ensemble: str) -> None:
         raise GomcFormatError(f"{conf.path}: the sum of move frequencies is {total:g}, not one")
 
 
-def write_gomc(outdir: str, s: System, type_names: list[str] | None = None,
-               prefix: str = "system") -> str:
-    """Write `s` as GOMC input files (control file, PDB and PSF per box, a
-    Mie-style parameter file) under `outdir`; returns the control file's path.
-    Box 1 holds the reservoir molecules of a GCMC system.  Atom masses are
-    written as 1.0 (a System carries none; the energies do not use them) and
-    each molecule's atoms are bonded as a chain so that it stays one
-    connected component when read back."""
-    os.makedirs(outdir, exist_ok=True)
+def gomc_box_records(s: System, type_names: list[str] | None = None
+                     ) -> tuple[list[PsfStructure], list[PdbFrame]]:
+    """Per box (box 1 holds a GCMC system's reservoir), the PSF structure and
+    the PDB frame that `write_gomc` writes for `s`, as GOMC reads them back
+    (coordinates rounded to the PDB's three decimals).  Atom masses are 1.0 (a
+    System carries none; the energies do not use them) and each molecule's
+    atoms are bonded as a chain so that it stays one connected component."""
     nt = len(s.sigma)
     names = type_names or [f"T{i}" for i in range(nt)]
-    write_parameters(os.path.join(outdir, f"{prefix}.par"),
-                     {names[i]: NonBonded(float(s.epsilon[i]), float(s.sigma[i]), float(s.n_rep[i]))
-                      for i in range(nt)})
     starts = s.mol_start()
     mb = s.mol_box if s.mol_box is not None else np.zeros(s.nmols, np.int32)
     nbox = 2 if s.ensemble == 1 else 1
-    L = s.box
+    psfs, pdbs = [], []
     for b in range(nbox):
         seg, resid, res, nm, typ, q, mass, xyz, bonds = [], [], [], [], [], [], [], [], []
         r = 0
@@ -671,13 +700,41 @@ def write_gomc(outdir: str, s: System, type_names: list[str] | None = None,
                 q.append(k.charges[j])
                 mass.append(1.0)
                 xyz.append(s.xyz[starts[m] + j])
-            # a chain of bonds keeps the molecule one connected component
             bonds.extend([first + j, first + j + 1] for j in range(len(k.atom_types) - 1))
-        xyz = np.array(xyz).reshape(-1, 3)
-        write_pdb(os.path.join(outdir, f"{prefix}_BOX_{b}.pdb"), xyz, nm, res, resid,
-                  cell=L, box_index=b)
-        write_psf(os.path.join(outdir, f"{prefix}_BOX_{b}.psf"), seg, resid, res, nm, typ, q,
-                  mass, np.array(bonds, np.int64).reshape(-1, 2))
+        xyz = np.array([[float("%.3f" % v) for v in p] for p in xyz]).reshape(-1, 3)
+        n = len(nm)
+        psfs.append(PsfStructure(seg, np.array(resid, np.int64), res, nm, typ, np.array(q),
+                                 np.array(mass), np.array(bonds, np.int64).reshape(-1, 2)))
+        pdbs.append(PdbFrame(nm, res, np.array(resid, np.int64), ["A"] * n, xyz,
+                             np.full(n, float(b)), np.zeros(n), tuple(s.box),
+                             (90.0, 90.0, 90.0)))
+    return psfs, pdbs
+
+
+def write_gomc(outdir: str, s: System, type_names: list[str] | None = None,
+               prefix: str = "system") -> str:
+    """Write `s` as GOMC input files (control file, PDB and PSF per box, a
+    Mie-style parameter file) under `outdir`; returns the control file's path.
+    The boxes are `gomc_box_records(s, type_names)`."""
+    os.makedirs(outdir, exist_ok=True)
+    nt = len(s.sigma)
+    names = type_names or [f"T{i}" for i in range(nt)]
+    write_parameters(os.path.join(outdir, f"{prefix}.par"),
+                     {names[i]: NonBonded(float(s.epsilon[i]), float(s.sigma[i]), float(s.n_rep[i]))
+                      for i in range(nt)})
+    nbox = 2 if s.ensemble == 1 else 1
+    L = s.box
+    starts = s.mol_start()
+    mb = s.mol_box if s.mol_box is not None else np.zeros(s.nmols, np.int32)
+    psfs, _ = gomc_box_records(s, names)
+    for b in range(nbox):
+        ps = psfs[b]
+        xyz = np.array([s.xyz[starts[m] + j] for m in range(s.nmols) if mb[m] == b
+                        for j in range(len(s.kinds[s.mol_kind[m]].atom_types))]).reshape(-1, 3)
+        write_pdb(os.path.join(outdir, f"{prefix}_BOX_{b}.pdb"), xyz, ps.name, ps.resname,
+                  ps.resid, cell=L, box_index=b)
+        write_psf(os.path.join(outdir, f"{prefix}_BOX_{b}.psf"), ps.segment, ps.resid,
+                  ps.resname, ps.name, ps.type, ps.charge, ps.mass, ps.bonds)
     conf = os.path.join(outdir, "in.conf")
     tot = s.move_disp + s.move_rot + s.move_mp + s.move_swap + s.move_vol
     with open(conf, "w") as f:
