@@ -1012,12 +1012,16 @@ def restore_box(system, restart: dict, device: int = 0, precision: int = 0):
     file_xyz = np.concatenate(restart["coords"]) if restart["coords"] else np.zeros((0, 3))
     if len(file_xyz) != int(lens.sum()):
         raise GomcCheckpointError("restart coordinates do not hold every atom")
-    L = np.array(restart["cells"][0], np.float64)
+    # the cell: the system's, except under NPT, whose volume moves leave it
+    # in the cell file (GOMC reads it there when restarting from an XSC file)
+    L = np.array(restart["cells"][0] if ens == 2 else s.box, np.float64)
     xyz = np.zeros_like(file_xyz)
     for i, m in enumerate(order):
         xyz[starts[m]:starts[m + 1]] = file_xyz[rs[i]:rs[i + 1]]
-    # back into [0, L) (BoxDimensions::WrapPBC)
+    # back into [0, L) (BoxDimensions::WrapPBC); a coordinate that rounds to
+    # L on the way back is taken to 0 as the box requires
     xyz = np.where(xyz >= L, xyz - L, np.where(xyz < 0.0, xyz + L, xyz))
+    xyz = np.where(xyz >= L, xyz - L, xyz)
     sys2 = replace(s, box=tuple(float(v) for v in L), xyz=xyz,
                    mol_box=mol_box if ens == 1 else s.mol_box,
                    seed=int(chk["seed_value"]), _keep=[])

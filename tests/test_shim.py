@@ -100,17 +100,17 @@ def test_shim_methods_equal_the_oracle(tmp_path):
     # coordinates give no further change
     assert out["recip_after_accept"][0] == pytest.approx(tot[3] + d, rel=1e-11)
     assert abs(out["recip_after_accept"][1]) <= 1e-9 * abs(tot[3])
-    en, real, _ = o.particle_inter(0, -0.8476, ptrials)
-    assert np.allclose(np.array(out["particle_en_plus_1"]) - 1.0, en, rtol=1e-10, atol=1e-8)
-    assert np.allclose(np.array(out["particle_real_plus_2"]) - 2.0, real, rtol=1e-10, atol=1e-8)
-    cv, cs, _, _ = o.cell_csr()
-    assert np.array_equal(out["cell_vector"], cv) and np.array_equal(out["cell_start"], cs)
-    assert out["bad_mol_error"] == -1
-    # the committed molecule is part of the box: totals move by the trial's change
-    o2 = Oracle(s, threads=4)
+    # the rest runs after the commit: the oracle holds the committed molecule
+    from dataclasses import replace
     xyz = s.xyz.copy()
     st = s.mol_start()[mol]
     xyz[st:st + 3] = trial
-    from dataclasses import replace
-    after = Oracle(replace(s, xyz=xyz, _keep=[]), threads=4).system_total()
-    assert np.allclose(out["total_after"], after, rtol=1e-11, atol=0)
+    oc = Oracle(replace(s, xyz=xyz, _keep=[]), threads=4)
+    en, real, _ = oc.particle_inter(0, -0.8476, ptrials)
+    assert np.allclose(np.array(out["particle_en_plus_1"]) - 1.0, en, rtol=1e-10, atol=1e-8)
+    assert np.allclose(np.array(out["particle_real_plus_2"]) - 2.0, real, rtol=1e-10, atol=1e-8)
+    cv, cs, _, _ = oc.cell_csr()
+    assert np.array_equal(out["cell_vector"], cv) and np.array_equal(out["cell_start"], cs)
+    assert out["bad_mol_error"] == -1
+    # the committed molecule is part of the box: totals move by the trial's change
+    assert np.allclose(out["total_after"], oc.system_total(), rtol=1e-11, atol=0)
