@@ -41,6 +41,7 @@ SIGNATURES = {
                                       _dp, _dp, _ip]),
     "gomc_recip_init": (C.c_int, [_vp, _dp]),
     "gomc_recip_sums": (C.c_int, [_vp, _dp]),
+    "gomc_recip_sums_noalloc": (C.c_int, [_vp, _dp]),
     "gomc_set_mol_box": (C.c_int, [_vp, C.c_int, C.c_int]),
     "gomc_mol_recip": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "gomc_swap_recip": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),

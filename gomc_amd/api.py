@@ -140,6 +140,13 @@ class Box:
         self._check(self.lib.gomc_recip_sums(self.h, C.byref(e)))
         return e.value
 
+    def recip_sums_noalloc(self) -> float:
+        """recip_sums with its scratch in the library's static device array
+        (no device allocation in the call)."""
+        e = C.c_double()
+        self._check(self.lib.gomc_recip_sums_noalloc(self.h, C.byref(e)))
+        return e.value
+
     def recip_accept(self):
         self._check(self.lib.gomc_recip_accept(self.h))
 

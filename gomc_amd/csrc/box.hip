@@ -791,9 +791,9 @@ int BuildKSegments(gomc_box *b) {
   return Sync(b);
 }
 
-// BoxReciprocalSums into the "new" sums; returns the reciprocal energy.
-int RecipSums(gomc_box *b, double *erecip) {
-  if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+// The box's arrays for the two structure-factor passes; the scratch
+// (part, partials, counter) is filled in by the caller.
+static SumsArgs BoxSumsArgs(const gomc_box *b) {
   SumsArgs a;
   a.x = b->x;
   a.y = b->y;
@@ -810,12 +810,19 @@ int RecipSums(gomc_box *b, double *erecip) {
   a.nseg = b->nsegDev;
   a.sumR = b->sumRnew;
   a.sumI = b->sumInew;
+  a.result = b->devRes;
+  return a;
+}
+
+// BoxReciprocalSums into the "new" sums; returns the reciprocal energy.
+int RecipSums(gomc_box *b, double *erecip) {
+  if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+  SumsArgs a = BoxSumsArgs(b);
   const int nblocks = std::max(1, (b->K + kBlock - 1) / kBlock);
   int rc = EnsurePartials(b, nblocks);
   if (rc) return rc;
   a.partials = b->partials;
   a.counter = b->counter;
-  a.result = b->devRes;
   const size_t need = size_t(SumsChunks(b->nseg, b->natoms, b->K)) * size_t(b->K) * 2;
   if (need > b->sumPartCap) {
     if (b->sumPart) (void)hipFree(b->sumPart);
@@ -832,7 +839,74 @@ int RecipSums(gomc_box *b, double *erecip) {
   return GOMC_OK;
 }
 
+// ------------------------------------------- allocation-free variant --
+// gomc_recip_sums_noalloc runs the same two passes as RecipSums with its
+// scratch -- the block counter, the second pass's per-block partials and
+// the first pass's per-chunk sums -- in `scratch`, one file-scope device
+// array that each GPU holds until the process ends.  Layout: counter at
+// byte 0, partials at byte 256 (kNoAllocMaxK / kBlock entries), per-chunk
+// sums after them (SumsChunks keeps chunks x K within kSumsPartCap, so
+// 2 kSumsPartCap doubles hold them for K up to kNoAllocMaxK).  Every call on
+// one GPU shares the array: calls that overlap on one GPU race.
+constexpr int kNoAllocMaxK = 1 << 20;
+constexpr size_t kNoAllocPartialsOff = 256;
+constexpr size_t kNoAllocPartOff =
+    kNoAllocPartialsOff + sizeof(Partial4) * size_t(kNoAllocMaxK / kBlock);
+constexpr size_t kNoAllocPartDoubles = 2 * kSumsPartCap;
+constexpr size_t kScratchBytes = kNoAllocPartOff + sizeof(double) * kNoAllocPartDoubles;
+static_assert(kNoAllocPartOff % 256 == 0, "per-chunk sums must stay 256-byte aligned");
 
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static std::mutex gScratchMu;
+static unsigned char *gScratch[kScratchDevices] = {};
+
+// Device address of this GPU's `scratch`, looked up once per device with
+// that device current (the array's host-side name is not a device address).
+static int ScratchBase(gomc_box *b, unsigned char **base) {
+  if (b->device < 0 || b->device >= kScratchDevices)
+    return Fail(b, GOMC_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(gScratchMu);
+  if (!gScratch[b->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return Fail(b, GOMC_ERR_HIP,
+                  std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    gScratch[b->device] = static_cast<unsigned char *>(p);
+  }
+  *base = gScratch[b->device];
+  return GOMC_OK;
+}
+
+// The block counter must read zero when the second pass starts; a fill
+// kernel, not a memset, so that a captured stream replays it.
+__global__ void ZeroCounterKernel(unsigned *counter) { *counter = 0u; }
+
+int RecipSumsNoAlloc(gomc_box *b, double *erecip) {
+  if (!b->recipReady) return Fail(b, GOMC_ERR_STATE, "k-vectors not initialised");
+  if (b->K > kNoAllocMaxK)
+    return Fail(b, GOMC_ERR_ARG,
+                "gomc_recip_sums_noalloc takes at most 1048576 k-vectors (use gomc_recip_sums)");
+  const size_t need = size_t(SumsChunks(b->nseg, b->natoms, b->K)) * size_t(b->K) * 2;
+  if (need > kNoAllocPartDoubles)
+    return Fail(b, GOMC_ERR_ARG, "structure-factor chunks exceed the scratch array");
+  unsigned char *base = nullptr;
+  int rc = ScratchBase(b, &base);
+  if (rc) return rc;
+  SumsArgs a = BoxSumsArgs(b);
+  a.counter = reinterpret_cast<unsigned *>(base);
+  a.partials = reinterpret_cast<Partial4 *>(base + kNoAllocPartialsOff);
+  a.part = reinterpret_cast<double *>(base + kNoAllocPartOff);
+  hipLaunchKernelGGL(ZeroCounterKernel, dim3(1), dim3(1), 0, b->stream, a.counter);
+  HIP_TRY(b, hipGetLastError());
+  HIP_TRY(b, LaunchRecipSums(a, b->stream));
+  rc = Sync(b);
+  if (rc) return rc;
+  *erecip = b->hostRes[2];
+  return GOMC_OK;
+}
 
 static double BoxHostMinImage(double d, int axis, const gomc_box *b) {
   const double ax = b->geom.L[axis], half = b->geom.half[axis];
@@ -1272,6 +1346,14 @@ int gomc_recip_sums(gomc_box *b, double *erecip) {
   int rc = FlushPending(b);
   if (rc) return rc;
   return RecipSums(b, erecip);
+}
+
+int gomc_recip_sums_noalloc(gomc_box *b, double *erecip) {
+  if (!b || !erecip) return GOMC_ERR_ARG;
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);
+  if (rc) return rc;
+  return RecipSumsNoAlloc(b, erecip);
 }
 
 int gomc_recip_accept(gomc_box *b) {

@@ -161,6 +161,14 @@ int gomc_recip_init(gomc_box *b, double *erecip);
  * CallBoxReciprocalSumsGPU.  Fills the "new" sums from the current
  * coordinates; gomc_recip_accept makes them the reference sums. */
 int gomc_recip_sums(gomc_box *b, double *erecip);
+/* gomc_recip_sums without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same kernels and results; the scratch
+ * (per-chunk sums, block partials, counter) lives in one 32.1 MiB
+ * file-scope device array per GPU, held until the process ends, so calls
+ * that overlap on one GPU race.  Refuses boxes of more than 2^20 k-vectors
+ * (GOMC_ERR_ARG).  The box must have been set up by gomc_recip_init. */
+int gomc_recip_sums_noalloc(gomc_box *b, double *erecip);
 /* Ewald::MolReciprocal (Ewald.cpp:409-473); replaces CallMolReciprocalGPU
  * (src/GPU/CalculateEwaldCUDAKernel.cuh:31-36).  *enew = reciprocal energy
  * with molecule `mol` at (nx,ny,nz); the caller's delta is
