@@ -84,6 +84,7 @@ struct DevState {
   // block, once the workgroup has computed it (nextReady), is mt[mtCur ^ 1]
   uint32_t mt[2][kMtN];
   int mtIdx, mtCur, nextReady, status;
+  int recipSwapped;  // the launch ended with the two structure-factor arrays swapped
   double pot[7];  // inter, tc, real, recip, self, corr, total
   double scale[2][kDevKinds];
   uint64_t tries[2][kDevKinds], acc[2][kDevKinds], tTries[2][kDevKinds], tAcc[2][kDevKinds];
@@ -516,26 +517,53 @@ NOINLINE __device__ void ComboEnergies(const DevBox &B, Ctl &c) {
   __syncthreads();
 }
 
+// Waves of a translate/rotate trial that take the pair terms while the rest
+// take the k-vectors, in proportion to 2 N_live distance tests per atom
+// against kRecipWeight K phase products per atom (at least two waves each).
+constexpr double kRecipWeight = 1.0;
+__device__ __forceinline__ int PairWaves(int nLive, int K) {
+  const double wp = 2.0 * double(nLive), wr = kRecipWeight * double(K);
+  const int pw = int(double(kDevWaves) * wp / (wp + wr) + 0.5);
+  return pw < 2 ? 2 : (pw > kDevWaves - 2 ? kDevWaves - 2 : pw);
+}
+
 // One translate / rotate trial's energies: every atom against the old and
 // new positions (CalculateEnergy::MoleculeInter, CalculateEnergy.cpp:581-686)
-// and every k-vector's delta (Ewald::MolReciprocal, Ewald.cpp:409-473).
+// and every k-vector's delta (Ewald::MolReciprocal, Ewald.cpp:409-473), the
+// pair waves and the k-vector waves running side by side.
 template <int P>
 NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c, PhaseTab &tab) {
   double lj = 0.0, real = 0.0, ov = 0.0, e = 0.0;
   const int st = c.st, len = c.len, ntypes = B.ntypes;
   const BoxGeom g = B.g;
-  const GD *x = G(B.x), *y = G(B.y), *z = G(B.z), *q = G(B.q);
-  const GI *type = G(B.type);
-  const GD *sigSq = G(B.sigSq), *epsCn = G(B.epsCn), *nHalf = G(B.nHalf);
-  // (atom p, old) then (atom p, new) for each p, the molecule's data read
-  // from LDS; in-cutoff pairs go through the wave's queue
-  {
-    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const bool doRecip = c.doRecip;
+  if (doRecip) {  // new positions are table rows [0, len), old ones [kDevKindAtoms, +len)
+    __shared__ double tpx[kDevPos], tpy[kDevPos], tpz[kDevPos];
+    if (threadIdx.x < kDevKindAtoms) {
+      const int p = threadIdx.x;
+      tpx[p] = c.nx[p];
+      tpy[p] = c.ny[p];
+      tpz[p] = c.nz[p];
+      tpx[kDevKindAtoms + p] = c.ox[p];
+      tpy[kDevKindAtoms + p] = c.oy[p];
+      tpz[kDevKindAtoms + p] = c.oz[p];
+    }
+    __syncthreads();
+    BuildPhases<P>(B, tpx, tpy, tpz, kDevPos, tab);  // ends with a barrier
+  }
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const int pw = doRecip ? PairWaves(c.nLive, B.K) : kDevWaves;
+  if (wave < pw) {
+    // (atom p, old) then (atom p, new) for each p, the molecule's data read
+    // from LDS; in-cutoff pairs go through the wave's queue
+    const GD *x = G(B.x), *y = G(B.y), *z = G(B.z), *q = G(B.q);
+    const GI *type = G(B.type);
+    const GD *sigSq = G(B.sigSq), *epsCn = G(B.epsCn), *nHalf = G(B.nHalf);
     WaveQ &wq = sQ[wave];
     int cnt = 0;
     const int nLive = c.nLive;
     const GI *live = G(B.live);
-    for (int base = wave * kWave; base < nLive; base += kDevThreads) {
+    for (int base = wave * kWave; base < nLive; base += pw * kWave) {
       const int li = base + lane;
       const int j = li < nLive ? live[li] : 0;
       const bool valid = li < nLive && !(j >= st && j < st + len);
@@ -565,27 +593,14 @@ NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c, PhaseTab &tab
     }
     DrainQ<P>(g, wq, cnt, lane, sigSq, epsCn, nHalf, lj, real);
     __builtin_amdgcn_wave_barrier();
-  }
-  if (c.doRecip) {
+  } else {
+    // every k-vector's new structure factor and energy term, beside the pairs
     const GD *pref = G(B.pref);
     const GD *sumR = G(B.sumR), *sumI = G(B.sumI);
     const GI *kpack = G(B.kpack);
     GDW *newR = GW(B.newR), *newI = GW(B.newI);
-    {  // new positions are table rows [0, len), old ones [kDevKindAtoms, +len)
-      __shared__ double tpx[kDevPos], tpy[kDevPos], tpz[kDevPos];
-      if (threadIdx.x < kDevKindAtoms) {
-        const int p = threadIdx.x;
-        tpx[p] = c.nx[p];
-        tpy[p] = c.ny[p];
-        tpz[p] = c.nz[p];
-        tpx[kDevKindAtoms + p] = c.ox[p];
-        tpy[kDevKindAtoms + p] = c.oy[p];
-        tpz[kDevKindAtoms + p] = c.oz[p];
-      }
-      __syncthreads();
-      BuildPhases<P>(B, tpx, tpy, tpz, kDevPos, tab);
-    }
-    for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
+    const int r0 = pw * kWave, rn = kDevThreads - r0;
+    for (int k = int(threadIdx.x) - r0; k < B.K; k += rn) {
       double sRN = 0.0, sIN = 0.0, sRO = 0.0, sIO = 0.0;
       const int code = kpack[k];
       const int ix = code & 0xff, iy = ((code >> 8) & 0xff) - 128, iz = ((code >> 16) & 0xff) - 128;
@@ -641,13 +656,16 @@ NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, Ctl &c, PhaseTab &tab)
   BlockSum4(c, e, 0.0, 0.0, 0.0);
 }
 
-__device__ void CommitRecip(const DevBox &B) {
-  GDW *sumR = GW(B.sumR), *sumI = GW(B.sumI);
-  const GD *newR = G(B.newR), *newI = G(B.newI);
-  for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
-    sumR[k] = newR[k];
-    sumI[k] = newI[k];
-  }
+// An accepted trial's structure factors become the box's (thread 0): the
+// trial phases wrote every k of newR/newI, so the two arrays trade places
+// instead of K values being copied.
+__device__ void SwapRecip(DevBox &B) {
+  double *t = B.sumR;
+  B.sumR = B.newR;
+  B.newR = t;
+  t = B.sumI;
+  B.sumI = B.newI;
+  B.newI = t;
 }
 
 // ------------------------------------------------- thread-0 serial pieces --
@@ -1214,7 +1232,7 @@ NOINLINE __device__ void TransferFinish(const DevBox &B, DevState &s, Ctl &c, in
 
 // The MoleculeTransfer move after PickArbDist (gcmc.cpp TransferMove).
 template <int P>
-__device__ void TransferMoveD(const DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
+__device__ void TransferMoveD(DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
   if (threadIdx.x == 0) TransferPick(B, s, c, i);
   __syncthreads();
   if (c.op != kOpTransfer) return;
@@ -1246,7 +1264,7 @@ __device__ void TransferMoveD(const DevBox &B, DevState &s, Ctl &c, PhaseTab &ta
       B.z[st + p] = c.nm.pz[p];
       B.mol[st + p] = c.dst == 0 ? m : -1;
     }
-    if (c.recipLose != 0.0 || c.recipGain != 0.0) CommitRecip(B);
+    if ((c.recipLose != 0.0 || c.recipGain != 0.0) && threadIdx.x == 0) SwapRecip(B);
   }
   __syncthreads();
   if (threadIdx.x == 0 && c.commit && c.found < 0) {  // lookup out of step: stop here
@@ -1371,7 +1389,7 @@ NOINLINE __device__ void MolAccept(const DevBox &B, DevState &s, Ctl &c, int64_t
 }
 
 template <int P>
-__device__ void MolMoveD(const DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
+__device__ void MolMoveD(DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
   if (threadIdx.x == 0) MolPick(B, s, c, i);
   __syncthreads();
   if (c.op != kOpMol) return;
@@ -1385,7 +1403,7 @@ __device__ void MolMoveD(const DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, in
       B.y[c.st + p] = c.ny[p];
       B.z[c.st + p] = c.nz[p];
     }
-    if (B.g.ewald) CommitRecip(B);
+    if (B.g.ewald && threadIdx.x == 0) SwapRecip(B);
   }
   __syncthreads();
 }
@@ -1444,6 +1462,7 @@ __global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *b
     S.sumN[1] += B.nkinds > 1 ? double(NumKindInBox(B, S, 1, 0)) : 0.0;
     S.samples++;
   }
+  if (threadIdx.x == 0) S.recipSwapped = B.sumR != boxes[blockIdx.x].sumR;
   __syncthreads();
   {
     const uint32_t *src = reinterpret_cast<const uint32_t *>(&S);
@@ -1698,6 +1717,10 @@ int DevDownload(gomc_box *b) {
   HIP_TRY(b, hipMemcpyAsync(b->hy.data(), b->y, bytes, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->hz.data(), b->z, bytes, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(b, hipStreamSynchronize(b->stream));
+  if (s.recipSwapped) {  // the box's structure factors ended in the other array
+    std::swap(b->sumRref, b->sumRnew);
+    std::swap(b->sumIref, b->sumInew);
+  }
   const int K = b->nkinds;
   b->mt.SetState(s.mt[s.mtCur], s.mtIdx);
   b->pot.inter = s.pot[0];
