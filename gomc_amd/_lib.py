@@ -42,6 +42,7 @@ SIGNATURES = {
     "gomc_recip_init": (C.c_int, [_vp, _dp]),
     "gomc_recip_sums": (C.c_int, [_vp, _dp]),
     "gomc_recip_sums_noalloc": (C.c_int, [_vp, _dp]),
+    "gomc_set_device_team": (C.c_int, [_vp, C.c_int]),
     "gomc_set_mol_box": (C.c_int, [_vp, C.c_int, C.c_int]),
     "gomc_mol_recip": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
     "gomc_swap_recip": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]),

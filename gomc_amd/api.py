@@ -140,6 +140,10 @@ class Box:
         self._check(self.lib.gomc_recip_sums(self.h, C.byref(e)))
         return e.value
 
+    def set_device_team(self, team: int):
+        """Workgroups per box in the device move loop (0: by k-vector count)."""
+        self._check(self.lib.gomc_set_device_team(self.h, int(team)))
+
     def recip_sums_noalloc(self) -> float:
         """recip_sums with its scratch in the library's static device array
         (no device allocation in the call)."""

@@ -131,6 +131,7 @@ struct gomc_box {
   int device = 0;
   hipStream_t stream = nullptr;
   int precision = 0;  // 1: trial kernels with FP32 pair terms and phases, FP64 sums
+  int devTeam = 0;    // workgroups per box in the device move loop (0: by k-vector count)
   std::string err;
 
   // ---- topology and force field (host) ----

@@ -42,6 +42,14 @@ constexpr int kTab = 2 * kDevNMax + 1;   // phase-table entries per axis
 constexpr int kDevPos = 2 * kDevKindAtoms;  // positions of one trial: new, then old
 constexpr int kDisplace = 0, kRotate = 1, kMolTransfer = 10;
 constexpr double kSmallWeight = 1.0e-38;  // lib/NumLib.h:19
+// A box's moves may run on a team of workgroups: every member runs the whole
+// move (draws, pair terms, CBMC stages, the Metropolis test) on the same
+// data and so takes the same decisions, and the k-vectors are split among
+// the members, whose partial reciprocal energies are exchanged once per move
+// through device-coherent atomics (TeamExchange).
+constexpr int kDevTeamMax = 8;
+constexpr int kDevXchgWords = 2 * kDevTeamMax * 2;  // [parity][member]{value, seq}
+constexpr int64_t kDevSpinLimit = int64_t(1) << 22;  // polls before a wait fails
 
 // Read-only description of one box for a launch.
 struct DevBox {
@@ -69,7 +77,11 @@ struct DevBox {
   int *molBox;
   int *live, *livePos;  // atoms of the box's molecules (any order), each atom's place or -1
   int nLive0;
-  uint32_t *lookup;
+  uint32_t *lookup;     // live, livePos and lookup are this member's own copies
+  // team: members, this member's rank, its k-vectors [k0, k1), the box's
+  // exchange block (kDevXchgWords words, zero at the launch's start)
+  int team, rank, k0, k1;
+  uint64_t *xchg;
   double *com;  // [3][nmols]
   struct DevState *st;
   int64_t step0, nsteps;
@@ -95,7 +107,7 @@ struct DevState {
   int64_t samples, moves, accepted;
 };
 
-enum : int { kStErr = 1, kStEmptyReservoir = 2, kStWeights = 3, kStMoveKind = 4 };
+enum : int { kStErr = 1, kStEmptyReservoir = 2, kStWeights = 3, kStMoveKind = 4, kStTeam = 5 };
 
 // One trial molecule (cbmc::TrialMol).
 struct DTrial {
@@ -126,6 +138,7 @@ struct Ctl {
   DTrial nm, om;
   int64_t done, cycleLeft;
   int cyclesDone;
+  uint64_t xseq;  // the team's next exchange (1, 2, ... in every member)
   double red[kDevWaves][4];
   double res[4];
 };
@@ -325,6 +338,63 @@ __device__ void BlockSum4(Ctl &c, double a, double b, double d, double e) {
     double r = 0.0;
     for (int w = 0; w < kDevWaves; ++w) r += c.red[w][threadIdx.x];
     c.res[threadIdx.x] = r;
+  }
+  __syncthreads();
+}
+
+// The team's sum of one value per member, in member order, into c.res[slot]
+// of every member (all threads call it at the same point of a move; with a
+// team of one it does nothing).  Each member's thread 0 publishes its value,
+// then the exchange's sequence number; lanes 0..team-1 of wave 0 wait for
+// the other members' numbers and read their values.  Every move of a team
+// makes exactly one exchange, so the numbers run 1, 2, ... alike in every
+// member and a member is at most one exchange ahead of the slowest: two
+// parities of slots suffice.  A wait that outlasts kDevSpinLimit polls fails
+// the launch (status kStTeam) instead of hanging.
+NOINLINE __device__ void TeamExchange(const DevBox &B, DevState &s, Ctl &c, int slot) {
+  if (B.team <= 1) return;
+  if (threadIdx.x < kWave) {
+    const int lane = threadIdx.x;
+    const uint64_t seq = c.xseq;
+    uint64_t *blk = B.xchg + (seq & 1) * (kDevTeamMax * 2);
+    const double v = slot >= 0 ? c.res[slot] : 0.0;  // slot -1: a move without energies
+    if (lane == 0) {
+      __hip_atomic_exchange(blk + 2 * B.rank, __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_exchange(blk + 2 * B.rank + 1, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    double mine = 0.0;
+    int ok = 1;
+    if (lane < B.team) {
+      if (lane == B.rank) {
+        mine = v;
+      } else {
+        for (int64_t n = 0;; ++n) {
+          if (__hip_atomic_load(blk + 2 * lane + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) ==
+              seq)
+            break;
+          if (n > kDevSpinLimit) {
+            ok = 0;
+            break;
+          }
+        }
+        if (ok)
+          mine = __builtin_bit_cast(
+              double, __hip_atomic_load(blk + 2 * lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+      }
+    }
+    const bool all = __all(ok) != 0;
+    double total = 0.0;
+    for (int q = 0; q < B.team; ++q) total += __shfl(mine, q, kWave);
+    if (lane == 0) {
+      if (all) {
+        if (slot >= 0) c.res[slot] = total;
+      } else {
+        s.status = kStTeam;
+        c.abort = 1;
+      }
+      c.xseq = seq + 1;
+    }
   }
   __syncthreads();
 }
@@ -532,7 +602,7 @@ __device__ __forceinline__ int PairWaves(int nLive, int K) {
 // and every k-vector's delta (Ewald::MolReciprocal, Ewald.cpp:409-473), the
 // pair waves and the k-vector waves running side by side.
 template <int P>
-NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c, PhaseTab &tab) {
+NOINLINE __device__ void MolTrialEnergies(const DevBox &B, DevState &s, Ctl &c, PhaseTab &tab) {
   double lj = 0.0, real = 0.0, ov = 0.0, e = 0.0;
   const int st = c.st, len = c.len, ntypes = B.ntypes;
   const BoxGeom g = B.g;
@@ -552,7 +622,9 @@ NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c, PhaseTab &tab
     BuildPhases<P>(B, tpx, tpy, tpz, kDevPos, tab);  // ends with a barrier
   }
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-  const int pw = doRecip ? PairWaves(c.nLive, B.K) : kDevWaves;
+  // the same split in every member of a team (its pair sums, computed by
+  // each member in full, must agree to the last bit)
+  const int pw = doRecip ? PairWaves(c.nLive, (B.K + B.team - 1) / B.team) : kDevWaves;
   if (wave < pw) {
     // (atom p, old) then (atom p, new) for each p, the molecule's data read
     // from LDS; in-cutoff pairs go through the wave's queue
@@ -600,7 +672,7 @@ NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c, PhaseTab &tab
     const GI *kpack = G(B.kpack);
     GDW *newR = GW(B.newR), *newI = GW(B.newI);
     const int r0 = pw * kWave, rn = kDevThreads - r0;
-    for (int k = int(threadIdx.x) - r0; k < B.K; k += rn) {
+    for (int k = B.k0 + int(threadIdx.x) - r0; k < B.k1; k += rn) {
       double sRN = 0.0, sIN = 0.0, sRO = 0.0, sIO = 0.0;
       const int code = kpack[k];
       const int ix = code & 0xff, iy = ((code >> 8) & 0xff) - 128, iz = ((code >> 16) & 0xff) - 128;
@@ -623,11 +695,12 @@ NOINLINE __device__ void MolTrialEnergies(const DevBox &B, Ctl &c, PhaseTab &tab
     }
   }
   BlockSum4(c, lj, real, e, ov);
+  TeamExchange(B, s, c, 2);  // the members' k-vector ranges
 }
 
 // Ewald::SwapDestRecip / SwapSourceRecip (Ewald.cpp:478-531,657-710).
 template <int P>
-NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, Ctl &c, PhaseTab &tab) {
+NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, DevState &s, Ctl &c, PhaseTab &tab) {
   double e = 0.0;
   const int len = c.len, sign = c.sign;
   const GD *pref = G(B.pref);
@@ -635,7 +708,7 @@ NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, Ctl &c, PhaseTab &tab)
   const GI *kpack = G(B.kpack);
   GDW *newR = GW(B.newR), *newI = GW(B.newI);
   BuildPhases<P>(B, c.nx, c.ny, c.nz, kDevKindAtoms, tab);
-  for (int k = threadIdx.x; k < B.K; k += kDevThreads) {
+  for (int k = B.k0 + int(threadIdx.x); k < B.k1; k += kDevThreads) {
     double sR = 0.0, sI = 0.0;
     const int code = kpack[k];
     const int ix = code & 0xff, iy = ((code >> 8) & 0xff) - 128, iz = ((code >> 16) & 0xff) - 128;
@@ -654,6 +727,7 @@ NOINLINE __device__ void SwapRecipEnergy(const DevBox &B, Ctl &c, PhaseTab &tab)
     e += (R * R + I * I) * pref[k];
   }
   BlockSum4(c, e, 0.0, 0.0, 0.0);
+  TeamExchange(B, s, c, 0);
 }
 
 // An accepted trial's structure factors become the box's (thread 0): the
@@ -1235,7 +1309,10 @@ template <int P>
 __device__ void TransferMoveD(DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
   if (threadIdx.x == 0) TransferPick(B, s, c, i);
   __syncthreads();
-  if (c.op != kOpTransfer) return;
+  if (c.op != kOpTransfer) {  // nothing to transfer: the move's exchange alone
+    if (!c.abort) TeamExchange(B, s, c, -1);
+    return;
+  }
   if (c.len == 1) {
     SingleStage<P>(B, s, c, c.nm, 0, true);
     if (c.abort) return;
@@ -1248,7 +1325,11 @@ __device__ void TransferMoveD(DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int
   if (c.abort) return;
   if (threadIdx.x == 0) TransferCalc(B, s, c);
   __syncthreads();
-  if (c.doRecip) SwapRecipEnergy<P>(B, c, tab);
+  if (c.doRecip)
+    SwapRecipEnergy<P>(B, s, c, tab);
+  else
+    TeamExchange(B, s, c, -1);  // one exchange per move
+  if (c.abort) return;
   if (threadIdx.x == 0) TransferAccept(B, s, c);
   __syncthreads();
   if (c.commit) {
@@ -1392,8 +1473,12 @@ template <int P>
 __device__ void MolMoveD(DevBox &B, DevState &s, Ctl &c, PhaseTab &tab, int64_t i) {
   if (threadIdx.x == 0) MolPick(B, s, c, i);
   __syncthreads();
-  if (c.op != kOpMol) return;
-  MolTrialEnergies<P>(B, c, tab);
+  if (c.op != kOpMol) {  // no molecule of the kind: the move's exchange alone
+    TeamExchange(B, s, c, -1);
+    return;
+  }
+  MolTrialEnergies<P>(B, s, c, tab);
+  if (c.abort) return;
   if (threadIdx.x == 0) MolAccept(B, s, c, i);
   __syncthreads();
   if (c.commit) {
@@ -1436,6 +1521,7 @@ __global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *b
     c.done = 0;
     c.cyclesDone = 0;
     c.cycleLeft = -1;
+    c.xseq = 1;
   }
   __syncthreads();
   while (true) {
@@ -1464,7 +1550,7 @@ __global__ void __launch_bounds__(kDevThreads) DeviceMovesKernel(const DevBox *b
   }
   if (threadIdx.x == 0) S.recipSwapped = B.sumR != boxes[blockIdx.x].sumR;
   __syncthreads();
-  {
+  if (B.rank == 0) {  // the members of a team hold the same state
     const uint32_t *src = reinterpret_cast<const uint32_t *>(&S);
     uint32_t *dst = reinterpret_cast<uint32_t *>(B.st);
     for (int w = threadIdx.x; w < int(sizeof(DevState) / 4); w += kDevThreads) dst[w] = src[w];
@@ -1489,6 +1575,8 @@ struct DevDriver {
   DevBox *list = nullptr;       // launch list when this box leads a launch
   int listCap = 0;
   DevState hstate;
+  int team = 0;                 // members; live, livePos and lookup hold one copy each
+  uint64_t *xchg = nullptr;     // the team's exchange block
 };
 
 void DevFree(gomc_box *b) {
@@ -1496,13 +1584,22 @@ void DevFree(gomc_box *b) {
   if (!d) return;
   for (void *p : {(void *)d->state, (void *)d->kpack, (void *)d->molKind, (void *)d->molBox,
                   (void *)d->live, (void *)d->livePos,
-                  (void *)d->lookup, (void *)d->com, (void *)d->trace, (void *)d->list})
+                  (void *)d->lookup, (void *)d->com, (void *)d->trace, (void *)d->list,
+                  (void *)d->xchg})
     if (p) (void)hipFree(p);
   delete d;
   b->dev = nullptr;
 }
 
 namespace {
+
+// Members of the box's team: set by gomc_set_device_team, else by the
+// k-vector count (the k-vectors are what the members share out).
+int DevTeamSize(const gomc_box *b) {
+  if (b->devTeam > 0) return std::min(b->devTeam, kDevTeamMax);
+  if (!b->ewald) return 1;
+  return b->K >= 16384 ? 4 : (b->K >= 8192 ? 2 : 1);
+}
 
 int DevCheck(gomc_box *b) {
   if (b->cached) return Fail(b, GOMC_ERR_STATE, "device move loop: EwaldCached mode is not supported");
@@ -1529,10 +1626,8 @@ int DevEnsure(gomc_box *b, int64_t traceMoves) {
     HIP_TRY(b, hipMalloc(&d->state, sizeof(DevState)));
     HIP_TRY(b, hipMalloc(&d->molKind, sizeof(int) * std::max(1, b->nmols)));
     HIP_TRY(b, hipMalloc(&d->molBox, sizeof(int) * std::max(1, b->nmols)));
-    HIP_TRY(b, hipMalloc(&d->lookup, sizeof(uint32_t) * std::max(1, b->nmols)));
     HIP_TRY(b, hipMalloc(&d->com, sizeof(double) * 3 * std::max(1, b->nmols)));
-    HIP_TRY(b, hipMalloc(&d->live, sizeof(int) * std::max(1, b->natoms)));
-    HIP_TRY(b, hipMalloc(&d->livePos, sizeof(int) * std::max(1, b->natoms)));
+    HIP_TRY(b, hipMalloc(&d->xchg, sizeof(uint64_t) * kDevXchgWords));
     HIP_TRY(b, hipMemcpy(d->molKind, b->molKind.data(), sizeof(int) * b->nmols, hipMemcpyHostToDevice));
     // index triples of the box's k-vectors, in their order
     std::vector<double> kx, ky, kz, pref;
@@ -1553,6 +1648,18 @@ int DevEnsure(gomc_box *b, int64_t traceMoves) {
                            hipMemcpyHostToDevice));
   }
   DevDriver *d = b->dev;
+  const int team = DevTeamSize(b);
+  if (team != d->team) {  // one copy of the lookup and the live-atom lists per member
+    for (void *p : {(void *)d->lookup, (void *)d->live, (void *)d->livePos})
+      if (p) (void)hipFree(p);
+    d->lookup = nullptr;
+    d->live = d->livePos = nullptr;
+    d->team = 0;
+    HIP_TRY(b, hipMalloc(&d->lookup, sizeof(uint32_t) * team * std::max(1, b->nmols)));
+    HIP_TRY(b, hipMalloc(&d->live, sizeof(int) * team * std::max(1, b->natoms)));
+    HIP_TRY(b, hipMalloc(&d->livePos, sizeof(int) * team * std::max(1, b->natoms)));
+    d->team = team;
+  }
   if (traceMoves > d->traceCap) {
     if (d->trace) (void)hipFree(d->trace);
     d->trace = nullptr;
@@ -1630,9 +1737,14 @@ int DevUpload(gomc_box *b, int64_t step0, int64_t nsteps, int cycles, bool trace
   h.molStart = b->molStartDev;
   h.molKind = d->molKind;
   h.molBox = d->molBox;
-  h.live = d->live;
+  h.live = d->live;  // rank 0's copies; Launch points each member at its own
   h.livePos = d->livePos;
   h.lookup = d->lookup;
+  h.team = d->team;
+  h.rank = 0;
+  h.k0 = 0;
+  h.k1 = b->ewald ? b->K : 0;
+  h.xchg = d->xchg;
   h.com = d->com;
   h.st = d->state;
   h.step0 = step0;
@@ -1691,10 +1803,16 @@ int DevUpload(gomc_box *b, int64_t step0, int64_t nsteps, int cycles, bool trace
   HIP_TRY(b, hipMemcpyAsync(d->state, &s, sizeof(s), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(d->com, com.data(), sizeof(double) * com.size(), hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(d->molBox, molBox.data(), sizeof(int) * b->nmols, hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(b, hipMemcpyAsync(d->lookup, b->lookup.data(), sizeof(uint32_t) * b->nmols, hipMemcpyHostToDevice, b->stream));
+  for (int r = 0; r < d->team; ++r)
+    HIP_TRY(b, hipMemcpyAsync(d->lookup + size_t(r) * b->nmols, b->lookup.data(),
+                              sizeof(uint32_t) * b->nmols, hipMemcpyHostToDevice, b->stream));
   if (!live.empty())
-    HIP_TRY(b, hipMemcpyAsync(d->live, live.data(), sizeof(int) * live.size(), hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(b, hipMemcpyAsync(d->livePos, livePos.data(), sizeof(int) * b->natoms, hipMemcpyHostToDevice, b->stream));
+    for (int r = 0; r < d->team; ++r)
+      HIP_TRY(b, hipMemcpyAsync(d->live + size_t(r) * b->natoms, live.data(),
+                                sizeof(int) * live.size(), hipMemcpyHostToDevice, b->stream));
+  for (int r = 0; r < d->team; ++r)
+    HIP_TRY(b, hipMemcpyAsync(d->livePos + size_t(r) * b->natoms, livePos.data(),
+                              sizeof(int) * b->natoms, hipMemcpyHostToDevice, b->stream));
   // the copies read host vectors that go out of scope: complete them now
   HIP_TRY(b, hipStreamSynchronize(b->stream));
   return GOMC_OK;
@@ -1761,31 +1879,82 @@ int DevDownload(gomc_box *b) {
       return Fail(b, GOMC_ERR_STATE, "CBMC weights inconsistent");
     case kStMoveKind:
       return Fail(b, GOMC_ERR_STATE, "move kind not served by this driver");
+    case kStTeam:
+      return Fail(b, GOMC_ERR_STATE, "device move loop: a team member did not answer");
     default:
       return Fail(b, GOMC_ERR_STATE, "device move loop failed");
   }
 }
 
+// Workgroups of the device move loop that the device holds at once.
+int ResidentWorkgroups(gomc_box *b, int *out) {
+  int perCU = 0, cus = 0;
+  hipError_t e = b->precision == 1
+                     ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, DeviceMovesKernel<1>,
+                                                                    kDevThreads, 0)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, DeviceMovesKernel<0>,
+                                                                    kDevThreads, 0);
+  if (e != hipSuccess) return Fail(b, GOMC_ERR_HIP, std::string("occupancy: ") + hipGetErrorString(e));
+  HIP_TRY(b, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+  *out = perCU * cus;
+  return GOMC_OK;
+}
+
+// The boxes' moves, each box on its team of workgroups.  A launch holds
+// whole teams and no more workgroups than the device holds at once, so a
+// member never waits for one that cannot be scheduled; boxes beyond that go
+// to the next launch (a box's result does not depend on its launch).
 int Launch(gomc_box *lead, const std::vector<gomc_box *> &boxes) {
   DevDriver *d = lead->dev;
-  const int n = int(boxes.size());
-  if (n > d->listCap) {
+  int cap = 0;
+  int rc = ResidentWorkgroups(lead, &cap);
+  if (rc) return rc;
+  if (cap > d->listCap) {
     if (d->list) (void)hipFree(d->list);
     d->list = nullptr;
     d->listCap = 0;
-    HIP_TRY(lead, hipMalloc(&d->list, sizeof(DevBox) * n));
-    d->listCap = n;
+    HIP_TRY(lead, hipMalloc(&d->list, sizeof(DevBox) * cap));
+    d->listCap = cap;
   }
-  std::vector<DevBox> v(n);
-  for (int i = 0; i < n; ++i) v[i] = boxes[i]->dev->hbox;
-  HIP_TRY(lead, hipMemcpyAsync(d->list, v.data(), sizeof(DevBox) * n, hipMemcpyHostToDevice,
-                               lead->stream));
-  if (lead->precision == 1)
-    hipLaunchKernelGGL(DeviceMovesKernel<1>, dim3(n), dim3(kDevThreads), 0, lead->stream, d->list);
-  else
-    hipLaunchKernelGGL(DeviceMovesKernel<0>, dim3(n), dim3(kDevThreads), 0, lead->stream, d->list);
-  HIP_TRY(lead, hipGetLastError());
-  HIP_TRY(lead, hipStreamSynchronize(lead->stream));
+  std::vector<DevBox> v;
+  size_t i = 0;
+  while (i < boxes.size()) {
+    v.clear();
+    for (; i < boxes.size(); ++i) {
+      gomc_box *b = boxes[i];
+      DevDriver *bd = b->dev;
+      const int team = bd->team;
+      if (team > cap)
+        return Fail(lead, GOMC_ERR_STATE, "device move loop: a team larger than the device holds");
+      if (int(v.size()) + team > cap) break;
+      if (team > 1)  // exchange numbers start again at 1
+        HIP_TRY(lead, hipMemsetAsync(bd->xchg, 0, sizeof(uint64_t) * kDevXchgWords, lead->stream));
+      const int K = bd->hbox.k1;
+      for (int r = 0; r < team; ++r) {
+        DevBox m = bd->hbox;
+        m.rank = r;
+        m.k0 = int(int64_t(K) * r / team);
+        m.k1 = int(int64_t(K) * (r + 1) / team);
+        m.live = bd->live + size_t(r) * b->natoms;
+        m.livePos = bd->livePos + size_t(r) * b->natoms;
+        m.lookup = bd->lookup + size_t(r) * b->nmols;
+        if (r > 0) {  // the leader records the moves
+          m.tKind = m.tMol = m.tAcc = m.tOvl = nullptr;
+          m.tE = nullptr;
+        }
+        v.push_back(m);
+      }
+    }
+    const int n = int(v.size());
+    HIP_TRY(lead, hipMemcpyAsync(d->list, v.data(), sizeof(DevBox) * n, hipMemcpyHostToDevice,
+                                 lead->stream));
+    if (lead->precision == 1)
+      hipLaunchKernelGGL(DeviceMovesKernel<1>, dim3(n), dim3(kDevThreads), 0, lead->stream, d->list);
+    else
+      hipLaunchKernelGGL(DeviceMovesKernel<0>, dim3(n), dim3(kDevThreads), 0, lead->stream, d->list);
+    HIP_TRY(lead, hipGetLastError());
+    HIP_TRY(lead, hipStreamSynchronize(lead->stream));
+  }
   return GOMC_OK;
 }
 
@@ -1831,6 +2000,12 @@ int64_t gomc_run_moves_device(gomc_box *b, int64_t start_step, int64_t nsteps, i
     if (drecip) drecip[i] = te[3 * i + 2];
   }
   return accepted;
+}
+
+int gomc_set_device_team(gomc_box *b, int team) {
+  if (!b || team < 0 || team > kDevTeamMax) return GOMC_ERR_ARG;
+  b->devTeam = team;
+  return GOMC_OK;
 }
 
 int gomc_run_moves_device_multi(gomc_box *const *boxes, int n, const int64_t *start_step,

@@ -281,8 +281,18 @@ int64_t gomc_run_moves_device(gomc_box *b, int64_t start_step, int64_t nsteps,
                               int32_t *kind, int32_t *mol, int32_t *acc,
                               int32_t *ovl, double *dlj, double *dreal,
                               double *drecip);
+/* Workgroups per box in the device move loop: 1..8, or 0 for the default
+ * (by k-vector count: 4 from 16384, 2 from 8192, else 1; 1 without Ewald).
+ * Every member runs the box's whole move on the same data and takes the
+ * same decisions; the k-vectors are split among the members, whose partial
+ * reciprocal energies are summed in member order once per move, so a
+ * box's results depend on its team size and on nothing else in a launch.
+ * No counterpart in the reference. */
+int gomc_set_device_team(gomc_box *b, int team);
 /* One launch of the device move loop over `n` boxes of one device and one
- * precision (a workgroup per box).  Box i runs from start_step[i]:
+ * precision (a team of workgroups per box, gomc_set_device_team; boxes
+ * beyond what the device holds at once go to a further launch).  Box i
+ * runs from start_step[i]:
  * nsteps[i] moves, or with cycles (may be NULL) up to cycles[i] cycles of
  * N_box moves each (N_box counted at the cycle's start, at least 1, never
  * more than nsteps[i] moves in all), adding its kind-0 and kind-1 molecule

@@ -159,3 +159,50 @@ def test_mixed_precision_device_loop_error_against_fp64():
         a, b = rm[key][:upto], rf[key][:upto]
         ok = rf["ovl"][:upto] == 0
         assert np.all(np.abs(a[ok] - b[ok]) <= 1e-6 * np.maximum(np.abs(b[ok]), e)), key
+
+
+@pytest.mark.parametrize("team", [2, 4, 8])
+def test_team_of_workgroups_same_sequence_as_oracle(team):
+    """A box on a team of workgroups (the k-vectors split among the members,
+    every member running the whole move): the oracle's sequences, for
+    translations, rotations and transfers with CBMC growth, and the running
+    energies equal to a recomputation of the final state."""
+    s = gcmc_small(seed=5)
+    b, o = Box(s), Oracle(s, threads=8)
+    b.set_device_team(team)
+    e0 = o.energies()
+    n = 2500
+    ng, rg = b.run(n, device=True)
+    no, ro = o.run(n, mode=0)
+    assert ro["acc"][ro["kind"] == 10].sum() > 10
+    same_records(rg, ro, e0)
+    assert ng == no
+    assert np.array_equal(b.mol_boxes()[1], o.mol_boxes()[1])
+    assert np.allclose(b.coords(), o.coords(), rtol=0, atol=1e-9)
+    assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=1e-6)
+    assert np.allclose(b.system_total(), b.energies(), rtol=1e-10, atol=1e-6)
+    # the box continues on one workgroup: the state the team left is whole
+    b.set_device_team(1)
+    _, rg2 = b.run(500, start_step=n, device=True)
+    _, ro2 = o.run(500, start_step=n, mode=0)
+    same_records(rg2, ro2, e0)
+
+
+def test_teamed_box_alone_equals_in_a_launch_with_others():
+    """A 60 A box takes a team of 4 by its k-vector count; run beside boxes
+    on one workgroup it ends bit for bit as when run alone, with cycle
+    sampling."""
+    big = systems.gcmc_mixture(200, 200, 60.0, fugacity_bar=(5.0, 5.0), reservoir=3300, seed=41)
+    specs = [gcmc_small(seed=31), big, gcmc_small(seed=32)]
+    together = [Box(s) for s in specs]
+    assert together[1].nkvec >= 16384
+    moves, acc, sums, samples = run_device(together, 20000, cycles=3)
+    for i, s in enumerate(specs):
+        alone = Box(s)
+        m1, a1, s1, n1 = run_device([alone], 20000, cycles=3)
+        assert moves[i] == m1[0] and acc[i] == a1[0] and samples[i] == n1[0] == 3
+        assert np.array_equal(sums[i], s1[0])
+        assert np.array_equal(together[i].coords(), alone.coords())
+        assert np.array_equal(together[i].energies(), alone.energies())
+    b = together[1]
+    assert np.allclose(b.system_total(), b.energies(), rtol=1e-10, atol=1e-6)
