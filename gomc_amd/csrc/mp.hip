@@ -34,9 +34,17 @@ __device__ __forceinline__ double Uneg11(uint64_t v) {
 // CalculateEnergy::BoxForce (CalculateEnergy.cpp:268-406).  One workgroup
 // per (cell, up to 64 atoms of it); its G waves take the 27 neighbour cells
 // round robin and their per-atom partial forces are summed in wave order
-// through LDS.  Energies count each pair once (i < j).  A box with few
-// workgroups uses G = 16 (at most two neighbour cells per wave), a larger
-// one G = 4 (more workgroups resident per CU).
+// through LDS.  A wave walks a neighbour cell 64 atoms at a time: each lane
+// loads one atom (position, charge, type, molecule) into the wave's LDS
+// tile, then every lane runs over the tile, so the neighbour data arrive
+// as one coalesced load per tile instead of a chain of dependent loads per
+// pair.  A lane first marks its in-cutoff partners of the tile, then
+// evaluates only those, in ascending tile order: the pairs of a lane are
+// summed in the same order as by a plain walk over the cell.
+// Energies count each pair once (i < j).  A box with few workgroups uses
+// G = 8 (at most four neighbour cells per wave; 512 threads leave the pair
+// math 256 VGPRs, where 1024 would spill), a larger one G = 4 (more
+// workgroups resident per CU).
 struct PairForceArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
@@ -51,10 +59,67 @@ struct PairForceArgs {
   double *result;
 };
 
+// The energies (counted when `count`) and virial factors of one in-cutoff
+// pair: FFParticle::CalcCoulomb / CalcCoulombVir (src/FFParticle.cpp:
+// 387-399, 432-446) and CalcEn / CalcVir (295-359).  erfc(a r) is
+// exp(-(a r)^2) erfcx(a r) with erfcx from the fitted polynomial of
+// erfcx_poly.hpp (coefficients `poly`, in LDS), and the same exponential is the virial's Gaussian: one
+// exp and one polynomial per pair instead of libm's erfc, erf and exp, whose
+// range branches split a wave.  (1 - erf is erfc up to rounding: the
+// reference's form differs from this one by about 1e-16 absolute.)
+struct PairTerms {
+  double eLJ, eReal, vLJ, vReal;
+};
+__device__ __forceinline__ PairTerms PairMath(const BoxGeom &g, const double *poly, double d2,
+                                              double qiqj, double sig, double eps, double nh,
+                                              double eps6, double nov6, bool count) {
+  PairTerms p{0.0, 0.0, 0.0, 0.0};
+  if (g.elect && qiqj != 0.0 && !(g.rCutCoulSq < d2)) {
+    const double dist = sqrt(d2);
+    if (g.ewald) {
+      const double x = g.alpha * dist;
+      double ec, gauss;
+      if (x <= kErfcxUse) {
+        const double t = (x - kErfcxMid) * kErfcxInvHalf;
+        double px = poly[kErfcxDeg];
+        // read from LDS each step: unrolled, the coefficients would be held
+        // in 58 VGPRs across the pair loop
+#pragma unroll 1
+        for (int k = kErfcxDeg - 1; k >= 0; --k) px = fma(px, t, poly[k]);
+        gauss = exp(-(x * x));
+        ec = gauss * px;
+      } else {
+        ec = erfc(x);
+        gauss = exp(-(x * x));
+      }
+      if (count) p.eReal = qiqj * ec / dist;
+      p.vReal = qiqj * (ec / dist + g.alpha * M_2_SQRTPI * gauss) / d2;
+    } else {
+      if (count) p.eReal = qiqj / dist;
+      p.vReal = qiqj / (d2 * dist);
+    }
+  }
+  if (!(g.rCutLJSq < d2)) {
+    if (count) p.eLJ = LJEnergy(d2, sig, eps, nh);
+    p.vLJ = LJVirial(d2, sig, eps6, nov6, nh);
+  }
+  return p;
+}
+
+struct PairTile {
+  double x[kChunk], y[kChunk], z[kChunk], q[kChunk];
+  int type[kChunk], mol[kChunk], idx[kChunk];
+};
+
 template <int kGroups>
 __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairForceArgs a) {
+  static_assert(kChunk == kWave, "one tile entry per lane");
   __shared__ double sf[kGroups][3][kChunk];
   __shared__ double red[kGroups * kChunk / kWave];
+  __shared__ PairTile tile[kGroups];
+  __shared__ double poly[kErfcxDeg + 1];
+  if (threadIdx.x <= kErfcxDeg) poly[threadIdx.x] = kErfcxPoly[threadIdx.x];
+  __syncthreads();
   const int lane = threadIdx.x % kChunk, grp = threadIdx.x / kChunk;
   const int cell = a.chunks[3 * blockIdx.x], off = a.chunks[3 * blockIdx.x + 1],
             cnt = a.chunks[3 * blockIdx.x + 2];
@@ -63,33 +128,60 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i], qi = a.q[i];
   const int ti = a.type[i], mi = valid ? a.mol[i] : -1;
   double fx = 0.0, fy = 0.0, fz = 0.0, lj = 0.0, real = 0.0;
+  PairTile &t = tile[grp];
   for (int nc = grp; nc < 27; nc += kGroups) {
     const int nb = a.nbr[cell * 27 + nc];
-    const int je = a.cellStart[nb + 1];
-    for (int jj = a.cellStart[nb]; jj < je; ++jj) {
-      const int j = a.cellVec[jj];
-      if (!valid || a.mol[j] == mi) continue;
-      const double dx = MinImage(xi - a.x[j], a.g.L[0], a.g.half[0]);
-      const double dy = MinImage(yi - a.y[j], a.g.L[1], a.g.half[1]);
-      const double dz = MinImage(zi - a.z[j], a.g.L[2], a.g.half[2]);
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      if (!(a.g.rCutSq > d2)) continue;
-      double vReal = 0.0, vLJ = 0.0;
-      if (a.g.elect) {
-        const double qiqj = qi * a.q[j] * kQQFact;
-        if (qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) {
-          if (i < j) real += CoulombEnergy(a.g, d2, qiqj);
-          vReal = CoulombVirial(a.g, d2, qiqj);
+    const int j0 = a.cellStart[nb], je = a.cellStart[nb + 1];
+    for (int base = j0; base < je; base += kChunk) {
+      const int n = min(kChunk, je - base);
+      if (lane < n) {
+        const int j = a.cellVec[base + lane];
+        t.idx[lane] = j;
+        t.x[lane] = a.x[j];
+        t.y[lane] = a.y[j];
+        t.z[lane] = a.z[j];
+        t.q[lane] = a.q[j];
+        t.type[lane] = a.type[j];
+        t.mol[lane] = a.mol[j];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // the tile's in-cutoff partners of this lane as a bit mask first, so
+      // the wave runs the pair math about as many times as the busiest lane
+      // has partners rather than once per tile entry
+      uint64_t near = 0;
+      if (valid)
+        for (int k = 0; k < n; ++k) {
+          const double dx = MinImage(xi - t.x[k], a.g.L[0], a.g.half[0]);
+          const double dy = MinImage(yi - t.y[k], a.g.L[1], a.g.half[1]);
+          const double dz = MinImage(zi - t.z[k], a.g.L[2], a.g.half[2]);
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          if (t.mol[k] != mi && a.g.rCutSq > d2) near |= uint64_t(1) << k;
         }
+      while (near) {
+        const int k = __builtin_ctzll(near);
+        near &= near - 1;
+        const int j = t.idx[k];
+        const double dx = MinImage(xi - t.x[k], a.g.L[0], a.g.half[0]);
+        const double dy = MinImage(yi - t.y[k], a.g.L[1], a.g.half[1]);
+        const double dz = MinImage(zi - t.z[k], a.g.L[2], a.g.half[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const int idx = ti + t.type[k] * a.ntypes;
+        const PairTerms p = PairMath(a.g, poly, d2, qi * t.q[k] * kQQFact, a.sigSq[idx],
+                                     a.epsCn[idx], a.nHalf[idx], a.epsCn6[idx], a.nOver6[idx],
+                                     i < j);
+        real += p.eReal;
+        lj += p.eLJ;
+        const double vLJ = p.vLJ, vReal = p.vReal;
+        fx += (dx * vLJ + dx * vReal);
+        fy += (dy * vLJ + dy * vReal);
+        fz += (dz * vLJ + dz * vReal);
       }
-      if (!(a.g.rCutLJSq < d2)) {
-        const int idx = ti + a.type[j] * a.ntypes;
-        if (i < j) lj += LJEnergy(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
-        vLJ = LJVirial(d2, a.sigSq[idx], a.epsCn6[idx], a.nOver6[idx], a.nHalf[idx]);
-      }
-      fx += (dx * vLJ + dx * vReal);
-      fy += (dy * vLJ + dy * vReal);
-      fz += (dz * vLJ + dz * vReal);
+      // every lane is done with the tile before the next one overwrites it
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
   sf[grp][0][lane] = fx;
@@ -99,15 +191,15 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
   part.lj = BlockSum(lj, red);  // synchronises the LDS writes above
   part.real = BlockSum(real, red);
   if (grp == 0 && valid) {
-    double s[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      s[d] = sf[0][d][lane];
-      for (int g = 1; g < kGroups; ++g) s[d] += sf[g][d][lane];
+    double sx = sf[0][0][lane], sy = sf[0][1][lane], sz = sf[0][2][lane];
+    for (int g = 1; g < kGroups; ++g) {
+      sx += sf[g][0][lane];
+      sy += sf[g][1][lane];
+      sz += sf[g][2][lane];
     }
-    a.fx[i] = s[0];
-    a.fy[i] = s[1];
-    a.fz[i] = s[2];
+    a.fx[i] = sx;
+    a.fy[i] = sy;
+    a.fz[i] = sz;
   }
   FinishPartials(part, a.partials, a.counter, a.result);
 }
@@ -568,7 +660,7 @@ int MPPairForce(gomc_box *b, int set, double *lj, double *real) {
   a.counter = b->counter;
   a.result = b->devRes;
   if (n < 256)
-    hipLaunchKernelGGL(PairForceKernel<16>, dim3(n), dim3(16 * kChunk), 0, b->stream, a);
+    hipLaunchKernelGGL(PairForceKernel<8>, dim3(n), dim3(8 * kChunk), 0, b->stream, a);
   else
     hipLaunchKernelGGL(PairForceKernel<4>, dim3(n), dim3(4 * kChunk), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
