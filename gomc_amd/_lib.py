@@ -35,6 +35,7 @@ SIGNATURES = {
     "gomc_get_coords": (C.c_int, [_vp, _dp, _dp, _dp]),
     "gomc_commit_mol": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp]),
     "gomc_build_cells": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "gomc_trial_cells": (C.c_int, [_vp, _ip, _ip, _ip]),
     "gomc_box_inter": (C.c_int, [_vp, _dp, _dp]),
     "gomc_mol_inter": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _ip]),
     "gomc_particle_inter": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,

@@ -170,6 +170,18 @@ class Box:
         self._check(self.lib.gomc_build_cells(self.h, ptr(cv), ptr(cs), ptr(ac)))
         return cv, cs, ac
 
+    def trial_cells(self):
+        """The device's trial cell table: (count[n_cells], slots[n_cells, cap])."""
+        info = np.zeros(7, dtype=np.int64)
+        self._check(self.lib.gomc_box_info(self.h, ptr(info)))
+        nc = int(info[3])  # a volume move may have changed the grid
+        cap = C.c_int32()
+        self._check(self.lib.gomc_trial_cells(self.h, C.byref(cap), None, None))
+        cnt = np.zeros(nc, dtype=np.int32)
+        slots = np.zeros(nc * cap.value, dtype=np.int32)
+        self._check(self.lib.gomc_trial_cells(self.h, C.byref(cap), ptr(cnt), ptr(slots)))
+        return cnt, slots.reshape(nc, cap.value)
+
     # ---- forces and the MultiParticle move ----
     def box_force(self):
         """(lj, real, atom forces (n_atoms, 3), molecule forces (n_mols, 3))."""

@@ -30,6 +30,12 @@ int HipFail(gomc_box *b, hipError_t e, const char *what) {
 namespace {
 
 // ---------------------------------------------------------------- trial --
+// Neighbour cells a trial walks when the box has more than 27 cells, and
+// edits of the trial cell table one launch carries.
+constexpr int kTrialCells = 96;
+constexpr int kTrialEdits = 64;
+constexpr uint32_t kNewBits = 16;  // mask bit kNewBits + p: new position p
+
 struct TrialArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
@@ -51,15 +57,60 @@ struct TrialArgs {
   int cstart, clen;
   double cx[kMaxMolAtoms], cy[kMaxMolAtoms], cz[kMaxMolAtoms];
   double *wx, *wy, *wz;
+  // cell walk (nlist > 0): the cells within one cell of any old or new
+  // position, each with the mask of the positions it neighbours
+  int nlist, tcCap;
+  int *tcCount, *tcSlots;
+  int lcell[kTrialCells];
+  uint32_t lmask[kTrialCells];
+  // trial cell table edits, applied by the last block
+  int nedits, tcSlotsN;
+  int edits[2 * kTrialEdits];
   Partial4 *partials;
   unsigned *counter;
   double *result;
 };
+static_assert(sizeof(TrialArgs) <= 4096, "trial arguments must fit the kernel argument buffer");
 
-// Blocks [0, nPairBlocks): one atom j per thread against the old and new
-// positions of the molecule (CalculateEnergy.cpp:593-679).  Blocks after
-// that: one k-vector per thread, the structure-factor delta and the new
-// per-k energy (Ewald.cpp:436-467).
+// Atom j at (xj, yj, zj) against the molecule's old positions p with mask
+// bit p set (their energies subtracted) and new positions with bit
+// kNewBits + p set (added), p ascending, old before new
+// (CalculateEnergy.cpp:593-679).
+template <int P>
+__device__ __forceinline__ void TrialAtom(const TrialArgs &a, int j, double xj, double yj,
+                                          double zj, uint32_t mask, double &lj, double &real,
+                                          double &ov) {
+  const double qj = a.q[j];
+  const int tj = a.type[j];
+  for (int p = 0; p < a.mlen; ++p) {
+    const int idx = a.mtype[p] + tj * a.ntypes;
+    const double qiqj = a.mq[p] * qj * kQQFact;
+    if (mask >> p & 1u) {
+      const double d2 = DistSq(a.g, a.ox[p], a.oy[p], a.oz[p], xj, yj, zj);
+      if (a.g.rCutSq > d2) {
+        if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += -Coul<P>(a.g, d2, qiqj);
+        if (!(a.g.rCutLJSq < d2)) lj += -LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+      }
+    }
+    if (mask >> (kNewBits + p) & 1u) {
+      const double d2 = DistSq(a.g, a.nx[p], a.ny[p], a.nz[p], xj, yj, zj);
+      if (a.g.rCutSq > d2) {
+        if (d2 < a.g.rCutLowSq) ov = 1.0;
+        if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += Coul<P>(a.g, d2, qiqj);
+        if (!(a.g.rCutLJSq < d2)) lj += LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+      }
+    }
+  }
+}
+
+// Blocks [0, nPairBlocks): the pair part.  Without a cell list one atom j
+// per thread against every old and new position; with one, one slot of a
+// listed cell per thread against the positions that cell neighbours
+// (CellList::EnumerateLocal, as MoleculeInter walks it), then one thread per
+// atom of the molecule accepted last (its cells are edited at the end of
+// this launch, so the walk skips it and takes it from the arguments).
+// Blocks after that: one k-vector per thread, the structure-factor delta and
+// the new per-k energy (Ewald.cpp:436-467).
 // P = 1: pair terms and phases in FP32, sums in FP64 (the mixed mode).
 template <int P>
 __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
@@ -73,37 +124,32 @@ __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
   }
   if (int(blockIdx.x) < a.nPairBlocks) {
     double lj = 0.0, real = 0.0, ov = 0.0;
-    const int j = blockIdx.x * kBlock + tid;
-    if (a.doPair && j < a.natoms && (j < a.mstart || j >= a.mstart + a.mlen) && a.mol[j] >= 0) {
-      double xj, yj, zj;
-      const int cj = j - a.cstart;
-      if (cj >= 0 && cj < a.clen) {
-        xj = a.cx[cj];
-        yj = a.cy[cj];
-        zj = a.cz[cj];
-      } else {
-        xj = a.x[j];
-        yj = a.y[j];
-        zj = a.z[j];
+    const int t = blockIdx.x * kBlock + tid;
+    const uint32_t all = ~0u;
+    if (a.doPair && a.nlist == 0) {
+      const int j = t;
+      if (j < a.natoms && (j < a.mstart || j >= a.mstart + a.mlen) && a.mol[j] >= 0) {
+        const int cj = j - a.cstart;
+        if (cj >= 0 && cj < a.clen)
+          TrialAtom<P>(a, j, a.cx[cj], a.cy[cj], a.cz[cj], all, lj, real, ov);
+        else
+          TrialAtom<P>(a, j, a.x[j], a.y[j], a.z[j], all, lj, real, ov);
       }
-      const double qj = a.q[j];
-      const int tj = a.type[j];
-      for (int p = 0; p < a.mlen; ++p) {
-        const int idx = a.mtype[p] + tj * a.ntypes;
-        const double qiqj = a.mq[p] * qj * kQQFact;
-        double d2 = DistSq(a.g, a.ox[p], a.oy[p], a.oz[p], xj, yj, zj);
-        if (a.g.rCutSq > d2) {
-          if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2))
-            real += -Coul<P>(a.g, d2, qiqj);
-          if (!(a.g.rCutLJSq < d2)) lj += -LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+    } else if (a.doPair) {
+      const int nslots = a.nlist * a.tcCap;
+      if (t < nslots) {
+        const int e = t / a.tcCap, s = t - e * a.tcCap;
+        const int c = a.lcell[e];
+        if (s < a.tcCount[c]) {
+          const int j = a.tcSlots[c * a.tcCap + s];
+          const bool self = j >= a.mstart && j < a.mstart + a.mlen;
+          const bool pend = j >= a.cstart && j < a.cstart + a.clen;
+          if (!self && !pend) TrialAtom<P>(a, j, a.x[j], a.y[j], a.z[j], a.lmask[e], lj, real, ov);
         }
-        d2 = DistSq(a.g, a.nx[p], a.ny[p], a.nz[p], xj, yj, zj);
-        if (a.g.rCutSq > d2) {
-          if (d2 < a.g.rCutLowSq) ov = 1.0;
-          if (a.g.elect && qiqj != 0.0 && !(a.g.rCutCoulSq < d2))
-            real += Coul<P>(a.g, d2, qiqj);
-          if (!(a.g.rCutLJSq < d2)) lj += LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
-        }
+      } else if (t - nslots < a.clen) {
+        const int cj = t - nslots, j = a.cstart + cj;
+        if ((j < a.mstart || j >= a.mstart + a.mlen) && a.mol[j] >= 0)
+          TrialAtom<P>(a, j, a.cx[cj], a.cy[cj], a.cz[cj], all, lj, real, ov);
       }
     }
     part.lj = BlockSum(lj, red);
@@ -150,25 +196,38 @@ __global__ void __launch_bounds__(kBlock) TrialKernel(const TrialArgs a) {
     }
     part.recip = BlockSum(e, red);
   }
-  FinishPartials(part, a.partials, a.counter, a.result);
+  // Every other block has published its partial, after its last read of
+  // the cell table: the edits can no longer be seen by this launch.
+  if (FinishPartials(part, a.partials, a.counter, a.result) && tid == 0)
+    for (int e = 0; e < a.nedits; ++e) {
+      const int at = a.edits[2 * e], v = a.edits[2 * e + 1];
+      if (at < a.tcSlotsN)
+        a.tcSlots[at] = v;
+      else
+        a.tcCount[at - a.tcSlotsN] = v;
+    }
+}
+
+// Trial cell table edits that no trial launch carries, in order.
+constexpr int kEditBatch = 256;
+struct EditArgs {
+  int *tcCount, *tcSlots;
+  int n, tcSlotsN;
+  int edits[2 * kEditBatch];
+};
+
+__global__ void CellEditKernel(const EditArgs a) {
+  if (threadIdx.x != 0) return;
+  for (int e = 0; e < a.n; ++e) {
+    const int at = a.edits[2 * e], v = a.edits[2 * e + 1];
+    if (at < a.tcSlotsN)
+      a.tcSlots[at] = v;
+    else
+      a.tcCount[at - a.tcSlotsN] = v;
+  }
 }
 
 // ------------------------------------------------------------ cell list --
-// CellList::PositionToCell (src/CellList.h:88-101).
-__device__ __forceinline__ int PositionToCell(double px, double py, double pz, const double *cs,
-                                              const int *edge) {
-  int ix = int(px / cs[0]), iy = int(py / cs[1]), iz = int(pz / cs[2]);
-  ix -= (ix == edge[0] ? 1 : 0);
-  iy -= (iy == edge[1] ? 1 : 0);
-  iz -= (iz == edge[2] ? 1 : 0);
-  // Coordinates are validated to [0, L) on upload; the clamp only keeps a
-  // bad value from indexing outside the cell arrays.
-  ix = min(max(ix, 0), edge[0] - 1);
-  iy = min(max(iy, 0), edge[1] - 1);
-  iz = min(max(iz, 0), edge[2] - 1);
-  return ix * edge[1] * edge[2] + iy * edge[2] + iz;
-}
-
 struct CellArgs {
   const double *x, *y, *z;
   const int *mol;  // < 0: reservoir atom, not gridded
@@ -185,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) CellCountKernel(const CellArgs a) {
     a.atomCell[i] = -1;
     return;
   }
-  int c = PositionToCell(a.x[i], a.y[i], a.z[i], a.cs, a.edge);
+  int c = CellOf(a.x[i], a.y[i], a.z[i], a.cs, a.edge);
   a.atomCell[i] = c;
   atomicAdd(&a.cellCount[c], 1);
 }
@@ -539,8 +598,11 @@ __global__ void __launch_bounds__(kBlock) CacheBuildKernel(const CacheArgs a) {
   }
 }
 
-// Trial positions of one atom against all atoms (CalculateEnergy.cpp:741-783):
-// one workgroup per trial.
+// Trial positions of one atom against the live atoms
+// (CalculateEnergy.cpp:741-783): one workgroup per trial.  With the trial
+// cell table (tcCap > 0) a workgroup walks the 27 cells around its
+// position's cell, as ParticleInter's EnumerateLocal does; without it, every
+// atom.
 struct ParticleArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
@@ -550,7 +612,28 @@ struct ParticleArgs {
   double charge;
   const double *tx, *ty, *tz;
   double *en, *real, *overlap;
+  int tcCap;
+  const int *tcCount, *tcSlots, *nbr;
+  double cs[3];
+  int edge[3];
 };
+
+template <int P>
+__device__ __forceinline__ void ParticleAtom(const ParticleArgs &a, int j, double px, double py,
+                                             double pz, double &lj, double &real, double &ov) {
+  const double d2 = DistSq(a.g, px, py, pz, a.x[j], a.y[j], a.z[j]);
+  if (a.g.rCutSq > d2) {
+    if (d2 < a.g.rCutLowSq) ov = 1.0;
+    if (!(a.g.rCutLJSq < d2)) {
+      const int idx = a.atype + a.type[j] * a.ntypes;
+      lj += LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
+    }
+    if (a.g.elect) {
+      const double qiqj = a.q[j] * a.charge * kQQFact;
+      if (qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += Coul<P>(a.g, d2, qiqj);
+    }
+  }
+}
 
 template <int P>
 __global__ void __launch_bounds__(kBlock) ParticleInterKernel(const ParticleArgs a) {
@@ -558,19 +641,20 @@ __global__ void __launch_bounds__(kBlock) ParticleInterKernel(const ParticleArgs
   const int t = blockIdx.x;
   const double px = a.tx[t], py = a.ty[t], pz = a.tz[t];
   double lj = 0.0, real = 0.0, ov = 0.0;
-  for (int j = threadIdx.x; j < a.natoms; j += kBlock) {
-    if (a.mol[j] == a.skipMol || a.mol[j] < 0) continue;
-    double d2 = DistSq(a.g, px, py, pz, a.x[j], a.y[j], a.z[j]);
-    if (a.g.rCutSq > d2) {
-      if (d2 < a.g.rCutLowSq) ov = 1.0;
-      if (!(a.g.rCutLJSq < d2)) {
-        int idx = a.atype + a.type[j] * a.ntypes;
-        lj += LJ<P>(d2, a.sigSq[idx], a.epsCn[idx], a.nHalf[idx]);
-      }
-      if (a.g.elect) {
-        double qiqj = a.q[j] * a.charge * kQQFact;
-        if (qiqj != 0.0 && !(a.g.rCutCoulSq < d2)) real += Coul<P>(a.g, d2, qiqj);
-      }
+  if (a.tcCap > 0) {
+    const int c0 = CellOf(px, py, pz, a.cs, a.edge);
+    for (int i = threadIdx.x; i < 27 * a.tcCap; i += kBlock) {
+      const int n = i / a.tcCap, s = i - n * a.tcCap;
+      const int c = a.nbr[c0 * 27 + n];
+      if (s >= a.tcCount[c]) continue;
+      const int j = a.tcSlots[c * a.tcCap + s];
+      if (a.mol[j] == a.skipMol) continue;
+      ParticleAtom<P>(a, j, px, py, pz, lj, real, ov);
+    }
+  } else {
+    for (int j = threadIdx.x; j < a.natoms; j += kBlock) {
+      if (a.mol[j] == a.skipMol || a.mol[j] < 0) continue;
+      ParticleAtom<P>(a, j, px, py, pz, lj, real, ov);
     }
   }
   lj = BlockSum(lj, red);
@@ -601,6 +685,126 @@ int Sync(gomc_box *b) {
   HIP_TRY(b, hipStreamSynchronize(b->stream));
   HIP_TRY(b, hipGetLastError());
   return GOMC_OK;
+}
+
+// ------------------------------------------------------ trial cell table --
+// A position the cell walk can take: inside the box, as the reference's
+// moves leave every trial position (WrapPBC).
+static bool InBox(const gomc_box *b, double x, double y, double z) {
+  return x >= 0.0 && x <= b->geom.L[0] && y >= 0.0 && y <= b->geom.L[1] && z >= 0.0 &&
+         z <= b->geom.L[2];
+}
+
+// The 27 cells within one cell of cell c (periodic; edges of at least 3
+// make them distinct).
+static void CellNeighbours(const gomc_box *b, int c, int out[27]) {
+  const int *E = b->edge;
+  const int ix = c / (E[1] * E[2]), iy = (c / E[2]) % E[1], iz = c % E[2];
+  int n = 0;
+  for (int dx = -1; dx <= 1; ++dx)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dz = -1; dz <= 1; ++dz)
+        out[n++] = ((ix + dx + E[0]) % E[0]) * E[1] * E[2] + ((iy + dy + E[1]) % E[1]) * E[2] +
+                   (iz + dz + E[2]) % E[2];
+}
+
+// Builds the table from the host coordinates of the live molecules and
+// uploads it: capacity the fullest cell plus a quarter (at least 8 more).
+int TrialCellsEnsure(gomc_box *b) {
+  if (b->tcValid) return GOMC_OK;
+  const int nc = b->ncell;
+  b->tcEdits.clear();
+  b->hTcCount.assign(nc, 0);
+  b->hTcAtomCell.assign(b->natoms, -1);
+  b->hTcAtomSlot.assign(b->natoms, -1);
+  for (int m = 0; m < b->nmols; ++m) {
+    if (!b->molBox.empty() && b->molBox[m] != 0) continue;
+    for (int a = b->molStart[m]; a < b->molStart[m] + b->molLen[m]; ++a) {
+      const int c = CellOf(b->hx[a], b->hy[a], b->hz[a], b->cellSize, b->edge);
+      b->hTcAtomCell[a] = c;
+      b->hTcAtomSlot[a] = b->hTcCount[c]++;
+    }
+  }
+  const int mx = nc > 0 ? *std::max_element(b->hTcCount.begin(), b->hTcCount.end()) : 0;
+  const int cap = mx + std::max(8, mx / 4);
+  if (size_t(nc) * size_t(cap) + size_t(nc) > size_t(INT32_MAX))
+    return Fail(b, GOMC_ERR_ARG, "trial cell table beyond 2^31 entries");
+  b->tcCap = cap;
+  b->hTcSlots.assign(size_t(nc) * cap, -1);
+  for (int a = 0; a < b->natoms; ++a)
+    if (b->hTcAtomCell[a] >= 0) b->hTcSlots[size_t(b->hTcAtomCell[a]) * cap + b->hTcAtomSlot[a]] = a;
+  if (size_t(nc) > b->tcCountAlloc) {
+    if (b->tcCount) (void)hipFree(b->tcCount);
+    b->tcCount = nullptr;
+    b->tcCountAlloc = 0;
+    HIP_TRY(b, hipMalloc(&b->tcCount, sizeof(int) * nc));
+    b->tcCountAlloc = nc;
+  }
+  if (b->hTcSlots.size() > b->tcSlotsAlloc) {
+    if (b->tcSlots) (void)hipFree(b->tcSlots);
+    b->tcSlots = nullptr;
+    b->tcSlotsAlloc = 0;
+    HIP_TRY(b, hipMalloc(&b->tcSlots, sizeof(int) * b->hTcSlots.size()));
+    b->tcSlotsAlloc = b->hTcSlots.size();
+  }
+  HIP_TRY(b, hipMemcpyAsync(b->tcCount, b->hTcCount.data(), sizeof(int) * nc,
+                            hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(b, hipMemcpyAsync(b->tcSlots, b->hTcSlots.data(), sizeof(int) * b->hTcSlots.size(),
+                            hipMemcpyHostToDevice, b->stream));
+  int rc = Sync(b);
+  if (rc) return rc;
+  b->tcValid = true;
+  return GOMC_OK;
+}
+
+// Applies the queued edits on the box's stream, ahead of whatever is
+// launched next; no wait.
+int TrialCellsFlushEdits(gomc_box *b) {
+  const int n = int(b->tcEdits.size() / 2);
+  for (int e0 = 0; e0 < n; e0 += kEditBatch) {
+    EditArgs a;
+    a.tcCount = b->tcCount;
+    a.tcSlots = b->tcSlots;
+    a.tcSlotsN = b->ncell * b->tcCap;
+    a.n = std::min(kEditBatch, n - e0);
+    std::copy(b->tcEdits.begin() + 2 * e0, b->tcEdits.begin() + 2 * (e0 + a.n), a.edits);
+    hipLaunchKernelGGL(CellEditKernel, dim3(1), dim3(kWave), 0, b->stream, a);
+    HIP_TRY(b, hipGetLastError());
+  }
+  b->tcEdits.clear();
+  return GOMC_OK;
+}
+
+// CellList::RemoveMol / AddMol (src/CellList.h:28-29) for an accepted
+// displacement of molecule m:
+// an atom that changes cell leaves its slot to its old cell's last atom and
+// takes the next slot of its new cell.  A full cell makes the table invalid
+// (rebuilt, larger, before its next use).
+static void TrialCellsCommit(gomc_box *b, int m, const double *x, const double *y,
+                             const double *z) {
+  if (!b->tcValid || !TrialCellsUseful(b)) return;
+  const int cap = b->tcCap, slotsN = b->ncell * cap;
+  std::vector<int> &E = b->tcEdits;
+  for (int p = 0; p < b->molLen[m]; ++p) {
+    const int a = b->molStart[m] + p;
+    const int c0 = b->hTcAtomCell[a], c1 = CellOf(x[p], y[p], z[p], b->cellSize, b->edge);
+    if (c0 == c1) continue;
+    if (c0 < 0 || b->hTcCount[c1] >= cap) {
+      TrialCellsInvalidate(b);
+      return;
+    }
+    const int slot = b->hTcAtomSlot[a], last = --b->hTcCount[c0];
+    const int moved = b->hTcSlots[size_t(c0) * cap + last];
+    b->hTcSlots[size_t(c0) * cap + slot] = moved;
+    b->hTcAtomSlot[moved] = slot;
+    b->hTcSlots[size_t(c0) * cap + last] = -1;
+    E.insert(E.end(), {c0 * cap + slot, moved, slotsN + c0, last});
+    const int ns = b->hTcCount[c1]++;
+    b->hTcSlots[size_t(c1) * cap + ns] = a;
+    b->hTcAtomSlot[a] = ns;
+    b->hTcAtomCell[a] = c1;
+    E.insert(E.end(), {c1 * cap + ns, a, slotsN + c1, ns + 1});
+  }
 }
 
 int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const double *oz,
@@ -639,7 +843,6 @@ int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const do
   }
   a.doPair = doPair;
   a.doRecip = doRecip && b->ewald;
-  a.nPairBlocks = doPair ? (b->natoms + kBlock - 1) / kBlock : 0;
   const int nRecip = a.doRecip ? (b->K + kBlock - 1) / kBlock : 0;
   a.K = b->K;
   a.kx = b->kx;
@@ -672,6 +875,60 @@ int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const do
   a.wx = b->x;
   a.wy = b->y;
   a.wz = b->z;
+  // the cells within one cell of every old and new position, when walking
+  // them is less work than a scan of every atom
+  a.nlist = 0;
+  if (doPair && TrialCellsUseful(b)) {
+    int rc = TrialCellsEnsure(b);
+    if (rc) return rc;
+    if (b->tcMark.size() != size_t(b->ncell)) b->tcMark.assign(b->ncell, -1);
+    bool fits = true;
+    for (int side = 0; side < 2 && fits; ++side)
+      for (int p = 0; p < len && fits; ++p) {
+        const double *px = side ? nx : ox, *py = side ? ny : oy, *pz = side ? nz : oz;
+        if (!InBox(b, px[p], py[p], pz[p])) {  // only the scan of every atom takes any image
+          fits = false;
+          break;
+        }
+        int nb[27];
+        CellNeighbours(b, CellOf(px[p], py[p], pz[p], b->cellSize, b->edge), nb);
+        const uint32_t bit = 1u << (side ? kNewBits + p : p);
+        for (int n = 0; n < 27; ++n) {
+          int &mk = b->tcMark[nb[n]];
+          if (mk < 0) {
+            if (a.nlist == kTrialCells) {
+              fits = false;
+              break;
+            }
+            mk = a.nlist;
+            a.lcell[a.nlist] = nb[n];
+            a.lmask[a.nlist++] = 0;
+          }
+          a.lmask[mk] |= bit;
+        }
+      }
+    for (int e = 0; e < a.nlist; ++e) b->tcMark[a.lcell[e]] = -1;
+    if (!fits || size_t(a.nlist) * b->tcCap + a.clen >= size_t(b->natoms)) a.nlist = 0;
+  }
+  a.tcCap = b->tcCap;
+  a.tcCount = b->tcCount;
+  a.tcSlots = b->tcSlots;
+  a.tcSlotsN = b->ncell * b->tcCap;
+  a.nedits = 0;
+  if (!b->tcEdits.empty()) {
+    if (b->tcEdits.size() <= size_t(2 * kTrialEdits)) {
+      a.nedits = int(b->tcEdits.size() / 2);
+      std::copy(b->tcEdits.begin(), b->tcEdits.end(), a.edits);
+    } else {
+      int rc = TrialCellsFlushEdits(b);
+      if (rc) return rc;
+    }
+  }
+  if (doPair)
+    a.nPairBlocks = a.nlist > 0 ? int((size_t(a.nlist) * b->tcCap + a.clen + kBlock - 1) / kBlock)
+                                : (b->natoms + kBlock - 1) / kBlock;
+  else
+    a.nPairBlocks = 0;
   int nblocks = a.nPairBlocks + nRecip;
   if (nblocks == 0) nblocks = 1;
   int rc = EnsurePartials(b, nblocks);
@@ -685,8 +942,12 @@ int LaunchTrial(gomc_box *b, int m, const double *ox, const double *oy, const do
     hipLaunchKernelGGL(TrialKernel<0>, dim3(nblocks), dim3(kBlock), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
   rc = Sync(b);
-  if (rc) return rc;
+  if (rc) {
+    TrialCellsInvalidate(b);
+    return rc;
+  }
   b->pendingMol = -1;
+  if (a.nedits) b->tcEdits.clear();
   for (int i = 0; i < 3; ++i) out[i] = b->hostRes[i];
   out[3] = b->hostRes[3] > 0.0 ? 1.0 : 0.0;  // any block saw a pair inside RcutLow
   return GOMC_OK;
@@ -972,6 +1233,8 @@ static void FreeAll(gomc_box *b) {
   if (b->virBuf) (void)hipFree(b->virBuf);
   if (b->sumPart) (void)hipFree(b->sumPart);
   b->sumPart = nullptr;
+  for (void *p : {(void *)b->tcCount, (void *)b->tcSlots})
+    if (p) (void)hipFree(p);
   b->sumPartCap = 0;
   for (void *p : {(void *)b->cacheC, (void *)b->cacheS, (void *)b->stageC, (void *)b->stageS})
     if (p) (void)hipFree(p);
@@ -1017,6 +1280,7 @@ int gomc_set_coords(gomc_box *b, const double *x, const double *y, const double 
   HIP_TRY(b, hipMemcpyAsync(b->y, y, bytes, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->z, z, bytes, hipMemcpyHostToDevice, b->stream));
   b->pendingMol = -1;
+  TrialCellsInvalidate(b);
   int rc = Sync(b);
   if (!rc && b->cached) rc = CacheBuild(b);
   return rc;
@@ -1041,9 +1305,11 @@ int gomc_commit_mol(gomc_box *b, int m, const double *x, const double *y, const 
     HIP_TRY(b, hipMemcpyAsync(b->x + ps, b->pend[0], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
     HIP_TRY(b, hipMemcpyAsync(b->y + ps, b->pend[1], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
     HIP_TRY(b, hipMemcpyAsync(b->z + ps, b->pend[2], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
-    int rc = Sync(b);
+    int rc = TrialCellsFlushEdits(b);
+    if (!rc) rc = Sync(b);
     if (rc) return rc;
   }
+  TrialCellsCommit(b, m, x, y, z);
   for (int p = 0; p < len; ++p) {
     b->hx[s + p] = b->pend[0][p] = x[p];
     b->hy[s + p] = b->pend[1][p] = y[p];
@@ -1064,15 +1330,18 @@ int FlushPending(gomc_box *b) {
   HIP_TRY(b, hipMemcpyAsync(b->x + ps, b->pend[0], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->y + ps, b->pend[1], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->z + ps, b->pend[2], sizeof(double) * pl, hipMemcpyHostToDevice, b->stream));
-  int rc = Sync(b);
+  int rc = TrialCellsFlushEdits(b);
+  if (!rc) rc = Sync(b);
   if (rc) return rc;
   b->pendingMol = -1;
   return GOMC_OK;
 }
 
-// Energies of `n` trial positions of one atom against every live atom
-// (CalculateEnergy::ParticleInter, src/CalculateEnergy.cpp:727-785); the
-// trial staging buffer is the box's and grows only when `n` does.
+// Energies of `n` trial positions of one atom against the live atoms
+// (CalculateEnergy::ParticleInter, src/CalculateEnergy.cpp:727-785), through
+// the trial cell table when 27 cells of it hold fewer slots than the box
+// has atoms; the trial staging buffer is the box's and grows only when `n`
+// does.
 int ParticleInterImpl(gomc_box *b, int atomType, double charge, int skipMol, int n,
                       const double *tx, const double *ty, const double *tz, double *en,
                       double *real, int32_t *overlap) {
@@ -1110,6 +1379,24 @@ int ParticleInterImpl(gomc_box *b, int atomType, double charge, int skipMol, int
   a.en = buf + 3 * n;
   a.real = buf + 4 * n;
   a.overlap = buf + 5 * n;
+  a.tcCap = 0;
+  if (TrialCellsUseful(b)) {
+    int rc = TrialCellsEnsure(b);
+    if (!rc) rc = TrialCellsFlushEdits(b);
+    if (rc) return rc;
+    bool inBox = true;
+    for (int t = 0; t < n; ++t) inBox = inBox && InBox(b, tx[t], ty[t], tz[t]);
+    if (inBox && size_t(27) * b->tcCap < size_t(b->natoms)) {
+      a.tcCap = b->tcCap;
+      a.tcCount = b->tcCount;
+      a.tcSlots = b->tcSlots;
+      a.nbr = b->nbr;
+      for (int i = 0; i < 3; ++i) {
+        a.cs[i] = b->cellSize[i];
+        a.edge[i] = b->edge[i];
+      }
+    }
+  }
   if (b->precision == 1)
     hipLaunchKernelGGL(ParticleInterKernel<1>, dim3(n), dim3(kBlock), 0, b->stream, a);
   else
@@ -1204,6 +1491,7 @@ int CacheBuild(gomc_box *b) {
 
 // Moves molecule m's atoms in or out of the interacting set on the device.
 int SetMolLive(gomc_box *b, int m, bool live) {
+  TrialCellsInvalidate(b);
   const int s = b->molStart[m], len = b->molLen[m];
   std::vector<int> ids(len, live ? m : -1);
   HIP_TRY(b, hipMemcpyAsync(b->mol + s, ids.data(), sizeof(int) * len, hipMemcpyHostToDevice,
@@ -1242,6 +1530,24 @@ int gomc_build_cells(gomc_box *b, int32_t *cell_vector, int32_t *cell_start, int
     HIP_TRY(b, hipMemcpyAsync(cell_start, b->cellStart, sizeof(int) * (b->ncell + 1), hipMemcpyDeviceToHost, b->stream));
   if (atom_cell)
     HIP_TRY(b, hipMemcpyAsync(atom_cell, b->atomCell, sizeof(int) * b->natoms, hipMemcpyDeviceToHost, b->stream));
+  return Sync(b);
+}
+
+int gomc_trial_cells(gomc_box *b, int32_t *cap, int32_t *count, int32_t *slots) {
+  if (!b || !cap) return GOMC_ERR_ARG;
+  if (!TrialCellsUseful(b))
+    return Fail(b, GOMC_ERR_STATE, "a box of 27 cells has no trial cell table");
+  (void)hipSetDevice(b->device);
+  int rc = FlushPending(b);  // applies the queued edits
+  if (!rc) rc = TrialCellsEnsure(b);
+  if (rc) return rc;
+  *cap = b->tcCap;
+  if (count)
+    HIP_TRY(b, hipMemcpyAsync(count, b->tcCount, sizeof(int) * b->ncell, hipMemcpyDeviceToHost,
+                              b->stream));
+  if (slots)
+    HIP_TRY(b, hipMemcpyAsync(slots, b->tcSlots, sizeof(int) * size_t(b->ncell) * b->tcCap,
+                              hipMemcpyDeviceToHost, b->stream));
   return Sync(b);
 }
 

@@ -46,6 +46,23 @@ __host__ __device__ inline int SumsChunks(int nseg, int natoms, int K) {
   return c > 1 ? c : 1;
 }
 
+// CellList::PositionToCell (src/CellList.h:88-101), on the host and the
+// device alike: both sides of the trial cell table must put a position in
+// the same cell.
+__host__ __device__ inline int CellOf(double px, double py, double pz, const double *cs,
+                                      const int *edge) {
+  int ix = int(px / cs[0]), iy = int(py / cs[1]), iz = int(pz / cs[2]);
+  ix -= (ix == edge[0] ? 1 : 0);
+  iy -= (iy == edge[1] ? 1 : 0);
+  iz -= (iz == edge[2] ? 1 : 0);
+  // Coordinates are validated to [0, L) on upload; the clamp only keeps a
+  // bad value from indexing outside the cell arrays.
+  ix = ix < 0 ? 0 : (ix > edge[0] - 1 ? edge[0] - 1 : ix);
+  iy = iy < 0 ? 0 : (iy > edge[1] - 1 ? edge[1] - 1 : iy);
+  iz = iz < 0 ? 0 : (iz > edge[2] - 1 ? edge[2] - 1 : iz);
+  return ix * edge[1] * edge[2] + iy * edge[2] + iz;
+}
+
 // Per-block partial of a fused trial: LJ, real, reciprocal, overlap.
 struct Partial4 {
   double lj, real, recip, overlap;
@@ -54,7 +71,8 @@ struct Partial4 {
 // Completion of a multi-block reduction: each block publishes its partial,
 // the last block to arrive sums all partials in block order and writes the
 // four sums to the host-mapped result, then re-arms the counter.
-__device__ inline void FinishPartials(const Partial4 &mine, Partial4 *partials,
+// Returns true in every thread of that last block.
+__device__ inline bool FinishPartials(const Partial4 &mine, Partial4 *partials,
                                       unsigned *counter, double *result) {
   __shared__ bool last;
   if (threadIdx.x == 0) {
@@ -81,6 +99,7 @@ __device__ inline void FinishPartials(const Partial4 &mine, Partial4 *partials,
     *counter = 0u;
     __threadfence_system();
   }
+  return last;
 }
 
 // MultiParticle state (src/moves/MultiParticle.h:37-58 and the
@@ -176,6 +195,19 @@ struct gomc_box {
   double pend[3][gomc::kMaxMolAtoms];
 
   std::vector<int> hCellCount;  // atoms per cell at the last build
+  // Trial cell table (plain trial path, boxes of more than 27 cells): per
+  // cell up to tcCap atom slots of the live atoms, in no particular order.
+  // The host copy (hTc*) is authoritative; the device copy trails it by the
+  // edits in tcEdits -- (address, value) pairs, an address below
+  // ncell * tcCap being a slot and above it a count -- which the next
+  // trial launch applies after its last reader is done, or a small edit
+  // kernel applies before anything else reads the table.
+  bool tcValid = false;
+  int tcCap = 0;
+  int *tcCount = nullptr, *tcSlots = nullptr;  // device [ncell], [ncell * tcCap]
+  size_t tcCountAlloc = 0, tcSlotsAlloc = 0;
+  std::vector<int> hTcCount, hTcSlots, hTcAtomCell, hTcAtomSlot, tcMark;
+  std::vector<int> tcEdits;
   // EwaldCached mode (src/EwaldCached.cpp): per-molecule structure factors
   // cacheC/S [nmols][K] of the current positions, and the trial molecule's
   // new terms staged until the move is accepted.
@@ -243,6 +275,16 @@ int EnsurePartials(gomc_box *b, int nblocks);
 int Sync(gomc_box *b);
 int BuildCells(gomc_box *b);
 int FlushPending(gomc_box *b);
+// Trial cell table (box.hip).  Valid only after TrialCellsEnsure; any
+// change of the coordinates or of which molecules live in the box other
+// than gomc_commit_mol calls TrialCellsInvalidate.
+inline bool TrialCellsUseful(const gomc_box *b) { return b->ncell > 27; }
+inline void TrialCellsInvalidate(gomc_box *b) {
+  b->tcValid = false;
+  b->tcEdits.clear();
+}
+int TrialCellsEnsure(gomc_box *b);
+int TrialCellsFlushEdits(gomc_box *b);
 // MultiParticle pieces (mp.hip).
 int MPEnsure(gomc_box *b);
 void MPFree(gomc_box *b);

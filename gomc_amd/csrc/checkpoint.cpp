@@ -338,6 +338,7 @@ int gomc_box_restore(gomc_box *b, const void *buf, int64_t size, int64_t *step) 
   DevFree(b);
   b->pendingMol = -1;
   b->stageMol = -1;
+  TrialCellsInvalidate(b);
   int rc = GOMC_OK;
   bool newGeom = vol != b->volume || volInv != b->volInv;
   for (int i = 0; i < 3; ++i) newGeom = newGeom || L[i] != b->geom.L[i] || H[i] != b->geom.half[i];

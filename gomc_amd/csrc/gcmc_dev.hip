@@ -1831,6 +1831,7 @@ int DevDownload(gomc_box *b) {
   HIP_TRY(b, hipMemcpyAsync(com.data(), d->com, sizeof(double) * com.size(), hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(b, hipMemcpyAsync(molBox.data(), d->molBox, sizeof(int) * b->nmols, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->lookup.data(), d->lookup, sizeof(uint32_t) * b->nmols, hipMemcpyDeviceToHost, b->stream));
+  TrialCellsInvalidate(b);  // the loop moved molecules and changed which ones live
   HIP_TRY(b, hipMemcpyAsync(b->hx.data(), b->x, bytes, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->hy.data(), b->y, bytes, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(b, hipMemcpyAsync(b->hz.data(), b->z, bytes, hipMemcpyDeviceToHost, b->stream));

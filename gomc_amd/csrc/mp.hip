@@ -850,6 +850,7 @@ static int MPAccept(gomc_box *b, bool result, bool evaluated, const double e[3])
     b->pot.recip = e[2];
     b->pot.Total();
     mp.ref = 1 - mp.ref;
+    TrialCellsInvalidate(b);  // every molecule moved
     if (b->ewald) gomc_recip_accept(b);
     const size_t na = sizeof(double) * b->natoms, nm = sizeof(double) * b->nmols;
     HIP_TRY(b, hipMemcpyAsync(b->hx.data(), b->x, na, hipMemcpyDeviceToHost, b->stream));

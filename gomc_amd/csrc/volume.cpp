@@ -58,6 +58,7 @@ void FreeGridArrays(const Grid &g) {
 // New grid arrays for the box's current geometry (the old ones stay with
 // the caller).
 int NewGrid(gomc_box *b) {
+  TrialCellsInvalidate(b);
   const double rCutBox = std::max(b->rCutLJ, b->rCutCoul);
   for (int i = 0; i < 3; ++i) {
     b->edge[i] = std::max(int(std::floor(b->geom.L[i] / rCutBox)), 3);
@@ -155,6 +156,7 @@ int VolumeMove(gomc_box *b, int &accepted, double dE[3]) {
   if (b->cached) return Fail(b, GOMC_ERR_STATE, "volume moves are not supported with the molecule cache");
   int rc = FlushPending(b);
   if (rc) return rc;
+  TrialCellsInvalidate(b);  // coordinates scale, and the grid may change
   const double delta = Sym(b->mt, b->moves.volScale);
   const double oldVol = b->volume, newVol = b->volume + delta;
   const double rCutBox = std::max(b->rCutLJ, b->rCutCoul);

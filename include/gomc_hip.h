@@ -104,6 +104,16 @@ int gomc_commit_mol(gomc_box *b, int mol, const double *x, const double *y,
  * cell_vector[n_atoms], cell_start[n_cells+1], atom_cell[n_atoms]. */
 int gomc_build_cells(gomc_box *b, int32_t *cell_vector, int32_t *cell_start,
                      int32_t *atom_cell);
+/* The trial cell table that gomc_trial and gomc_particle_inter walk in a
+ * box of more than 27 cells (MoleculeInter and ParticleInter over
+ * CellList::EnumerateLocal, src/CalculateEnergy.cpp:595,745): per cell up
+ * to *cap slots of live atoms, kept up to date by the accepted
+ * displacements (CellList::RemoveMol/AddMol, src/CellList.h:28-29).  Writes
+ * *cap; count[n_cells] and slots[n_cells * cap] (each NULL or sized for the
+ * cap from a call with both NULL) receive the device's copy.  A read-back
+ * for checking; no counterpart in the reference.  GOMC_ERR_STATE in a box
+ * of 27 cells, which the trials scan whole. */
+int gomc_trial_cells(gomc_box *b, int32_t *cap, int32_t *count, int32_t *slots);
 
 /* ---- real-space pair energies ---------------------------------------- */
 /* CalculateEnergy::BoxInter (src/CalculateEnergy.cpp:157-266); replaces
