@@ -69,12 +69,17 @@ struct Partial4 {
 };
 
 // Completion of a multi-block reduction: each block publishes its partial,
-// the last block to arrive sums all partials in block order and writes the
-// four sums to the host-mapped result, then re-arms the counter.
-// Returns true in every thread of that last block.
+// the last block to arrive sums all partials -- thread t the partials t,
+// t + blockDim, ... in that order, then the threads' sums by BlockSum's
+// fixed tree, so the order depends on the grid and block sizes only -- and
+// writes the four sums to the host-mapped result, then re-arms the counter.
+// The partials are read by every thread at once: one round of loads rather
+// than one dependent load per block.  Returns true in every thread of that
+// last block.
 __device__ inline bool FinishPartials(const Partial4 &mine, Partial4 *partials,
                                       unsigned *counter, double *result) {
   __shared__ bool last;
+  __shared__ double red[1024 / kWave];
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = mine;
     __threadfence();
@@ -82,22 +87,28 @@ __device__ inline bool FinishPartials(const Partial4 &mine, Partial4 *partials,
     last = (prev == gridDim.x - 1);
   }
   __syncthreads();
-  if (last && threadIdx.x == 0) {
+  if (last) {
     __threadfence();
     double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     volatile Partial4 *vp = partials;
-    for (unsigned i = 0; i < gridDim.x; ++i) {
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) {
       s0 += vp[i].lj;
       s1 += vp[i].real;
       s2 += vp[i].recip;
       s3 += vp[i].overlap;
     }
-    result[0] = s0;
-    result[1] = s1;
-    result[2] = s2;
-    result[3] = s3;
-    *counter = 0u;
-    __threadfence_system();
+    s0 = BlockSum(s0, red);
+    s1 = BlockSum(s1, red);
+    s2 = BlockSum(s2, red);
+    s3 = BlockSum(s3, red);
+    if (threadIdx.x == 0) {
+      result[0] = s0;
+      result[1] = s1;
+      result[2] = s2;
+      result[3] = s3;
+      *counter = 0u;
+      __threadfence_system();
+    }
   }
   return last;
 }
@@ -123,6 +134,8 @@ struct MPState {
   unsigned *wcounter = nullptr;
   double *fpart = nullptr;  // per-k-chunk reciprocal forces [chunk][xyz][natoms]
   size_t fpartCap = 0;
+  double *spart = nullptr;  // per-slice pair forces [slice][xyz][natoms]
+  size_t spartCap = 0;
 };
 
 // MoveSettings state of the driver (src/MoveSettings.cpp:19-258).

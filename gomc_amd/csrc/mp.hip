@@ -31,20 +31,25 @@ __device__ __forceinline__ double Uneg11(uint64_t v) {
 }
 
 // ------------------------------------------------------------ pair force --
-// CalculateEnergy::BoxForce (CalculateEnergy.cpp:268-406).  One workgroup
-// per (cell, up to 64 atoms of it); its G waves take the 27 neighbour cells
-// round robin and their per-atom partial forces are summed in wave order
-// through LDS.  A wave walks a neighbour cell 64 atoms at a time: each lane
-// loads one atom (position, charge, type, molecule) into the wave's LDS
-// tile, then every lane runs over the tile, so the neighbour data arrive
-// as one coalesced load per tile instead of a chain of dependent loads per
-// pair.  A lane first marks its in-cutoff partners of the tile, then
-// evaluates only those, in ascending tile order: the pairs of a lane are
-// summed in the same order as by a plain walk over the cell.
-// Energies count each pair once (i < j).  A box with few workgroups uses
-// G = 8 (at most four neighbour cells per wave; 512 threads leave the pair
-// math 256 VGPRs, where 1024 would spill), a larger one G = 4 (more
-// workgroups resident per CU).
+// CalculateEnergy::BoxForce (CalculateEnergy.cpp:268-406).  A chunk is a
+// cell's atoms 64 at a time (one per lane); its 27 neighbour cells, taken
+// 64 atoms at a time, make a list of tiles.  `slices` workgroups of G waves
+// share a chunk and its waves take the tiles round robin: wave w of slice s
+// takes tiles s G + w, s G + w + slices G, ...  A wave stages each tile
+// (position, charge, type, molecule of 64 neighbour atoms) in its LDS, one
+// coalesced load per tile instead of a chain of dependent loads per pair,
+// marks each lane's in-cutoff partners of the tile in a bit mask, then
+// evaluates only those, in ascending tile order.  A workgroup adds its
+// waves' per-atom forces in wave order through LDS; with one slice it
+// writes the chunk's forces, with several each slice writes its own and
+// PairForceSumKernel adds them in slice order.  Energies count each pair
+// once (i < j).  The host picks G and the slice count so that a box of any
+// size puts a few thousand waves on the device.
+constexpr int kPairChunksFor4 = 1024;  // from this many chunks: 4 waves per workgroup
+constexpr int kPairChunksFor8 = 256;   // from this many: 8; below: 16 and slices
+constexpr int kPairSmallWaves = 2048;  // waves a small box is split into, about
+constexpr int kPairMaxSlices = 8;
+
 struct PairForceArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
@@ -53,7 +58,9 @@ struct PairForceArgs {
   int ntypes;
   const int *cellStart, *cellVec, *nbr;
   const int *chunks;  // [3 * nchunks]: cell, first atom within the cell, count
+  int slices, natoms;
   double *fx, *fy, *fz;
+  double *spart;  // slices > 1: [slice][xyz][natoms]
   Partial4 *partials;
   unsigned *counter;
   double *result;
@@ -89,8 +96,8 @@ __device__ __forceinline__ PairTerms PairMath(const BoxGeom &g, const double *po
         gauss = exp(-(x * x));
         ec = gauss * px;
       } else {
-        ec = erfc(x);
         gauss = exp(-(x * x));
+        ec = gauss * ErfcxTail(x);
       }
       if (count) p.eReal = qiqj * ec / dist;
       p.vReal = qiqj * (ec / dist + g.alpha * M_2_SQRTPI * gauss) / d2;
@@ -100,8 +107,19 @@ __device__ __forceinline__ PairTerms PairMath(const BoxGeom &g, const double *po
     }
   }
   if (!(g.rCutLJSq < d2)) {
-    if (count) p.eLJ = LJEnergy(d2, sig, eps, nh);
-    p.vLJ = LJVirial(d2, sig, eps6, nov6, nh);
+    // FFParticle::CalcEn and CalcVir in their operation orders, the
+    // repulsion by MiePow rather than libm's pow
+    if (count) {
+      const double rRat2 = sig / d2;
+      const double attract = rRat2 * rRat2 * rRat2;
+      const double repulse = nh == 6.0 ? attract * attract : MiePow(rRat2, nh);
+      p.eLJ = eps * (repulse - attract);
+    }
+    const double rNeg2 = 1.0 / d2;
+    const double rRat2 = rNeg2 * sig;
+    const double attract = rRat2 * rRat2 * rRat2;
+    const double repulse = nh == 6.0 ? attract * attract : MiePow(rRat2, nh);
+    p.vLJ = eps6 * (nov6 * repulse - attract) * rNeg2;
   }
   return p;
 }
@@ -118,71 +136,87 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
   __shared__ double red[kGroups * kChunk / kWave];
   __shared__ PairTile tile[kGroups];
   __shared__ double poly[kErfcxDeg + 1];
+  __shared__ int nbS[27], nbE[27], tileS[28];
+  const int chunkId = blockIdx.x / a.slices, slice = blockIdx.x - chunkId * a.slices;
+  const int cell = a.chunks[3 * chunkId], off = a.chunks[3 * chunkId + 1],
+            cnt = a.chunks[3 * chunkId + 2];
   if (threadIdx.x <= kErfcxDeg) poly[threadIdx.x] = kErfcxPoly[threadIdx.x];
+  if (threadIdx.x < 27) {
+    const int nb = a.nbr[cell * 27 + threadIdx.x];
+    nbS[threadIdx.x] = a.cellStart[nb];
+    nbE[threadIdx.x] = a.cellStart[nb + 1];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int c = 0; c < 27; ++c) {
+      tileS[c] = t;
+      t += (nbE[c] - nbS[c] + kChunk - 1) / kChunk;
+    }
+    tileS[27] = t;
+  }
   __syncthreads();
   const int lane = threadIdx.x % kChunk, grp = threadIdx.x / kChunk;
-  const int cell = a.chunks[3 * blockIdx.x], off = a.chunks[3 * blockIdx.x + 1],
-            cnt = a.chunks[3 * blockIdx.x + 2];
   const bool valid = lane < cnt;
   const int i = valid ? a.cellVec[a.cellStart[cell] + off + lane] : 0;
   const double xi = a.x[i], yi = a.y[i], zi = a.z[i], qi = a.q[i];
   const int ti = a.type[i], mi = valid ? a.mol[i] : -1;
   double fx = 0.0, fy = 0.0, fz = 0.0, lj = 0.0, real = 0.0;
   PairTile &t = tile[grp];
-  for (int nc = grp; nc < 27; nc += kGroups) {
-    const int nb = a.nbr[cell * 27 + nc];
-    const int j0 = a.cellStart[nb], je = a.cellStart[nb + 1];
-    for (int base = j0; base < je; base += kChunk) {
-      const int n = min(kChunk, je - base);
-      if (lane < n) {
-        const int j = a.cellVec[base + lane];
-        t.idx[lane] = j;
-        t.x[lane] = a.x[j];
-        t.y[lane] = a.y[j];
-        t.z[lane] = a.z[j];
-        t.q[lane] = a.q[j];
-        t.type[lane] = a.type[j];
-        t.mol[lane] = a.mol[j];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // the tile's in-cutoff partners of this lane as a bit mask first, so
-      // the wave runs the pair math about as many times as the busiest lane
-      // has partners rather than once per tile entry
-      uint64_t near = 0;
-      if (valid)
-        for (int k = 0; k < n; ++k) {
-          const double dx = MinImage(xi - t.x[k], a.g.L[0], a.g.half[0]);
-          const double dy = MinImage(yi - t.y[k], a.g.L[1], a.g.half[1]);
-          const double dz = MinImage(zi - t.z[k], a.g.L[2], a.g.half[2]);
-          const double d2 = dx * dx + dy * dy + dz * dz;
-          if (t.mol[k] != mi && a.g.rCutSq > d2) near |= uint64_t(1) << k;
-        }
-      while (near) {
-        const int k = __builtin_ctzll(near);
-        near &= near - 1;
-        const int j = t.idx[k];
+  const int ntiles = tileS[27], stride = a.slices * kGroups;
+  int c = 0;
+  for (int tt = slice * kGroups + grp; tt < ntiles; tt += stride) {
+    while (tileS[c + 1] <= tt) ++c;
+    const int base = nbS[c] + (tt - tileS[c]) * kChunk;
+    const int n = min(kChunk, nbE[c] - base);
+    if (lane < n) {
+      const int j = a.cellVec[base + lane];
+      t.idx[lane] = j;
+      t.x[lane] = a.x[j];
+      t.y[lane] = a.y[j];
+      t.z[lane] = a.z[j];
+      t.q[lane] = a.q[j];
+      t.type[lane] = a.type[j];
+      t.mol[lane] = a.mol[j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // the tile's in-cutoff partners of this lane as a bit mask first, so
+    // the wave runs the pair math about as many times as the busiest lane
+    // has partners rather than once per tile entry
+    uint64_t near = 0;
+    if (valid)
+      for (int k = 0; k < n; ++k) {
         const double dx = MinImage(xi - t.x[k], a.g.L[0], a.g.half[0]);
         const double dy = MinImage(yi - t.y[k], a.g.L[1], a.g.half[1]);
         const double dz = MinImage(zi - t.z[k], a.g.L[2], a.g.half[2]);
         const double d2 = dx * dx + dy * dy + dz * dz;
-        const int idx = ti + t.type[k] * a.ntypes;
-        const PairTerms p = PairMath(a.g, poly, d2, qi * t.q[k] * kQQFact, a.sigSq[idx],
-                                     a.epsCn[idx], a.nHalf[idx], a.epsCn6[idx], a.nOver6[idx],
-                                     i < j);
-        real += p.eReal;
-        lj += p.eLJ;
-        const double vLJ = p.vLJ, vReal = p.vReal;
-        fx += (dx * vLJ + dx * vReal);
-        fy += (dy * vLJ + dy * vReal);
-        fz += (dz * vLJ + dz * vReal);
+        if (t.mol[k] != mi && a.g.rCutSq > d2) near |= uint64_t(1) << k;
       }
-      // every lane is done with the tile before the next one overwrites it
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    while (near) {
+      const int k = __builtin_ctzll(near);
+      near &= near - 1;
+      const int j = t.idx[k];
+      const double dx = MinImage(xi - t.x[k], a.g.L[0], a.g.half[0]);
+      const double dy = MinImage(yi - t.y[k], a.g.L[1], a.g.half[1]);
+      const double dz = MinImage(zi - t.z[k], a.g.L[2], a.g.half[2]);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const int idx = ti + t.type[k] * a.ntypes;
+      const PairTerms p = PairMath(a.g, poly, d2, qi * t.q[k] * kQQFact, a.sigSq[idx],
+                                   a.epsCn[idx], a.nHalf[idx], a.epsCn6[idx], a.nOver6[idx],
+                                   i < j);
+      real += p.eReal;
+      lj += p.eLJ;
+      const double vLJ = p.vLJ, vReal = p.vReal;
+      fx += (dx * vLJ + dx * vReal);
+      fy += (dy * vLJ + dy * vReal);
+      fz += (dz * vLJ + dz * vReal);
     }
+    // every lane is done with the tile before the next one overwrites it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   sf[grp][0][lane] = fx;
   sf[grp][1][lane] = fy;
@@ -197,11 +231,37 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
       sy += sf[g][1][lane];
       sz += sf[g][2][lane];
     }
-    a.fx[i] = sx;
-    a.fy[i] = sy;
-    a.fz[i] = sz;
+    if (a.slices == 1) {
+      a.fx[i] = sx;
+      a.fy[i] = sy;
+      a.fz[i] = sz;
+    } else {
+      double *sp = a.spart + size_t(slice) * 3 * a.natoms;
+      sp[i] = sx;
+      sp[a.natoms + i] = sy;
+      sp[2 * size_t(a.natoms) + i] = sz;
+    }
   }
   FinishPartials(part, a.partials, a.counter, a.result);
+}
+
+// The slices' forces of every chunk atom, added in slice order.
+__global__ void __launch_bounds__(kChunk) PairForceSumKernel(const PairForceArgs a) {
+  const int cell = a.chunks[3 * blockIdx.x], off = a.chunks[3 * blockIdx.x + 1],
+            cnt = a.chunks[3 * blockIdx.x + 2];
+  if (int(threadIdx.x) >= cnt) return;
+  const int i = a.cellVec[a.cellStart[cell] + off + threadIdx.x];
+  const size_t n = a.natoms;
+  double sx = a.spart[i], sy = a.spart[n + i], sz = a.spart[2 * n + i];
+  for (int s = 1; s < a.slices; ++s) {
+    const double *sp = a.spart + size_t(s) * 3 * n;
+    sx += sp[i];
+    sy += sp[n + i];
+    sz += sp[2 * n + i];
+  }
+  a.fx[i] = sx;
+  a.fy[i] = sy;
+  a.fz[i] = sz;
 }
 
 // ------------------------------------------------------ reciprocal force --
@@ -546,7 +606,7 @@ namespace gomc {
 void MPFree(gomc_box *b) {
   MPState &mp = b->mp;
   std::vector<void *> ptrs = {mp.xn, mp.yn, mp.zn, mp.inRange, mp.fail, mp.chunks, mp.wpart,
-                              mp.wcounter, mp.fpart};
+                              mp.wcounter, mp.fpart, mp.spart};
   for (int s = 0; s < 2; ++s)
     for (int d = 0; d < 3; ++d) {
       ptrs.push_back(mp.com[s][d]);
@@ -633,7 +693,26 @@ int MPPairForce(gomc_box *b, int set, double *lj, double *real) {
   }
   HIP_TRY(b, hipMemcpyAsync(mp.chunks, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice,
                             b->stream));
-  rc = EnsurePartials(b, n);
+  // waves per workgroup and slices per chunk: a large box fills the device
+  // with chunks alone, a small one splits each chunk's tiles over slices
+  int G = 8, slices = 1;
+  if (n >= kPairChunksFor4) {
+    G = 4;
+  } else if (n < kPairChunksFor8) {
+    G = 16;
+    slices = std::min(kPairMaxSlices, std::max(1, (kPairSmallWaves + 16 * n - 1) / (16 * n)));
+  }
+  if (slices > 1) {
+    const size_t need = size_t(slices) * 3 * size_t(b->natoms);
+    if (need > mp.spartCap) {
+      if (mp.spart) (void)hipFree(mp.spart);
+      mp.spart = nullptr;
+      mp.spartCap = 0;
+      HIP_TRY(b, hipMalloc(&mp.spart, sizeof(double) * need));
+      mp.spartCap = need;
+    }
+  }
+  rc = EnsurePartials(b, n * slices);
   if (rc) return rc;
   PairForceArgs a;
   a.g = b->geom;
@@ -659,11 +738,21 @@ int MPPairForce(gomc_box *b, int set, double *lj, double *real) {
   a.partials = b->partials;
   a.counter = b->counter;
   a.result = b->devRes;
-  if (n < 256)
-    hipLaunchKernelGGL(PairForceKernel<8>, dim3(n), dim3(8 * kChunk), 0, b->stream, a);
+  a.slices = slices;
+  a.natoms = b->natoms;
+  a.spart = mp.spart;
+  const dim3 grid(n * slices);
+  if (G == 16)
+    hipLaunchKernelGGL(PairForceKernel<16>, grid, dim3(16 * kChunk), 0, b->stream, a);
+  else if (G == 8)
+    hipLaunchKernelGGL(PairForceKernel<8>, grid, dim3(8 * kChunk), 0, b->stream, a);
   else
-    hipLaunchKernelGGL(PairForceKernel<4>, dim3(n), dim3(4 * kChunk), 0, b->stream, a);
+    hipLaunchKernelGGL(PairForceKernel<4>, grid, dim3(4 * kChunk), 0, b->stream, a);
   HIP_TRY(b, hipGetLastError());
+  if (slices > 1) {
+    hipLaunchKernelGGL(PairForceSumKernel, dim3(n), dim3(kChunk), 0, b->stream, a);
+    HIP_TRY(b, hipGetLastError());
+  }
   rc = Sync(b);
   if (rc) return rc;
   *lj = b->hostRes[0];

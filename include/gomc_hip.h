@@ -423,8 +423,9 @@ const char *gomc_server_last_error(const gomc_server *sv);
  * GCMC lane: builder threads make each such job a device box in arrival
  * order while the budget holds its bytes, and every built job is advanced,
  * by whole MC cycles, in each launch of the device move loop
- * (gomc_run_moves_device_multi: one workgroup per job, the job's own
- * MT19937 stream) at the server's precision. */
+ * (gomc_run_moves_device_multi: a team of workgroups per job by its
+ * k-vector count, the job's own MT19937 stream) at the server's
+ * precision. */
 int gomc_server_submit(gomc_server *sv, const gomc_system_desc *d,
                        int64_t moves, int64_t *job_id);
 /* status 0 queued, 1 running, 2 done, 3 failed; times = {submitted,

@@ -2,7 +2,8 @@
 (BoxForce, BoxForceReciprocal, CalculateTorque, and the whole
 MultiParticle move inside GOMC's move loop).
 
-Tolerances (FP64): pair energies 1e-11 relative; every force and torque
+Tolerances (FP64), also for Mie exponents other than 12 and Ewald
+tolerances that take alpha r beyond 4: pair energies 1e-11 relative; every force and torque
 component within 1e-9 of the largest component of its array (the sums run
 in a different order on the device); the move loop with MultiParticle
 moves keeps the identical move-kind, move-type, accept/reject and
@@ -35,6 +36,25 @@ CASES = [(300, 10.0, 11), (1000, 7.0, 3)]  # 3 and 4 cells per edge
 @pytest.mark.parametrize("nmol,rcut,seed", CASES)
 def test_box_force_matches_oracle(nmol, rcut, seed):
     s = systems.water_box(nmol, rcut=rcut, seed=seed)
+    b, o = Box(s), Oracle(s, threads=8)
+    lj, real, aF, mF = b.box_force()
+    olj, oreal, oaF, omF = o.box_force()
+    assert abs(lj - olj) <= 1e-11 * abs(olj)
+    assert abs(real - oreal) <= 1e-11 * abs(oreal)
+    assert rel_close(aF, oaF)
+    assert rel_close(mF, omF)
+
+
+@pytest.mark.parametrize("n_rep,tol", [(13.0, 1e-5), (16.0, 1e-5), (15.3, 1e-5), (12.0, 1e-9)])
+def test_box_force_mie_exponents_and_tight_ewald(n_rep, tol):
+    """The pair-force kernel's own forms: Mie repulsion (sigma^2/r^2)^(n/2)
+    by products for whole and half-integer n/2 and by exp(n/2 log x)
+    otherwise, against the oracle's pow; erfc(alpha r) beyond alpha r = 4
+    (tolerance 1e-9: alpha rc = 4.55) by the continued fraction, against
+    libm's erfc."""
+    from dataclasses import replace
+    s = systems.water_box(400, rcut=9.0, seed=17, tolerance=tol)
+    s = replace(s, n_rep=[n_rep, 12.0], _keep=[])
     b, o = Box(s), Oracle(s, threads=8)
     lj, real, aF, mF = b.box_force()
     olj, oreal, oaF, omF = o.box_force()
