@@ -249,6 +249,50 @@ __global__ void __launch_bounds__(kBlock) CellCountKernel(const CellArgs a) {
   atomicAdd(&a.cellCount[c], 1);
 }
 
+// The same two passes for grids of at most kCellLdsMax cells: a workgroup of
+// 1024 atoms counts its atoms per cell in LDS and adds each nonzero count to
+// the global one with a single atomic, and the scatter reserves one range
+// per (workgroup, cell) the same way and places its atoms inside it by LDS
+// atomics.  Global atomics per cell drop from one per atom to one per
+// workgroup; the order inside a cell is made ascending by CellSortKernel
+// either way.
+constexpr int kCellLdsMax = 6144;
+constexpr int kCellLdsBlock = 1024;
+
+__global__ void __launch_bounds__(kCellLdsBlock) CellCountLdsKernel(const CellArgs a) {
+  extern __shared__ int cnt[];
+  for (int c = threadIdx.x; c < a.ncell; c += kCellLdsBlock) cnt[c] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * kCellLdsBlock + threadIdx.x;
+  if (i < a.natoms) {
+    if (a.mol[i] < 0) {
+      a.atomCell[i] = -1;
+    } else {
+      const int c = CellOf(a.x[i], a.y[i], a.z[i], a.cs, a.edge);
+      a.atomCell[i] = c;
+      atomicAdd(&cnt[c], 1);
+    }
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < a.ncell; c += kCellLdsBlock)
+    if (cnt[c]) atomicAdd(&a.cellCount[c], cnt[c]);
+}
+
+__global__ void __launch_bounds__(kCellLdsBlock) CellScatterLdsKernel(const CellArgs a) {
+  extern __shared__ int cnt[];  // [ncell] counts, then [ncell] range starts
+  int *base = cnt + a.ncell;
+  for (int c = threadIdx.x; c < a.ncell; c += kCellLdsBlock) cnt[c] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * kCellLdsBlock + threadIdx.x;
+  const int c = i < a.natoms ? a.atomCell[i] : -1;
+  const int r = c >= 0 ? atomicAdd(&cnt[c], 1) : 0;
+  __syncthreads();
+  for (int k = threadIdx.x; k < a.ncell; k += kCellLdsBlock)
+    if (cnt[k]) base[k] = atomicAdd(&a.cellCursor[k], cnt[k]);
+  __syncthreads();
+  if (c >= 0) a.cellVec[base[c] + r] = i;
+}
+
 // Exclusive scan of the per-cell counts (one workgroup; ncell is small).
 __global__ void __launch_bounds__(1024) CellScanKernel(const CellArgs a) {
   __shared__ int chunkSum[1024];
@@ -971,10 +1015,18 @@ int BuildCells(gomc_box *b) {
   a.cellVec = b->cellVec;
   a.atomCell = b->atomCell;
   HIP_TRY(b, hipMemsetAsync(b->cellCount, 0, sizeof(int) * b->ncell, b->stream));
-  const int g = (b->natoms + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(CellCountKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
-  hipLaunchKernelGGL(CellScanKernel, dim3(1), dim3(1024), 0, b->stream, a);
-  hipLaunchKernelGGL(CellScatterKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
+  if (b->ncell <= kCellLdsMax) {
+    const int g = (b->natoms + kCellLdsBlock - 1) / kCellLdsBlock;
+    const size_t lds = sizeof(int) * size_t(b->ncell);
+    hipLaunchKernelGGL(CellCountLdsKernel, dim3(g), dim3(kCellLdsBlock), lds, b->stream, a);
+    hipLaunchKernelGGL(CellScanKernel, dim3(1), dim3(1024), 0, b->stream, a);
+    hipLaunchKernelGGL(CellScatterLdsKernel, dim3(g), dim3(kCellLdsBlock), 2 * lds, b->stream, a);
+  } else {
+    const int g = (b->natoms + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(CellCountKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
+    hipLaunchKernelGGL(CellScanKernel, dim3(1), dim3(1024), 0, b->stream, a);
+    hipLaunchKernelGGL(CellScatterKernel, dim3(g), dim3(kBlock), 0, b->stream, a);
+  }
   // largest cell bounds the LDS of the sort
   std::vector<int> counts(b->ncell);
   HIP_TRY(b, hipMemcpyAsync(counts.data(), b->cellCount, sizeof(int) * b->ncell,

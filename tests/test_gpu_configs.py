@@ -4,8 +4,10 @@
   the kind, molecule, overlap and accept sequences of the first 10^4 moves
   equal the oracle's under the same MT19937 stream; per-move energy
   changes within 1e-10 relative; energies within 1e-10 relative.
-* 10,000 waters in a 66.9 A box (rc 12 A, Ewald 1e-6, about 10^4
-  k-vectors; MultiParticle 1:1 with translate/rotate): the whole-box
+* 10,000 waters in a 66.9 A box (rc 12 A, Ewald 1e-6, 30,902 k-vectors,
+  125 cells): 2000 translate/rotate moves and then 12 moves with
+  MultiParticle 1:1 have the oracle's sequences, per-move energy changes
+  within 1e-10 relative; with MultiParticle 1:1 with translate/rotate: the whole-box
   totals and the pair, reciprocal and molecule forces and torques against
   the oracle at full size (forces within 1e-9 of the largest component);
   after a run of moves that includes MultiParticle moves, the running
@@ -78,6 +80,31 @@ def test_large_box_10000_waters_forces_and_mp_bookkeeping():
     ng, rg = b.run(60)
     assert (rg["kind"] == 2).sum() >= 10 and ng > 0
     assert np.allclose(b.system_total(), b.energies(), rtol=1e-10, atol=1e-6)
+
+
+def test_10000_waters_same_sequence_as_oracle():
+    """Full size: translate/rotate moves walk 125 cells and 30,902 k-vectors;
+    then MultiParticle moves 1:1 with them, each against the oracle."""
+    from dataclasses import replace
+    s = replace(systems.config2(), move_mp=0.0, move_disp=0.5, move_rot=0.5, _keep=[])
+    b, o = Box(s), Oracle(s, threads=16)
+    n = 2000
+    ng, rg = b.run(n)
+    no, ro = o.run(n, mode=0)
+    same_sequence(rg, ro, o.energies())
+    assert ng == no and ng > 500
+    assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=0)
+    del b, o
+    s = systems.config2()
+    b, o = Box(s), Oracle(s, threads=16)
+    n = 12
+    ng, rg = b.run(n)
+    no, ro = o.run(n, mode=0)
+    assert (ro["kind"] == 2).sum() >= 4
+    same_sequence(rg, ro, o.energies())
+    assert ng == no
+    assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=0)
+    assert np.allclose(b.coords(), o.coords(), rtol=0, atol=1e-9)
 
 
 def test_gcmc_40A_box_same_sequence_as_oracle():
