@@ -280,9 +280,10 @@ def config_rows(threads: int, with_cpu: bool) -> dict:
     s3 = systems.config3()
     r, k = gpu_rate(s3, 10000)
     rd, _ = gpu_rate(s3, 10000, device=True)
-    row = dict(workload="GCMC methane/CO2, L 40 A, 300 K, rc 10 A, Ewald 1e-5, 1200 "
-                        "molecules seeded, 40 % transfers (CBMC 16/8), 30 % translate, "
-                        "30 % rotate", gpu_host_driver_moves_per_s=r,
+    row = dict(workload="GCMC methane/CO2, L 40 A, 300 K, rc 10 A, Ewald 1e-5, fugacities "
+                        "960/400 bar (the box's densest state, about 980 molecules), 1000 "
+                        "seeded, 40 % transfers (CBMC 16/8), 30 % translate, 30 % rotate",
+               gpu_host_driver_moves_per_s=r,
                gpu_device_loop_moves_per_s=rd, moves_by_kind=k, gpu_sample="10000 moves")
     if with_cpu:
         row.update(cpu_moves_per_s=cpu_rate(s3, 3000, threads), cpu_threads=threads,

@@ -315,11 +315,15 @@ def gcmc_mixture(n_ch4: int, n_co2: int, box: float, temperature: float = 300.0,
 
 
 def config3(seed: int = 12345) -> System:
-    """GCMC methane + CO2, 40 A box, 300 K, fugacities chosen for about 2000
-    molecules in total; the box is seeded uniformly with 600 + 600 molecules
-    (random sequential insertion cannot reach 2000 at liquid-like density)
-    and the transfers fill it; 40 % transfers with 16 first-atom trials, 30 %
-    translate, 30 % rotate, Ewald tolerance 1e-5."""
-    return gcmc_mixture(600, 600, 40.0, temperature=300.0, fugacity_bar=(60.0, 25.0),
+    """GCMC methane + CO2, 40 A box, 300 K, at the densest state the box
+    reaches: fugacities 960 / 400 bar hold about 980 molecules (283 CH4,
+    693 CO2; device move loop, mean over 5e5 moves after 5e5).  The molecule
+    count saturates there (the same ratio at 60 / 25 bar: 241; x8: 920;
+    x32: 1023): 2000 molecules in (40 A)^3 would be 0.031 per A^3, about
+    twice the number density of liquid methane or CO2 at 300 K, which no
+    chemical potential reaches.  Seeded uniformly with 500 + 500 molecules;
+    40 % transfers with 16 first-atom trials, 30 % translate, 30 % rotate,
+    Ewald tolerance 1e-5."""
+    return gcmc_mixture(500, 500, 40.0, temperature=300.0, fugacity_bar=(960.0, 400.0),
                         reservoir=1500, seed=seed, cbmc_first=16, cbmc_nth=8, cbmc_ang=100)
 

@@ -13,8 +13,9 @@
   after a run of moves that includes MultiParticle moves, the running
   energies equal a from-scratch recomputation (1e-10 relative) -- the
   size-independent check of the MP transform and acceptance bookkeeping.
-* GCMC methane / CO2 in a 40 A box (1200 molecules seeded, fugacities for
-  about 2000, a reservoir of 3000): totals with the reservoir excluded, and
+* GCMC methane / CO2 in a 40 A box (1000 molecules seeded, fugacities for
+  its densest state, about 980 molecules; a reservoir of 3000): totals with
+  the reservoir excluded, and
   10^4 moves (transfers with CBMC growth, translations, rotations) with
   the oracle's kind, molecule, overlap and accept sequences; final
   membership identical.
@@ -121,6 +122,26 @@ def test_gcmc_40A_box_same_sequence_as_oracle():
     n2, box2 = o.mol_boxes()
     assert n1 == n2 and np.array_equal(box1, box2)
     assert np.allclose(b.energies(), o.energies(), rtol=1e-10, atol=1e-6)
+
+
+def test_config3_box_holds_its_dense_state():
+    """Config 3's fugacities hold the 40 A box near 980 molecules (the count
+    saturates there; 2000 would be twice the liquid number density): from
+    the 1000 seeded, 3e5 device-loop moves keep the mean over the last half
+    within 880-1080 molecules, CO2 the majority."""
+    s = systems.config3()
+    b = Box(s)
+    kinds = np.asarray(s.mol_kind)
+    counts, step = [], 0
+    for _ in range(30):
+        b.run(10000, start_step=step, record=False, device=True)
+        step += 10000
+        _, box = b.mol_boxes()
+        counts.append((int(((box == 0) & (kinds == 0)).sum()), int(((box == 0) & (kinds == 1)).sum())))
+    h = np.array(counts[15:])
+    total = h.sum(1).mean()
+    assert 880 <= total <= 1080, total
+    assert h[:, 1].mean() > h[:, 0].mean()
 
 
 def test_config4_batch_cycle_and_traced_boxes():
