@@ -149,10 +149,8 @@ __device__ __noinline__ void Propose(const BatchBox &sb, BatchState &st, MoveSha
     const double bb = st.scale[1];
     const double theta = u[2] * (2.0 * bb) - bb;
     const double cu = u[3] * 2.0 - 1.0;
-    const double phi = u[4] * (2.0 * M_PI);
     const double root = sqrt(1.0 - cu * cu);
-    double sp, cp;
-    sincos(phi, &sp, &cp);
+    const double sp = u[6], cp = u[7];  // sincos(2 pi u[4]), made with the draws
     const double ax[3] = {root * cp, root * sp, cu};
     double sn, c;
     sincos(theta, &sn, &c);
@@ -165,7 +163,10 @@ __device__ __noinline__ void Propose(const BatchBox &sb, BatchState &st, MoveSha
   }
 }
 
-// Philox draws of one move: u[0..5] (two blocks, counter {step, 0|1}).
+// Philox draws of one move: u[0..5] (two blocks, counter {step, 0|1}), and
+// sin, cos of a rotation's axis azimuth 2 pi u[4] in u[6], u[7] (a function
+// of the draw alone, so made here, 64 moves ahead, not by thread 0 within
+// the move).
 __device__ __noinline__ void MoveDraws(uint64_t seed, uint64_t step, double *out) {
   const U64x4 r0 = Philox(step, 0, 0, 0, seed, kPhiloxTag);
   const U64x4 r1 = Philox(step, 1, 0, 0, seed, kPhiloxTag);
@@ -175,6 +176,8 @@ __device__ __noinline__ void MoveDraws(uint64_t seed, uint64_t step, double *out
   out[3] = U01(r0.v[3]);
   out[4] = U01(r1.v[0]);
   out[5] = U01(r1.v[1]);
+  const double phi = out[4] * (2.0 * M_PI);
+  sincos(phi, &out[6], &out[7]);
 }
 
 // Phase tables e^{i (c_d n) r_d}, n = 0..nmax_d, for the moving molecule's
@@ -683,7 +686,7 @@ template <int PREC>
 __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a) {
   using T = typename Prec<PREC>::T;
   __shared__ BatchBox sb;
-  __shared__ double draws[kDrawChunk][6];
+  __shared__ double draws[kDrawChunk][8];
   __shared__ CplxT<T> tab[2 * kBatchMaxAtoms][3][kBatchMaxN + 1];
   __shared__ MoveShared ms;
   __shared__ double red[kBatchThreads / kWave][4];
@@ -851,7 +854,11 @@ __global__ void __launch_bounds__(kBatchThreads) McBatchKernel(const BatchArgs a
           }
         }
       }
-      __syncthreads();
+      // No barrier here: every read of these writes by another thread comes
+      // after the next move's first barrier (after the proposal; the trial
+      // coordinates are read by the same threads < len that write them
+      // here), and nothing before that barrier overwrites what the commit
+      // reads (ms.newp is next written after it, ms.newCom by this thread 0).
     }
 
     if (pend && ewald)
