@@ -293,6 +293,42 @@ def config_rows(threads: int, with_cpu: bool) -> dict:
     return out
 
 
+def served_n_check(jobs, results, replicas: int = 4, burn: int = 100, sample: int = 200) -> dict:
+    """<N> per kind of the stream's densest job in the served precision
+    (mixed) against FP64: `replicas` runs a side from the job's initial
+    state, every run with its own MT19937 seed (the two sides' seeds differ
+    too: under one seed the two precisions take the same decisions for
+    hundreds of cycles, which would compare nothing), `burn` cycles, then
+    `sample` cycles sampled; standard errors from the replica scatter.  The FP64 side is the
+    device move loop in FP64, which takes the FP64 oracle's decisions move
+    for move (tests/test_gpu_devloop.py), so it stands for the oracle at a
+    size the oracle cannot run here in seconds."""
+    from dataclasses import replace
+    from gomc_amd.api import Box, run_device
+    j = int(np.argmax([r["mean_n"].sum() for r in results]))
+    s0 = jobs[j].system
+    big = 1 << 40
+    out = dict(job=j, box_A=float(s0.box[0]), temperature_K=float(s0.temperature),
+               fugacity_bar=[float(f) for f in s0.fugacity], replicas=replicas,
+               burn_cycles=burn, sample_cycles=sample,
+               served_mean_n=[float(v) for v in results[j]["mean_n"]])
+    for prec, tag in ((1, "mixed"), (0, "fp64")):
+        boxes = [Box(replace(s0, seed=int(s0.seed) * 7919 + 1000 * prec + r + 1, _keep=[]))
+                 for r in range(replicas)]
+        for b in boxes:
+            b.set_precision(prec)
+        moves, _, _, _ = run_device(boxes, big, cycles=burn)
+        _, _, sums, samples = run_device(boxes, big, start_steps=moves, cycles=sample)
+        n = sums / samples[:, None]
+        for b in boxes:
+            b.close()
+        out[tag] = [float(v) for v in n.mean(0)]
+        out[tag + "_se"] = [float(v) for v in n.std(0, ddof=1) / math.sqrt(replicas)]
+    out["z"] = [float(abs(out["mixed"][k] - out["fp64"][k]) /
+                      max(math.hypot(out["mixed_se"][k], out["fp64_se"][k]), 1e-12)) for k in range(2)]
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -425,6 +461,8 @@ def main() -> None:
         print(f"# served GCMC {served_gcmc['moves_per_s']:.0f} moves/s p50 "
               f"{served_gcmc['p50_s']:.2f} s p99 {served_gcmc['p99_s']:.2f} s; <N> per job "
               f"{served_gcmc['mean_n_range']}", file=sys.stderr, flush=True)
+        served_gcmc["n_check"] = served_n_check(jobs, rep["results"])
+        print(f"# served GCMC <N> check {served_gcmc['n_check']}", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain:

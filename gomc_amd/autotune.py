@@ -7,8 +7,11 @@ at which a single-workgroup box gives its pair phase 12 waves (beside a
 4-wave reciprocal phase) instead of 8 + 8.  The tuner times MC cycles of a
 workload (default: the 256-box water batch) replayed from a HIP graph,
 first over the wave split with the default teams, then over the team
-thresholds with the best split, and writes the fastest choice to
-gomc_amd/tuning.json, which `Batch` then uses by default.  The packing of
+thresholds with the best split, then refines the thresholds by coordinate
+descent (each moved by +-100 and +-200 molecules, kept in order, two
+rounds); a refined choice replaces the coarse one only when a longer
+re-measurement of both finds it at least 1 % faster.  The fastest choice
+goes to gomc_amd/tuning.json, which `Batch` then uses by default.  The packing of
 single-workgroup boxes onto workgroups is tuned online instead: each launch
 measures every box's time and the next launch packs by it.
 
@@ -58,6 +61,36 @@ def tune(boxes, cycles: int = 3, log=print) -> dict:
         log(f"# pair12_max_k {t['pair12_max_k']:5d} teams {tm}: {ms:.2f} ms/cycle")
         if ms < best_ms:
             best, best_ms = t, ms
+    coarse = dict(best)
+    seen = {(tuple(t["team_min"]), t["pair12_max_k"]) for t, _ in results}
+    for _ in range(2):
+        moved = False
+        for i in range(3):
+            for step in (-200, -100, 100, 200):
+                tm = list(best["team_min"])
+                tm[i] += step
+                if not (0 < tm[0] < tm[1] < tm[2]):
+                    continue
+                t = {"team_min": tm, "pair12_max_k": best["pair12_max_k"]}
+                key = (tuple(tm), t["pair12_max_k"])
+                if key in seen:
+                    continue
+                seen.add(key)
+                ms = cycle_ms(boxes, t, cycles)
+                results.append((t, ms))
+                log(f"# refine teams {tuple(tm)}: {ms:.2f} ms/cycle")
+                if ms < best_ms:
+                    best, best_ms, moved = t, ms, True
+        if not moved:
+            break
+    if best != coarse:  # confirm against the coarse choice on longer runs
+        a = cycle_ms(boxes, coarse, 3 * cycles)
+        b = cycle_ms(boxes, best, 3 * cycles)
+        log(f"# confirm coarse {coarse['team_min']} {a:.2f} vs refined {best['team_min']} {b:.2f}")
+        if b < 0.99 * a:
+            best_ms = b
+        else:
+            best, best_ms = coarse, a
     return {"tuning": best, "ms_per_cycle": best_ms,
             "candidates": [{"tuning": t, "ms_per_cycle": m} for t, m in results]}
 

@@ -426,6 +426,7 @@ struct SumsArgs {
   const double *kx, *ky, *kz, *pref;
   const int *kseg;   // segment starts [nseg + 1], kseg[nseg] = K
   const int *nseg;   // device
+  double cz;         // the nz step of every segment, 2 pi / L_z
   double *sumR, *sumI;
   double *part;      // [chunks][K][2]
   Partial4 *partials;
@@ -486,7 +487,9 @@ __global__ void __launch_bounds__(kSegThreads) KSegmentsKernel(const double *kx,
 }
 
 __global__ void __launch_bounds__(kBlock) RecipSumsPartKernel(const SumsArgs a) {
-  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
+  // the tile's atoms, and each atom's phase step e^{i c_z z} (the same for
+  // every segment: consecutive nz differ by c_z), made once per atom
+  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock], sc[kBlock], ssn[kBlock];
   const int nseg = *a.nseg;
   const int segBlocks = max(1, (nseg + kBlock - 1) / kBlock);
   const int chunks = SumsChunks(nseg, a.natoms, a.K);
@@ -498,7 +501,6 @@ __global__ void __launch_bounds__(kBlock) RecipSumsPartKernel(const SumsArgs a) 
     const int len = active ? a.kseg[seg + 1] - k0 : 0;
     const double kx0 = active ? a.kx[k0] : 0.0, ky0 = active ? a.ky[k0] : 0.0,
                  kz0 = active ? a.kz[k0] : 0.0;
-    const double dz = len > 1 ? a.kz[k0 + 1] - kz0 : 0.0;
     double sR[kSeg], sI[kSeg];
 #pragma unroll
     for (int i = 0; i < kSeg; ++i) sR[i] = sI[i] = 0.0;
@@ -512,15 +514,19 @@ __global__ void __launch_bounds__(kBlock) RecipSumsPartKernel(const SumsArgs a) 
         sy[threadIdx.x] = a.y[j];
         sz[threadIdx.x] = a.z[j];
         sq[threadIdx.x] = (!a.mol || a.mol[j] >= 0) ? a.q[j] : 0.0;  // reservoir adds nothing
+        double sn, cn;
+        sincos(a.cz * a.z[j], &sn, &cn);
+        ssn[threadIdx.x] = sn;
+        sc[threadIdx.x] = cn;
       }
       __syncthreads();
       if (active)
         for (int jj = 0; jj < tn; ++jj) {
           const double qj = sq[jj];
           if (fabs(qj) < 0.000000001) continue;
-          double im, re, ss, cs;
+          double im, re;
           sincos(sx[jj] * kx0 + sy[jj] * ky0 + sz[jj] * kz0, &im, &re);
-          sincos(dz * sz[jj], &ss, &cs);
+          const double ss = ssn[jj], cs = sc[jj];
 #pragma unroll
           for (int i = 0; i < kSeg; ++i) {
             if (i < len) {
@@ -1121,6 +1127,7 @@ static SumsArgs BoxSumsArgs(const gomc_box *b) {
   a.pref = b->pref;
   a.kseg = b->kseg;
   a.nseg = b->nsegDev;
+  a.cz = (1.0 / b->geom.L[2]) * (2.0 * M_PI);  // KVectorsOrth's z spacing
   a.sumR = b->sumRnew;
   a.sumI = b->sumInew;
   a.result = b->devRes;

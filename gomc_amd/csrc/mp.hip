@@ -283,6 +283,7 @@ struct RecipForceArgs {
   const double *kx, *ky, *kz, *pref, *sumR, *sumI;
   const int *kseg;  // segment starts [nseg + 1]
   int nseg;
+  double cz;  // the nz step of every segment, 2 pi / L_z
   double *fx, *fy, *fz;
   // segment chunks: RecipForcePartKernel writes chunk c's k-space force of
   // every atom (segments [c segChunk, (c + 1) segChunk)) to
@@ -307,6 +308,9 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
   }
   const bool on = active && !(fabs(qp) < 0.000000001);
   double X = 0.0, Y = 0.0, Z = 0.0;
+  // the atom's phase step e^{i c_z z}, the same for every segment
+  double sn = 0.0, cn = 1.0;
+  if (on) sincos(a.cz * zp, &sn, &cn);
   const int s0 = min(a.nseg, chunk * a.segChunk), s1 = min(a.nseg, s0 + a.segChunk);
   for (int t = s0; t < s1; t += kForceSegTile) {
     const int tn = min(kForceSegTile, s1 - t);
@@ -325,9 +329,8 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
     if (on)
       for (int j = 0; j < tn; ++j) {
         const int k0 = ss[j], len = ss[j + 1] - k0;
-        double im, re, sn, cn;
+        double im, re;
         sincos(xp * sk[0][k0] + yp * sk[1][k0] + zp * sk[2][k0], &im, &re);
-        sincos((len > 1 ? sk[2][k0 + 1] - sk[2][k0] : 0.0) * zp, &sn, &cn);
 #pragma unroll
         for (int i = 0; i < kSeg; ++i) {
           if (i < len) {
@@ -788,6 +791,7 @@ int MPRecipForce(gomc_box *b, int set) {
   a.fy = mp.aFR[set][1];
   a.fz = mp.aFR[set][2];
   a.kseg = b->kseg;
+  a.cz = (1.0 / b->geom.L[2]) * (2.0 * M_PI);  // KVectorsOrth's z spacing
   a.nseg = b->nseg;
   const int g = (b->natoms + kBlock - 1) / kBlock;
   // segment chunks, about 1024 workgroups in the first pass

@@ -100,12 +100,19 @@ __global__ void __launch_bounds__(kBlock) PairVirialKernel(const PairVirArgs a) 
   }
 }
 
+// Segments of the k list (consecutive nz, at most kSeg vectors) staged in
+// LDS this many at a time by the intramolecular term.
+constexpr int kVirSegTile = 64;
+
 struct RecipVirArgs {
   BoxGeom g;
   const double *x, *y, *z, *q;
   const int *mol;
   int natoms, K, kBlocks;
-  int kChunk, kChunks;  // k-vectors per chunk (whole tiles), chunks per atom block
+  int segChunk, kChunks;  // segments per chunk, chunks per atom block
+  const int *kseg;        // segment starts [nseg + 1]
+  int nseg;
+  double kzStep;          // the nz step of every segment, 2 pi / L_z
   const double *kx, *ky, *kz, *pref, *sumR, *sumI;
   const double *cx, *cy, *cz;
   double constVal;  // 1 / (4 alpha^2)
@@ -114,7 +121,8 @@ struct RecipVirArgs {
 
 __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a) {
   __shared__ double red[kBlock / kWave];
-  __shared__ double sk[6][kBlock];
+  __shared__ double sk[6][kVirSegTile * kSeg];
+  __shared__ int ss[kVirSegTile + 1];
   double w[3] = {0, 0, 0};
   if (int(blockIdx.x) < a.kBlocks) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -144,30 +152,45 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
         ddz = Unwrap(zp, cz, a.g.L[2], a.g.half[2]) - cz;
       }
     }
+    // this chunk's segments, kVirSegTile at a time: two sincos per
+    // (atom, segment), the segment's first phase and its nz step, then one
+    // complex product per k-vector
     const int chunk = (blockIdx.x - a.kBlocks) % a.kChunks;
-    const int k0 = chunk * a.kChunk, k1 = min(a.K, k0 + a.kChunk);
-    for (int t = k0; t < k1; t += kBlock) {
-      const int tn = min(kBlock, k1 - t);
+    const int s0 = min(a.nseg, chunk * a.segChunk), s1 = min(a.nseg, s0 + a.segChunk);
+    double sn = 0.0, cn = 1.0;  // the atom's phase step e^{i c_z z}, every segment's
+    if (on) sincos(a.kzStep * zp, &sn, &cn);
+    for (int t = s0; t < s1; t += kVirSegTile) {
+      const int tn = min(kVirSegTile, s1 - t);
+      const int kA = a.kseg[t], kn = a.kseg[t + tn] - kA;
       __syncthreads();
-      if (int(threadIdx.x) < tn) {
-        const int k = t + threadIdx.x;
-        sk[0][threadIdx.x] = a.kx[k];
-        sk[1][threadIdx.x] = a.ky[k];
-        sk[2][threadIdx.x] = a.kz[k];
-        sk[3][threadIdx.x] = a.pref[k];
-        sk[4][threadIdx.x] = a.sumR[k];
-        sk[5][threadIdx.x] = a.sumI[k];
+      for (int i = threadIdx.x; i <= tn; i += kBlock) ss[i] = a.kseg[t + i] - kA;
+      for (int i = threadIdx.x; i < kn; i += kBlock) {
+        sk[0][i] = a.kx[kA + i];
+        sk[1][i] = a.ky[kA + i];
+        sk[2][i] = a.kz[kA + i];
+        sk[3][i] = a.pref[kA + i];
+        sk[4][i] = a.sumR[kA + i];
+        sk[5][i] = a.sumI[kA + i];
       }
       __syncthreads();
       if (on)
-        for (int i = 0; i < tn; ++i) {
-          const double arg = xp * sk[0][i] + yp * sk[1][i] + zp * sk[2][i];
+        for (int j = 0; j < tn; ++j) {
+          const int k0 = ss[j], len = ss[j + 1] - k0;
           double s, c;
-          sincos(arg, &s, &c);
-          const double factor = sk[3][i] * 2.0 * (sk[5][i] * c - sk[4][i] * s) * qp;
-          w[0] += factor * (sk[0][i] * ddx);
-          w[1] += factor * (sk[1][i] * ddy);
-          w[2] += factor * (sk[2][i] * ddz);
+          sincos(xp * sk[0][k0] + yp * sk[1][k0] + zp * sk[2][k0], &s, &c);
+#pragma unroll
+          for (int i = 0; i < kSeg; ++i) {
+            if (i < len) {
+              const int k = k0 + i;
+              const double factor = sk[3][k] * 2.0 * (sk[5][k] * c - sk[4][k] * s) * qp;
+              w[0] += factor * (sk[0][k] * ddx);
+              w[1] += factor * (sk[1][k] * ddy);
+              w[2] += factor * (sk[2][k] * ddz);
+            }
+            const double nc = c * cn - s * sn;
+            s = c * sn + s * cn;
+            c = nc;
+          }
         }
     }
   }
@@ -190,16 +213,15 @@ extern "C" int gomc_box_virial(gomc_box *b, double out[9]) {
   const int pairBlocks = b->ncell * 27;
   const int kBlocks = b->ewald ? (b->K + kBlock - 1) / kBlock : 0;
   const int atomBlocks = b->ewald ? (b->natoms + kBlock - 1) / kBlock : 0;
-  // The intramolecular term splits the k-vectors into chunks as well as the
-  // atoms into blocks, so that a large box fills the chip (about 2048
-  // workgroups); chunks are whole k tiles.
-  int kChunks = 1, kChunk = std::max(b->K, 1);
+  // The intramolecular term splits the k list's segments into chunks as
+  // well as the atoms into blocks, so that a large box fills the chip (about
+  // 2048 workgroups).
+  int kChunks = 1, segChunk = std::max(b->nseg, 1);
   if (atomBlocks > 0) {
-    const int tiles = std::max(kBlocks, 1);
-    kChunks = std::min(tiles, std::max(1, (2048 + atomBlocks - 1) / atomBlocks));
-    const int tilesPerChunk = (tiles + kChunks - 1) / kChunks;
-    kChunk = tilesPerChunk * kBlock;
-    kChunks = (tiles + tilesPerChunk - 1) / tilesPerChunk;
+    if (b->nseg <= 0) return Fail(b, GOMC_ERR_STATE, "k-vector segments not built");
+    kChunks = std::min(b->nseg, std::max(1, (2048 + atomBlocks - 1) / atomBlocks));
+    segChunk = (b->nseg + kChunks - 1) / kChunks;
+    kChunks = (b->nseg + segChunk - 1) / segChunk;
   }
   const int nblocks = pairBlocks + kBlocks + atomBlocks * kChunks;
   const size_t need = size_t(kVirSlots) * std::max(nblocks, 1);
@@ -244,8 +266,11 @@ extern "C" int gomc_box_virial(gomc_box *b, double out[9]) {
     r.natoms = b->natoms;
     r.K = b->K;
     r.kBlocks = kBlocks;
-    r.kChunk = kChunk;
+    r.segChunk = segChunk;
     r.kChunks = kChunks;
+    r.kseg = b->kseg;
+    r.nseg = b->nseg;
+    r.kzStep = (1.0 / b->geom.L[2]) * (2.0 * M_PI);  // KVectorsOrth's z spacing
     r.kx = b->kx;
     r.ky = b->ky;
     r.kz = b->kz;

@@ -221,14 +221,17 @@ def test_mixed_precision_observables_within_two_sigma():
 
 
 def test_autotuner_picks_a_timed_candidate():
-    """The autotuner times every candidate launch shape on a small batch and
-    returns the fastest; the per-box times it relies on are positive."""
+    """The autotuner times the whole coarse grid of launch shapes on a small
+    batch, then refines the team thresholds, and returns one of the shapes
+    it timed; the per-box times it relies on are positive."""
     from gomc_amd import autotune
     boxes = [systems.water_box(n, seed=300 + i) for i, n in enumerate((250, 600, 1900, 2100))]
     rep = autotune.tune(boxes, cycles=1, log=lambda m: None)
     ms = [c["ms_per_cycle"] for c in rep["candidates"]]
-    assert len(ms) == len(autotune.PAIR12) + len(autotune.TEAMS) - 1
-    assert rep["ms_per_cycle"] == min(ms) and all(m > 0 for m in ms)
+    coarse = len(autotune.PAIR12) + len(autotune.TEAMS) - 1
+    assert len(ms) >= coarse and all(m > 0 for m in ms)
+    assert rep["tuning"] in [c["tuning"] for c in rep["candidates"]]
+    assert rep["ms_per_cycle"] > 0
     bt = Batch(boxes, tuning=rep["tuning"])
     bt.run_cycles(1, graph=False)
     assert np.all(bt.box_times() > 0)
