@@ -140,7 +140,7 @@ struct gomc_batch {
   int grid = 0;        // workgroups per launch
   int gridCap = 0;     // workgroups that fit on the device at once
   bool teams = true;   // false: every box runs on one workgroup (serving)
-  gomc_batch_tuning tune{{1800, 2900, 3500}, 2048};
+  gomc_batch_tuning tune{{1800, 2700, 3500}, 1024};
   uint64_t *boxTicks = nullptr;
   std::vector<double> usPerMove;  // measured per-box cost of the last launch (0: none)
   std::vector<double> lastUs;     // per-box time of the last launch

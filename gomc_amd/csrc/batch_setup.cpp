@@ -338,7 +338,7 @@ int TeamSize(const gomc_batch *bt, int nmols) {
   // Per-move cost of one workgroup grows about 10 us + 6 ns per molecule
   // (measured on MI355X, water); a team of T repeats ~7 us of it per member
   // (draws, trial coordinates, tables, the exchange) and splits the rest.
-  // The default thresholds {1800, 2900, 3500} keep a 256-box cycle near 50 ms.
+  // The default thresholds {1800, 2700, 3500} keep a 256-box cycle near 49 ms.
   const int32_t *t = bt->tune.team_min;
   if (t[2] > 0 && nmols >= t[2]) return 8;
   if (t[1] > 0 && nmols >= t[1]) return 4;

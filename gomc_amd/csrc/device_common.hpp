@@ -236,6 +236,64 @@ __device__ __forceinline__ double Rsqrt64(double d2) {
   return y;
 }
 
+// The energies (counted when `count`) and virial factors of one in-cutoff
+// pair: FFParticle::CalcCoulomb / CalcCoulombVir (src/FFParticle.cpp:
+// 387-399, 432-446) and CalcEn / CalcVir (295-359).  erfc(a r) is
+// exp(-(a r)^2) erfcx(a r) with erfcx from the fitted polynomial of
+// erfcx_poly.hpp (coefficients `poly`, in LDS), and the same exponential is the virial's Gaussian: one
+// exp and one polynomial per pair instead of libm's erfc, erf and exp, whose
+// range branches split a wave.  (1 - erf is erfc up to rounding: the
+// reference's form differs from this one by about 1e-16 absolute.)
+struct PairTerms {
+  double eLJ, eReal, vLJ, vReal;
+};
+__device__ __forceinline__ PairTerms PairMath(const BoxGeom &g, const double *poly, double d2,
+                                              double qiqj, double sig, double eps, double nh,
+                                              double eps6, double nov6, bool count) {
+  PairTerms p{0.0, 0.0, 0.0, 0.0};
+  if (g.elect && qiqj != 0.0 && !(g.rCutCoulSq < d2)) {
+    const double dist = sqrt(d2);
+    if (g.ewald) {
+      const double x = g.alpha * dist;
+      double ec, gauss;
+      if (x <= kErfcxUse) {
+        const double t = (x - kErfcxMid) * kErfcxInvHalf;
+        double px = poly[kErfcxDeg];
+        // read from LDS each step: unrolled, the coefficients would be held
+        // in 58 VGPRs across the pair loop
+#pragma unroll 1
+        for (int k = kErfcxDeg - 1; k >= 0; --k) px = fma(px, t, poly[k]);
+        gauss = exp(-(x * x));
+        ec = gauss * px;
+      } else {
+        gauss = exp(-(x * x));
+        ec = gauss * ErfcxTail(x);
+      }
+      if (count) p.eReal = qiqj * ec / dist;
+      p.vReal = qiqj * (ec / dist + g.alpha * M_2_SQRTPI * gauss) / d2;
+    } else {
+      if (count) p.eReal = qiqj / dist;
+      p.vReal = qiqj / (d2 * dist);
+    }
+  }
+  if (!(g.rCutLJSq < d2)) {
+    // FFParticle::CalcEn and CalcVir in their operation orders, the
+    // repulsion by MiePow rather than libm's pow
+    if (count) {
+      const double rRat2 = sig / d2;
+      const double attract = rRat2 * rRat2 * rRat2;
+      const double repulse = nh == 6.0 ? attract * attract : MiePow(rRat2, nh);
+      p.eLJ = eps * (repulse - attract);
+    }
+    const double rNeg2 = 1.0 / d2;
+    const double rRat2 = rNeg2 * sig;
+    const double attract = rRat2 * rRat2 * rRat2;
+    const double repulse = nh == 6.0 ? attract * attract : MiePow(rRat2, nh);
+    p.vLJ = eps6 * (nov6 * repulse - attract) * rNeg2;
+  }
+  return p;
+}
+
 // The pair terms and phases at precision P: 0 is FP64 throughout, 1 the
 // FP32 forms above (sums stay FP64 in every caller).
 template <int P>

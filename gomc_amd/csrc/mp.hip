@@ -38,15 +38,20 @@ __device__ __forceinline__ double Uneg11(uint64_t v) {
 // takes tiles s G + w, s G + w + slices G, ...  A wave stages each tile
 // (position, charge, type, molecule of 64 neighbour atoms) in its LDS, one
 // coalesced load per tile instead of a chain of dependent loads per pair,
-// marks each lane's in-cutoff partners of the tile in a bit mask, then
-// evaluates only those, in ascending tile order.  A workgroup adds its
+// and marks each lane's in-cutoff partners of the tile in a bit mask.  It
+// then lists the tile's pairs in (home atom, tile entry) order and
+// evaluates them 64 at a time, so the pair math runs about (pairs / 64)
+// times per tile rather than as often as the busiest lane has partners;
+// the force contributions of a batch are summed per home atom by a
+// segmented scan and added to that atom's sum in the wave's LDS, an order
+// fixed by the positions alone.  A workgroup adds its
 // waves' per-atom forces in wave order through LDS; with one slice it
 // writes the chunk's forces, with several each slice writes its own and
 // PairForceSumKernel adds them in slice order.  Energies count each pair
 // once (i < j).  The host picks G and the slice count so that a box of any
 // size puts a few thousand waves on the device.
 constexpr int kPairChunksFor4 = 1024;  // from this many chunks: 4 waves per workgroup
-constexpr int kPairChunksFor8 = 256;   // from this many: 8; below: 16 and slices
+constexpr int kPairChunksFor8 = 256;   // from this many: 8; below: 8 and slices
 constexpr int kPairSmallWaves = 2048;  // waves a small box is split into, about
 constexpr int kPairMaxSlices = 8;
 
@@ -66,68 +71,25 @@ struct PairForceArgs {
   double *result;
 };
 
-// The energies (counted when `count`) and virial factors of one in-cutoff
-// pair: FFParticle::CalcCoulomb / CalcCoulombVir (src/FFParticle.cpp:
-// 387-399, 432-446) and CalcEn / CalcVir (295-359).  erfc(a r) is
-// exp(-(a r)^2) erfcx(a r) with erfcx from the fitted polynomial of
-// erfcx_poly.hpp (coefficients `poly`, in LDS), and the same exponential is the virial's Gaussian: one
-// exp and one polynomial per pair instead of libm's erfc, erf and exp, whose
-// range branches split a wave.  (1 - erf is erfc up to rounding: the
-// reference's form differs from this one by about 1e-16 absolute.)
-struct PairTerms {
-  double eLJ, eReal, vLJ, vReal;
-};
-__device__ __forceinline__ PairTerms PairMath(const BoxGeom &g, const double *poly, double d2,
-                                              double qiqj, double sig, double eps, double nh,
-                                              double eps6, double nov6, bool count) {
-  PairTerms p{0.0, 0.0, 0.0, 0.0};
-  if (g.elect && qiqj != 0.0 && !(g.rCutCoulSq < d2)) {
-    const double dist = sqrt(d2);
-    if (g.ewald) {
-      const double x = g.alpha * dist;
-      double ec, gauss;
-      if (x <= kErfcxUse) {
-        const double t = (x - kErfcxMid) * kErfcxInvHalf;
-        double px = poly[kErfcxDeg];
-        // read from LDS each step: unrolled, the coefficients would be held
-        // in 58 VGPRs across the pair loop
-#pragma unroll 1
-        for (int k = kErfcxDeg - 1; k >= 0; --k) px = fma(px, t, poly[k]);
-        gauss = exp(-(x * x));
-        ec = gauss * px;
-      } else {
-        gauss = exp(-(x * x));
-        ec = gauss * ErfcxTail(x);
-      }
-      if (count) p.eReal = qiqj * ec / dist;
-      p.vReal = qiqj * (ec / dist + g.alpha * M_2_SQRTPI * gauss) / d2;
-    } else {
-      if (count) p.eReal = qiqj / dist;
-      p.vReal = qiqj / (d2 * dist);
-    }
-  }
-  if (!(g.rCutLJSq < d2)) {
-    // FFParticle::CalcEn and CalcVir in their operation orders, the
-    // repulsion by MiePow rather than libm's pow
-    if (count) {
-      const double rRat2 = sig / d2;
-      const double attract = rRat2 * rRat2 * rRat2;
-      const double repulse = nh == 6.0 ? attract * attract : MiePow(rRat2, nh);
-      p.eLJ = eps * (repulse - attract);
-    }
-    const double rNeg2 = 1.0 / d2;
-    const double rRat2 = rNeg2 * sig;
-    const double attract = rRat2 * rRat2 * rRat2;
-    const double repulse = nh == 6.0 ? attract * attract : MiePow(rRat2, nh);
-    p.vLJ = eps6 * (nov6 * repulse - attract) * rNeg2;
-  }
-  return p;
-}
-
 struct PairTile {
   double x[kChunk], y[kChunk], z[kChunk], q[kChunk];
   int type[kChunk], mol[kChunk], idx[kChunk];
 };
+
+// In-cutoff pairs of one tile listed per wave, evaluated 64 at a time.
+constexpr int kPairList = 512;  // list entries per wave and window
+
+// Sum of v over the lanes of each run of equal, non-decreasing keys, in a
+// fixed tree order (inclusive scan; the run's last lane holds the total).
+__device__ __forceinline__ double SegScan(double v, int key, int lane) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const double u = __shfl_up(v, d);
+    const int ku = __shfl_up(key, d);
+    if (lane >= d && ku == key) v += u;
+  }
+  return v;
+}
 
 template <int kGroups>
 __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairForceArgs a) {
@@ -135,17 +97,33 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
   __shared__ double sf[kGroups][3][kChunk];
   __shared__ double red[kGroups * kChunk / kWave];
   __shared__ PairTile tile[kGroups];
+  __shared__ PairTile home;
+  __shared__ uint16_t plist[kGroups][kPairList];
   __shared__ double poly[kErfcxDeg + 1];
   __shared__ int nbS[27], nbE[27], tileS[28];
   const int chunkId = blockIdx.x / a.slices, slice = blockIdx.x - chunkId * a.slices;
   const int cell = a.chunks[3 * chunkId], off = a.chunks[3 * chunkId + 1],
             cnt = a.chunks[3 * chunkId + 2];
+  const int lane = threadIdx.x % kChunk, grp = threadIdx.x / kChunk;
+  const bool valid = lane < cnt;
+  const int i = valid ? a.cellVec[a.cellStart[cell] + off + lane] : 0;
   if (threadIdx.x <= kErfcxDeg) poly[threadIdx.x] = kErfcxPoly[threadIdx.x];
   if (threadIdx.x < 27) {
     const int nb = a.nbr[cell * 27 + threadIdx.x];
     nbS[threadIdx.x] = a.cellStart[nb];
     nbE[threadIdx.x] = a.cellStart[nb + 1];
   }
+  if (grp == 0) {
+    home.x[lane] = a.x[i];
+    home.y[lane] = a.y[i];
+    home.z[lane] = a.z[i];
+    home.q[lane] = a.q[i];
+    home.type[lane] = a.type[i];
+    home.idx[lane] = i;
+  }
+  sf[grp][0][lane] = 0.0;
+  sf[grp][1][lane] = 0.0;
+  sf[grp][2][lane] = 0.0;
   __syncthreads();
   if (threadIdx.x == 0) {
     int t = 0;
@@ -156,13 +134,11 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
     tileS[27] = t;
   }
   __syncthreads();
-  const int lane = threadIdx.x % kChunk, grp = threadIdx.x / kChunk;
-  const bool valid = lane < cnt;
-  const int i = valid ? a.cellVec[a.cellStart[cell] + off + lane] : 0;
-  const double xi = a.x[i], yi = a.y[i], zi = a.z[i], qi = a.q[i];
-  const int ti = a.type[i], mi = valid ? a.mol[i] : -1;
-  double fx = 0.0, fy = 0.0, fz = 0.0, lj = 0.0, real = 0.0;
+  const double xi = home.x[lane], yi = home.y[lane], zi = home.z[lane];
+  const int mi = valid ? a.mol[i] : -1;
+  double lj = 0.0, real = 0.0;
   PairTile &t = tile[grp];
+  uint16_t *list = plist[grp];
   const int ntiles = tileS[27], stride = a.slices * kGroups;
   int c = 0;
   for (int tt = slice * kGroups + grp; tt < ntiles; tt += stride) {
@@ -182,9 +158,6 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // the tile's in-cutoff partners of this lane as a bit mask first, so
-    // the wave runs the pair math about as many times as the busiest lane
-    // has partners rather than once per tile entry
     uint64_t near = 0;
     if (valid)
       for (int k = 0; k < n; ++k) {
@@ -194,33 +167,66 @@ __global__ void __launch_bounds__(kGroups * kChunk) PairForceKernel(const PairFo
         const double d2 = dx * dx + dy * dy + dz * dz;
         if (t.mol[k] != mi && a.g.rCutSq > d2) near |= uint64_t(1) << k;
       }
-    while (near) {
-      const int k = __builtin_ctzll(near);
-      near &= near - 1;
-      const int j = t.idx[k];
-      const double dx = MinImage(xi - t.x[k], a.g.L[0], a.g.half[0]);
-      const double dy = MinImage(yi - t.y[k], a.g.L[1], a.g.half[1]);
-      const double dz = MinImage(zi - t.z[k], a.g.L[2], a.g.half[2]);
-      const double d2 = dx * dx + dy * dy + dz * dz;
-      const int idx = ti + t.type[k] * a.ntypes;
-      const PairTerms p = PairMath(a.g, poly, d2, qi * t.q[k] * kQQFact, a.sigSq[idx],
-                                   a.epsCn[idx], a.nHalf[idx], a.epsCn6[idx], a.nOver6[idx],
-                                   i < j);
-      real += p.eReal;
-      lj += p.eLJ;
-      const double vLJ = p.vLJ, vReal = p.vReal;
-      fx += (dx * vLJ + dx * vReal);
-      fy += (dy * vLJ + dy * vReal);
-      fz += (dz * vLJ + dz * vReal);
+    // the wave's pairs in (lane, k) order: lane's pairs start at `pos`
+    const int mine = __popcll(near);
+    int incl = mine;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int u = __shfl_up(incl, d);
+      if (lane >= d) incl += u;
+    }
+    const int total = __shfl(incl, kWave - 1);
+    int pos = incl - mine;
+    for (int b = 0; b < total; b += kPairList) {
+      while (near && pos < b + kPairList) {
+        list[pos - b] = uint16_t((lane << 6) | __builtin_ctzll(near));
+        near &= near - 1;
+        ++pos;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int nb = min(kPairList, total - b);
+      for (int r = 0; r < nb; r += kWave) {
+        const bool on = r + lane < nb;
+        const int e = on ? list[r + lane] : 0;
+        const int owner = on ? e >> 6 : kWave, k = e & 63;
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        if (on) {
+          const double dx = MinImage(home.x[owner] - t.x[k], a.g.L[0], a.g.half[0]);
+          const double dy = MinImage(home.y[owner] - t.y[k], a.g.L[1], a.g.half[1]);
+          const double dz = MinImage(home.z[owner] - t.z[k], a.g.L[2], a.g.half[2]);
+          const double d2 = dx * dx + dy * dy + dz * dz;
+          const int idx = home.type[owner] + t.type[k] * a.ntypes;
+          const PairTerms p = PairMath(a.g, poly, d2, home.q[owner] * t.q[k] * kQQFact,
+                                       a.sigSq[idx], a.epsCn[idx], a.nHalf[idx], a.epsCn6[idx],
+                                       a.nOver6[idx], home.idx[owner] < t.idx[k]);
+          real += p.eReal;
+          lj += p.eLJ;
+          fx = dx * p.vLJ + dx * p.vReal;
+          fy = dy * p.vLJ + dy * p.vReal;
+          fz = dz * p.vLJ + dz * p.vReal;
+        }
+        fx = SegScan(fx, owner, lane);
+        fy = SegScan(fy, owner, lane);
+        fz = SegScan(fz, owner, lane);
+        const int nextOwner = __shfl_down(owner, 1);
+        if (on && (lane == kWave - 1 || nextOwner != owner)) {
+          sf[grp][0][owner] += fx;
+          sf[grp][1][owner] += fy;
+          sf[grp][2][owner] += fz;
+        }
+      }
+      // the list is read before the next window overwrites it
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     // every lane is done with the tile before the next one overwrites it
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  sf[grp][0][lane] = fx;
-  sf[grp][1][lane] = fy;
-  sf[grp][2][lane] = fz;
   Partial4 part{0, 0, 0, 0};
   part.lj = BlockSum(lj, red);  // synchronises the LDS writes above
   part.real = BlockSum(real, red);
@@ -702,8 +708,7 @@ int MPPairForce(gomc_box *b, int set, double *lj, double *real) {
   if (n >= kPairChunksFor4) {
     G = 4;
   } else if (n < kPairChunksFor8) {
-    G = 16;
-    slices = std::min(kPairMaxSlices, std::max(1, (kPairSmallWaves + 16 * n - 1) / (16 * n)));
+    slices = std::min(kPairMaxSlices, std::max(1, (kPairSmallWaves + 8 * n - 1) / (8 * n)));
   }
   if (slices > 1) {
     const size_t need = size_t(slices) * 3 * size_t(b->natoms);
@@ -745,9 +750,7 @@ int MPPairForce(gomc_box *b, int set, double *lj, double *real) {
   a.natoms = b->natoms;
   a.spart = mp.spart;
   const dim3 grid(n * slices);
-  if (G == 16)
-    hipLaunchKernelGGL(PairForceKernel<16>, grid, dim3(16 * kChunk), 0, b->stream, a);
-  else if (G == 8)
+  if (G == 8)
     hipLaunchKernelGGL(PairForceKernel<8>, grid, dim3(8 * kChunk), 0, b->stream, a);
   else
     hipLaunchKernelGGL(PairForceKernel<4>, grid, dim3(4 * kChunk), 0, b->stream, a);

@@ -6,7 +6,10 @@
 // PairVirialKernel: one workgroup per (cell, neighbour slot), the same
 // enumeration as BoxInterKernel: for every inter-molecular pair inside the
 // box cutoff, F/r of the LJ and erfc-Coulomb terms times the atom
-// separation's and the minimum-image centre separation's components.
+// separation's and the minimum-image centre separation's components.  A
+// wave takes (64 cell atoms, 64 neighbour atoms) items and queues their
+// in-cutoff pairs in its LDS, so the pair terms run once per 64 queued
+// pairs, not once per 64 tested.
 // RecipVirialKernel: blocks [0, kBlocks) one k-vector per thread (the
 // k-space term); the others one atom per thread over one chunk of LDS-staged
 // k data (the intramolecular term: q 2 pref (S_I cos - S_R sin) k_d times
@@ -33,66 +36,132 @@ struct PairVirArgs {
   double *part;                // [blocks][kVirSlots]
 };
 
-// FFParticle::CalcCoulombVir with the Coulomb cutoff (FFParticle.cpp:401-446).
-__device__ __forceinline__ double CoulVirGated(const BoxGeom &g, double d2, double qiqj) {
-  if (g.rCutCoulSq < d2) return 0.0;
-  return CoulombVirial(g, d2, qiqj);
+constexpr int kUnroll = 4;  // neighbour entries tested per step of the pair virial
+
+// 64 atoms of a cell staged in a wave's LDS, with their molecules' centres.
+struct VirTile {
+  double x[kWave], y[kWave], z[kWave], q[kWave], cx[kWave], cy[kWave], cz[kWave];
+  int idx[kWave], type[kWave], mol[kWave];
+};
+
+__device__ __forceinline__ void StageVirTile(const PairVirArgs &a, VirTile &t, int first, int n,
+                                             int lane) {
+  if (lane < n) {
+    const int j = a.cellVec[first + lane], m = a.mol[j];
+    t.idx[lane] = j;
+    t.x[lane] = a.x[j];
+    t.y[lane] = a.y[j];
+    t.z[lane] = a.z[j];
+    t.q[lane] = a.q[j];
+    t.type[lane] = a.type[j];
+    t.mol[lane] = m;
+    t.cx[lane] = a.cx[m];
+    t.cy[lane] = a.cy[m];
+    t.cz[lane] = a.cz[m];
+  }
 }
 
-__global__ void __launch_bounds__(kBlock) PairVirialKernel(const PairVirArgs a) {
-  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock];
-  __shared__ int sidx[kBlock], stype[kBlock], smol[kBlock];
-  __shared__ double red[kBlock / kWave];
+// The virial terms of one in-cutoff pair (entry u of tile I, entry k of
+// tile J) added to v: FFParticle::CalcCoulombVir and CalcVir
+// (FFParticle.cpp:295-359, 401-446) times the atom and centre separations'
+// components, the Coulomb term with erfc(a r) = exp(-(a r)^2) erfcx(a r)
+// where the reference writes 1 - erf.
+__device__ __forceinline__ void PairVirial(const PairVirArgs &a, const double *poly,
+                                           const VirTile &I, int u, const VirTile &J, int k,
+                                           double v[6]) {
+  const double dx = MinImage(I.x[u] - J.x[k], a.g.L[0], a.g.half[0]);
+  const double dy = MinImage(I.y[u] - J.y[k], a.g.L[1], a.g.half[1]);
+  const double dz = MinImage(I.z[u] - J.z[k], a.g.L[2], a.g.half[2]);
+  const double d2 = dx * dx + dy * dy + dz * dz;
+  const double cdx = MinImage(I.cx[u] - J.cx[k], a.g.L[0], a.g.half[0]);
+  const double cdy = MinImage(I.cy[u] - J.cy[k], a.g.L[1], a.g.half[1]);
+  const double cdz = MinImage(I.cz[u] - J.cz[k], a.g.L[2], a.g.half[2]);
+  const int idx = I.type[u] + J.type[k] * a.ntypes;
+  const PairTerms t = PairMath(a.g, poly, d2, I.q[u] * J.q[k], a.sigSq[idx], 0.0, a.nHalf[idx],
+                               a.epsCn6[idx], a.nOver6[idx], false);
+  v[0] += t.vLJ * (dx * cdx);
+  v[1] += t.vLJ * (dy * cdy);
+  v[2] += t.vLJ * (dz * cdz);
+  v[3] += t.vReal * (dx * cdx);
+  v[4] += t.vReal * (dy * cdy);
+  v[5] += t.vReal * (dz * cdz);
+}
+
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The block's (cell chunk of 64, neighbour tile of 64) items go to its
+// waves round robin.  A wave stages both tiles in its LDS, each lane holds
+// one cell atom and walks the neighbour tile kUnroll entries at a time, and
+// the in-cutoff pairs are queued; the pair terms run once per 64 queued
+// pairs (and once for an item's rest).  The sums' order is fixed by the
+// positions.
+__global__ void __launch_bounds__(kBlock, 4) PairVirialKernel(const PairVirArgs a) {
+  constexpr int kWaves = kBlock / kWave;
+  __shared__ VirTile tileI[kWaves], tileJ[kWaves];
+  __shared__ int pairList[kWaves][(kUnroll + 1) * kWave];
+  __shared__ double red[kWaves];
+  __shared__ double poly[kErfcxDeg + 1];
+  if (threadIdx.x <= kErfcxDeg) poly[threadIdx.x] = kErfcxPoly[threadIdx.x];
+  __syncthreads();
   const int cell = blockIdx.x / 27, nb = a.nbr[blockIdx.x];
   const int cs = a.cellStart[cell], ce = a.cellStart[cell + 1];
   const int ns = a.cellStart[nb], ne = a.cellStart[nb + 1];
-  const int ni = ce - cs;
+  const int ic = (ce - cs + kWave - 1) / kWave, jc = (ne - ns + kWave - 1) / kWave;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  VirTile &I = tileI[w], &J = tileJ[w];
+  int *q = pairList[w];
+  const uint64_t below = (uint64_t(1) << lane) - 1;
   double v[6] = {0, 0, 0, 0, 0, 0};  // LJ xx yy zz, real xx yy zz
-  for (int tile = ns; tile < ne; tile += kBlock) {
-    const int tn = min(kBlock, ne - tile);
-    if (threadIdx.x < tn) {
-      const int j = a.cellVec[tile + threadIdx.x];
-      sidx[threadIdx.x] = j;
-      sx[threadIdx.x] = a.x[j];
-      sy[threadIdx.x] = a.y[j];
-      sz[threadIdx.x] = a.z[j];
-      sq[threadIdx.x] = a.q[j];
-      stype[threadIdx.x] = a.type[j];
-      smol[threadIdx.x] = a.mol[j];
-    }
-    __syncthreads();
-    for (int p = threadIdx.x; p < ni * tn; p += kBlock) {
-      const int i = a.cellVec[cs + p / tn], jj = p % tn, j = sidx[jj];
-      const int mi = a.mol[i], mj = smol[jj];
-      if (i < j && mi != mj) {
-        const double dx = MinImage(a.x[i] - sx[jj], a.g.L[0], a.g.half[0]);
-        const double dy = MinImage(a.y[i] - sy[jj], a.g.L[1], a.g.half[1]);
-        const double dz = MinImage(a.z[i] - sz[jj], a.g.L[2], a.g.half[2]);
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        if (a.g.rCutSq > d2) {
-          const double cdx = MinImage(a.cx[mi] - a.cx[mj], a.g.L[0], a.g.half[0]);
-          const double cdy = MinImage(a.cy[mi] - a.cy[mj], a.g.L[1], a.g.half[1]);
-          const double cdz = MinImage(a.cz[mi] - a.cz[mj], a.g.L[2], a.g.half[2]);
-          if (a.g.elect) {
-            const double qiqj = a.q[i] * sq[jj];
-            if (qiqj != 0.0) {
-              const double pRF = CoulVirGated(a.g, d2, qiqj);
-              v[3] += pRF * (dx * cdx);
-              v[4] += pRF * (dy * cdy);
-              v[5] += pRF * (dz * cdz);
-            }
-          }
-          if (!(a.g.rCutLJSq < d2)) {
-            const int idx = a.type[i] + stype[jj] * a.ntypes;
-            const double pVF = LJVirial(d2, a.sigSq[idx], a.epsCn6[idx], a.nOver6[idx], a.nHalf[idx]);
-            v[0] += pVF * (dx * cdx);
-            v[1] += pVF * (dy * cdy);
-            v[2] += pVF * (dz * cdz);
+  for (int item = w; item < ic * jc; item += kWaves) {
+    const int i0 = cs + (item / jc) * kWave, j0 = ns + (item % jc) * kWave;
+    const int ni = min(kWave, ce - i0), nj = min(kWave, ne - j0);
+    WaveSync();  // the previous item's reads are done
+    StageVirTile(a, I, i0, ni, lane);
+    StageVirTile(a, J, j0, nj, lane);
+    WaveSync();
+    const bool iv = lane < ni;
+    const int i = iv ? I.idx[lane] : 0, mi = iv ? I.mol[lane] : -1;
+    const double xi = iv ? I.x[lane] : 0.0, yi = iv ? I.y[lane] : 0.0, zi = iv ? I.z[lane] : 0.0;
+    // kUnroll tile entries per step, each lane testing its atom against
+    // them; the wave queues the in-cutoff pairs (i < j, other molecule)
+    int queued = 0;
+    for (int k0 = 0; k0 < nj || queued > 0; k0 += kUnroll) {
+      if (k0 < nj) {
+        bool hit[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const int k = k0 + u;
+          hit[u] = false;
+          if (iv && k < nj && i < J.idx[k] && mi != J.mol[k]) {
+            const double dx = MinImage(xi - J.x[k], a.g.L[0], a.g.half[0]);
+            const double dy = MinImage(yi - J.y[k], a.g.L[1], a.g.half[1]);
+            const double dz = MinImage(zi - J.z[k], a.g.L[2], a.g.half[2]);
+            hit[u] = a.g.rCutSq > dx * dx + dy * dy + dz * dz;
           }
         }
+#pragma unroll
+        for (int u = 0; u < kUnroll; ++u) {
+          const uint64_t m = __ballot(hit[u]);
+          if (hit[u]) q[queued + __popcll(m & below)] = (lane << 6) | (k0 + u);
+          queued += __popcll(m);
+        }
+      }
+      // full batches, or the item's last pairs
+      while (queued >= kWave || (queued > 0 && k0 + kUnroll >= nj)) {
+        WaveSync();
+        const int n = min(queued, kWave);
+        if (lane < n) {
+          const int e = q[queued - n + lane];
+          PairVirial(a, poly, I, e >> 6, J, e & 63, v);
+        }
+        queued -= n;
+        WaveSync();
       }
     }
-    __syncthreads();
   }
   for (int c = 0; c < 6; ++c) {
     const double s = BlockSum(v[c], red);
