@@ -413,8 +413,9 @@ __global__ void __launch_bounds__(kBlock) BoxInterKernel(const BoxInterArgs a) {
 // consecutive nz at fixed (nx, ny).  KSegmentsKernel cuts it into segments:
 // pieces of a run of at most kSeg vectors.  Along a segment the phase of an
 // atom advances by one complex product, e^{i k_{n+1} . r} =
-// e^{i k_n . r} e^{i dz z}, so a segment costs two sincos per atom (its
-// first vector and the step) instead of one per vector.  The first pass
+// e^{i k_n . r} e^{i dz z}, with dz = 2 pi / L_z for every segment, so a
+// segment costs one sincos per atom (its first vector; the step's phase is
+// made once per atom) and one complex product per vector.  The first pass
 // takes one segment per thread and one chunk of atoms per work item
 // (per-chunk sums into part[chunk][k][2]); the second adds the chunks in
 // order, one k per thread.  Both passes read the segment count from device
@@ -530,11 +531,11 @@ __global__ void __launch_bounds__(kBlock) RecipSumsPartKernel(const SumsArgs a) 
 #pragma unroll
           for (int i = 0; i < kSeg; ++i) {
             if (i < len) {
-              sR[i] += qj * re;
-              sI[i] += qj * im;
+              sR[i] = fma(qj, re, sR[i]);
+              sI[i] = fma(qj, im, sI[i]);
             }
-            const double nr = re * cs - im * ss;
-            im = re * ss + im * cs;
+            const double nr = fma(re, cs, -(im * ss));
+            im = fma(re, ss, im * cs);
             re = nr;
           }
         }

@@ -274,10 +274,11 @@ __global__ void __launch_bounds__(kChunk) PairForceSumKernel(const PairForceArgs
 // Ewald::BoxForceReciprocal (Ewald.cpp:1496-1596), one atom per thread:
 // the k-space force F_j = sum_k 2 q_j prefact_k (sin ReS - cos ImS) k over
 // the current structure factors, walked segment by segment (pieces of at
-// most kSeg consecutive-nz k-vectors, built with the box's k-vectors):
-// two sincos per segment (its first vector and the nz step), then one
-// complex product per vector.  The k data of a tile of segments is staged
-// through LDS.  Then the intramolecular erf correction is subtracted and
+// most kSeg consecutive-nz k-vectors, built with the box's k-vectors): one
+// sincos per segment (its first vector; the nz step's phase is the atom's
+// own), then one complex product and six FMAs per vector against
+// pref S_R k and pref S_I k, staged per tile of segments in LDS; the 2 q
+// factor is applied once at the end.  Then the intramolecular erf correction is subtracted and
 // the chunks added in order.
 constexpr int kForceSegTile = 64;  // segments per LDS tile (at most 1024 k)
 
@@ -300,7 +301,10 @@ struct RecipForceArgs {
 };
 
 __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceArgs a) {
+  // per k of the tile: pref S_R k_d and pref S_I k_d (d = x, y, z); per
+  // segment: its first k-vector
   __shared__ double sk[6][kForceSegTile * kSeg];
+  __shared__ double sf[3][kForceSegTile];
   __shared__ int ss[kForceSegTile + 1];
   const int chunk = blockIdx.x % a.kChunks;
   const int p = blockIdx.x / a.kChunks * kBlock + threadIdx.x;
@@ -322,40 +326,53 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
     const int tn = min(kForceSegTile, s1 - t);
     const int kA = a.kseg[t], kn = a.kseg[t + tn] - kA;
     __syncthreads();
-    for (int i = threadIdx.x; i <= tn; i += kBlock) ss[i] = a.kseg[t + i] - kA;
+    for (int i = threadIdx.x; i <= tn; i += kBlock) {
+      const int k = a.kseg[t + i];
+      ss[i] = k - kA;
+      if (i < tn) {
+        sf[0][i] = a.kx[k];
+        sf[1][i] = a.ky[k];
+        sf[2][i] = a.kz[k];
+      }
+    }
     for (int i = threadIdx.x; i < kn; i += kBlock) {
-      sk[0][i] = a.kx[kA + i];
-      sk[1][i] = a.ky[kA + i];
-      sk[2][i] = a.kz[kA + i];
-      sk[3][i] = a.pref[kA + i];
-      sk[4][i] = a.sumR[kA + i];
-      sk[5][i] = a.sumI[kA + i];
+      const int k = kA + i;
+      const double pr = a.pref[k] * a.sumR[k], pi = a.pref[k] * a.sumI[k];
+      sk[0][i] = pr * a.kx[k];
+      sk[1][i] = pi * a.kx[k];
+      sk[2][i] = pr * a.ky[k];
+      sk[3][i] = pi * a.ky[k];
+      sk[4][i] = pr * a.kz[k];
+      sk[5][i] = pi * a.kz[k];
     }
     __syncthreads();
     if (on)
       for (int j = 0; j < tn; ++j) {
         const int k0 = ss[j], len = ss[j + 1] - k0;
         double im, re;
-        sincos(xp * sk[0][k0] + yp * sk[1][k0] + zp * sk[2][k0], &im, &re);
+        sincos(xp * sf[0][j] + yp * sf[1][j] + zp * sf[2][j], &im, &re);
 #pragma unroll
         for (int i = 0; i < kSeg; ++i) {
           if (i < len) {
             const int k = k0 + i;
-            const double factor = 2.0 * qp * sk[3][k] * 1.0 * (im * sk[4][k] - re * sk[5][k]);
-            X += factor * sk[0][k];
-            Y += factor * sk[1][k];
-            Z += factor * sk[2][k];
+            X = fma(im, sk[0][k], X);
+            X = fma(-re, sk[1][k], X);
+            Y = fma(im, sk[2][k], Y);
+            Y = fma(-re, sk[3][k], Y);
+            Z = fma(im, sk[4][k], Z);
+            Z = fma(-re, sk[5][k], Z);
           }
-          const double nr = re * cn - im * sn;
-          im = re * sn + im * cn;
+          const double nr = fma(re, cn, -(im * sn));
+          im = fma(re, sn, im * cn);
           re = nr;
         }
       }
   }
   if (active) {
-    a.part[(size_t(chunk) * 3 + 0) * a.natoms + p] = X;
-    a.part[(size_t(chunk) * 3 + 1) * a.natoms + p] = Y;
-    a.part[(size_t(chunk) * 3 + 2) * a.natoms + p] = Z;
+    const double q2 = 2.0 * qp;
+    a.part[(size_t(chunk) * 3 + 0) * a.natoms + p] = q2 * X;
+    a.part[(size_t(chunk) * 3 + 1) * a.natoms + p] = q2 * Y;
+    a.part[(size_t(chunk) * 3 + 2) * a.natoms + p] = q2 * Z;
   }
 }
 

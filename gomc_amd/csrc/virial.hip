@@ -190,7 +190,8 @@ struct RecipVirArgs {
 
 __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a) {
   __shared__ double red[kBlock / kWave];
-  __shared__ double sk[6][kVirSegTile * kSeg];
+  __shared__ double sk[6][kVirSegTile * kSeg];  // pref S_R k_d, pref S_I k_d per k
+  __shared__ double sf[3][kVirSegTile];           // each segment's first k-vector
   __shared__ int ss[kVirSegTile + 1];
   double w[3] = {0, 0, 0};
   if (int(blockIdx.x) < a.kBlocks) {
@@ -221,47 +222,65 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
         ddz = Unwrap(zp, cz, a.g.L[2], a.g.half[2]) - cz;
       }
     }
-    // this chunk's segments, kVirSegTile at a time: two sincos per
-    // (atom, segment), the segment's first phase and its nz step, then one
-    // complex product per k-vector
+    // this chunk's segments, kVirSegTile at a time: one sincos per
+    // (atom, segment), the segment's first phase (the nz step's phase is
+    // the atom's own), then one complex product and six FMAs per k-vector
+    // against pref S_R k and pref S_I k; 2 q times the offset at the end
     const int chunk = (blockIdx.x - a.kBlocks) % a.kChunks;
     const int s0 = min(a.nseg, chunk * a.segChunk), s1 = min(a.nseg, s0 + a.segChunk);
     double sn = 0.0, cn = 1.0;  // the atom's phase step e^{i c_z z}, every segment's
     if (on) sincos(a.kzStep * zp, &sn, &cn);
+    double W[3] = {0.0, 0.0, 0.0};
     for (int t = s0; t < s1; t += kVirSegTile) {
       const int tn = min(kVirSegTile, s1 - t);
       const int kA = a.kseg[t], kn = a.kseg[t + tn] - kA;
       __syncthreads();
-      for (int i = threadIdx.x; i <= tn; i += kBlock) ss[i] = a.kseg[t + i] - kA;
+      for (int i = threadIdx.x; i <= tn; i += kBlock) {
+        const int k = a.kseg[t + i];
+        ss[i] = k - kA;
+        if (i < tn) {
+          sf[0][i] = a.kx[k];
+          sf[1][i] = a.ky[k];
+          sf[2][i] = a.kz[k];
+        }
+      }
       for (int i = threadIdx.x; i < kn; i += kBlock) {
-        sk[0][i] = a.kx[kA + i];
-        sk[1][i] = a.ky[kA + i];
-        sk[2][i] = a.kz[kA + i];
-        sk[3][i] = a.pref[kA + i];
-        sk[4][i] = a.sumR[kA + i];
-        sk[5][i] = a.sumI[kA + i];
+        const int k = kA + i;
+        const double pr = a.pref[k] * a.sumR[k], pi = a.pref[k] * a.sumI[k];
+        sk[0][i] = pr * a.kx[k];
+        sk[1][i] = pi * a.kx[k];
+        sk[2][i] = pr * a.ky[k];
+        sk[3][i] = pi * a.ky[k];
+        sk[4][i] = pr * a.kz[k];
+        sk[5][i] = pi * a.kz[k];
       }
       __syncthreads();
       if (on)
         for (int j = 0; j < tn; ++j) {
           const int k0 = ss[j], len = ss[j + 1] - k0;
           double s, c;
-          sincos(xp * sk[0][k0] + yp * sk[1][k0] + zp * sk[2][k0], &s, &c);
+          sincos(xp * sf[0][j] + yp * sf[1][j] + zp * sf[2][j], &s, &c);
 #pragma unroll
           for (int i = 0; i < kSeg; ++i) {
             if (i < len) {
               const int k = k0 + i;
-              const double factor = sk[3][k] * 2.0 * (sk[5][k] * c - sk[4][k] * s) * qp;
-              w[0] += factor * (sk[0][k] * ddx);
-              w[1] += factor * (sk[1][k] * ddy);
-              w[2] += factor * (sk[2][k] * ddz);
+              W[0] = fma(c, sk[1][k], W[0]);
+              W[0] = fma(-s, sk[0][k], W[0]);
+              W[1] = fma(c, sk[3][k], W[1]);
+              W[1] = fma(-s, sk[2][k], W[1]);
+              W[2] = fma(c, sk[5][k], W[2]);
+              W[2] = fma(-s, sk[4][k], W[2]);
             }
-            const double nc = c * cn - s * sn;
-            s = c * sn + s * cn;
+            const double nc = fma(c, cn, -(s * sn));
+            s = fma(c, sn, s * cn);
             c = nc;
           }
         }
     }
+    const double q2 = 2.0 * qp;
+    w[0] = q2 * (W[0] * ddx);
+    w[1] = q2 * (W[1] * ddy);
+    w[2] = q2 * (W[2] * ddz);
   }
   for (int c = 0; c < 3; ++c) {
     const double s = BlockSum(w[c], red);
