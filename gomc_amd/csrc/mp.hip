@@ -303,7 +303,7 @@ struct RecipForceArgs {
 __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceArgs a) {
   // per k of the tile: pref S_R k_d and pref S_I k_d (d = x, y, z); per
   // segment: its first k-vector
-  __shared__ double sk[6][kForceSegTile * kSeg];
+  __shared__ double2 sk[3][kForceSegTile * kSeg];  // (pref S_R k_d, pref S_I k_d), one 16-byte read
   __shared__ double sf[3][kForceSegTile];
   __shared__ int ss[kForceSegTile + 1];
   const int chunk = blockIdx.x % a.kChunks;
@@ -338,12 +338,9 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
     for (int i = threadIdx.x; i < kn; i += kBlock) {
       const int k = kA + i;
       const double pr = a.pref[k] * a.sumR[k], pi = a.pref[k] * a.sumI[k];
-      sk[0][i] = pr * a.kx[k];
-      sk[1][i] = pi * a.kx[k];
-      sk[2][i] = pr * a.ky[k];
-      sk[3][i] = pi * a.ky[k];
-      sk[4][i] = pr * a.kz[k];
-      sk[5][i] = pi * a.kz[k];
+      sk[0][i] = make_double2(pr * a.kx[k], pi * a.kx[k]);
+      sk[1][i] = make_double2(pr * a.ky[k], pi * a.ky[k]);
+      sk[2][i] = make_double2(pr * a.kz[k], pi * a.kz[k]);
     }
     __syncthreads();
     if (on)
@@ -355,12 +352,13 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
         for (int i = 0; i < kSeg; ++i) {
           if (i < len) {
             const int k = k0 + i;
-            X = fma(im, sk[0][k], X);
-            X = fma(-re, sk[1][k], X);
-            Y = fma(im, sk[2][k], Y);
-            Y = fma(-re, sk[3][k], Y);
-            Z = fma(im, sk[4][k], Z);
-            Z = fma(-re, sk[5][k], Z);
+            const double2 vx = sk[0][k], vy = sk[1][k], vz = sk[2][k];
+            X = fma(im, vx.x, X);
+            X = fma(-re, vx.y, X);
+            Y = fma(im, vy.x, Y);
+            Y = fma(-re, vy.y, Y);
+            Z = fma(im, vz.x, Z);
+            Z = fma(-re, vz.y, Z);
           }
           const double nr = fma(re, cn, -(im * sn));
           im = fma(re, sn, im * cn);

@@ -190,7 +190,7 @@ struct RecipVirArgs {
 
 __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a) {
   __shared__ double red[kBlock / kWave];
-  __shared__ double sk[6][kVirSegTile * kSeg];  // pref S_R k_d, pref S_I k_d per k
+  __shared__ double2 sk[3][kVirSegTile * kSeg];  // (pref S_R k_d, pref S_I k_d) per k
   __shared__ double sf[3][kVirSegTile];           // each segment's first k-vector
   __shared__ int ss[kVirSegTile + 1];
   double w[3] = {0, 0, 0};
@@ -247,12 +247,9 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
       for (int i = threadIdx.x; i < kn; i += kBlock) {
         const int k = kA + i;
         const double pr = a.pref[k] * a.sumR[k], pi = a.pref[k] * a.sumI[k];
-        sk[0][i] = pr * a.kx[k];
-        sk[1][i] = pi * a.kx[k];
-        sk[2][i] = pr * a.ky[k];
-        sk[3][i] = pi * a.ky[k];
-        sk[4][i] = pr * a.kz[k];
-        sk[5][i] = pi * a.kz[k];
+        sk[0][i] = make_double2(pr * a.kx[k], pi * a.kx[k]);
+        sk[1][i] = make_double2(pr * a.ky[k], pi * a.ky[k]);
+        sk[2][i] = make_double2(pr * a.kz[k], pi * a.kz[k]);
       }
       __syncthreads();
       if (on)
@@ -264,12 +261,13 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
           for (int i = 0; i < kSeg; ++i) {
             if (i < len) {
               const int k = k0 + i;
-              W[0] = fma(c, sk[1][k], W[0]);
-              W[0] = fma(-s, sk[0][k], W[0]);
-              W[1] = fma(c, sk[3][k], W[1]);
-              W[1] = fma(-s, sk[2][k], W[1]);
-              W[2] = fma(c, sk[5][k], W[2]);
-              W[2] = fma(-s, sk[4][k], W[2]);
+              const double2 vx = sk[0][k], vy = sk[1][k], vz = sk[2][k];
+              W[0] = fma(c, vx.y, W[0]);
+              W[0] = fma(-s, vx.x, W[0]);
+              W[1] = fma(c, vy.y, W[1]);
+              W[1] = fma(-s, vy.x, W[1]);
+              W[2] = fma(c, vz.y, W[2]);
+              W[2] = fma(-s, vz.x, W[2]);
             }
             const double nc = fma(c, cn, -(s * sn));
             s = fma(c, sn, s * cn);
