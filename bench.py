@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 # HBM bytes per launch of McBatchKernel<0> on the default 256-box workload,
 # from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a known-size copy
 # (profiles/McBatchKernel_water256_fp64_hbm.md); reported for that workload.
-PROFILED_TRAFFIC = {"bytes": 158.3e9, "source": "profiles/McBatchKernel_water256_fp64_hbm.md"}
+PROFILED_TRAFFIC = {"bytes": 167.5e9, "source": "profiles/McBatchKernel_water256_fp64_hbm.md"}
 # Fixed per-term FLOP weights of the algorithmic work model: distance +
 # minimum image 12, erfc-Coulomb 30, LJ 15, sincos + 2 FMA 24, per-k energy 6.
 W_DIST, W_COUL, W_LJ, W_SINCOS, W_KENERGY = 12, 30, 15, 24, 6
