@@ -427,7 +427,9 @@ struct SumsArgs {
   const double *kx, *ky, *kz, *pref;
   const int *kseg;   // segment starts [nseg + 1], kseg[nseg] = K
   const int *nseg;   // device
-  double cz;         // the nz step of every segment, 2 pi / L_z
+  double cx, cy, cz;  // KVectorsOrth's spacings 2 pi / L_d (cz: every segment's nz step)
+  int nmax[3];        // KVectorsOrth's |n_d| bounds: the phase tables' ranges
+  int tabAtoms;       // atoms per LDS tile (their phase tables fill the dynamic LDS)
   double *sumR, *sumI;
   double *part;      // [chunks][K][2]
   Partial4 *partials;
@@ -487,47 +489,78 @@ __global__ void __launch_bounds__(kSegThreads) KSegmentsKernel(const double *kx,
   }
 }
 
+constexpr int kSumsTabBytes = 48 * 1024;  // dynamic LDS of the phase tables
+constexpr int kSumsTabMaxAtoms = 64;
+
+__host__ __device__ inline int SumsTabEntries(const int nmax[3]) {
+  return nmax[0] + nmax[1] + nmax[2] + 3;
+}
+
 __global__ void __launch_bounds__(kBlock) RecipSumsPartKernel(const SumsArgs a) {
-  // the tile's atoms, and each atom's phase step e^{i c_z z} (the same for
-  // every segment: consecutive nz differ by c_z), made once per atom
-  __shared__ double sx[kBlock], sy[kBlock], sz[kBlock], sq[kBlock], sc[kBlock], ssn[kBlock];
+  // per atom of the tile: e^{i n c_d r_d} for n = 0..nmax_d on each axis (a
+  // negative n is the conjugate), so the first phase of a segment at
+  // (nx, ny, nz0) is two complex products instead of a sincos; the nz step
+  // is the z table's n = 1
+  extern __shared__ double2 tab[];
+  __shared__ double sq[kSumsTabMaxAtoms];
   const int nseg = *a.nseg;
   const int segBlocks = max(1, (nseg + kBlock - 1) / kBlock);
   const int chunks = SumsChunks(nseg, a.natoms, a.K);
   const int aChunk = (a.natoms + chunks - 1) / chunks;
+  const int E = SumsTabEntries(a.nmax), oy = a.nmax[0] + 1, oz = oy + a.nmax[1] + 1;
+  const int T = a.tabAtoms;
   for (int item = blockIdx.x; item < segBlocks * chunks; item += gridDim.x) {
     const int seg = (item % segBlocks) * kBlock + threadIdx.x, chunk = item / segBlocks;
     const bool active = seg < nseg;
     const int k0 = active ? a.kseg[seg] : 0;
     const int len = active ? a.kseg[seg + 1] - k0 : 0;
-    const double kx0 = active ? a.kx[k0] : 0.0, ky0 = active ? a.ky[k0] : 0.0,
-                 kz0 = active ? a.kz[k0] : 0.0;
+    // the segment's integer triple (k = c n, one rounding, so rint recovers n)
+    const int nx = active ? min(a.nmax[0], max(0, int(rint(a.kx[k0] / a.cx)))) : 0;
+    const int ny = active ? min(a.nmax[1], max(-a.nmax[1], int(rint(a.ky[k0] / a.cy)))) : 0;
+    const int nz = active ? min(a.nmax[2], max(-a.nmax[2], int(rint(a.kz[k0] / a.cz)))) : 0;
+    const double syn = ny < 0 ? -1.0 : 1.0, szn = nz < 0 ? -1.0 : 1.0;
+    const int iy = oy + abs(ny), iz = oz + abs(nz);
     double sR[kSeg], sI[kSeg];
 #pragma unroll
     for (int i = 0; i < kSeg; ++i) sR[i] = sI[i] = 0.0;
     const int a0 = min(a.natoms, chunk * aChunk), a1 = min(a.natoms, a0 + aChunk);
-    for (int tile = a0; tile < a1; tile += kBlock) {
-      const int tn = min(kBlock, a1 - tile);
+    for (int tile = a0; tile < a1; tile += T) {
+      const int tn = min(T, a1 - tile);
       __syncthreads();
       if (int(threadIdx.x) < tn) {
         const int j = tile + threadIdx.x;
-        sx[threadIdx.x] = a.x[j];
-        sy[threadIdx.x] = a.y[j];
-        sz[threadIdx.x] = a.z[j];
         sq[threadIdx.x] = (!a.mol || a.mol[j] >= 0) ? a.q[j] : 0.0;  // reservoir adds nothing
-        double sn, cn;
-        sincos(a.cz * a.z[j], &sn, &cn);
-        ssn[threadIdx.x] = sn;
-        sc[threadIdx.x] = cn;
+      }
+      // each (atom, axis) table by one thread: n = 1 by sincos, then one
+      // complex product per n
+      for (int w = threadIdx.x; w < 3 * tn; w += kBlock) {
+        const int at = w / 3, ax = w - 3 * at, j = tile + at;
+        const double r = ax == 0 ? a.x[j] : (ax == 1 ? a.y[j] : a.z[j]);
+        const double c = ax == 0 ? a.cx : (ax == 1 ? a.cy : a.cz);
+        const int o = ax == 0 ? 0 : (ax == 1 ? oy : oz), n1 = a.nmax[ax];
+        double2 *t = tab + size_t(at) * E + o;
+        double s1, c1;
+        sincos(c * r, &s1, &c1);
+        t[0] = make_double2(1.0, 0.0);
+        double re = c1, im = s1;
+        for (int n = 1; n <= n1; ++n) {
+          t[n] = make_double2(re, im);
+          const double nr = fma(re, c1, -(im * s1));
+          im = fma(re, s1, im * c1);
+          re = nr;
+        }
       }
       __syncthreads();
       if (active)
         for (int jj = 0; jj < tn; ++jj) {
           const double qj = sq[jj];
           if (fabs(qj) < 0.000000001) continue;
-          double im, re;
-          sincos(sx[jj] * kx0 + sy[jj] * ky0 + sz[jj] * kz0, &im, &re);
-          const double ss = ssn[jj], cs = sc[jj];
+          const double2 *t = tab + size_t(jj) * E;
+          const double2 ex = t[nx], ey = t[iy], ez = t[iz], st = t[oz + 1];
+          const double eyi = syn * ey.y, ezi = szn * ez.y;
+          const double pr = fma(ex.x, ey.x, -(ex.y * eyi)), pi = fma(ex.x, eyi, ex.y * ey.x);
+          double re = fma(pr, ez.x, -(pi * ezi)), im = fma(pr, ezi, pi * ez.x);
+          const double ss = st.y, cs = st.x;
 #pragma unroll
           for (int i = 0; i < kSeg; ++i) {
             if (i < len) {
@@ -1087,7 +1120,8 @@ int BoxInter(gomc_box *b, double *lj, double *real) {
 // counter (zero, re-armed by the last block) and result [4] (result[2]
 // receives the energy).
 hipError_t LaunchRecipSums(const SumsArgs &a, hipStream_t s) {
-  hipLaunchKernelGGL(RecipSumsPartKernel, dim3(kSumsGrid), dim3(kBlock), 0, s, a);
+  const size_t tabBytes = size_t(a.tabAtoms) * SumsTabEntries(a.nmax) * sizeof(double2);
+  hipLaunchKernelGGL(RecipSumsPartKernel, dim3(kSumsGrid), dim3(kBlock), tabBytes, s, a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int nblocks = std::max(1, (a.K + kBlock - 1) / kBlock);
@@ -1128,7 +1162,12 @@ static SumsArgs BoxSumsArgs(const gomc_box *b) {
   a.pref = b->pref;
   a.kseg = b->kseg;
   a.nseg = b->nsegDev;
-  a.cz = (1.0 / b->geom.L[2]) * (2.0 * M_PI);  // KVectorsOrth's z spacing
+  a.cx = (1.0 / b->geom.L[0]) * (2.0 * M_PI);  // KVectorsOrth's spacings and ranges
+  a.cy = (1.0 / b->geom.L[1]) * (2.0 * M_PI);
+  a.cz = (1.0 / b->geom.L[2]) * (2.0 * M_PI);
+  for (int i = 0; i < 3; ++i) a.nmax[i] = int(b->recipRcut * b->geom.L[i] / (2.0 * M_PI)) + 1;
+  a.tabAtoms = std::max(1, std::min(kSumsTabMaxAtoms, kSumsTabBytes / int(SumsTabEntries(a.nmax) *
+                                                                           sizeof(double2))));
   a.sumR = b->sumRnew;
   a.sumI = b->sumInew;
   a.result = b->devRes;
