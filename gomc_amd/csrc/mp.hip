@@ -275,8 +275,9 @@ __global__ void __launch_bounds__(kChunk) PairForceSumKernel(const PairForceArgs
 // the k-space force F_j = sum_k 2 q_j prefact_k (sin ReS - cos ImS) k over
 // the current structure factors, walked segment by segment (pieces of at
 // most kSeg consecutive-nz k-vectors, built with the box's k-vectors): one
-// sincos per segment (its first vector; the nz step's phase is the atom's
-// own), then one complex product and six FMAs per vector against
+// sincos per run of consecutive nz (a segment continuing a run starts at
+// the phase the previous segment rotated to; the nz step's phase is the
+// atom's own), then one complex product and six FMAs per vector against
 // pref S_R k and pref S_I k, staged per tile of segments in LDS; the 2 q
 // factor is applied once at the end.  Then the intramolecular erf correction is subtracted and
 // the chunks added in order.
@@ -322,6 +323,10 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
   double sn = 0.0, cn = 1.0;
   if (on) sincos(a.cz * zp, &sn, &cn);
   const int s0 = min(a.nseg, chunk * a.segChunk), s1 = min(a.nseg, s0 + a.segChunk);
+  // a segment that continues the previous one's run (same nx, ny after a
+  // full segment) starts at the phase the previous one rotated to
+  double im = 0.0, re = 1.0, pkx = 0.0, pky = 0.0;
+  int plen = 0;
   for (int t = s0; t < s1; t += kForceSegTile) {
     const int tn = min(kForceSegTile, s1 - t);
     const int kA = a.kseg[t], kn = a.kseg[t + tn] - kA;
@@ -346,8 +351,12 @@ __global__ void __launch_bounds__(kBlock) RecipForcePartKernel(const RecipForceA
     if (on)
       for (int j = 0; j < tn; ++j) {
         const int k0 = ss[j], len = ss[j + 1] - k0;
-        double im, re;
-        sincos(xp * sf[0][j] + yp * sf[1][j] + zp * sf[2][j], &im, &re);
+        const double kx0 = sf[0][j], ky0 = sf[1][j];
+        if (!(plen == kSeg && kx0 == pkx && ky0 == pky))
+          sincos(xp * kx0 + yp * ky0 + zp * sf[2][j], &im, &re);
+        pkx = kx0;
+        pky = ky0;
+        plen = len;
 #pragma unroll
         for (int i = 0; i < kSeg; ++i) {
           if (i < len) {

@@ -222,15 +222,19 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
         ddz = Unwrap(zp, cz, a.g.L[2], a.g.half[2]) - cz;
       }
     }
-    // this chunk's segments, kVirSegTile at a time: one sincos per
-    // (atom, segment), the segment's first phase (the nz step's phase is
-    // the atom's own), then one complex product and six FMAs per k-vector
+    // this chunk's segments, kVirSegTile at a time: one sincos per run of
+    // consecutive nz, for its first phase (the nz step's phase is the
+    // atom's own), then one complex product and six FMAs per k-vector
     // against pref S_R k and pref S_I k; 2 q times the offset at the end
     const int chunk = (blockIdx.x - a.kBlocks) % a.kChunks;
     const int s0 = min(a.nseg, chunk * a.segChunk), s1 = min(a.nseg, s0 + a.segChunk);
     double sn = 0.0, cn = 1.0;  // the atom's phase step e^{i c_z z}, every segment's
     if (on) sincos(a.kzStep * zp, &sn, &cn);
     double W[3] = {0.0, 0.0, 0.0};
+    // a segment that continues the previous one's run (same nx, ny after a
+    // full segment) starts at the phase the previous one rotated to
+    double s = 0.0, c = 1.0, pkx = 0.0, pky = 0.0;
+    int plen = 0;
     for (int t = s0; t < s1; t += kVirSegTile) {
       const int tn = min(kVirSegTile, s1 - t);
       const int kA = a.kseg[t], kn = a.kseg[t + tn] - kA;
@@ -255,8 +259,12 @@ __global__ void __launch_bounds__(kBlock) RecipVirialKernel(const RecipVirArgs a
       if (on)
         for (int j = 0; j < tn; ++j) {
           const int k0 = ss[j], len = ss[j + 1] - k0;
-          double s, c;
-          sincos(xp * sf[0][j] + yp * sf[1][j] + zp * sf[2][j], &s, &c);
+          const double kx0 = sf[0][j], ky0 = sf[1][j];
+          if (!(plen == kSeg && kx0 == pkx && ky0 == pky))
+            sincos(xp * kx0 + yp * ky0 + zp * sf[2][j], &s, &c);
+          pkx = kx0;
+          pky = ky0;
+          plen = len;
 #pragma unroll
           for (int i = 0; i < kSeg; ++i) {
             if (i < len) {
