@@ -259,10 +259,10 @@ __device__ __forceinline__ PairTerms PairMath(const BoxGeom &g, const double *po
       if (x <= kErfcxUse) {
         const double t = (x - kErfcxMid) * kErfcxInvHalf;
         double px = poly[kErfcxDeg];
-        // read from LDS each step, four steps' reads issued together: fully
+        // read from LDS each step, eight steps' reads issued together: fully
         // unrolled, the coefficients would be held in 58 VGPRs across the
         // pair loop
-#pragma unroll 4
+#pragma unroll 8
         for (int k = kErfcxDeg - 1; k >= 0; --k) px = fma(px, t, poly[k]);
         gauss = exp(-(x * x));
         ec = gauss * px;
