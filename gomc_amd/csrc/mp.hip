@@ -44,9 +44,9 @@ __device__ __forceinline__ double Uneg11(uint64_t v) {
 // times per tile rather than as often as the busiest lane has partners;
 // the force contributions of a batch are summed per home atom by a
 // segmented scan and added to that atom's sum in the wave's LDS, an order
-// fixed by the positions alone.  A workgroup adds its
-// waves' per-atom forces in wave order through LDS; with one slice it
-// writes the chunk's forces, with several each slice writes its own and
+// fixed by the positions alone.  A workgroup adds its waves' per-atom
+// forces in wave order through LDS; with one slice it writes the chunk's
+// forces, with several each slice writes its own and
 // PairForceSumKernel adds them in slice order.  Energies count each pair
 // once (i < j).  The host picks G and the slice count so that a box of any
 // size puts a few thousand waves on the device.
