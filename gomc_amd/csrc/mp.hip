@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(kChunk) PairForceSumKernel(const PairForceArgs
 // pref S_R k and pref S_I k, staged per tile of segments in LDS; the 2 q
 // factor is applied once at the end.  Then the intramolecular erf correction is subtracted and
 // the chunks added in order.
-constexpr int kForceSegTile = 64;  // segments per LDS tile (at most 1024 k)
+constexpr int kForceSegTile = 32;  // segments per LDS tile (at most 512 k)
 
 struct RecipForceArgs {
   BoxGeom g;

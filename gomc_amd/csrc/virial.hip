@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(kBlock, 4) PairVirialKernel(const PairVirArgs 
 
 // Segments of the k list (consecutive nz, at most kSeg vectors) staged in
 // LDS this many at a time by the intramolecular term.
-constexpr int kVirSegTile = 64;
+constexpr int kVirSegTile = 32;
 
 struct RecipVirArgs {
   BoxGeom g;
